@@ -1,0 +1,249 @@
+"""GPU parity of selection / compaction, hash joins, sorting, gathers and code remaps against the
+CPU oracle (bit-exact: these are integer / index results)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import helpers as H
+import warpstore_amd as ws
+from warpstore_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def colview(buf, t):
+    v = abi.ColView()
+    v.p, v.t = buf.p, t
+    return v
+
+
+def hcolview(arr, t):
+    v = abi.ColView()
+    v.p, v.t = arr.ctypes.data, t
+    return v
+
+
+@pytest.mark.parametrize("mode", ["interp", "jit"])
+def test_select_multi(gpu, mode):
+    H.say(f"case select_multi {mode}")
+    n = 250_000
+    rng = np.random.default_rng(7)
+    a = rng.integers(-1000, 1000, n).astype(np.int32)
+    x = rng.random(n)
+    x[rng.random(n) < 0.03] = np.nan
+    deleted = rng.random(n) < 0.1
+    t = H.Table({"a": a, "x": x}, [70_000, 0, 1, 179_999], deleted)
+    b = abi.EnvBuilder()
+    sa, sx = b.slot(abi.T_I32), b.slot(abi.T_F64)
+    where = b.prog([(abi.OP_COL, sa), b.ilit(100), (abi.OP_GT,), (abi.OP_COL, sx), b.dlit(0.5), (abi.OP_LT,), (abi.OP_OR,)])
+    slots = [("a", abi.T_I32, 0), ("x", abi.T_F64, 0)]
+    dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
+    nb = len(t.sizes)
+    bn = np.array(t.sizes, np.int64)
+    jit = None
+    if mode == "jit":
+        gpu.ws_jit_set_mode(abi.JIT_SYNC)
+        fp = abi.AggPlan()
+        fp.has_where, fp.where = 1, where
+        jit = C.c_void_p()
+        ws.check(gpu.ws_jit_get(C.byref(b.env), 2, C.byref(fp), abi.JF_FLAGM, b"filter", C.byref(jit)))
+        assert jit.value
+    out = C.c_void_p()
+    per = np.zeros(nb, np.int64)
+    cnt = C.c_int64()
+    ws.check(gpu.ws_select_multi(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(dev.dels, C.c_void_p),
+                                 C.cast(dev.views, C.c_void_p), 2, C.byref(out), ws.ptr(per), jit, C.byref(cnt)))
+    got = ws.DevBuf(cnt.value * 4, tmp=True, adopt=out.value).to_array(np.int32, cnt.value)
+    exp = np.zeros(n, np.int32)
+    eper = np.zeros(nb, np.int64)
+    m = H.oracle().or_select(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(host.dels, C.c_void_p),
+                             C.cast(host.views, C.c_void_p), 2, ws.ptr(exp), ws.ptr(eper))
+    assert cnt.value == m
+    assert np.array_equal(got, exp[:m])
+    assert np.array_equal(per, eper)
+
+
+def test_select_single_env_cols(gpu):
+    H.say("case select single")
+    n = 100_003
+    v = np.arange(n, dtype=np.int64) * 7 % 1000
+    dv = ws.DevBuf.from_array(v)
+    b = abi.EnvBuilder()
+    s = b.slot(abi.T_I64)
+    b.env.cols[s].p = dv.p
+    where = b.prog([(abi.OP_COL, s), b.ilit(500), (abi.OP_GE,)])
+    out = ws.DevBuf(n * 4)
+    cnt = C.c_int64()
+    ws.check(gpu.ws_select(C.byref(b.env), 1, where, n, None, out.p, None, C.byref(cnt)))
+    got = out.to_array(np.int32, cnt.value)
+    assert np.array_equal(got, np.nonzero(v >= 500)[0].astype(np.int32))
+
+
+def run_join(L, bkey, nb, pkey, np_, outer=0):
+    ob, op, m = C.c_void_p(), C.c_void_p(), C.c_int64()
+    if outer:
+        ws.check(L.ws_hash_join_outer(bkey, nb, pkey, np_, C.byref(ob), C.byref(op), 1 if outer == 2 else 0, C.byref(m)))
+    else:
+        ws.check(L.ws_hash_join(bkey, nb, pkey, np_, C.byref(ob), C.byref(op), C.byref(m)))
+    if m.value == 0:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32)
+    b = ws.DevBuf(m.value * 4, tmp=True, adopt=ob.value).to_array(np.int32, m.value)
+    p = ws.DevBuf(m.value * 4, tmp=True, adopt=op.value).to_array(np.int32, m.value)
+    return b, p
+
+
+def oracle_join(bk, bt, pk, pt, outer=0):
+    cap = len(bk) * len(pk) + len(pk) + len(bk) + 1
+    ob = np.zeros(min(cap, 10_000_000), np.int32)
+    op = np.zeros_like(ob)
+    m = H.oracle().or_join(C.byref(hcolview(bk, bt)), len(bk), C.byref(hcolview(pk, pt)), len(pk), outer, ws.ptr(ob),
+                           ws.ptr(op), len(ob))
+    assert m >= 0
+    return ob[:m], op[:m]
+
+
+@pytest.mark.parametrize("outer", [0, 1, 2])
+def test_hash_join(gpu, outer):
+    H.say(f"case join outer={outer}")
+    rng = np.random.default_rng(13)
+    bk = rng.integers(0, 5000, 20_000).astype(np.int64)  # duplicate build keys
+    bk[::97] = abi.NULL_I64
+    pk = rng.integers(-100, 6000, 300_000).astype(np.int64)
+    pk[::101] = abi.NULL_I64
+    db, dp = ws.DevBuf.from_array(bk), ws.DevBuf.from_array(pk)
+    gb, gp = run_join(gpu, colview(db, abi.T_I64N), len(bk), colview(dp, abi.T_I64N), len(pk), outer)
+    eb, ep = oracle_join(bk, abi.T_I64N, pk, abi.T_I64N, outer)
+    # probe order: the GPU output is ascending in the probe index (chain order inside a row)
+    if outer < 2:
+        assert np.all(np.diff(gp) >= 0)
+    order = np.lexsort((gb, gp))
+    order_e = np.lexsort((eb, ep))
+    assert np.array_equal(gp[order], ep[order_e]) and np.array_equal(gb[order], eb[order_e])
+
+
+def test_hash_join_parts_offsets(gpu):
+    H.say("case join parts")
+    rng = np.random.default_rng(17)
+    P = 4
+    bsz = [3000, 0, 2500, 4000]
+    psz = [50_000, 20_000, 0, 33_333]
+    bk = np.concatenate([rng.integers(0, 2000, s) + 10_000 * i for i, s in enumerate(bsz)]).astype(np.int32)
+    pk = np.concatenate([rng.integers(0, 2500, s) + 10_000 * i for i, s in enumerate(psz)]).astype(np.int32)
+    boff = np.concatenate([[0], np.cumsum(bsz)]).astype(np.int64)
+    poff = np.concatenate([[0], np.cumsum(psz)]).astype(np.int64)
+    db, dp = ws.DevBuf.from_array(bk), ws.DevBuf.from_array(pk)
+    ob, op, m = C.c_void_p(), C.c_void_p(), C.c_int64()
+    off = np.zeros(P + 1, np.int64)
+    ws.check(gpu.ws_hash_join_parts(colview(db, abi.T_I32), ws.ptr(boff), P, colview(dp, abi.T_I32), ws.ptr(poff), P,
+                                    C.byref(ob), C.byref(op), ws.ptr(off), C.byref(m)))
+    gb = ws.DevBuf(m.value * 4, tmp=True, adopt=ob.value).to_array(np.int32, m.value)
+    gp = ws.DevBuf(m.value * 4, tmp=True, adopt=op.value).to_array(np.int32, m.value)
+    eb, ep = oracle_join(bk, abi.T_I32, pk, abi.T_I32)  # keys are partition-disjoint: same pair set
+    assert m.value == len(eb)
+    order = np.lexsort((gb, gp))
+    order_e = np.lexsort((eb, ep))
+    assert np.array_equal(gp[order], ep[order_e]) and np.array_equal(gb[order], eb[order_e])
+    for p in range(P + 1):  # out_off[p] = first output pair of probe partition p
+        assert off[p] == np.searchsorted(gp, poff[p]) if p < P else off[p] == m.value
+
+
+@pytest.mark.parametrize("m", [1, 1000, 300_001])
+def test_sort_perm(gpu, m):
+    H.say(f"case sort m={m}")
+    rng = np.random.default_rng(m)
+    k0 = rng.integers(0, 50, m).astype(np.float64)
+    k0[rng.random(m) < 0.05] = np.nan
+    k1 = rng.standard_normal(m)
+    k1[rng.random(m) < 0.05] = -0.0
+    k1[rng.random(m) < 0.05] = 0.0
+    k2 = np.floor(rng.random(m) * 3)
+    keys = [k0, k1, k2]
+    desc = np.array([1, 0, 1], np.uint8)
+    dks = [ws.DevBuf.from_array(k) for k in keys]
+    kp = (C.c_void_p * 3)(*[d.p for d in dks])
+    perm = C.c_void_p()
+    ws.check(gpu.ws_sort_perm(kp, ws.ptr(desc), 3, m, C.byref(perm)))
+    got = ws.DevBuf(m * 4, tmp=True, adopt=perm.value).to_array(np.int32, m)
+    exp = np.zeros(m, np.int32)
+    hp = (C.c_void_p * 3)(*[k.ctypes.data for k in keys])
+    H.oracle().or_sort_perm(hp, ws.ptr(desc), 3, m, ws.ptr(exp))
+    assert np.array_equal(got, exp)
+
+
+def test_gathers_and_codes(gpu):
+    H.say("case gathers")
+    rng = np.random.default_rng(19)
+    n = 100_000
+    src = rng.integers(-2 ** 40, 2 ** 40, n).astype(np.int64)
+    idx = rng.integers(0, n, 50_000).astype(np.int32)
+    ds, di = ws.DevBuf.from_array(src), ws.DevBuf.from_array(idx)
+    out = ws.DevBuf(len(idx) * 8)
+    ws.check(gpu.ws_gather(ds.p, abi.P_I64, di.p, len(idx), out.p))
+    assert np.array_equal(out.to_array(np.int64), src[idx])
+    idxn = idx.copy()
+    idxn[::3] = -1
+    din = ws.DevBuf.from_array(idxn)
+    ws.check(gpu.ws_gather_null(ds.p, abi.P_I64, din.p, len(idx), out.p))
+    exp = np.where(idxn < 0, abi.NULL_I64, src[np.maximum(idxn, 0)])
+    assert np.array_equal(out.to_array(np.int64), exp)
+    # encoded gather (u16 frame of reference) and gather across batches
+    small = rng.integers(1000, 1000 + 60_000, n).astype(np.int32)
+    t = H.Table({"s": small}, [40_000, 60_000])
+    t.encode("s")
+    bound = H.Bound(t, [("s", abi.T_I32, 0)], True)
+    boff = np.array([0, 40_000], np.int64)
+    out32 = ws.DevBuf(len(idx) * 4)
+    ws.check(gpu.ws_gather_multi(C.cast(bound.views, C.c_void_p), ws.ptr(boff), 2, abi.P_I32, 4, di.p, len(idx), out32.p))
+    assert np.array_equal(out32.to_array(np.int32), small[idx])
+    v0 = bound.views[0]
+    idx0 = idx[idx < 40_000]
+    di0 = ws.DevBuf.from_array(idx0)
+    ws.check(gpu.ws_gather_enc(C.byref(v0), abi.P_I32, di0.p, len(idx0), out32.p))
+    assert np.array_equal(out32.to_array(np.int32, len(idx0)), small[idx0])
+    # dictionary code remap
+    codes = rng.integers(-3, 120, n).astype(np.int32)
+    codes[::11] = abi.NULL_I32
+    cmap = rng.integers(0, 1000, 100).astype(np.int32)
+    dc, dm, do = ws.DevBuf.from_array(codes), ws.DevBuf.from_array(cmap), ws.DevBuf(n * 4)
+    ws.check(gpu.ws_remap_codes(dc.p, n, dm.p, len(cmap), do.p))
+    exp = np.zeros(n, np.int32)
+    H.oracle().or_remap_codes(ws.ptr(codes), n, ws.ptr(cmap), len(cmap), ws.ptr(exp))
+    assert np.array_equal(do.to_array(np.int32), exp)
+    # join-filter key bitmap
+    keys = rng.integers(0, 5000, 3000).astype(np.int32)
+    dk = ws.DevBuf.from_array(keys)
+    nbits = 4000
+    bits = ws.DevBuf(((nbits + 31) // 32) * 4)
+    ws.check(gpu.ws_key_bitmap(colview(dk, abi.T_I32), len(keys), 100, nbits, bits.p))
+    w = bits.to_array(np.uint32)
+    got = np.nonzero(np.unpackbits(w.view(np.uint8), bitorder="little")[:nbits])[0]
+    kk = keys.astype(np.int64) - 100
+    assert np.array_equal(got, np.unique(kk[(kk >= 0) & (kk < nbits)]))
+    # int64 bits -> double
+    bv = rng.integers(-2 ** 50, 2 ** 50, 1000).astype(np.int64)
+    db2, dd = ws.DevBuf.from_array(bv), ws.DevBuf(8000)
+    ws.check(gpu.ws_bits_to_double(db2.p, 1000, dd.p))
+    assert np.array_equal(dd.to_array(np.float64), bv.astype(np.float64))
+
+
+def test_compact_column_major(gpu):
+    """GpuAggregator::compact leaves the groups in VRAM, column-major."""
+    H.say("case compact column-major")
+    k, v, p = H.gen_c1(50_000)
+    t = H.Table({"k": k, "v": v, "p": p}, 1 << 20)
+    b = abi.EnvBuilder()
+    sk, sv, sp = b.slot(abi.T_I32), b.slot(abi.T_I64), b.slot(abi.T_F64)
+    plan = H.plan_c1(b, sk, sv, sp)
+    slots = [("k", abi.T_I32, 0), ("v", abi.T_I64, 0), ("p", abi.T_F64, 0)]
+    dev = H.Bound(t, slots, True)
+    h, _ = H.gpu_aggregate(plan, b.env, dev, cap=1 << 12, fetch=False)
+    try:
+        kp, ap, g = C.c_void_p(), C.c_void_p(), C.c_int64()
+        ws.check(gpu.ws_agg_compact(h, C.byref(kp), C.byref(ap), C.byref(g)))
+        keys = ws.DevBuf(g.value * 8, tmp=True, adopt=kp.value).to_array(np.float64, g.value)
+        accs = ws.DevBuf(g.value * 48, tmp=True, adopt=ap.value).to_array(np.float64, g.value * 6).reshape(6, g.value).T
+    finally:
+        gpu.ws_agg_destroy(h)
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, H.Bound(t, slots, False))
+    H.assert_groups_equal(plan, keys.reshape(-1, 1), accs.copy(), ok, oa)

@@ -1,0 +1,937 @@
+// Kernel bodies of the fused scan pipeline, written for CDNA4 (gfx950): 64-lane wavefronts,
+// 160 KiB of LDS per CU, HBM3E streaming.
+//
+// The bodies are templates over a row evaluator `Ev` (expressions of one query) and are built twice:
+//   - ahead of time with `InterpEval`, which runs the engine's stack bytecode per row;
+//   - at run time by hipRTC with a typed evaluator generated from the same bytecode (ws_jit.cpp).
+// An evaluator supplies:
+//   static constexpr int kKeys, kAggs, kRows;   // loop bounds, rows per thread per tile
+//   Ev(const ScanDesc&);  void bind(const ws_colview* cols);   // per batch
+//   int nkeys(), naggs(), op(a); bool has_where(), agg_int(a), key_int(k);
+//   bool where(r); double key(k, r); i64 ikey(k, r); double agg(a, r); i64 iagg(a, r); uint8_t dkind(a);
+// Semantics follow the reference kernels (src/kernels.cuh): NULL-skipping combines, exact int64 /
+// int128 integer accumulation, the split BIGINT sum, and 64-bit hash group identity.
+//
+// Must stay compilable by hipRTC: no host headers.
+#pragma once
+#include "ws_vm.h"
+
+namespace ws {
+
+constexpr u64 kEmpty = ~0ull;
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ launch descriptor
+
+// One per launch, in device memory (uploaded together with the batch tables in one copy).
+// Rows of all batches are cut into tiles of `tile_rows`; tiles never straddle batches, so a tile
+// is (batch, first row, row count). Blocks walk tiles in increasing order.
+struct ScanDesc {
+  int32_t nb, ncols;
+  int32_t tile_rows;
+  int32_t mode;                  // dense: 0 global atomics, 1 per-thread private, 2 LDS copies
+  i64 ntiles;
+  const int64_t* n;              // [nb] rows per batch
+  const uint32_t* const* del;    // [nb] deleted-row bitmaps (bit set = deleted), may be null
+  const ws_colview* cols;        // [nb * ncols] column views per batch
+  const int64_t* tile0;          // [nb + 1] first tile of each batch
+  const int64_t* row0;           // [nb] first global row of each batch (select: output indices)
+  const ws_expr_env* env;        // device copy of the query's expression environment
+  const ws_agg_plan* plan;       // device copy of the aggregation plan
+  const ws_dense_spec* ds;       // device copy of the dense spec (dense kernels)
+  int32_t lds_slots;             // hash: LDS table slots (power of two); dense: copies
+  int32_t copy_stride;           // dense LDS copies: bytes per copy
+  int32_t slot_off[WS_MAX_AGGS]; // dense LDS copies: byte offset of each slot's array in a copy
+};
+
+// Global hash table of one grouped aggregation (row-major accumulators per slot).
+struct GTable {
+  u64* keys;   // [cap] group hash, kEmpty = free
+  double* kv;  // [cap * nkeys] key values (int keys as int64 bits)
+  double* acc; // [cap * naggs] accumulators (agg_bits slots as int64 bits)
+  u64 mask;    // cap - 1
+  int* flags;  // [0] overflow, [1] dense key out of range
+  u64* count;  // distinct groups inserted
+};
+
+// ------------------------------------------------------------------ small helpers
+
+__device__ __forceinline__ u64 mix64(u64 x) {  // splitmix64 finalizer
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return x;
+}
+__device__ __forceinline__ bool is_deleted(const uint32_t* del, i64 r) {
+  return del && ((del[r >> 5] >> (r & 31)) & 1u);
+}
+__device__ __forceinline__ double agg_identity(int op) {
+  return op == WS_AGG_MIN ? bitsd(0x7ff0000000000000ll) : op == WS_AGG_MAX ? bitsd((i64)0xfff0000000000000ull) : 0.0;
+}
+__device__ __forceinline__ i64 int_identity(int op) { return op == WS_AGG_MIN ? kI64Max : op == WS_AGG_MAX ? kI64Min : 0; }
+__device__ __forceinline__ double slot_identity(bool bits, int op) { return bits ? bitsd(int_identity(op)) : agg_identity(op); }
+// NULL-skipping combine of doubles (NaN = NULL): fmin / fmax ignore NaN, sums skip it.
+__device__ __forceinline__ double combine_d(int op, double a, double b) {
+  if (op == WS_AGG_MIN) return fmin(a, b);
+  if (op == WS_AGG_MAX) return fmax(a, b);
+  return b != b ? a : a != a ? b : a + b;
+}
+__device__ __forceinline__ i64 combine_i(int op, i64 a, i64 b) {
+  return op == WS_AGG_MIN ? (b < a ? b : a) : op == WS_AGG_MAX ? (b > a ? b : a) : (i64)((u64)a + (u64)b);
+}
+// A slot value as stored: exact int64 bits (agg_bits) or a double.
+__device__ __forceinline__ double combine_slot(bool bits, int op, double a, double b) {
+  return bits ? bitsd(combine_i(op, dbits(a), dbits(b))) : combine_d(op, a, b);
+}
+// Integer aggregate argument as the slot accumulates it (reference kernels.cuh:65-69): a NULL of a
+// nullable int column becomes the neutral value; a split BIGINT SUM keeps its low / high half.
+__device__ __forceinline__ i64 int_part(const ws_agg_plan& p, int a, i64 v) {
+  if (p.agg_null[a] && v == (p.agg_null[a] == WS_T_I32N ? (i64)WS_NULL_I32 : (i64)WS_NULL_I64))
+    return int_identity(p.ops[a]);
+  if (!p.agg_bits[a]) return v;
+  return p.agg_part[a] == 1 ? (v & 0xffffffffll) : p.agg_part[a] == 2 ? (v >> 32) : v;
+}
+
+// Atomic combines into a slot. SCOPE is __HIP_MEMORY_SCOPE_WORKGROUP for LDS tables and
+// __HIP_MEMORY_SCOPE_AGENT for global ones. Double sums use the hardware FP64 atomic add (the
+// buffers are coarse-grained device memory); double min / max are compare-and-swap loops; NaN
+// inputs (NULL) are skipped.
+template <int SCOPE>
+__device__ __forceinline__ void atomic_combine_d(int op, double* addr, double v) {
+  if (v != v) return;
+  if (op == WS_AGG_SUM || op == WS_AGG_COUNT) {
+    __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, SCOPE);
+    return;
+  }
+  u64* a = (u64*)addr;
+  u64 old = __hip_atomic_load(a, __ATOMIC_RELAXED, SCOPE);
+  for (;;) {
+    const double cur = bitsd((i64)old);
+    const double nv = op == WS_AGG_MIN ? fmin(cur, v) : fmax(cur, v);
+    if (nv == cur) return;
+    if (__hip_atomic_compare_exchange_strong(a, &old, (u64)dbits(nv), __ATOMIC_RELAXED, __ATOMIC_RELAXED, SCOPE)) return;
+  }
+}
+template <int SCOPE>
+__device__ __forceinline__ void atomic_combine_i(int op, i64* addr, i64 v) {
+  if (op == WS_AGG_MIN) __hip_atomic_fetch_min(addr, v, __ATOMIC_RELAXED, SCOPE);
+  else if (op == WS_AGG_MAX) __hip_atomic_fetch_max(addr, v, __ATOMIC_RELAXED, SCOPE);
+  else __hip_atomic_fetch_add((u64*)addr, (u64)v, __ATOMIC_RELAXED, SCOPE);
+}
+template <int SCOPE>
+__device__ __forceinline__ void atomic_combine_slot(bool bits, int op, double* addr, double v) {
+  if (bits) atomic_combine_i<SCOPE>(op, (i64*)addr, dbits(v));
+  else atomic_combine_d<SCOPE>(op, addr, v);
+}
+#define WS_LDS __HIP_MEMORY_SCOPE_WORKGROUP
+#define WS_DEV __HIP_MEMORY_SCOPE_AGENT
+
+// ------------------------------------------------------------------ wave / block reductions
+
+__device__ __forceinline__ i64 wave_sum_i64(i64 v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ u64 wave_sum_u64(u64 v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ i64 wave_combine_i(int op, i64 v) {
+  for (int o = 32; o > 0; o >>= 1) v = combine_i(op, v, (i64)__shfl_xor((long long)v, o, kWave));
+  return v;
+}
+__device__ __forceinline__ double wave_combine_d(int op, double v) {
+  for (int o = 32; o > 0; o >>= 1) v = combine_d(op, v, __shfl_xor(v, o, kWave));
+  return v;
+}
+// 128-bit signed sum carried as (lo, hi).
+__device__ __forceinline__ void add128(u64& lo, i64& hi, u64 blo, i64 bhi) {
+  const u64 s = lo + blo;
+  hi += bhi + (s < lo ? 1 : 0);
+  lo = s;
+}
+__device__ __forceinline__ void wave_sum128(u64& lo, i64& hi) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const u64 olo = __shfl_xor(lo, o, kWave);
+    const i64 ohi = (i64)__shfl_xor((long long)hi, o, kWave);
+    add128(lo, hi, olo, ohi);
+  }
+}
+
+// ------------------------------------------------------------------ tile walking
+
+// Per-block cursor over increasing tile ids: which batch a tile is in, and its rows.
+struct TileRows {
+  int b;
+  i64 first, count;
+};
+__device__ __forceinline__ int batch_of_tile(const ScanDesc& d, i64 t, int hint) {
+  int b = hint < 0 ? 0 : hint;
+  while (b + 1 < d.nb && d.tile0[b + 1] <= t) ++b;
+  return b;
+}
+__device__ __forceinline__ int batch_of_tile_bsearch(const ScanDesc& d, i64 t) {
+  int lo = 0, hi = d.nb - 1;  // last batch with tile0 <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.tile0[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__device__ __forceinline__ TileRows tile_rows(const ScanDesc& d, i64 t, int b) {
+  TileRows tr;
+  tr.b = b;
+  tr.first = (t - d.tile0[b]) * (i64)d.tile_rows;
+  const i64 left = d.n[b] - tr.first;
+  tr.count = left < d.tile_rows ? left : d.tile_rows;
+  return tr;
+}
+// Block-wide: stage batch b's column views in LDS (all threads call).
+__device__ __forceinline__ void stage_batch(const ScanDesc& d, int b, ws_colview* scols) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)b * d.ncols + i];
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ interpreter evaluator
+
+// Out-of-line VM call: inlined into every unrolled key / aggregate evaluation, its stack and the
+// opcode switch would multiply register pressure (and spill) across the whole kernel.
+template <bool kFull>
+__device__ __noinline__ double eval_prog_ni(const ws_expr_env* env, const ws_colview* cols, ws_prog pg, i64 row) {
+  return eval_prog_t<kFull>(*env, cols, pg, row);
+}
+
+// Runs the engine's bytecode per row (AOT fallback while a plan compiles, or when codegen is off).
+// Plan limits: WS_INTERP_KEYS keys, WS_INTERP_AGGS aggregates. A program that is one column or one
+// literal (the common case for keys and aggregate arguments) is read inline. kFull = false is the
+// lean variant for plans without scalar-function opcodes (see prog_is_simple).
+template <bool kFull>
+struct InterpEvalT {
+  static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 1;
+  const ws_expr_env* env;
+  const ws_agg_plan* plan;
+  const ws_colview* cols = nullptr;
+  __device__ InterpEvalT(const ScanDesc& d) : env(d.env), plan(d.plan) {}
+  __device__ void bind(const ws_colview* c) { cols = c; }
+  __device__ int nkeys() const { return plan->nkeys; }
+  __device__ int naggs() const { return plan->naggs; }
+  __device__ int op(int a) const { return plan->ops[a]; }
+  __device__ bool agg_int(int a) const { return plan->agg_int[a] != 0; }
+  __device__ bool key_int(int k) const { return plan->key_int[k] != 0; }
+  __device__ bool has_where() const { return plan->has_where != 0; }
+  __device__ uint8_t dkind(int a, const ws_dense_spec& ds) const { return ds.kind[a]; }
+  __device__ double run(ws_prog p, i64 r) const {
+    if (p.len == 1) {
+      const ws_instr in = env->code[p.start];
+      if (in.op == WS_OP_COL) return load_col(cols[in.arg], r);
+      if (in.op == WS_OP_CONST) return in.is_int ? (double)dbits(env->consts[in.arg]) : env->consts[in.arg];
+    }
+    return eval_prog_ni<kFull>(env, cols, p, r);
+  }
+  __device__ bool where(i64 r) const { return !plan->has_where || truthy(run(plan->where, r)); }
+  __device__ double key(int k, i64 r) const { return run(plan->keys[k], r); }
+  // A plain integer column reads exactly (BIGINT above 2^53); other integer programs round.
+  __device__ i64 ikey(int k, i64 r) const {
+    const ws_prog p = plan->keys[k];
+    const ws_instr in = env->code[p.start];
+    if (p.len == 1 && in.op == WS_OP_COL && cols[in.arg].t != WS_T_F64) return col_int(cols[in.arg], r);
+    return (i64)key(k, r);
+  }
+  __device__ double agg(int a, i64 r) const { return plan->ops[a] == WS_AGG_COUNT ? 1.0 : run(plan->aggs[a], r); }
+  __device__ i64 iagg(int a, i64 r) const {
+    const ws_prog p = plan->aggs[a];
+    const ws_instr in = env->code[p.start];
+    if (plan->ops[a] != WS_AGG_COUNT && p.len == 1 && in.op == WS_OP_COL && cols[in.arg].t != WS_T_F64)
+      return col_int(cols[in.arg], r);
+    return (i64)agg(a, r);
+  }
+};
+typedef InterpEvalT<true> InterpEval;
+typedef InterpEvalT<false> InterpEvalLean;
+
+// ------------------------------------------------------------------ no GROUP BY
+
+// Per-block partial of a no-keys aggregation: per slot a 128-bit (lo, hi) pair for integer sums,
+// the int64 min / max, or the double value; slot 0 counts rows.
+struct NoKeysPartial {
+  u64 lo[WS_MAX_AGGS];
+  i64 hi[WS_MAX_AGGS];
+};
+
+// Filter + aggregate without GROUP BY. Each thread accumulates exactly (int128 sums, int64
+// min / max, NULL-skipping doubles), the block reduces through wave shuffles and LDS, writes one
+// partial, and the last block to finish folds all partials in block order into `out` (so FP sums
+// are deterministic for a given input and grid). out[a] holds the low 64 bits / value, out[64 + a]
+// the high 64 bits of exact integer sums (reference kernels.cuh:188-302 keeps the same layout).
+template <class Ev>
+__device__ void nokeys_body(const ScanDesc& d, double* out, NoKeysPartial* parts, unsigned* done) {
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  __shared__ u64 s_lo[WS_MAX_AGGS][4];
+  __shared__ i64 s_hi[WS_MAX_AGGS][4];
+  __shared__ bool s_last;
+  Ev ev(d);
+  const ws_agg_plan& plan = *d.plan;
+  const int NA = ev.naggs();
+  u64 lo[Ev::kAggs];
+  i64 hi[Ev::kAggs];
+  double facc[Ev::kAggs];
+#pragma unroll
+  for (int a = 0; a < Ev::kAggs; ++a) {
+    const int op = a < NA ? ev.op(a) : WS_AGG_SUM;
+    lo[a] = (u64)int_identity(op);
+    hi[a] = 0;
+    facc[a] = agg_identity(op);
+  }
+  i64 cnt = 0;
+  int cur = -1;
+  for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+    const int b = batch_of_tile(d, t, cur);
+    if (b != cur) {
+      stage_batch(d, b, scols);
+      ev.bind(scols);
+      cur = b;
+    }
+    const TileRows tr = tile_rows(d, t, b);
+    const uint32_t* del = d.del[b];
+#pragma unroll
+    for (int j = 0; j < Ev::kRows; ++j) {
+      for (i64 i = threadIdx.x + (i64)j * blockDim.x; i < tr.count; i += (i64)Ev::kRows * blockDim.x) {
+        const i64 r = tr.first + i;
+        if (is_deleted(del, r)) continue;
+        if (ev.has_where() && !ev.where(r)) continue;
+        ++cnt;
+#pragma unroll
+        for (int a = 0; a < Ev::kAggs; ++a) {
+          if (a >= NA) break;
+          const int op = ev.op(a);
+          if (op == WS_AGG_COUNT) continue;
+          if (ev.agg_int(a)) {
+            const i64 v = int_part(plan, a, ev.iagg(a, r));
+            if (op == WS_AGG_SUM) add128(lo[a], hi[a], (u64)v, v < 0 ? -1 : 0);
+            else lo[a] = (u64)combine_i(op, (i64)lo[a], v);
+          } else {
+            facc[a] = combine_d(op, facc[a], ev.agg(a, r));
+          }
+        }
+      }
+    }
+  }
+  // Block reduction: wave shuffles, then LDS across the block's waves (<= 4 waves of 64).
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  for (int a = 0; a < NA && a < Ev::kAggs; ++a) {
+    const int op = ev.op(a);
+    u64 vlo;
+    i64 vhi = 0;
+    if (op == WS_AGG_COUNT) {
+      vlo = (u64)wave_sum_i64(cnt);
+    } else if (ev.agg_int(a) && op == WS_AGG_SUM) {
+      vlo = lo[a];
+      vhi = hi[a];
+      wave_sum128(vlo, vhi);
+    } else if (ev.agg_int(a)) {
+      vlo = (u64)wave_combine_i(op, (i64)lo[a]);
+    } else {
+      vlo = (u64)dbits(wave_combine_d(op, facc[a]));
+    }
+    if (lane == 0) {
+      s_lo[a][wave] = vlo;
+      s_hi[a][wave] = vhi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < NA) {
+    const int a = threadIdx.x;
+    const int op = plan.ops[a];
+    u64 vlo = s_lo[a][0];
+    i64 vhi = s_hi[a][0];
+    for (int w = 1; w < nw; ++w) {
+      if (op == WS_AGG_COUNT || (plan.agg_int[a] && op == WS_AGG_SUM)) add128(vlo, vhi, s_lo[a][w], s_hi[a][w]);
+      else if (plan.agg_int[a]) vlo = (u64)combine_i(op, (i64)vlo, (i64)s_lo[a][w]);
+      else vlo = (u64)dbits(combine_d(op, bitsd((i64)vlo), bitsd((i64)s_lo[a][w])));
+    }
+    parts[blockIdx.x].lo[a] = vlo;
+    parts[blockIdx.x].hi[a] = vhi;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // Last block: fold the partials in block order; thread a owns slot a.
+  if (threadIdx.x < NA) {
+    const int a = threadIdx.x;
+    const int op = plan.ops[a];
+    const bool isum = op == WS_AGG_COUNT || (plan.agg_int[a] && op == WS_AGG_SUM);
+    u64 vlo = isum ? 0 : (u64)(plan.agg_int[a] ? int_identity(op) : dbits(agg_identity(op)));
+    i64 vhi = 0;
+    for (unsigned blk = 0; blk < gridDim.x; ++blk) {
+      const volatile NoKeysPartial* p = parts + blk;
+      const u64 plo = p->lo[a];
+      const i64 phi = p->hi[a];
+      if (isum) add128(vlo, vhi, plo, phi);
+      else if (plan.agg_int[a]) vlo = (u64)combine_i(op, (i64)vlo, (i64)plo);
+      else vlo = (u64)dbits(combine_d(op, bitsd((i64)vlo), bitsd((i64)plo)));
+    }
+    // Fold into the running result (earlier windows of the same aggregation).
+    if (op == WS_AGG_COUNT) {
+      out[a] = out[a] + (double)(i64)vlo;
+    } else if (plan.agg_int[a] && op == WS_AGG_SUM) {
+      u64 olo = (u64)dbits(out[a]);
+      i64 ohi = dbits(out[WS_MAX_AGGS + a]);
+      add128(olo, ohi, vlo, vhi);
+      out[a] = bitsd((i64)olo);
+      out[WS_MAX_AGGS + a] = bitsd(ohi);
+    } else if (plan.agg_int[a]) {
+      out[a] = bitsd(combine_i(op, dbits(out[a]), (i64)vlo));
+    } else {
+      out[a] = combine_d(op, out[a], bitsd((i64)vlo));
+    }
+  }
+}
+
+// ------------------------------------------------------------------ dense GROUP BY
+
+// Row's dense group index g = sum((key_k - kmin_k) * stride_k); -1 if a key is outside its range
+// (stale statistics: the caller is told through flags[1] instead of writing out of bounds).
+template <class Ev>
+__device__ __forceinline__ i64 dense_index(const Ev& ev, const ws_dense_spec& ds, int NK, i64 r) {
+  i64 g = 0;
+#pragma unroll
+  for (int k = 0; k < Ev::kKeys; ++k) {
+    if (k >= NK) break;
+    const i64 o = ev.ikey(k, r) - ds.kmin[k];
+    if (o < 0 || o >= ds.range[k]) return -1;
+    g += o * ds.stride[k];
+  }
+  return g;
+}
+__device__ __forceinline__ int dk_op(uint8_t k) {
+  return (k == WS_DK_MIN_I || k == WS_DK_MIN_D) ? WS_AGG_MIN : (k == WS_DK_MAX_I || k == WS_DK_MAX_D) ? WS_AGG_MAX : WS_AGG_SUM;
+}
+__device__ __forceinline__ bool dk_int(uint8_t k) {
+  return k == WS_DK_COUNT || k == WS_DK_SUM_I || k == WS_DK_MIN_I || k == WS_DK_MAX_I;
+}
+// Per-row value of slot a as the dense accumulators hold it: int64 for integer kinds, double
+// otherwise (NaN = NULL, skipped by the combine).
+template <class Ev>
+__device__ __forceinline__ void dense_value(const Ev& ev, const ws_agg_plan& plan, uint8_t k, int a, i64 r, i64& iv, double& dv) {
+  if (k == WS_DK_COUNT) iv = 1;
+  else if (dk_int(k)) iv = int_part(plan, a, ev.iagg(a, r));
+  else dv = ev.agg(a, r);
+}
+// Block partial of one dense slot -> the value written to the partials (or combined globally):
+// integer kinds as int64 bits, double kinds as double bits.
+// Final slot encoding (reference kernels.cuh:489-506, 616-630): agg_bits -> int64 bits; otherwise a
+// double ((double) of the exact integer).
+
+// Dense filter + GROUP BY. Three physical layouts, chosen by the host planner per launch:
+//   mode 1: per-thread private accumulators in LDS ([slot][thread], no atomics per row);
+//   mode 2: `copies` LDS copies of all group accumulators, copy = wave % copies, LDS atomics;
+//   mode 0: straight to global memory with atomics.
+// Modes 1 and 2 end with one block partial per slot written to `parts` ([block][G * NA], int64
+// or double bits), folded by `dense_reduce` in block order.
+template <class Ev>
+__device__ void dense_body(const ScanDesc& d, double* gacc, i64* parts, int* flags) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  Ev ev(d);
+  const ws_agg_plan& plan = *d.plan;
+  const ws_dense_spec& ds = *d.ds;
+  const int NK = ev.nkeys(), NA = ev.naggs();
+  const i64 G = ds.G;
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int mode = d.mode;
+  const int C = d.lds_slots;
+  // ---- init the LDS accumulators
+  if (mode == 1) {
+    i64* sl = (i64*)ws_smem;
+    const int S = (int)G * NA;
+    for (int s = 0; s < S; ++s) {
+      const uint8_t k = ds.kind[s % NA];
+      const i64 init = dk_int(k) ? int_identity(dk_op(k)) : dbits(agg_identity(dk_op(k)));
+      sl[(i64)s * T + tid] = init;
+    }
+  } else if (mode == 2) {
+    for (int c = 0; c < C; ++c) {
+      unsigned char* cb = ws_smem + (i64)c * d.copy_stride;
+      for (int a = 0; a < NA; ++a) {
+        const uint8_t k = ds.kind[a];
+        unsigned char* arr = cb + d.slot_off[a];
+        for (i64 g = tid; g < G; g += T) {
+          if (k == WS_DK_COUNT) ((unsigned*)arr)[g] = 0u;
+          else if (dk_int(k)) ((i64*)arr)[g] = int_identity(dk_op(k));
+          else ((double*)arr)[g] = agg_identity(dk_op(k));
+        }
+      }
+    }
+  }
+  const int wave = tid / kWave;
+  unsigned char* mine = mode == 2 ? ws_smem + (i64)(wave % C) * d.copy_stride : nullptr;
+  bool bad = false;
+  int cur = -1;
+  for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+    const int b = batch_of_tile(d, t, cur);
+    if (b != cur) {
+      stage_batch(d, b, scols);
+      ev.bind(scols);
+      cur = b;
+    }
+    const TileRows tr = tile_rows(d, t, b);
+    const uint32_t* del = d.del[b];
+    for (i64 i = tid; i < tr.count; i += T) {
+      const i64 r = tr.first + i;
+      if (is_deleted(del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      const i64 g = dense_index(ev, ds, NK, r);
+      if (g < 0) {
+        bad = true;
+        continue;
+      }
+      if (mode == 1) {
+        i64* sl = (i64*)ws_smem;
+        const i64 base = g * NA * T + tid;
+#pragma unroll
+        for (int a = 0; a < Ev::kAggs; ++a) {
+          if (a >= NA) break;
+          const uint8_t k = ev.dkind(a, ds);
+          i64 iv = 0;
+          double dv = 0;
+          dense_value(ev, plan, k, a, r, iv, dv);
+          i64& slot = sl[base + (i64)a * T];
+          if (dk_int(k)) slot = combine_i(dk_op(k), slot, iv);
+          else slot = dbits(combine_d(dk_op(k), bitsd(slot), dv));
+        }
+      } else if (mode == 2) {
+#pragma unroll
+        for (int a = 0; a < Ev::kAggs; ++a) {
+          if (a >= NA) break;
+          const uint8_t k = ev.dkind(a, ds);
+          unsigned char* arr = mine + d.slot_off[a];
+          i64 iv = 0;
+          double dv = 0;
+          dense_value(ev, plan, k, a, r, iv, dv);
+          if (k == WS_DK_COUNT) __hip_atomic_fetch_add(&((unsigned*)arr)[g], 1u, __ATOMIC_RELAXED, WS_LDS);
+          else if (dk_int(k)) atomic_combine_i<WS_LDS>(dk_op(k), &((i64*)arr)[g], iv);
+          else atomic_combine_d<WS_LDS>(dk_op(k), &((double*)arr)[g], dv);
+        }
+      } else {
+        double* row = gacc + g * NA;
+#pragma unroll
+        for (int a = 0; a < Ev::kAggs; ++a) {
+          if (a >= NA) break;
+          const uint8_t k = ev.dkind(a, ds);
+          i64 iv = 0;
+          double dv = 0;
+          dense_value(ev, plan, k, a, r, iv, dv);
+          if (plan.agg_bits[a]) atomic_combine_i<WS_DEV>(dk_op(k), (i64*)&row[a], iv);
+          else if (dk_int(k)) atomic_combine_d<WS_DEV>(a == 0 ? WS_AGG_SUM : dk_op(k), &row[a], (double)iv);
+          else atomic_combine_d<WS_DEV>(dk_op(k), &row[a], dv);
+        }
+      }
+    }
+  }
+  if (bad) flags[1] = 1;
+  if (mode == 0) return;
+  __syncthreads();
+  // ---- block partial: one value per (group, slot), written coalesced to parts[block][g * NA + a]
+  const i64 S = G * NA;
+  i64* out = parts + (i64)blockIdx.x * S;
+  if (mode == 1) {
+    const i64* sl = (const i64*)ws_smem;
+    // One wave per slot: lanes sum / combine the T private copies.
+    const int lane = tid & (kWave - 1), nw = T / kWave;
+    for (i64 s = wave; s < S; s += nw) {
+      const uint8_t k = ds.kind[s % NA];
+      const int op = dk_op(k);
+      if (dk_int(k)) {
+        i64 x = int_identity(op);
+        for (int t = lane; t < T; t += kWave) x = combine_i(op, x, sl[s * T + t]);
+        x = wave_combine_i(op, x);
+        if (lane == 0) out[s] = x;
+      } else {
+        double x = agg_identity(op);
+        for (int t = lane; t < T; t += kWave) x = combine_d(op, x, bitsd(sl[s * T + t]));
+        x = wave_combine_d(op, x);
+        if (lane == 0) out[s] = dbits(x);
+      }
+    }
+  } else {
+    for (i64 s = tid; s < S; s += T) {
+      const i64 g = s / NA;
+      const int a = (int)(s - g * NA);
+      const uint8_t k = ds.kind[a];
+      const int op = dk_op(k);
+      if (k == WS_DK_COUNT) {
+        i64 x = 0;
+        for (int c = 0; c < C; ++c) x += ((const unsigned*)(ws_smem + (i64)c * d.copy_stride + d.slot_off[a]))[g];
+        out[s] = x;
+      } else if (dk_int(k)) {
+        i64 x = int_identity(op);
+        for (int c = 0; c < C; ++c) x = combine_i(op, x, ((const i64*)(ws_smem + (i64)c * d.copy_stride + d.slot_off[a]))[g]);
+        out[s] = x;
+      } else {
+        double x = agg_identity(op);
+        for (int c = 0; c < C; ++c) x = combine_d(op, x, ((const double*)(ws_smem + (i64)c * d.copy_stride + d.slot_off[a]))[g]);
+        out[s] = dbits(x);
+      }
+    }
+  }
+}
+
+// Folds the dense block partials in block order into gacc (row-major [g][NA]), on top of what
+// earlier windows left there. A block whose count for g is 0 contributes nothing to g, as in the
+// reference's per-block flush, so slot values of groups without rows keep their identities.
+__device__ void dense_reduce_body(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int nparts,
+                                  double* gacc) {
+  const int NA = plan.naggs;
+  const i64 S = ds.G * NA;
+  for (i64 s = blockIdx.x * (i64)blockDim.x + threadIdx.x; s < S; s += (i64)gridDim.x * blockDim.x) {
+    const i64 g = s / NA;
+    const int a = (int)(s - g * NA);
+    const uint8_t k = ds.kind[a];
+    const int op = dk_op(k);
+    const bool bits = plan.agg_bits[a] != 0;
+    if (dk_int(k)) {
+      i64 x = op == WS_AGG_SUM ? 0 : int_identity(op);
+      bool any = false;
+      for (int p = 0; p < nparts; ++p) {
+        if (parts[(i64)p * S + g * NA] == 0) continue;  // the block saw no row of g
+        any = true;
+        x = combine_i(op, x, parts[(i64)p * S + s]);
+      }
+      if (!any) continue;
+      if (bits) gacc[s] = bitsd(combine_i(op, dbits(gacc[s]), x));
+      else gacc[s] = combine_d(a == 0 ? WS_AGG_SUM : op, gacc[s], (double)x);
+    } else {
+      double x = agg_identity(op);
+      bool any = false;
+      for (int p = 0; p < nparts; ++p) {
+        if (parts[(i64)p * S + g * NA] == 0) continue;
+        any = true;
+        x = combine_d(op, x, bitsd(parts[(i64)p * S + s]));
+      }
+      if (!any) continue;
+      gacc[s] = combine_d(op, gacc[s], x);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ hash GROUP BY
+
+// Inserts one group contribution into the global table; false when the table overflowed (too
+// full or a probe chain longer than 256), which the caller answers by retrying bigger.
+__device__ __forceinline__ bool global_insert(const GTable& t, const ws_agg_plan& plan, int NK, int NA, u64 h,
+                                              const double* kv, const double* vals, unsigned* new_groups) {
+  u64 s = h & t.mask;
+  for (int probe = 0; probe < 256; ++probe) {
+    u64 prev = kEmpty;
+    const bool claimed = __hip_atomic_compare_exchange_strong(&t.keys[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+    if (claimed) {
+      for (int k = 0; k < NK; ++k) t.kv[s * NK + k] = kv[k];
+      __hip_atomic_fetch_add(new_groups, 1u, __ATOMIC_RELAXED, WS_LDS);
+    }
+    if (claimed || prev == h) {
+      for (int a = 0; a < NA; ++a) atomic_combine_slot<WS_DEV>(plan.agg_bits[a] != 0, plan.ops[a], &t.acc[s * NA + a], vals[a]);
+      return true;
+    }
+    s = (s + 1) & t.mask;
+  }
+  t.flags[0] = 1;
+  return false;
+}
+
+// Hash filter + GROUP BY. Group identity is the 64-bit hash of the keys (as in the reference,
+// kernels.cuh:340-347): h = fold(mix64(h ^ bits(key_k)) + k) from 0x9e3779b97f4a7c15.
+// Each block keeps an open-addressing table of `lds_slots` groups in LDS for the whole launch
+// (hot groups of skewed keys are combined there with LDS atomics); a row whose group finds no
+// LDS slot within 8 probes goes straight to the global table. At the end the block flushes its
+// LDS groups into the global table, and adds its count of new global groups once.
+template <class Ev>
+__device__ void grouped_body(const ScanDesc& d, GTable gt) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  __shared__ unsigned s_new;
+  __shared__ int s_stop;
+  Ev ev(d);
+  const ws_agg_plan& plan = *d.plan;
+  const int NK = ev.nkeys(), NA = ev.naggs();
+  const int S = d.lds_slots;
+  u64* lk = (u64*)ws_smem;
+  double* lkv = (double*)(lk + S);
+  double* lacc = lkv + (i64)S * NK;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    lk[i] = kEmpty;
+    for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
+  }
+  if (threadIdx.x == 0) s_new = 0;
+  int cur = -1;
+  for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+    // Overflowed: the whole aggregation is redone bigger, stop early (block-uniform decision).
+    __syncthreads();
+    if (threadIdx.x == 0) s_stop = *(volatile int*)gt.flags;
+    __syncthreads();
+    if (s_stop) break;
+    const int b = batch_of_tile(d, t, cur);
+    if (b != cur) {
+      stage_batch(d, b, scols);
+      ev.bind(scols);
+      cur = b;
+    }
+    const TileRows tr = tile_rows(d, t, b);
+    const uint32_t* del = d.del[b];
+    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+      const i64 r = tr.first + i;
+      if (is_deleted(del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      double kv[Ev::kKeys];
+      double av[Ev::kAggs];
+      u64 h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+      for (int k = 0; k < Ev::kKeys; ++k) {
+        if (k >= NK) break;
+        kv[k] = ev.key_int(k) ? bitsd(ev.ikey(k, r)) : ev.key(k, r) + 0.0;
+        h = mix64(h ^ (u64)dbits(kv[k])) + (u64)k;
+      }
+      if (h == kEmpty) h ^= 1;
+#pragma unroll
+      for (int a = 0; a < Ev::kAggs; ++a) {
+        if (a >= NA) break;
+        av[a] = plan.agg_bits[a] ? bitsd(int_part(plan, a, ev.iagg(a, r))) : ev.agg(a, r);
+      }
+      unsigned s = (unsigned)(h & (u64)(S - 1));
+      bool done = false;
+      for (int probe = 0; probe < 8; ++probe) {
+        u64 prev = kEmpty;
+        const bool claimed = __hip_atomic_compare_exchange_strong(&lk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (claimed)
+          for (int k = 0; k < NK; ++k) lkv[(i64)s * NK + k] = kv[k];
+        if (claimed || prev == h) {
+#pragma unroll
+          for (int a = 0; a < Ev::kAggs; ++a) {
+            if (a >= NA) break;
+            atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, ev.op(a), &lacc[(i64)s * NA + a], av[a]);
+          }
+          done = true;
+          break;
+        }
+        s = (s + 1) & (unsigned)(S - 1);
+      }
+      if (!done) global_insert(gt, plan, NK, NA, h, kv, av, &s_new);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) s_stop = *(volatile int*)gt.flags;
+  __syncthreads();
+  for (int i = threadIdx.x; i < S && !s_stop; i += blockDim.x)
+    if (lk[i] != kEmpty) global_insert(gt, plan, NK, NA, lk[i], &lkv[(i64)i * NK], &lacc[(i64)i * NA], &s_new);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_new) {
+    const u64 total = atomicAdd((unsigned long long*)gt.count, (unsigned long long)s_new) + s_new;
+    if (total * 10 > (gt.mask + 1) * 7) gt.flags[0] = 1;  // load above 0.7: retry bigger
+  }
+}
+
+// ------------------------------------------------------------------ filter + stream compaction
+
+// Tile status word for decoupled look-back: [63:62] 0 = not ready, 1 = tile count, 2 = inclusive
+// prefix; [61:0] the value.
+constexpr u64 kLbAgg = 1ull << 62, kLbInc = 2ull << 62, kLbMask = (1ull << 62) - 1;
+constexpr int kSelRows = 4;  // rows per thread of a select tile: tile = 4 * blockDim rows
+
+// Wave-wide decoupled look-back, called by one full wave of the block that owns tile t: publishes
+// the tile's count, then sums the counts of earlier tiles (64 per step) until an inclusive prefix is
+// found, publishes its own inclusive prefix and returns the exclusive one (in every lane).
+// Predecessor tiles were handed out earlier by an atomic counter, so they are running and publish;
+// a wait that still does not end is reported through *err instead of hanging the device.
+__device__ __forceinline__ u64 lookback_exclusive(u64* state, i64 t, u64 tile_count, int* err) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane == 0)
+    __hip_atomic_store(&state[t], (t == 0 ? kLbInc : kLbAgg) | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  u64 excl = 0;
+  i64 pos = t - 1;
+  unsigned spins = 0;
+  while (pos >= 0) {
+    const i64 idx = pos - lane;
+    const u64 st = idx >= 0 ? __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kLbInc;
+    if (__ballot((st >> 62) == 0)) {
+      if (++spins > (1u << 22)) {
+        if (lane == 0) *err = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    const u64 incl = __ballot((st >> 62) == 2);
+    if (incl) {
+      const int first = __ffsll((unsigned long long)incl) - 1;  // nearest inclusive predecessor
+      excl += wave_sum_u64(lane <= first ? (st & kLbMask) : 0);
+      break;
+    }
+    excl += wave_sum_u64(st & kLbMask);
+    pos -= kWave;
+  }
+  if (lane == 0 && t > 0)
+    __hip_atomic_store(&state[t], kLbInc | (excl + tile_count), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// Slot compaction: out_idx gets, in ascending order, every i in [0, n) with pred(i); total[0] the
+// count. One tile of kSelRows * blockDim slots per block, tiles in counter order.
+template <class P>
+__device__ void compact_body(const P& pred, i64 n, u64* state, unsigned* tile_ctr, int32_t* out_idx, u64* total,
+                             int* err) {
+  __shared__ unsigned s_tile;
+  __shared__ unsigned s_grp[kSelRows * 16];
+  __shared__ u64 s_excl;
+  if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const i64 t = s_tile;
+  const i64 tile = (i64)kSelRows * blockDim.x;
+  if (t * tile >= n) return;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  bool pass[kSelRows];
+  unsigned rank[kSelRows];
+#pragma unroll
+  for (int j = 0; j < kSelRows; ++j) {
+    const i64 i = t * tile + (i64)j * blockDim.x + threadIdx.x;
+    pass[j] = i < n && pred(i);
+    const u64 m = __ballot(pass[j]);
+    rank[j] = (unsigned)__popcll(m & ((1ull << lane) - 1));
+    if (lane == 0) s_grp[j * nw + wave] = (unsigned)__popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned run = 0;
+    for (int g = 0; g < kSelRows * nw; ++g) {
+      const unsigned c = s_grp[g];
+      s_grp[g] = run;
+      run += c;
+    }
+    s_excl = run;
+  }
+  __syncthreads();
+  const u64 tile_count = s_excl;
+  __syncthreads();
+  if (wave == 0) {
+    const u64 excl = lookback_exclusive(state, t, tile_count, err);
+    if (lane == 0) {
+      s_excl = excl;
+      if ((t + 1) * tile >= n) *total = excl + tile_count;
+    }
+  }
+  __syncthreads();
+  const u64 base = s_excl;
+#pragma unroll
+  for (int j = 0; j < kSelRows; ++j)
+    if (pass[j]) out_idx[base + s_grp[j * nw + wave] + rank[j]] = (int32_t)(t * tile + (i64)j * blockDim.x + threadIdx.x);
+}
+
+// Single-pass WHERE + compaction over all batches (replaces a flag array + Flagged select): each
+// block takes the next tile id from a counter, evaluates the filter for its rows, ranks passing
+// rows with wave ballots and an LDS scan, learns its output offset by decoupled look-back over
+// earlier tiles, and writes ascending global row indices (row0[batch] + row). Per-batch counts are
+// accumulated in per_batch[b]; total[0] gets the count of all selected rows.
+template <class Ev>
+__device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, int32_t* out_idx, u64* per_batch,
+                            u64* total, int* err) {
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  __shared__ unsigned s_tile;
+  __shared__ unsigned s_wcount[16];
+  __shared__ u64 s_excl;
+  if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+  __syncthreads();
+  const i64 t = s_tile;
+  if (t >= d.ntiles) return;
+  const int b = batch_of_tile_bsearch(d, t);
+  for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)b * d.ncols + i];
+  __syncthreads();
+  Ev ev(d);
+  ev.bind(scols);
+  const TileRows tr = tile_rows(d, t, b);
+  const uint32_t* del = d.del[b];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  // Each thread owns kRows consecutive chunks of blockDim rows: row i = j * blockDim + tid.
+  bool pass[kSelRows];
+  unsigned mycount[kSelRows];
+#pragma unroll
+  for (int j = 0; j < kSelRows; ++j) {
+    const i64 i = (i64)j * blockDim.x + threadIdx.x;
+    const i64 r = tr.first + i;
+    pass[j] = i < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r));
+  }
+  // Rank: rows are ordered (j, wave, lane); per (j, wave) a ballot, then an LDS prefix over the
+  // (j, wave) groups in that order.
+  __shared__ unsigned s_grp[4 * 16];
+  unsigned local_rank[kSelRows];
+#pragma unroll
+  for (int j = 0; j < kSelRows; ++j) {
+    const u64 m = __ballot(pass[j]);
+    local_rank[j] = (unsigned)__popcll(m & ((1ull << lane) - 1));
+    mycount[j] = (unsigned)__popcll(m);
+    if (lane == 0) s_grp[j * nw + wave] = mycount[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned run = 0;
+    for (int g = 0; g < kSelRows * nw; ++g) {
+      const unsigned c = s_grp[g];
+      s_grp[g] = run;
+      run += c;
+    }
+    s_wcount[0] = run;  // tile count
+  }
+  __syncthreads();
+  const u64 tile_count = s_wcount[0];
+  // Decoupled look-back (wave 0) for this tile's output offset.
+  if (wave == 0) {
+    const u64 excl = lookback_exclusive(state, t, tile_count, err);
+    if (lane == 0) {
+      s_excl = excl;
+      if (tile_count) atomicAdd((unsigned long long*)&per_batch[b], (unsigned long long)tile_count);
+      if (t == d.ntiles - 1) *total = excl + tile_count;
+    }
+  }
+  __syncthreads();
+  const u64 base = s_excl;
+  const i64 g0 = d.row0[b] + tr.first;
+#pragma unroll
+  for (int j = 0; j < kSelRows; ++j) {
+    if (!pass[j]) continue;
+    const i64 i = (i64)j * blockDim.x + threadIdx.x;
+    out_idx[base + s_grp[j * nw + wave] + local_rank[j]] = (int32_t)(g0 + i);
+  }
+}
+
+}  // namespace ws
+
+// ------------------------------------------------------------------ kernel entry points
+// The AOT build instantiates every kernel with InterpEval; a compiled plan instantiates only the one
+// it needs with its generated evaluator (extern "C" so the module symbol has a fixed name).
+#define WS_KERNEL_NOKEYS(EV, NAME)                                                                          \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, double* out,               \
+                                                         ws::NoKeysPartial* parts, unsigned* done) {        \
+    ws::nokeys_body<EV>(*d, out, parts, done);                                                               \
+  }
+#define WS_KERNEL_DENSE(EV, NAME)                                                                           \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, double* gacc, long long* parts, \
+                                                         int* flags) {                                       \
+    ws::dense_body<EV>(*d, gacc, parts, flags);                                                              \
+  }
+#define WS_KERNEL_GROUPED(EV, NAME)                                                                         \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, ws::GTable gt) {            \
+    ws::grouped_body<EV>(*d, gt);                                                                            \
+  }
+#define WS_KERNEL_SELECT(EV, NAME)                                                                          \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, unsigned long long* state,  \
+                                                         unsigned* tile_ctr, int* out_idx,                  \
+                                                         unsigned long long* per_batch,                     \
+                                                         unsigned long long* total, int* err) {             \
+    ws::select_body<EV>(*d, state, tile_ctr, out_idx, per_batch, total, err);                               \
+  }

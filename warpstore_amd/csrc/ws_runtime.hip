@@ -1,0 +1,430 @@
+// Device setup, memory, staging, profiling and the C ABI entry points for them
+// (reference src/gpu.cu:32-186 covers the same ground for CUDA).
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <mutex>
+
+#include "ws_internal.h"
+
+namespace ws {
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string t_last_error;
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+void hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  (void)hipGetLastError();  // clear the sticky error so the next call starts clean
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) throw Error("GPU out of memory", WS_ERR_OOM);
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error: %s at %s (%s:%d)", hipGetErrorString(e), what, file, line);
+  throw Error(buf, WS_ERR_INTERNAL);
+}
+
+// ------------------------------------------------------------------ device
+
+static std::once_flag g_init_once;
+static int g_cus = 256;
+static std::string g_arch = "gfx950";
+static bool g_pool_ok = false;
+static bool g_init_failed = false;
+static std::string g_init_error;
+
+static void init_device() {
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) {
+    g_init_failed = true;
+    g_init_error = "no HIP device available";
+    (void)hipGetLastError();
+    return;
+  }
+  e = hipSetDevice(0);
+  hipDeviceProp_t prop;
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, 0);
+  if (e != hipSuccess) {
+    g_init_failed = true;
+    g_init_error = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return;
+  }
+  g_cus = prop.multiProcessorCount;
+  g_arch = prop.gcnArchName;
+  // Query temporaries come from the stream-ordered pool, which keeps freed memory for the next
+  // queries (release threshold: never); a failed allocation trims the pool and retries.
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
+    uint64_t thresh = UINT64_MAX;
+    g_pool_ok = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thresh) == hipSuccess;
+  }
+  (void)hipGetLastError();
+}
+
+void ensure_init() {
+  std::call_once(g_init_once, init_device);
+  if (g_init_failed) throw Error(g_init_error, WS_ERR_INTERNAL);
+}
+int cu_count() {
+  ensure_init();
+  return g_cus;
+}
+const std::string& arch_name() {
+  ensure_init();
+  return g_arch;
+}
+
+// ------------------------------------------------------------------ memory
+
+static std::atomic<int64_t> g_vram_used{0};
+static int64_t g_vram_limit = INT64_MAX;
+
+void trim_pool() {
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  (void)hipDeviceSynchronize();
+  (void)hipMemPoolTrimTo(pool, 0);
+  (void)hipGetLastError();
+}
+
+void* tmp_alloc(size_t bytes) {
+  ensure_init();
+  if (!bytes) return nullptr;
+  void* p = nullptr;
+  if (g_pool_ok) {
+    if (hipMallocAsync(&p, bytes, stream()) != hipSuccess) {
+      (void)hipGetLastError();
+      trim_pool();
+      WS_CK(hipMallocAsync(&p, bytes, stream()));
+    }
+  } else {
+    WS_CK(hipMalloc(&p, bytes));
+  }
+  return p;
+}
+void tmp_free(void* p) {
+  if (!p) return;
+  if (g_pool_ok) (void)hipFreeAsync(p, stream());
+  else {
+    (void)hipStreamSynchronize(stream());
+    (void)hipFree(p);
+  }
+}
+
+struct Stage {
+  uint8_t* buf = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+  ~Stage() {
+    // Thread exit: the buffer may still feed an in-flight copy; wait for it before freeing.
+    if (ev) {
+      if (pending) (void)hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
+    }
+    if (buf) (void)hipHostFree(buf);
+  }
+};
+static thread_local Stage t_stage;
+
+uint8_t* pinned_stage(size_t bytes) {
+  ensure_init();
+  Stage& s = t_stage;
+  if (!s.ev) WS_CK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+  if (s.pending) {
+    WS_CK(hipEventSynchronize(s.ev));
+    s.pending = false;
+  }
+  if (bytes > s.cap) {
+    if (s.buf) WS_CK(hipHostFree(s.buf));
+    s.buf = nullptr;
+    s.cap = std::max<size_t>(bytes, 64 << 10);
+    WS_CK(hipHostMalloc((void**)&s.buf, s.cap, hipHostMallocDefault));
+  }
+  return s.buf;
+}
+void pinned_stage_done() {
+  WS_CK(hipEventRecord(t_stage.ev, stream()));
+  t_stage.pending = true;
+}
+void h2d_sync(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  WS_CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+  WS_CK(hipStreamSynchronize(stream()));
+}
+void d2h_sync(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  WS_CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+  WS_CK(hipStreamSynchronize(stream()));
+}
+void h2d_async_staged(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  uint8_t* st = pinned_stage(bytes);
+  std::memcpy(st, src, bytes);
+  WS_CK(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, stream()));
+  pinned_stage_done();
+}
+uint8_t* upload_blob(const Blob& b) {
+  uint8_t* d = (uint8_t*)tmp_alloc(std::max<size_t>(b.host.size(), 16));
+  try {
+    h2d_async_staged(d, b.host.data(), b.host.size());
+  } catch (...) {
+    tmp_free(d);
+    throw;
+  }
+  return d;
+}
+
+// ------------------------------------------------------------------ profiling
+
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+static thread_local bool t_prof = false;
+static thread_local std::vector<ProfRec> t_prof_recs;
+static thread_local std::vector<hipEvent_t> t_prof_free;
+
+bool prof_enabled() { return t_prof; }
+static hipEvent_t prof_event() {
+  if (!t_prof_free.empty()) {
+    hipEvent_t e = t_prof_free.back();
+    t_prof_free.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  WS_CK(hipEventCreate(&e));
+  return e;
+}
+ProfScope::ProfScope(const char* kernel) {
+  if (!t_prof) return;
+  ProfRec r{kernel, prof_event(), prof_event()};
+  WS_CK(hipEventRecord(r.a, stream()));
+  slot = (int)t_prof_recs.size();
+  t_prof_recs.push_back(r);
+}
+ProfScope::~ProfScope() {
+  if (slot >= 0) (void)hipEventRecord(t_prof_recs[slot].b, stream());
+}
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) hip_fail(e, what, __FILE__, __LINE__);
+}
+
+// ------------------------------------------------------------------ fills
+
+template <class T>
+__global__ void fill_k(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+void fill_u64(u64* p, int64_t n, u64 v) {
+  if (n <= 0) return;
+  ProfScope ps("fill");
+  hipLaunchKernelGGL(fill_k<u64>, dim3(grid_for(n)), dim3(256), 0, stream(), p, n, v);
+  check_launch("fill_u64");
+}
+void fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  if (n <= 0) return;
+  ProfScope ps("fill");
+  hipLaunchKernelGGL(fill_k<uint32_t>, dim3(grid_for(n)), dim3(256), 0, stream(), p, n, v);
+  check_launch("fill_u32");
+}
+void fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+  if (n <= 0) return;
+  ProfScope ps("fill");
+  hipLaunchKernelGGL(fill_k<uint8_t>, dim3(grid_for(n)), dim3(256), 0, stream(), p, n, v);
+  check_launch("fill_u8");
+}
+void fill_i32(int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return;
+  ProfScope ps("fill");
+  hipLaunchKernelGGL(fill_k<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream(), p, n, v);
+  check_launch("fill_i32");
+}
+
+}  // namespace ws
+
+using namespace ws;
+
+// ------------------------------------------------------------------ C ABI: errors, device, memory
+
+extern "C" {
+
+const char* ws_last_error(void) { return t_last_error.c_str(); }
+
+int ws_init(void) { return guard([] { ensure_init(); }); }
+int ws_sm_count(void) {
+  try {
+    return cu_count();
+  } catch (...) {
+    return 0;
+  }
+}
+int ws_arch(void) {
+  try {
+    const std::string& a = arch_name();  // "gfx950..." -> 950
+    int v = 0;
+    for (size_t i = 3; i < a.size() && isdigit((unsigned char)a[i]); ++i) v = v * 10 + (a[i] - '0');
+    return v;
+  } catch (...) {
+    return 0;
+  }
+}
+int ws_describe(char* buf, size_t cap) {
+  return guard([&] {
+    ensure_init();
+    hipDeviceProp_t prop;
+    WS_CK(hipGetDeviceProperties(&prop, 0));
+    size_t fr = 0, tot = 0;
+    WS_CK(hipMemGetInfo(&fr, &tot));
+    snprintf(buf, cap, "%s (%s, %d CUs, %.1f GiB free / %.1f GiB)", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+             fr / 1073741824.0, tot / 1073741824.0);
+  });
+}
+
+int ws_dev_alloc(size_t bytes, void** out) {
+  return guard([&] {
+    ensure_init();
+    *out = nullptr;
+    if (bytes) WS_CK(hipMalloc(out, bytes));
+  });
+}
+void ws_dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+int ws_tmp_alloc(size_t bytes, void** out) { return guard([&] { *out = tmp_alloc(bytes); }); }
+void ws_trim_pool(void) { trim_pool(); }
+void ws_tmp_free(void* p) { tmp_free(p); }
+int ws_h2d(void* dst, const void* src, size_t b) { return guard([&] { h2d_sync(dst, src, b); }); }
+int ws_d2h(void* dst, const void* src, size_t b) { return guard([&] { d2h_sync(dst, src, b); }); }
+int ws_d2d(void* dst, const void* src, size_t b) {
+  return guard([&] {
+    if (!b) return;
+    WS_CK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, stream()));
+    WS_CK(hipStreamSynchronize(stream()));
+  });
+}
+int ws_sync(void) { return guard([] { WS_CK(hipStreamSynchronize(stream())); }); }
+
+// VRAM tiering: columns always live in device memory here; the host tier (mapped host memory)
+// is not provided, so a column that does not fit reports out of memory.
+void ws_set_vram_limit(int64_t bytes) { g_vram_limit = bytes; }
+int64_t ws_vram_used(void) { return g_vram_used; }
+double ws_vram_fill(void) {
+  if (g_vram_limit != INT64_MAX) return (double)g_vram_used / (double)g_vram_limit;
+  size_t fr = 0, tot = 1;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return 1.0 - (double)fr / (double)tot;
+}
+int ws_place_column(void* host, size_t bytes, int host_only, int* mapped, void** out) {
+  return guard([&] {
+    *mapped = 0;
+    *out = nullptr;
+    if (!bytes) return;
+    if (host_only || g_vram_used + (int64_t)bytes > g_vram_limit)
+      throw Error("column does not fit the VRAM budget (no host tier)", WS_ERR_OOM);
+    void* p = nullptr;
+    WS_CK(hipMalloc(&p, bytes));
+    g_vram_used += (int64_t)bytes;
+    if (hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(p);
+      g_vram_used -= (int64_t)bytes;
+      throw Error("column upload failed", WS_ERR_INTERNAL);
+    }
+    *out = p;
+  });
+}
+void ws_release_column(void* dev, void* host, size_t bytes, int mapped) {
+  (void)host;
+  if (!dev || mapped) return;
+  (void)hipFree(dev);
+  g_vram_used -= (int64_t)bytes;
+}
+void* ws_vram_acquire(const void* host, size_t bytes) {
+  if (!bytes || g_vram_used + (int64_t)bytes > g_vram_limit) return nullptr;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(p);
+    return nullptr;
+  }
+  g_vram_used += (int64_t)bytes;
+  return p;
+}
+void ws_vram_release(void* dev, size_t bytes) {
+  if (!dev) return;
+  (void)hipFree(dev);
+  g_vram_used -= (int64_t)bytes;
+}
+void* ws_host_map(void* host, size_t bytes) {
+  (void)host, (void)bytes;
+  return nullptr;  // no mapped host tier
+}
+void ws_host_unmap(void* host) { (void)host; }
+int64_t ws_vram_limit(void) { return g_vram_limit; }
+int64_t ws_vram_free(void) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return (int64_t)fr;
+}
+void* ws_patch_column(const void* old_dev, size_t bytes, int width, const int32_t* rows, const void* vals, int64_t m) {
+  (void)old_dev, (void)bytes, (void)width, (void)rows, (void)vals, (void)m;
+  return nullptr;  // DML stays on the engine's host path
+}
+void* ws_update_column(const void* old_col, const void* src, int64_t n, int w, int f64, const int32_t* rows, int64_t m,
+                       ws_gpu_update* u) {
+  (void)old_col, (void)src, (void)n, (void)w, (void)f64, (void)rows, (void)m;
+  if (u) u->bad = 0;
+  return nullptr;
+}
+
+// ------------------------------------------------------------------ C ABI: profiling
+
+void ws_prof_enable(int on) { t_prof = on != 0; }
+void ws_prof_reset(void) {
+  for (auto& r : t_prof_recs) {
+    (void)hipEventSynchronize(r.b);
+    t_prof_free.push_back(r.a);
+    t_prof_free.push_back(r.b);
+  }
+  t_prof_recs.clear();
+}
+int ws_prof_collect(ws_prof_row* rows, int cap) {
+  std::vector<ws_prof_row> agg;
+  for (auto& r : t_prof_recs) {
+    float ms = 0;
+    if (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    auto it = std::find_if(agg.begin(), agg.end(), [&](const ws_prof_row& x) { return strcmp(x.name, r.name) == 0; });
+    if (it == agg.end()) {
+      ws_prof_row x{};
+      snprintf(x.name, sizeof x.name, "%s", r.name);
+      agg.push_back(x);
+      it = agg.end() - 1;
+    }
+    it->launches += 1;
+    it->total_ms += ms;
+  }
+  const int n = (int)std::min<size_t>(agg.size(), (size_t)std::max(cap, 0));
+  for (int i = 0; i < n; ++i) rows[i] = agg[i];
+  return (int)agg.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// Stable multi-key sort permutation for ORDER BY (reference src/gpu.cu:1318-1386, which uses CUB's
+// radix sort). Each ORDER BY key, last to first, becomes an order-preserving u64 (NULL -> -inf,
+// -0 -> 0, descending keys complemented) and the (key, row) pairs go through a stable LSD radix
+// sort with 8-bit digits. Digit positions where every key agrees are skipped (one reduction up
+// front), so small-integer or narrow-range keys cost only the passes over their varying bytes.
+// A pass: per-tile digit histograms (LDS atomics) -> one exclusive scan of the digit-major
+// histogram table -> scatter, ranking equal digits inside a tile with wave ballots so the order of
+// equal keys is kept.
+#include <algorithm>
+
+#include "ws_internal.h"
+
+namespace ws {
+
+constexpr int kSBlock = 256;
+constexpr int kSItems = 16;                       // items per thread per tile
+constexpr int kSTile = kSBlock * kSItems;         // 4096 keys per tile
+constexpr int kScanTile = kSelRows * kSBlock;     // 1024 values per scan tile
+
+__device__ __forceinline__ u64 order_key(double v, bool desc) {
+  v = v != v ? -__builtin_huge_val() : (v == 0 ? 0.0 : v);
+  const u64 b = (u64)dbits(v);
+  const u64 u = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+  return desc ? ~u : u;
+}
+
+__global__ void seq_k(int32_t* p, int64_t m) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) p[j] = (int32_t)j;
+}
+__global__ void sort_key_k(const int32_t* perm, const double* key, int64_t m, int desc, u64* out) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
+    out[j] = order_key(key[perm[j]], desc != 0);
+}
+// OR over (key ^ key[0]): the bits that vary between keys.
+__global__ void varying_bits_k(const u64* k, int64_t m, unsigned long long* out) {
+  const u64 k0 = k[0];
+  u64 acc = 0;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) acc |= k[j] ^ k0;
+  for (int o = 32; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0 && acc) atomicOr(out, (unsigned long long)acc);
+}
+
+__global__ void __launch_bounds__(kSBlock) digit_hist_k(const u64* keys, int64_t m, int shift, int64_t ntiles, uint32_t* hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSTile;
+  for (int j = 0; j < kSItems; ++j) {
+    const int64_t i = base + (int64_t)j * kSBlock + threadIdx.x;
+    if (i < m) __hip_atomic_fetch_add(&h[(keys[i] >> shift) & 255], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of n u32 values (single pass, decoupled look-back).
+__global__ void __launch_bounds__(kSBlock) scan_u32_k(const uint32_t* in, uint32_t* out, int64_t n, u64* state, unsigned* ctr,
+                                                      int* err) {
+  __shared__ unsigned s_tile;
+  __shared__ u64 s_w[kSBlock / 64];
+  __shared__ u64 s_excl;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1u);
+  __syncthreads();
+  const i64 t = s_tile;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Each thread owns kSelRows consecutive values.
+  const int64_t i0 = t * kScanTile + (int64_t)threadIdx.x * kSelRows;
+  u64 v[kSelRows];
+  u64 sum = 0;
+  for (int j = 0; j < kSelRows; ++j) {
+    v[j] = i0 + j < n ? in[i0 + j] : 0;
+    sum += v[j];
+  }
+  u64 incl = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const u64 y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  u64 before = 0, tile_total = 0;
+  for (int w = 0; w < kSBlock / 64; ++w) {
+    if (w < wave) before += s_w[w];
+    tile_total += s_w[w];
+  }
+  if (wave == 0) {
+    const u64 excl = lookback_exclusive(state, t, tile_total, err);
+    if (lane == 0) s_excl = excl;
+  }
+  __syncthreads();
+  u64 run = s_excl + before + incl - sum;
+  for (int j = 0; j < kSelRows; ++j) {
+    if (i0 + j < n) out[i0 + j] = (uint32_t)run;
+    run += v[j];
+  }
+}
+
+// Stable scatter of one digit pass.
+__global__ void __launch_bounds__(kSBlock) digit_scatter_k(const u64* kin, const int32_t* vin, int64_t m, int shift, int64_t ntiles,
+                                                           const uint32_t* offs, u64* kout, int32_t* vout) {
+  __shared__ uint32_t s_base[256];      // global position of this tile's first key of each digit
+  __shared__ uint32_t s_run[256];       // keys of each digit already placed by earlier rounds
+  __shared__ uint32_t s_wc[kSBlock / 64][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  s_base[threadIdx.x] = offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+  s_run[threadIdx.x] = 0;
+  const int64_t base = (int64_t)blockIdx.x * kSTile;
+  for (int j = 0; j < kSItems; ++j) {
+    for (int w = 0; w < kSBlock / 64; ++w) s_wc[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = base + (int64_t)j * kSBlock + threadIdx.x;
+    const bool valid = i < m;
+    const u64 k = valid ? kin[i] : 0;
+    const int32_t v = valid ? vin[i] : 0;
+    const unsigned dg = (unsigned)((k >> shift) & 255);
+    // Lanes of this wave holding the same digit (and valid).
+    u64 peers = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const u64 bal = __ballot((dg >> b) & 1);
+      peers &= ((dg >> b) & 1) ? bal : ~bal;
+    }
+    const unsigned rank = (unsigned)__popcll(peers & ((1ull << lane) - 1));
+    if (valid && rank == 0) s_wc[wave][dg] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0;
+      for (int w = 0; w < wave; ++w) before += s_wc[w][dg];
+      const uint32_t pos = s_base[dg] + s_run[dg] + before + rank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int w = 0; w < kSBlock / 64; ++w) tot += s_wc[w][threadIdx.x];
+    s_run[threadIdx.x] += tot;
+    __syncthreads();
+  }
+}
+
+static void scan_u32(const uint32_t* in, uint32_t* out, int64_t n) {
+  const int64_t nt = (n + kScanTile - 1) / kScanTile;
+  DevBuf<u64> st(nt + 2);
+  fill_u64(st.p, st.n, 0);
+  hipLaunchKernelGGL(scan_u32_k, dim3((unsigned)nt), dim3(kSBlock), 0, stream(), in, out, n, st.p, (unsigned*)(st.p + nt),
+                     (int*)(st.p + nt + 1));
+  check_launch("scan_u32");
+}
+
+// Stable ascending sort of (keys, vals) by keys; results end in (keys, vals) (buffers swapped
+// with the scratch pair as passes run).
+static void radix_sort_pairs(DevBuf<u64>& keys, DevBuf<int32_t>& vals, DevBuf<u64>& k2, DevBuf<int32_t>& v2, int64_t m) {
+  DevBuf<unsigned long long> vary(1);
+  fill_u64((u64*)vary.p, 1, 0);
+  hipLaunchKernelGGL(varying_bits_k, dim3(grid_for(m)), dim3(256), 0, stream(), keys.p, m, vary.p);
+  check_launch("varying_bits");
+  unsigned long long bits = 0;
+  d2h_sync(&bits, vary.p, 8);
+  const int64_t ntiles = (m + kSTile - 1) / kSTile;
+  DevBuf<uint32_t> hist(256 * ntiles), offs(256 * ntiles);
+  for (int pass = 0; pass < 8; ++pass) {
+    const int shift = 8 * pass;
+    if (((bits >> shift) & 255) == 0) continue;  // every key has the same digit here
+    hipLaunchKernelGGL(digit_hist_k, dim3((unsigned)ntiles), dim3(kSBlock), 0, stream(), keys.p, m, shift, ntiles, hist.p);
+    check_launch("digit_hist");
+    scan_u32(hist.p, offs.p, 256 * ntiles);
+    hipLaunchKernelGGL(digit_scatter_k, dim3((unsigned)ntiles), dim3(kSBlock), 0, stream(), keys.p, vals.p, m, shift, ntiles,
+                       offs.p, k2.p, v2.p);
+    check_launch("digit_scatter");
+    std::swap(keys, k2);
+    std::swap(vals, v2);
+  }
+}
+
+// keys[i] sorted stably (ascending unless desc[i]); keys[0] most significant.
+static DevBuf<int32_t> sort_perm(const std::vector<std::pair<const double*, bool>>& keys, int64_t m) {
+  DevBuf<int32_t> perm(std::max<int64_t>(m, 1));
+  if (m <= 0) return perm;
+  ProfScope ps("sort_perm");
+  hipLaunchKernelGGL(seq_k, dim3(grid_for(m)), dim3(256), 0, stream(), perm.p, m);
+  check_launch("seq");
+  DevBuf<int32_t> perm2(m);
+  DevBuf<u64> k1(m), k2(m);
+  for (int i = (int)keys.size() - 1; i >= 0; --i) {
+    hipLaunchKernelGGL(sort_key_k, dim3(grid_for(m)), dim3(256), 0, stream(), perm.p, keys[i].first, m, keys[i].second ? 1 : 0,
+                       k1.p);
+    check_launch("sort_key");
+    radix_sort_pairs(k1, perm, k2, perm2, m);
+  }
+  return perm;
+}
+
+template <class T>
+__global__ void to_double_k(const T* src, int64_t n, double* dst) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) dst[j] = (double)src[j];
+}
+
+}  // namespace ws
+
+using namespace ws;
+
+extern "C" {
+
+int ws_sort_perm(const double* const* keys, const uint8_t* desc, int nkeys, int64_t m, int32_t** perm) {
+  return guard([&] {
+    *perm = nullptr;
+    std::vector<std::pair<const double*, bool>> k;
+    for (int i = 0; i < nkeys; ++i) k.push_back({keys[i], desc && desc[i]});
+    if (m > INT32_MAX) throw Error("sort: too many rows", WS_ERR_INTERNAL);
+    *perm = sort_perm(k, m).release();
+  });
+}
+int ws_sort_perm_host(const void* host_col, int ptype, int64_t n, int32_t* out) {
+  return guard([&] {
+    if (n <= 0) return;
+    const size_t es = ptype == WS_P_I32 ? 4 : 8;
+    DevBuf<uint8_t> raw(n * (int64_t)es);
+    h2d_sync(raw.p, host_col, n * es);
+    DevBuf<double> k(n);
+    if (ptype == WS_P_I32)
+      hipLaunchKernelGGL(to_double_k<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream(), (const int32_t*)raw.p, n, k.p);
+    else if (ptype == WS_P_I64)
+      hipLaunchKernelGGL(to_double_k<int64_t>, dim3(grid_for(n)), dim3(256), 0, stream(), (const int64_t*)raw.p, n, k.p);
+    else
+      WS_CK(hipMemcpyAsync(k.p, raw.p, 8 * (size_t)n, hipMemcpyDeviceToDevice, stream()));
+    check_launch("to_double");
+    DevBuf<int32_t> perm = sort_perm({{k.p, false}}, n);
+    d2h_sync(out, perm.p, 4 * (size_t)n);
+  });
+}
+
+}  // extern "C"

@@ -191,6 +191,7 @@ int ws_jit_compile_check(const ws_expr_env* env, int ncols, const ws_agg_plan* p
  * the engine's PROFILE stages (engine_core.h:4016-4028). While enabled, every kernel launch of the
  * calling thread is bracketed by events; ws_prof_collect synchronizes and sums them per kernel. */
 void ws_prof_enable(int on);
+void ws_prof_only(const char* kernel); /* restrict recording to one kernel name (NULL or "" = all) */
 void ws_prof_reset(void);
 /* Totals per kernel name: returns the number of kernels (at most cap). */
 typedef struct ws_prof_row {
@@ -199,6 +200,9 @@ typedef struct ws_prof_row {
   double total_ms;
 } ws_prof_row;
 int ws_prof_collect(ws_prof_row* rows, int cap);
+/* Tooling, no counterpart in the reference: a plain device-to-device streaming copy kernel
+ * (bytes a multiple of 16), the known-size reference for calibrating hardware byte counters. */
+int ws_copy_kernel(void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,8 @@
 // Pinned by: golden vectors from the reference's own devtypes.h compiled on the host (the VM,
 // decoders, dates, checked arithmetic), brute force and closed forms in tests/test_oracle.py.
 #include <algorithm>
+#include <array>
+#include <set>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -311,8 +313,11 @@ double run(const ws_expr_env& env, const ws_colview* cols, ws_prog pg, i64 row) 
       case WS_OP_INSORTED: {
         double x = pop();
         if (std::isnan(x)) { st.push_back(x); break; }
+        // lower bound by '<', then an exact '==' (a NaN entry never matches)
         const double* a = env.maps[in.arg];
-        st.push_back(std::binary_search(a, a + env.map_size[in.arg], x) ? 1.0 : 0.0);
+        const double* e = a + env.map_size[in.arg];
+        const double* it = std::lower_bound(a, e, x);
+        st.push_back(it != e && *it == x ? 1.0 : 0.0);
         break;
       }
       case WS_OP_VDOT: case WS_OP_VEUC: {
@@ -605,6 +610,75 @@ void or_remap_codes(const int32_t* in, int64_t n, const int32_t* map, int64_t ma
     int32_t c = in[j];
     out[j] = c == WS_NULL_I32 ? WS_NULL_I32 : (c >= 0 && c < map_n ? map[c] : -2);
   }
+}
+
+// Query-level restatements of the measured shapes (plain host loops, one thread).
+
+// SELECT user_id, COUNT(*), SUM(dur), COUNT(DISTINCT url) FROM h WHERE event_date < date_lt AND
+// region < region_lt GROUP BY user_id: every group, ordered by count desc then user_id asc, as
+// rows (user_id, count, sum(dur), distinct urls). The two-phase plan of the engine
+// (engine_select.cpp:1403-1492) is restated as (user, url) pairs sorted and counted.
+int64_t or_c2_query(const int64_t* user, const int32_t* region, const int32_t* date, const int32_t* url, const int32_t* dur,
+                    int64_t n, int32_t date_lt, int32_t region_lt, int64_t* out, int64_t cap) {
+  struct R {
+    int64_t user;
+    int32_t url, dur;
+  };
+  std::vector<R> rows;
+  rows.reserve((size_t)n);
+  for (int64_t i = 0; i < n; ++i)
+    if (date[i] < date_lt && region[i] < region_lt) rows.push_back({user[i], url[i], dur[i]});
+  std::sort(rows.begin(), rows.end(), [](const R& a, const R& b) { return a.user != b.user ? a.user < b.user : a.url < b.url; });
+  std::vector<std::array<int64_t, 4>> g;
+  for (size_t i = 0; i < rows.size();) {
+    size_t j = i;
+    int64_t cnt = 0, sum = 0, distinct = 0;
+    while (j < rows.size() && rows[j].user == rows[i].user) {
+      if (j == i || rows[j].url != rows[j - 1].url) ++distinct;
+      ++cnt;
+      sum += rows[j].dur;
+      ++j;
+    }
+    g.push_back({rows[i].user, cnt, sum, distinct});
+    i = j;
+  }
+  std::sort(g.begin(), g.end(), [](const std::array<int64_t, 4>& a, const std::array<int64_t, 4>& b) {
+    return a[1] != b[1] ? a[1] > b[1] : a[0] < b[0];
+  });
+  if ((int64_t)g.size() > cap) return -(int64_t)g.size();
+  for (size_t i = 0; i < g.size(); ++i)
+    for (int k = 0; k < 4; ++k) out[i * 4 + k] = g[i][k];
+  return (int64_t)g.size();
+}
+
+// SELECT f.flag, COUNT(*), SUM(f.price), SUM(f.qty) FROM fact f JOIN dim d ON f.fk = d.key WHERE
+// d.attr < attr_lt GROUP BY f.flag: rows (flag, count, raw price sum, qty sum) ordered by flag.
+int64_t or_c3_query(const int32_t* fk, const int64_t* qty, const int64_t* price, const int32_t* flag, int64_t nf,
+                    const int32_t* dkey, const int32_t* dattr, int64_t nd, int32_t attr_lt, int64_t* out, int64_t cap) {
+  std::multiset<int32_t> keys;
+  for (int64_t i = 0; i < nd; ++i)
+    if (dattr[i] < attr_lt) keys.insert(dkey[i]);
+  std::vector<int32_t> sorted(keys.begin(), keys.end());
+  std::map<int32_t, std::array<__int128, 3>> g;
+  for (int64_t i = 0; i < nf; ++i) {
+    auto rg = std::equal_range(sorted.begin(), sorted.end(), fk[i]);
+    const int64_t mult = rg.second - rg.first;
+    if (!mult) continue;
+    auto& a = g[flag[i]];
+    a[0] += mult;
+    a[1] += (__int128)price[i] * mult;
+    a[2] += (__int128)qty[i] * mult;
+  }
+  if ((int64_t)g.size() > cap) return -(int64_t)g.size();
+  int64_t r = 0;
+  for (auto& [f, a] : g) {
+    out[r * 4 + 0] = f;
+    out[r * 4 + 1] = (int64_t)a[0];
+    out[r * 4 + 2] = (int64_t)a[1];
+    out[r * 4 + 3] = (int64_t)a[2];
+    ++r;
+  }
+  return r;
 }
 
 }  // extern "C"

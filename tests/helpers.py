@@ -334,3 +334,52 @@ def gen_c2(n: int, seed: int = 2, n_users: int = 1000000, n_urls: int = 100000):
     dur = np.empty(n, np.int32)
     g.wsg_c2(seed, n, n_users, n_urls, ws.ptr(user), ws.ptr(region), ws.ptr(date), ws.ptr(url), ws.ptr(dur))
     return user, region, date, url, dur
+
+
+# ------------------------------------------------------------------ query-level oracles
+
+def oracle_c2(user, region, date, url, dur, date_lt=19723 + 300, region_lt=224):
+    O = oracle()
+    O.or_c2_query.restype = I64
+    O.or_c2_query.argtypes = [P, P, P, P, P, I64, C.c_int32, C.c_int32, P, I64]
+    cap = len(np.unique(user)) + 1
+    out = np.zeros((cap, 4), np.int64)
+    g = O.or_c2_query(ws.ptr(user), ws.ptr(region), ws.ptr(date), ws.ptr(url), ws.ptr(dur), len(user), date_lt, region_lt,
+                      ws.ptr(out), cap)
+    assert g >= 0
+    return out[:g]
+
+
+def oracle_c3(fk, qty, price, flag, dkey, dattr, attr_lt=50):
+    O = oracle()
+    O.or_c3_query.restype = I64
+    O.or_c3_query.argtypes = [P, P, P, P, I64, P, P, I64, C.c_int32, P, I64]
+    out = np.zeros((256, 4), np.int64)
+    g = O.or_c3_query(ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag), len(fk), ws.ptr(dkey), ws.ptr(dattr), len(dkey),
+                      attr_lt, ws.ptr(out), 256)
+    assert g >= 0
+    return out[:g]
+
+
+def check_topn(got: np.ndarray, full: np.ndarray) -> None:
+    """ORDER BY count DESC LIMIT n parity with ties: the counts equal the oracle's top counts,
+    and every returned row is exactly the oracle's row for that key (tie order is unspecified)."""
+    n = len(got)
+    assert n == min(len(full), n)
+    assert np.array_equal(got[:, 1], full[:n, 1]), (got[:, 1], full[:n, 1])
+    by_key = {int(r[0]): r for r in full}
+    for r in got:
+        assert np.array_equal(r, by_key[int(r[0])]), (r, by_key.get(int(r[0])))
+
+
+def gen_c3(n_fact: int, n_dim: int, seed: int = 3):
+    g = ws.datagen()
+    dkey = np.empty(n_dim, np.int32)
+    dattr = np.empty(n_dim, np.int32)
+    g.wsg_c3_dim(seed, n_dim, ws.ptr(dkey), ws.ptr(dattr))
+    fk = np.empty(n_fact, np.int32)
+    qty = np.empty(n_fact, np.int64)
+    price = np.empty(n_fact, np.int64)
+    flag = np.empty(n_fact, np.int32)
+    g.wsg_c3_fact(seed, n_fact, n_dim, ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag))
+    return fk, qty, price, flag, dkey, dattr

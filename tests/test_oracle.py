@@ -1,0 +1,322 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted as the parity reference:
+  - against golden vectors recorded from the reference's own expression semantics (its
+    devtypes.h compiled on the host, tests/golden/make_golden.py), and against that harness
+    directly when it is built here;
+  - against brute force / closed forms (numpy group-bys, nested-loop joins, stable sorts,
+    Hinnant's civil-date identities, 128-bit arithmetic);
+  - against the known answers of the reference's smoke test (tests/smoke.py:20-35)."""
+import ctypes as C
+import json
+import os
+import random
+import struct
+
+import numpy as np
+import pytest
+
+import helpers as H
+import warpstore_amd as ws
+from warpstore_amd import abi
+from golden.make_golden import Case, encoded_cases, gen_cases, NROWS
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "vm_golden.json")
+
+
+def fbits(x: float) -> str:
+    return struct.pack("<d", x).hex()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def test_vm_matches_reference_golden(golden):
+    O = H.oracle()
+    bad = []
+    for i, case in enumerate(golden["vm"]):
+        code = [tuple(c) for c in case["code"]]
+        c = Case(case["cols"], code, case["sets"], case["maps"])
+        for r in range(NROWS):
+            got = fbits(O.or_eval_prog(C.byref(c.env), None, c.prog, r))
+            if got != case["expect"][r]:
+                bad.append((i, r, got, case["expect"][r], code))
+    assert not bad, f"{len(bad)} mismatches, first: {bad[:3]}"
+
+
+def test_vm_matches_reference_harness_live():
+    R = H.ref_lib()
+    if R is None:
+        pytest.skip("reference harness not built here (no reference tree)")
+    O = H.oracle()
+    for case in gen_cases(seed=99, n=120):
+        c = Case(case["cols"], case["code"], case["sets"], case["maps"])
+        for r in range(NROWS):
+            a = fbits(O.or_eval_prog(C.byref(c.env), None, c.prog, r))
+            b = fbits(R.ref_eval_prog(C.addressof(c.env), None, 0, c.prog.len, r))
+            assert a == b, (case["code"], r, a, b)
+
+
+def test_decoders_match_reference(golden):
+    O = H.oracle()
+    for d in golden["decode"]:
+        blob = np.frombuffer(bytes.fromhex(d["blob"]), np.uint8).copy()
+        v = abi.ColView()
+        v.p, v.t, v.scale, v.enc, v.off = blob.ctypes.data, d["t"], d["scale"], d["enc"], d["off"]
+        for r in range(d["rows"]):
+            assert fbits(O.or_value(C.byref(v), r)) == d["load"][r], (d["enc"], d["t"], r)
+            if d["enc"] < 4:
+                assert O.or_stored_int(C.byref(v), r) == d["colint"][r]
+
+
+def test_dates_checked_and_functions(golden):
+    O = H.oracle()
+    for z, y, m, d, back in golden["dates"]:
+        yy, mm, dd = C.c_int64(), C.c_uint(), C.c_uint()
+        O.or_civil_from_days(z, C.byref(yy), C.byref(mm), C.byref(dd))
+        assert (yy.value, mm.value, dd.value) == (y, m, d)
+        assert O.or_days_from_civil(y, m, d) == back == z
+    for op, a, b, r, f in golden["checked"]:
+        flag = C.c_int(0)
+        assert O.or_checked(op, a, b, C.byref(flag)) == r and flag.value == f
+    for fn, x, want in golden["call1"]:
+        xv = struct.unpack("<d", bytes.fromhex(x))[0]
+        assert fbits(O.or_call1(fn, xv)) == want, (fn, xv)
+    for fn, a, b, want in golden["call2"]:
+        av, bv = (struct.unpack("<d", bytes.fromhex(h))[0] for h in (a, b))
+        assert fbits(O.or_call2(fn, av, bv)) == want, (fn, av, bv)
+
+
+def test_civil_dates_closed_forms():
+    O = H.oracle()
+    assert O.or_days_from_civil(1970, 1, 1) == 0
+    assert O.or_days_from_civil(2000, 3, 1) == 11017
+    assert O.or_days_from_civil(2024, 1, 1) == 19723
+    # consecutive days: +1 each, month lengths of a leap year
+    d0 = O.or_days_from_civil(2024, 1, 1)
+    lengths = [31, 29, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31]
+    acc = d0
+    for m, ln in enumerate(lengths, start=1):
+        assert O.or_days_from_civil(2024, m, 1) == acc
+        acc += ln
+    assert O.or_days_from_civil(2025, 1, 1) == acc
+
+
+def test_checked_arithmetic_closed_form():
+    O = H.oracle()
+    rng = random.Random(3)
+    for _ in range(2000):
+        a = rng.randint(-2 ** 63, 2 ** 63 - 1)
+        b = rng.choice([rng.randint(-2 ** 63, 2 ** 63 - 1), rng.randint(-1000, 1000)])
+        for op, exact in ((0, a + b), (1, a - b), (2, a * b)):
+            flag = C.c_int(0)
+            r = O.or_checked(op, a, b, C.byref(flag))
+            wrapped = (exact + 2 ** 63) % 2 ** 64 - 2 ** 63
+            assert r == wrapped and flag.value == (exact != wrapped)
+
+
+def _agg_case(rng, n, nulls=True):
+    key = rng.integers(0, 37, n).astype(np.int32)
+    v = rng.integers(-2 ** 40, 2 ** 40, n).astype(np.int64)
+    if nulls:
+        v[rng.random(n) < 0.1] = abi.NULL_I64
+    x = rng.standard_normal(n)
+    x[rng.random(n) < 0.05] = np.nan
+    deleted = rng.random(n) < 0.15
+    return key, v, x, deleted
+
+
+def test_aggregate_matches_numpy_bruteforce():
+    rng = np.random.default_rng(21)
+    n = 20_000
+    key, v, x, deleted = _agg_case(rng, n)
+    t = H.Table({"k": key, "v": v, "x": x}, [7000, 0, 13000], deleted)
+    b = abi.EnvBuilder()
+    sk, sv, sx = b.slot(abi.T_I32), b.slot(abi.T_I64N), b.slot(abi.T_F64)
+    plan = abi.AggPlan()
+    plan.has_where = 1
+    plan.where = b.prog([(abi.OP_COL, sx), b.dlit(1.5), (abi.OP_LT,)])  # NULL x fails the filter
+    plan.nkeys, plan.keys[0], plan.key_int[0] = 1, b.col(sk), 1
+    ops = [abi.AGG_COUNT, abi.AGG_SUM, abi.AGG_SUM, abi.AGG_MIN, abi.AGG_MAX, abi.AGG_SUM, abi.AGG_MAX]
+    plan.naggs = len(ops)
+    for a, o in enumerate(ops):
+        plan.ops[a] = o
+    plan.agg_int[0] = 1
+    for a in (1, 2, 3, 4):
+        plan.aggs[a], plan.agg_int[a], plan.agg_bits[a], plan.agg_null[a] = b.col(sv), 1, 1, abi.T_I64N
+    plan.agg_part[1], plan.agg_part[2] = 1, 2
+    plan.aggs[5] = plan.aggs[6] = b.col(sx)
+    host = H.Bound(t, [("k", abi.T_I32, 0), ("v", abi.T_I64N, 0), ("x", abi.T_F64, 0)], False)
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, host)
+    live = ~deleted & (x < 1.5)
+    keys = np.unique(key[live])
+    assert np.array_equal(ok[:, 0].view(np.int64), keys)
+    for i, kk in enumerate(keys):
+        m = live & (key == kk)
+        vv = v[m][v[m] != abi.NULL_I64]
+        assert oa[i, 0] == m.sum()
+        lo = sum(int(q) & 0xffffffff for q in vv)
+        hi = sum(int(q) >> 32 for q in vv)
+        assert oa[i, 1:2].view(np.int64)[0] == lo and oa[i, 2:3].view(np.int64)[0] == hi
+        assert hi * 2 ** 32 + lo == sum(int(q) for q in vv)  # the split recombines exactly
+        if len(vv):
+            assert oa[i, 3:4].view(np.int64)[0] == vv.min() and oa[i, 4:5].view(np.int64)[0] == vv.max()
+        np.testing.assert_allclose(oa[i, 5], x[m].sum(), rtol=1e-12)
+        assert oa[i, 6] == x[m].max()
+
+
+def test_nokeys_128bit_sum_closed_form():
+    n = 5000
+    v = np.full(n, 2 ** 62 + 12345, np.int64)
+    t = H.Table({"v": v}, n)
+    b = abi.EnvBuilder()
+    s = b.slot(abi.T_I64)
+    plan = abi.AggPlan()
+    plan.naggs = 2
+    plan.ops[0], plan.ops[1] = abi.AGG_COUNT, abi.AGG_SUM
+    plan.agg_int[0] = plan.agg_int[1] = 1
+    plan.aggs[1] = b.col(s)
+    _, oa, hi = H.oracle_aggregate(plan, b.env, H.Bound(t, [("v", abi.T_I64, 0)], False))
+    total = (int(hi[1]) << 64) + int(oa[0, 1:2].view(np.uint64)[0])
+    assert total == n * (2 ** 62 + 12345)
+
+
+def test_smoke_known_answers():
+    """The reference's smoke test table (tests/smoke.py:16-35), names dictionary-encoded."""
+    names = ["apple", "banana", "cherry"]
+    name = np.array([0, 1, 2, 0], np.int32)
+    price = np.array([1.5, 0.25, 3.0, 2.5])
+    O = H.oracle()
+    day = lambda y, m, d: O.or_days_from_civil(y, m, d)  # noqa: E731
+    d = np.array([day(2024, 1, 1), day(2024, 2, 1), day(2024, 3, 1), day(2024, 4, 1)], np.int32)
+    t = H.Table({"name": name, "price": price, "d": d}, 4)
+    host = H.Bound(t, [("name", abi.T_I32, 0), ("price", abi.T_F64, 0), ("d", abi.T_I32, 0)], False)
+    # GROUP BY name -> COUNT(*), SUM(price), MIN(d)
+    b = abi.EnvBuilder()
+    sn, sp, sd = b.slot(abi.T_I32), b.slot(abi.T_F64), b.slot(abi.T_I32)
+    plan = abi.AggPlan()
+    plan.nkeys, plan.keys[0], plan.key_int[0] = 1, b.col(sn), 1
+    plan.naggs = 3
+    plan.ops[0], plan.ops[1], plan.ops[2] = abi.AGG_COUNT, abi.AGG_SUM, abi.AGG_MIN
+    plan.agg_int[0] = 1
+    plan.aggs[1] = b.col(sp)
+    plan.aggs[2], plan.agg_int[2] = b.col(sd), 1
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, host)
+    got = [(names[int(k)], int(a[0]), a[1], int(a[2])) for k, a in zip(ok[:, 0].view(np.int64), oa)]
+    assert got == [("apple", 2, 4.0, day(2024, 1, 1)), ("banana", 1, 0.25, day(2024, 2, 1)),
+                   ("cherry", 1, 3.0, day(2024, 3, 1))]
+    # WHERE name LIKE 'b%' OR price > 2 -> COUNT(*), SUM(price) = (3, 5.75); LIKE is a dictionary bitmap
+    like = np.array([0b010], np.uint32)
+    b = abi.EnvBuilder()
+    sn, sp = b.slot(abi.T_I32), b.slot(abi.T_F64)
+    si = b.add_set(like.ctypes.data, 3)
+    plan = abi.AggPlan()
+    plan.has_where = 1
+    plan.where = b.prog([(abi.OP_COL, sn), (abi.OP_INSET, si), (abi.OP_COL, sp), b.dlit(2.0), (abi.OP_GT,), (abi.OP_OR,)])
+    plan.naggs = 2
+    plan.ops[0], plan.ops[1] = abi.AGG_COUNT, abi.AGG_SUM
+    plan.agg_int[0] = 1
+    plan.aggs[1] = b.col(sp)
+    _, oa, _ = H.oracle_aggregate(plan, b.env, H.Bound(t, [("name", abi.T_I32, 0), ("price", abi.T_F64, 0)], False))
+    assert (int(oa[0, 0]), oa[0, 1]) == (3, 5.75)
+    # GROUP BY CASE WHEN price > 10 THEN 1 ELSE 0 END after the +10 update of apples -> (0, 2), (1, 2)
+    price2 = np.array([11.5, 0.25, 3.0, 12.5])
+    t2 = H.Table({"price": price2}, 4)
+    b = abi.EnvBuilder()
+    sp = b.slot(abi.T_F64)
+    plan = abi.AggPlan()
+    plan.nkeys = 1
+    plan.keys[0] = b.prog([(abi.OP_COL, sp), b.dlit(10.0), (abi.OP_GT,), b.ilit(1), b.ilit(0), (abi.OP_SELECT,)])
+    plan.naggs, plan.ops[0], plan.agg_int[0] = 1, abi.AGG_COUNT, 1
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, H.Bound(t2, [("price", abi.T_F64, 0)], False))
+    assert [(k, int(a[0])) for k, a in zip(ok[:, 0], oa)] == [(0.0, 2), (1.0, 2)]
+    # YEAR(d) GROUP BY -> (2024, 4)
+    b = abi.EnvBuilder()
+    sd = b.slot(abi.T_I32)
+    plan = abi.AggPlan()
+    plan.nkeys, plan.keys[0] = 1, b.prog([(abi.OP_COL, sd), (abi.OP_YEAR,)])
+    plan.naggs, plan.ops[0], plan.agg_int[0] = 1, abi.AGG_COUNT, 1
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, H.Bound(t, [("d", abi.T_I32, 0)], False))
+    assert [(k, int(a[0])) for k, a in zip(ok[:, 0], oa)] == [(2024.0, 4)]
+
+
+def test_select_join_sort_bruteforce():
+    O = H.oracle()
+    rng = np.random.default_rng(5)
+    # joins: nested loops on small inputs
+    bk = rng.integers(0, 30, 60).astype(np.int32)
+    bk[::13] = abi.NULL_I32
+    pk = rng.integers(-5, 40, 200).astype(np.int32)
+    pk[::17] = abi.NULL_I32
+    for outer in (0, 1, 2):
+        cap = 100000
+        ob, op = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        bv, pv = abi.ColView(), abi.ColView()
+        bv.p, bv.t, pv.p, pv.t = bk.ctypes.data, abi.T_I32N, pk.ctypes.data, abi.T_I32N
+        m = O.or_join(C.byref(bv), len(bk), C.byref(pv), len(pk), outer, ob.ctypes.data, op.ctypes.data, cap)
+        exp = []
+        matched = set()
+        for j, k in enumerate(pk):
+            hits = [i for i, kb in enumerate(bk) if k != abi.NULL_I32 and kb == k]
+            matched.update(hits)
+            exp += [(j, i) for i in hits] or ([(j, -1)] if outer else [])
+        if outer == 2:
+            exp += [(-1, i) for i in range(len(bk)) if i not in matched]
+        assert sorted(zip(op[:m], ob[:m])) == sorted(exp)
+    # stable sort: numpy lexsort with the NULL / -0 / descending conventions
+    m = 5000
+    k0 = rng.integers(0, 9, m).astype(np.float64)
+    k0[rng.random(m) < 0.1] = np.nan
+    k1 = np.round(rng.standard_normal(m), 1)
+    k1[k1 == 0] = -0.0
+    desc = np.array([1, 0], np.uint8)
+    perm = np.zeros(m, np.int32)
+    O.or_sort_perm((C.c_void_p * 2)(k0.ctypes.data, k1.ctypes.data), desc.ctypes.data, 2, m, perm.ctypes.data)
+    n0 = np.where(np.isnan(k0), -np.inf, k0)
+    n1 = np.where(k1 == 0, 0.0, k1)
+    exp = np.lexsort((np.arange(m), n1, -n0))
+    assert np.array_equal(perm, exp)
+
+
+def test_c2_query_oracle_vs_numpy():
+    user, region, date, url, dur = H.gen_c2(200_000, n_users=5000, n_urls=3000)
+    full = H.oracle_c2(user, region, date, url, dur)
+    m = (date < 19723 + 300) & (region < 224)
+    u, inv = np.unique(user[m], return_inverse=True)
+    cnt = np.bincount(inv)
+    s = np.bincount(inv, weights=dur[m].astype(np.float64)).astype(np.int64)
+    pairs = np.unique(np.stack([user[m], url[m]], 1), axis=0)
+    dcount = dict(zip(*np.unique(pairs[:, 0], return_counts=True)))
+    exp = sorted(((int(a), int(b), int(c), int(dcount[a])) for a, b, c in zip(u, cnt, s)), key=lambda r: (-r[1], r[0]))
+    assert [tuple(r) for r in full] == exp
+
+
+def test_c3_query_oracle_vs_numpy():
+    fk, qty, price, flag, dkey, dattr = H.gen_c3(300_000, 20_000)
+    got = H.oracle_c3(fk, qty, price, flag, dkey, dattr)
+    sel = set(dkey[dattr < 50].tolist())
+    m = np.isin(fk, list(sel))
+    exp = []
+    for f in np.unique(flag[m]):
+        mm = m & (flag == f)
+        exp.append([f, mm.sum(), int(price[mm].sum()), int(qty[mm].sum())])
+    assert np.array_equal(got, np.array(exp, np.int64))
+
+
+def test_encoded_decode_roundtrip_via_generator():
+    """Frame-of-reference encoding decodes back to the original values (NULL sentinel kept)."""
+    rng = np.random.default_rng(9)
+    g = ws.datagen()
+    O = H.oracle()
+    for dtype, lo, hi, tcode in ((np.int32, 0, 250, abi.T_I32N), (np.int32, -1000, 60000, abi.T_I32N),
+                                 (np.int64, 10 ** 12, 10 ** 12 + 3 * 10 ** 9, abi.T_I64N)):
+        x = rng.integers(lo, hi, 3000).astype(dtype)
+        x[::29] = abi.NULL_I32 if dtype == np.int32 else abi.NULL_I64
+        out = np.zeros(16 + x.nbytes + 16, np.uint8)
+        e = g.wsg_encode_int(x.ctypes.data, 1 if dtype == np.int64 else 0, len(x), out.ctypes.data)
+        assert e in (1, 2, 3)
+        v = abi.ColView()
+        v.p, v.t, v.enc = out.ctypes.data, tcode, e
+        back = np.array([O.or_stored_int(C.byref(v), r) for r in range(len(x))])
+        assert np.array_equal(back, x.astype(np.int64))

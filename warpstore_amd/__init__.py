@@ -82,7 +82,8 @@ _SIGS = {
     "ws_jit_clear_cache": (None, []),
     "ws_jit_compile_check": (I, [C.POINTER(ExprEnv), I, C.POINTER(AggPlan), I, C.c_char_p, C.c_char_p, SZ,
                                  C.POINTER(SZ)]),
-    "ws_prof_enable": (None, [I]), "ws_prof_reset": (None, []), "ws_prof_collect": (I, [C.POINTER(abi.ProfRow), I]),
+    "ws_prof_enable": (None, [I]), "ws_prof_only": (None, [C.c_char_p]), "ws_prof_reset": (None, []), "ws_prof_collect": (I, [C.POINTER(abi.ProfRow), I]),
+    "ws_copy_kernel": (I, [P, P, SZ]),
 }
 
 

@@ -307,7 +307,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
     double* out = A->nk_out;
     if (jf) {
       void* args[] = {&up.ddesc, &out, &parts.p, &done};
-      jit_launch_raw(jf, grid, kBlock, 0, args, "agg_nokeys");
+      jit_launch_raw(jf, grid, kBlock, 0, args, "agg_nokeys_jit");
     } else {
       ProfScope ps("agg_nokeys");
       hipLaunchKernelGGL(lean ? ws_l_nokeys : ws_i_nokeys, dim3(grid), dim3(kBlock), 0, stream(), up.ddesc, out, parts.p, done);
@@ -335,7 +335,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
     i64* pp = parts.p;
     if (jf) {
       void* args[] = {&up.ddesc, &gacc, &pp, &flags};
-      jit_launch_raw(jf, (unsigned)grid, kBlock, dp.lds, args, "agg_dense");
+      jit_launch_raw(jf, (unsigned)grid, kBlock, dp.lds, args, "agg_dense_jit");
     } else {
       ProfScope ps("agg_dense");
       hipLaunchKernelGGL(lean ? ws_l_dense : ws_i_dense, dim3((unsigned)grid), dim3(kBlock), dp.lds, stream(), up.ddesc, gacc, pp,
@@ -363,7 +363,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   GTable t{A->keys, A->kv, A->acc, (u64)(A->cap - 1), A->flags, A->count};
   if (jf) {
     void* args[] = {&up.ddesc, &t};
-    jit_launch_raw(jf, grid, kBlock, lds, args, "agg_hash");
+    jit_launch_raw(jf, grid, kBlock, lds, args, "agg_hash_jit");
   } else {
     ProfScope ps("agg_hash");
     hipLaunchKernelGGL(lean ? ws_l_grouped : ws_i_grouped, dim3(grid), dim3(kBlock), lds, stream(), up.ddesc, t);
@@ -573,7 +573,7 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
   void* jf = jit ? jit_function(jit, fn) : nullptr;
   if (jf) {
     void* args[] = {&up.ddesc, &st.p, &ctr, &out_idx, &pb, &total, &err};
-    jit_launch_raw(jf, (unsigned)ntiles, kBlock, 0, args, "select");
+    jit_launch_raw(jf, (unsigned)ntiles, kBlock, 0, args, "select_jit");
   } else {
     ProfScope ps("select");
     const bool lean = !has_where || prog_is_simple(env, where);

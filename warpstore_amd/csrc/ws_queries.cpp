@@ -1,0 +1,619 @@
+// Engine-side drivers of the measured query shapes, written against the drop-in gdb:: operator
+// surface (include/warpstore_amd/gpu.h, jit.h) exactly the way the engine's planner calls it:
+//   - storage: GPU-resident segments of 2^20 rows per column (reference storage.h:33), optional
+//     frame-of-reference encoding per segment column (storage.cu:1138-1170);
+//   - bytecode compile of WHERE / keys / aggregate arguments into an ExprEnv (engine_core.h:1485);
+//   - aggregate slot layout, dense-vs-hash sizing and the overflow retry (engine_select.cpp:647-724,
+//     1130-1235, 1493-1613);
+//   - COUNT(DISTINCT) in two GPU phases and post-aggregation ORDER BY .. LIMIT on the GPU
+//     (engine_select.cpp:1239-1373, 1403-1492);
+//   - materialize with a join-filter bitmap, hash join and the join output gathers
+//     (engine_core.h:4345-4611, engine_select.cpp:1015-1041).
+// The host side above the C ABI: it owns no kernels and calls only the public surface.
+#define WARPSTORE_AMD_STANDALONE 1
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "warpstore_amd/gpu.h"
+#include "warpstore_amd/jit.h"
+
+extern "C" int wsg_encode_int(const void* x, int i64, int64_t n, uint8_t* out);
+
+namespace {
+
+using namespace gdb;
+constexpr int64_t kSegRows = 1 << 20;
+
+// ------------------------------------------------------------------ storage
+
+struct SegCol {
+  void* dev = nullptr;  // raw column, or header + codes when enc != 0
+  uint8_t enc = 0;
+};
+struct DevColumn {
+  int t = WS_T_I32;  // type code of the column's views
+  std::vector<SegCol> segs;
+  int64_t mn = INT64_MAX, mx = INT64_MIN;  // min / max of integer columns (colRange)
+};
+struct DevTable {
+  std::vector<int64_t> seg_rows;
+  std::vector<DevColumn> cols;
+  int64_t rows = 0;
+  ~DevTable() {
+    for (auto& c : cols)
+      for (auto& s : c.segs) devFree(s.dev);
+  }
+  ColView view(int c, size_t seg) const {
+    ColView v{};
+    v.p = cols[c].segs[seg].dev;
+    v.t = (uint8_t)cols[c].t;
+    v.enc = cols[c].segs[seg].enc;
+    v.off = 0;
+    return v;
+  }
+};
+
+int width_of(int t) { return (t == WS_T_I32 || t == WS_T_I32N) ? 4 : 8; }
+
+// ------------------------------------------------------------------ bytecode compiler (subset)
+
+struct Comp {
+  ExprEnv env;
+  int ncode = 0, nconst = 0, ncols = 0;
+  Comp() { std::memset(&env, 0, sizeof env); }
+  int slot(int t, uint8_t enc_hint = 0) {
+    const int s = ncols++;
+    env.cols[s].t = (uint8_t)t;
+    env.cols[s].enc = enc_hint;
+    return s;
+  }
+  int const_int(int64_t v) {
+    double d;
+    std::memcpy(&d, &v, 8);
+    for (int i = 0; i < nconst; ++i)
+      if (std::memcmp(&env.consts[i], &d, 8) == 0) return i;
+    env.consts[nconst] = d;
+    return nconst++;
+  }
+  int const_dbl(double v) {
+    for (int i = 0; i < nconst; ++i)
+      if (env.consts[i] == v) return i;
+    env.consts[nconst] = v;
+    return nconst++;
+  }
+  struct I {
+    uint8_t op, is_int;
+    uint16_t arg;
+  };
+  Prog prog(std::initializer_list<I> code) {
+    Prog p;
+    p.start = (int16_t)ncode;
+    for (const I& i : code) env.code[ncode++] = ws_instr{i.op, i.is_int, i.arg};
+    p.len = (int16_t)(ncode - p.start);
+    return p;
+  }
+  I col(int s) { return I{WS_OP_COL, 0, (uint16_t)s}; }
+  I ilit(int64_t v) { return I{WS_OP_CONST, 1, (uint16_t)const_int(v)}; }
+  static I op(int o) { return I{(uint8_t)o, 0, 0}; }
+};
+
+// Per batch: the env with the slots bound to the segment's columns (bindView, engine_core.h:4073).
+ExprEnv bind_segment(const Comp& c, const DevTable& t, const std::vector<int>& slot_col, size_t seg) {
+  ExprEnv e = c.env;
+  for (size_t s = 0; s < slot_col.size(); ++s) {
+    ColView v = t.view(slot_col[s], seg);
+    v.t = c.env.cols[s].t;
+    e.cols[s] = v;
+  }
+  return e;
+}
+
+int dense_kind(int op, bool is_int) {
+  return op == WS_AGG_COUNT ? WS_DK_COUNT
+         : op == WS_AGG_SUM ? (is_int ? WS_DK_SUM_I : WS_DK_SUM_D)
+         : op == WS_AGG_MIN ? (is_int ? WS_DK_MIN_I : WS_DK_MIN_D)
+                            : (is_int ? WS_DK_MAX_I : WS_DK_MAX_D);
+}
+
+// Dense spec decision for integer keys with known ranges (engine_select.cpp:1169-1229).
+bool plan_dense(const AggPlan& plan, const std::vector<std::pair<int64_t, int64_t>>& ranges, int64_t rows_in, DenseSpec& ds) {
+  std::memset(&ds, 0, sizeof ds);
+  int64_t G = 1;
+  for (int k = 0; k < plan.nkeys; ++k) {
+    const uint64_t span = (uint64_t)(ranges[k].second - ranges[k].first);
+    if (span >= (1ull << 26)) return false;
+    const int64_t range = (int64_t)span + 1;
+    if (G > (1ll << 26) / range) return false;
+    ds.kmin[k] = ranges[k].first;
+    ds.range[k] = range;
+    ds.stride[k] = G;
+    G *= range;
+  }
+  int off = 0;
+  bool int_only = true;
+  for (int a = 0; a < plan.naggs; ++a) {
+    ds.kind[a] = (uint8_t)dense_kind(plan.ops[a], plan.agg_int[a] != 0);
+    ds.slot_off[a] = off;
+    const int64_t bytes = G * (ds.kind[a] == WS_DK_COUNT ? 4 : 8);
+    off += (int)std::min<int64_t>((bytes + 7) / 8 * 8, 1 << 30);
+    int_only &= ds.kind[a] == WS_DK_COUNT || ds.kind[a] == WS_DK_SUM_I || ds.kind[a] == WS_DK_MIN_I || ds.kind[a] == WS_DK_MAX_I;
+  }
+  if (G * plan.naggs <= 88) {
+    ds.priv = 1;
+    ds.G = G;
+  } else if (off <= 48 * 1024 && G < 64 && !int_only) {
+    return false;  // hash aggregation
+  } else if (off <= 48 * 1024) {
+    ds.copies = (int)std::min<int64_t>(8, (48 * 1024) / off);
+    ds.copy_bytes = off;
+    ds.G = G;
+  } else if (G <= std::max<int64_t>(4 * rows_in, 1 << 16) && G * (8 + 8 * plan.nkeys + 8 * plan.naggs) <= (int64_t)512 << 20) {
+    ds.copies = 0;
+    ds.G = G;
+  } else {
+    return false;
+  }
+  return true;
+}
+
+int64_t pow2_at_least(int64_t v, int64_t lo) {
+  int64_t c = lo;
+  while (c < v) c <<= 1;
+  return c;
+}
+
+// ------------------------------------------------------------------ C1: filter + dense GROUP BY
+
+// SELECT k, COUNT(*), SUM(v), MIN(v), MAX(v), AVG(p) FROM t WHERE k < 500 AND v > 1000 GROUP BY k
+// Table columns: 0 k (INT), 1 v (BIGINT), 2 p (DOUBLE). Result rows: k, count, sum, min, max as
+// int64 and avg as double.
+int64_t query_c1(const DevTable& t, int64_t* out_i, double* out_d, int64_t cap_rows, bool dense_on) {
+  Comp c;
+  const int sk = c.slot(WS_T_I32), sv = c.slot(WS_T_I64), sp = c.slot(WS_T_F64);
+  const std::vector<int> slot_col = {0, 1, 2};
+  AggPlan plan;
+  std::memset(&plan, 0, sizeof plan);
+  plan.has_where = 1;
+  plan.where = c.prog({c.col(sk), c.ilit(500), Comp::op(WS_OP_LT), c.col(sv), c.ilit(1000), Comp::op(WS_OP_GT),
+                       Comp::op(WS_OP_AND)});
+  plan.nkeys = 1;
+  plan.keys[0] = c.prog({c.col(sk)});
+  plan.key_int[0] = (uint8_t)progIsInt(c.env, plan.keys[0]);
+  const uint8_t ops[] = {WS_AGG_COUNT, WS_AGG_SUM, WS_AGG_SUM, WS_AGG_MIN, WS_AGG_MAX, WS_AGG_SUM};
+  plan.naggs = 6;
+  for (int a = 0; a < 6; ++a) plan.ops[a] = ops[a];
+  for (int a = 1; a <= 4; ++a) plan.aggs[a] = c.prog({c.col(sv)});
+  plan.aggs[5] = c.prog({c.col(sp)});
+  plan.agg_part[1] = 1, plan.agg_part[2] = 2;
+  plan.agg_int[0] = 1;
+  for (int a = 1; a < 6; ++a) {
+    plan.agg_int[a] = (uint8_t)progIsInt(c.env, plan.aggs[a]);
+    plan.agg_bits[a] = (uint8_t)(plan.agg_int[a] && a <= 4);  // BIGINT column: exact int64 bits
+  }
+  DenseSpec ds;
+  const bool dense = dense_on && plan_dense(plan, {{t.cols[0].mn, t.cols[0].mx}}, t.rows, ds);
+  plan.dense_priv = dense ? (ds.priv ? 1 : 2) : 0;
+  const double est = std::min<double>((double)(t.cols[0].mx - t.cols[0].mn + 1), (double)std::max<int64_t>(t.rows, 1));
+  int64_t cap = 1024;
+  const int64_t max_cap = (1ll << 30) / (8 + 8 * plan.nkeys + 8 * plan.naggs);
+  while (cap < est * 2 && cap * 2 <= max_cap) cap <<= 1;
+  const JitProgram* jit = jitGet(c.env, c.ncols, plan, dense ? JF_DENSE : JF_GROUPED, "aggregate");
+  std::vector<double> keys, accs;
+  int64_t g = 0;
+  for (;;) {
+    GpuAggregator agg(plan, cap, dense ? &ds : nullptr);
+    for (size_t s = 0; s < t.seg_rows.size(); ++s) {
+      ExprEnv e = bind_segment(c, t, slot_col, s);
+      agg.addBatch(e, c.ncols, t.seg_rows[s], nullptr);
+    }
+    agg.run(c.env, jit);
+    if (agg.overflowed()) {
+      cap = pow2_at_least(std::max<int64_t>(cap * 4, agg.groupCount() * 4), 1);
+      continue;
+    }
+    g = agg.fetch(keys, accs);
+    break;
+  }
+  int64_t n = std::min(g, cap_rows);
+  for (int64_t i = 0; i < n; ++i) {
+    const double* a = &accs[i * 6];
+    int64_t kbits, lo, hi, mn, mx;
+    std::memcpy(&kbits, &keys[i], 8);
+    std::memcpy(&lo, &a[1], 8);
+    std::memcpy(&hi, &a[2], 8);
+    std::memcpy(&mn, &a[3], 8);
+    std::memcpy(&mx, &a[4], 8);
+    const __int128 sum = (__int128)hi * ((__int128)1 << 32) + lo;  // engine_core.h:1972-1983
+    out_i[i * 5 + 0] = kbits;
+    out_i[i * 5 + 1] = (int64_t)a[0];
+    out_i[i * 5 + 2] = (int64_t)sum;
+    out_i[i * 5 + 3] = mn;
+    out_i[i * 5 + 4] = mx;
+    out_d[i] = a[5] / a[0];
+  }
+  return g;
+}
+
+// ------------------------------------------------------------------ C2: COUNT DISTINCT + top 10
+
+// SELECT user_id, COUNT(*) c, SUM(dur), COUNT(DISTINCT url) FROM h WHERE event_date < 19723+300 AND
+// region < 224 GROUP BY user_id ORDER BY c DESC LIMIT 10.
+// Table columns: 0 user_id (BIGINT), 1 region (INT, may be encoded), 2 event_date (INT), 3 url
+// (INT codes), 4 dur (INT). Result rows: user_id, count, sum(dur), count distinct url.
+struct C2Stats {
+  int64_t pairs = 0, groups = 0, cap1 = 0, retries = 0;
+};
+int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
+  Comp c;
+  // slots in the engine's compile order: WHERE, keys, aggregates, then the DISTINCT argument
+  const int s_date = c.slot(WS_T_I32), s_region = c.slot(WS_T_I32, t.cols[1].segs[0].enc);
+  const int s_user = c.slot(WS_T_I64), s_dur = c.slot(WS_T_I32), s_url = c.slot(WS_T_I32);
+  const std::vector<int> slot_col = {2, 1, 0, 4, 3};
+  AggPlan plan;
+  std::memset(&plan, 0, sizeof plan);
+  plan.has_where = 1;
+  plan.where = c.prog({c.col(s_date), c.ilit(19723 + 300), Comp::op(WS_OP_LT), c.col(s_region), c.ilit(224),
+                       Comp::op(WS_OP_LT), Comp::op(WS_OP_AND)});
+  plan.nkeys = 1;
+  plan.keys[0] = c.prog({c.col(s_user)});
+  plan.key_int[0] = (uint8_t)progIsInt(c.env, plan.keys[0]);
+  plan.naggs = 2;
+  plan.ops[0] = WS_AGG_COUNT;
+  plan.ops[1] = WS_AGG_SUM;
+  plan.aggs[1] = c.prog({c.col(s_dur)});
+  plan.agg_int[0] = 1;
+  plan.agg_int[1] = (uint8_t)progIsInt(c.env, plan.aggs[1]);  // INT column: no split, no bits
+  const Prog dist = c.prog({c.col(s_url)});
+  const bool dist_int = progIsInt(c.env, dist);
+  // phase 1: GROUP BY (user_id, url)
+  AggPlan plan1 = plan;
+  plan1.nkeys = 2;
+  plan1.keys[1] = dist;
+  plan1.key_int[1] = (uint8_t)dist_int;
+  plan1.dense_priv = 0;
+  const JitProgram* jit1 = jitGet(c.env, c.ncols, plan1, JF_GROUPED, "aggregate (distinct pairs)");
+  int64_t cap1 = pow2_at_least(std::min<int64_t>(std::max<int64_t>(t.rows, 1), 1 << 22), 1024);
+  DevVec<double> dk, da;
+  int64_t g1 = 0;
+  for (;;) {
+    GpuAggregator a1(plan1, cap1, nullptr);
+    for (size_t s = 0; s < t.seg_rows.size(); ++s) {
+      ExprEnv e = bind_segment(c, t, slot_col, s);
+      a1.addBatch(e, c.ncols, t.seg_rows[s], nullptr);
+    }
+    a1.run(c.env, jit1);
+    if (a1.overflowed()) {
+      cap1 = pow2_at_least(std::max<int64_t>(cap1 * 4, a1.groupCount() * 4), 1);
+      if (st) ++st->retries;
+      continue;
+    }
+    g1 = a1.compact(dk, da);
+    break;
+  }
+  // phase 2: GROUP BY user_id over the distinct pairs (interpreted, as the engine runs it)
+  const int nk = plan.nkeys, na = plan1.naggs;
+  ExprEnv env2;
+  std::memset(&env2, 0, sizeof env2);
+  int nc = 0;
+  for (int k = 0; k <= nk; ++k) {
+    const bool in = k < nk ? plan.key_int[k] != 0 : dist_int;
+    env2.cols[k].p = dk.p + (int64_t)k * g1;
+    env2.cols[k].t = (uint8_t)(in ? WS_T_I64 : WS_T_F64);
+    env2.code[nc++] = ws_instr{WS_OP_COL, (uint8_t)in, (uint16_t)k};
+  }
+  AggPlan plan2;
+  std::memset(&plan2, 0, sizeof plan2);
+  plan2.nkeys = nk;
+  for (int k = 0; k < nk; ++k) {
+    plan2.keys[k] = Prog{(int16_t)k, 1};
+    plan2.key_int[k] = plan.key_int[k];
+  }
+  for (int a = 0; a < na; ++a) {
+    env2.cols[nk + 1 + a].p = da.p + (int64_t)a * g1;
+    env2.cols[nk + 1 + a].t = (uint8_t)(plan1.agg_bits[a] ? WS_T_I64 : WS_T_F64);
+    plan2.aggs[a] = Prog{(int16_t)nc, 1};
+    env2.code[nc++] = ws_instr{WS_OP_COL, 0, (uint16_t)(nk + 1 + a)};
+    plan2.ops[a] = plan1.ops[a] == WS_AGG_COUNT ? WS_AGG_SUM : plan1.ops[a];
+    plan2.agg_int[a] = plan2.agg_bits[a] = plan1.agg_bits[a];
+  }
+  if (dist_int) {
+    plan2.aggs[na] = Prog{(int16_t)nc, 1};
+    env2.code[nc++] = ws_instr{WS_OP_CONST, 0, 0};
+    env2.consts[0] = 1.0;
+  } else {
+    plan2.aggs[na] = Prog{(int16_t)nc, 3};
+    env2.code[nc++] = ws_instr{WS_OP_COL, 0, (uint16_t)nk};
+    env2.code[nc++] = ws_instr{WS_OP_ISNULL, 0, 0};
+    env2.code[nc++] = ws_instr{WS_OP_NOT, 0, 0};
+  }
+  plan2.ops[na] = WS_AGG_SUM;
+  plan2.naggs = na + 1;
+  const int64_t cap2 = pow2_at_least(2 * g1, 1024);
+  GpuAggregator agg2(plan2, cap2, nullptr);
+  agg2.addBatch(env2, nk + 1 + na, g1, nullptr);
+  agg2.run(env2, nullptr);
+  if (agg2.overflowed()) throw SqlError("distinct phase 2 overflowed", 1105);
+  // post-aggregation: ORDER BY c DESC LIMIT n on the GPU over the compacted groups
+  DevVec<double> gk, ga;
+  const int64_t g = agg2.compact(gk, ga);
+  ExprEnv penv;
+  std::memset(&penv, 0, sizeof penv);
+  std::vector<DevVec<double>> conv;
+  for (int k = 0; k < nk; ++k) {
+    if (plan2.key_int[k]) {
+      conv.emplace_back(std::max<int64_t>(g, 1));
+      gpuBitsToDouble(gk.p + (int64_t)k * g, g, conv.back().p);
+      penv.cols[k].p = conv.back().p;
+    } else {
+      penv.cols[k].p = gk.p + (int64_t)k * g;
+    }
+    penv.cols[k].t = WS_T_F64;
+  }
+  for (int a = 0; a < plan2.naggs; ++a) {
+    penv.cols[nk + a].p = ga.p + (int64_t)a * g;
+    penv.cols[nk + a].t = WS_T_F64;
+  }
+  penv.code[0] = ws_instr{WS_OP_COL, 0, (uint16_t)(nk + 0)};  // c = COUNT(*) = slot 0
+  const Prog order{0, 1};
+  const int64_t mc = std::min<int64_t>(g, limit);
+  DevVec<double> okey(std::max<int64_t>(g, 1));
+  gpuEvalAt(penv, order, nullptr, g, okey.p);
+  DevVec<int32_t> perm = gpuSortPerm({{okey.p, true}}, g);
+  const int W = nk + plan2.naggs;
+  DevVec<double> tmp(std::max<int64_t>(mc * W, 1));
+  for (int col = 0; col < W; ++col)
+    gpuGather(col < nk ? gk.p + (int64_t)col * g : ga.p + (int64_t)(col - nk) * g, PType::F64, perm.p, mc,
+              tmp.p + (int64_t)col * mc);
+  const std::vector<double> all = tmp.download(mc * W);
+  for (int64_t i = 0; i < mc; ++i) {
+    int64_t user;
+    std::memcpy(&user, &all[i], 8);
+    out[i * 4 + 0] = user;
+    out[i * 4 + 1] = (int64_t)all[(size_t)(1 * mc + i)];
+    out[i * 4 + 2] = std::llround(all[(size_t)(2 * mc + i)]);
+    out[i * 4 + 3] = (int64_t)all[(size_t)(3 * mc + i)];
+  }
+  if (st) st->pairs = g1, st->groups = g, st->cap1 = cap1;
+  return mc;
+}
+
+// ------------------------------------------------------------------ C3: star join
+
+// A materialized relation: device columns of the selected rows.
+struct RelCol {
+  DevVec<uint8_t> data;
+  int t;
+};
+
+// materialize(table, WHERE) of the needed columns (engine_core.h:4345-4487): one fused filter
+// launch over all segments, then one gather per needed column.
+int64_t materialize(const DevTable& t, Comp& c, bool has_where, Prog where, const std::vector<int>& slot_col,
+                    const std::vector<int>& need, std::vector<RelCol>& out) {
+  const JitProgram* jit = nullptr;
+  if (has_where) {
+    AggPlan fp;
+    std::memset(&fp, 0, sizeof fp);
+    fp.has_where = 1;
+    fp.where = where;
+    jit = jitGet(c.env, c.ncols, fp, JF_FLAGM, "filter");
+  }
+  std::vector<int64_t> bn;
+  std::vector<const uint32_t*> bdel;
+  std::vector<ColView> bcols;
+  for (size_t s = 0; s < t.seg_rows.size(); ++s) {
+    bn.push_back(t.seg_rows[s]);
+    bdel.push_back(nullptr);
+    ExprEnv e = bind_segment(c, t, slot_col, s);
+    for (int k = 0; k < c.ncols; ++k) bcols.push_back(e.cols[k]);
+  }
+  DevVec<int32_t> sel;
+  std::vector<int64_t> per;
+  const int64_t m = gpuSelectMulti(c.env, has_where, where, bn, bdel, bcols, c.ncols, sel, per, jit);
+  std::vector<int64_t> boff(bn.size(), 0);
+  for (size_t b = 1; b < bn.size(); ++b) boff[b] = boff[b - 1] + bn[b - 1];
+  out.clear();
+  for (int col : need) {
+    RelCol rc;
+    rc.t = t.cols[col].t;
+    const int w = width_of(rc.t);
+    rc.data = DevVec<uint8_t>(std::max<int64_t>(m, 1) * w);
+    std::vector<ColView> src;
+    for (size_t s = 0; s < bn.size(); ++s) src.push_back(t.view(col, s));
+    const PType pt = w == 4 ? PType::I32 : PType::I64;
+    if (m > 0) gpuGatherMulti(src, boff, pt, w, sel.p, m, rc.data.p);
+    out.push_back(std::move(rc));
+  }
+  return m;
+}
+
+// SELECT f.flag, COUNT(*), SUM(f.price), SUM(f.qty) FROM fact f JOIN dim d ON f.fk = d.key
+// WHERE d.attr < 50 GROUP BY f.flag.
+// fact columns: 0 fk (INT), 1 qty (BIGINT), 2 price (DECIMAL(15,2) as int64), 3 flag (INT);
+// dim columns: 0 key (INT), 1 attr (INT). Result rows: flag, count, raw price sum (scaled by
+// 100), qty sum.
+struct C3Stats {
+  int64_t dim_rows = 0, fact_rows = 0, pairs = 0;
+};
+int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stats* st) {
+  // dim: materialize(dim, [attr < 50], {key})
+  Comp cd;
+  const int d_attr = cd.slot(WS_T_I32);
+  const Prog dwhere = cd.prog({cd.col(d_attr), cd.ilit(50), Comp::op(WS_OP_LT)});
+  std::vector<RelCol> drel;
+  const int64_t nd = materialize(dim, cd, true, dwhere, {1}, {0}, drel);
+  // join filter: the dim keys were reduced below half their table, fact is large: exact bitmap
+  // of the keys over the dim key range (engine_select.cpp:1015-1041)
+  const int64_t lo = dim.cols[0].mn, nbits = dim.cols[0].mx - dim.cols[0].mn + 1;
+  DevVec<uint32_t> bits((nbits + 31) / 32);
+  ColView kv{};
+  kv.p = drel[0].data.p;
+  kv.t = WS_T_I32;
+  gpuKeyBitmap(kv, nd, lo, nbits, bits.p);
+  // fact: materialize(fact, [INSET(fk - lo, bitmap)], {fk, price, qty, flag})
+  Comp cf;
+  const int f_fk = cf.slot(WS_T_I32);
+  cf.env.sets[0] = bits.p;
+  cf.env.set_size[0] = (int32_t)nbits;
+  const Prog fwhere = cf.prog({cf.col(f_fk), cf.ilit(lo), Comp::op(WS_OP_SUB), Comp::I{WS_OP_INSET, 0, 0}});
+  std::vector<RelCol> frel;
+  const int64_t nf = materialize(fact, cf, true, fwhere, {0}, {0, 2, 1, 3}, frel);
+  // hash join: build on the smaller side (dim), probe the filtered fact rows
+  DevVec<int32_t> ib, ip;
+  ColView bk{}, pk{};
+  bk.p = drel[0].data.p, bk.t = WS_T_I32;
+  pk.p = frel[0].data.p, pk.t = WS_T_I32;
+  const int64_t m = gpuHashJoin(bk, nd, pk, nf, ib, ip);
+  // joined relation: gather every needed column of both sides by pair index
+  std::vector<DevVec<uint8_t>> jcols;
+  auto gather_side = [&](const RelCol& rc, const int32_t* idx) {
+    const int w = width_of(rc.t);
+    jcols.emplace_back(std::max<int64_t>(m, 1) * w);
+    if (m) gpuGather(rc.data.p, w == 4 ? PType::I32 : PType::I64, idx, m, jcols.back().p);
+  };
+  gather_side(drel[0], ib.p);
+  for (auto& rc : frel) gather_side(rc, ip.p);
+  // aggregate the joined relation: keys flag; slots COUNT(*), SUM(#RAW price), SUM(qty) lo / hi
+  Comp ca;
+  const int j_flag = ca.slot(WS_T_I32), j_price = ca.slot(WS_T_DECIMAL), j_qty = ca.slot(WS_T_I64);
+  ca.env.cols[j_price].scale = 2;
+  AggPlan plan;
+  std::memset(&plan, 0, sizeof plan);
+  plan.nkeys = 1;
+  plan.keys[0] = ca.prog({ca.col(j_flag)});
+  plan.key_int[0] = (uint8_t)progIsInt(ca.env, plan.keys[0]);
+  plan.naggs = 4;
+  plan.ops[0] = WS_AGG_COUNT;
+  plan.ops[1] = plan.ops[2] = plan.ops[3] = WS_AGG_SUM;
+  plan.aggs[1] = ca.prog({Comp::I{WS_OP_COLRAW, 1, (uint16_t)j_price}});
+  plan.aggs[2] = plan.aggs[3] = ca.prog({ca.col(j_qty)});
+  plan.agg_part[2] = 1, plan.agg_part[3] = 2;
+  plan.agg_int[0] = 1;
+  for (int a = 1; a < 4; ++a) plan.agg_int[a] = (uint8_t)progIsInt(ca.env, plan.aggs[a]);
+  plan.agg_bits[2] = plan.agg_bits[3] = 1;  // BIGINT column SUM with GROUP BY: split halves
+  DenseSpec ds;
+  const bool dense = plan_dense(plan, {{fact.cols[3].mn, fact.cols[3].mx}}, m, ds);
+  plan.dense_priv = dense ? (ds.priv ? 1 : 2) : 0;
+  const JitProgram* jit = jitGet(ca.env, ca.ncols, plan, dense ? JF_DENSE : JF_GROUPED, "aggregate join result");
+  ExprEnv e = ca.env;
+  e.cols[j_flag].p = jcols[4].p;  // flag
+  e.cols[j_price].p = jcols[2].p;  // price
+  e.cols[j_qty].p = jcols[3].p;   // qty
+  int64_t cap = pow2_at_least(2 * 25, 1024);
+  std::vector<double> keys, accs;
+  int64_t g = 0;
+  for (;;) {
+    GpuAggregator agg(plan, cap, dense ? &ds : nullptr);
+    agg.addBatch(e, ca.ncols, m, nullptr);
+    agg.run(ca.env, jit);
+    if (agg.overflowed()) {
+      cap = pow2_at_least(std::max<int64_t>(cap * 4, agg.groupCount() * 4), 1);
+      continue;
+    }
+    g = agg.fetch(keys, accs);
+    break;
+  }
+  for (int64_t i = 0; i < g; ++i) {
+    int64_t kb, lo2, hi2;
+    std::memcpy(&kb, &keys[i], 8);
+    std::memcpy(&lo2, &accs[i * 4 + 2], 8);
+    std::memcpy(&hi2, &accs[i * 4 + 3], 8);
+    out[i * 4 + 0] = kb;
+    out[i * 4 + 1] = (int64_t)accs[i * 4 + 0];
+    out[i * 4 + 2] = (int64_t)accs[i * 4 + 1];  // exact: per-group raw sums < 2^53
+    out[i * 4 + 3] = (int64_t)((__int128)hi2 * ((__int128)1 << 32) + lo2);
+  }
+  if (st) st->dim_rows = nd, st->fact_rows = nf, st->pairs = m;
+  return g;
+}
+
+thread_local std::string t_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const SqlError& e) {
+    t_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    t_err = e.what();
+    return 1105;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wsq_last_error(void) { return t_err.c_str(); }
+
+// A GPU-resident table: ncols columns of types[] (WS_T_*), n rows from host arrays, cut into
+// 2^20-row segments; encode[c] != 0 stores column c frame-of-reference encoded per segment.
+int wsq_table_load(int ncols, const int* types, const int* encode, int64_t n, const void* const* host, void** out) {
+  return guard([&] {
+    auto t = std::make_unique<DevTable>();
+    t->rows = n;
+    for (int64_t r = 0; r < n; r += kSegRows) t->seg_rows.push_back(std::min<int64_t>(kSegRows, n - r));
+    t->cols.resize(ncols);
+    for (int c = 0; c < ncols; ++c) {
+      DevColumn& col = t->cols[c];
+      col.t = types[c];
+      const int w = width_of(col.t);
+      int64_t r0 = 0;
+      for (int64_t rows : t->seg_rows) {
+        const uint8_t* src = (const uint8_t*)host[c] + r0 * w;
+        for (int64_t i = 0; i < rows; ++i) {
+          if (col.t == WS_T_F64) break;
+          const int64_t v = w == 4 ? ((const int32_t*)src)[i] : ((const int64_t*)src)[i];
+          col.mn = std::min(col.mn, v);
+          col.mx = std::max(col.mx, v);
+        }
+        SegCol sc;
+        if (encode && encode[c]) {
+          std::vector<uint8_t> enc((size_t)(WS_ENC_HEADER + rows * w + 16));
+          sc.enc = (uint8_t)wsg_encode_int(src, w == 8, rows, enc.data());
+          if (!sc.enc) throw SqlError("column does not fit an encoding", 1105);
+          const size_t bytes = (size_t)(WS_ENC_HEADER + rows * (sc.enc == 1 ? 1 : sc.enc == 2 ? 2 : 4) + 16);
+          sc.dev = devAlloc(bytes);
+          h2d(sc.dev, enc.data(), bytes);
+        } else {
+          sc.dev = devAlloc((size_t)(rows * w));
+          h2d(sc.dev, src, (size_t)(rows * w));
+        }
+        col.segs.push_back(sc);
+        r0 += rows;
+      }
+    }
+    *out = t.release();
+  });
+}
+void wsq_table_free(void* t) { delete (DevTable*)t; }
+
+int wsq_c1(void* t, int dense_on, int64_t* out_i, double* out_d, int64_t cap_rows, int64_t* groups) {
+  return guard([&] { *groups = query_c1(*(DevTable*)t, out_i, out_d, cap_rows, dense_on != 0); });
+}
+// stats: pairs, groups, cap1, retries
+int wsq_c2(void* t, int64_t limit, int64_t* out, int64_t* nrows, int64_t* stats) {
+  return guard([&] {
+    C2Stats st;
+    *nrows = query_c2(*(DevTable*)t, out, limit, &st);
+    if (stats) stats[0] = st.pairs, stats[1] = st.groups, stats[2] = st.cap1, stats[3] = st.retries;
+  });
+}
+// stats: dim rows, fact rows after the join filter, join pairs
+int wsq_c3(void* fact, void* dim, int64_t* out, int64_t* groups, int64_t* stats) {
+  return guard([&] {
+    C3Stats st;
+    *groups = query_c3(*(DevTable*)fact, *(DevTable*)dim, out, &st);
+    if (stats) stats[0] = st.dim_rows, stats[1] = st.fact_rows, stats[2] = st.pairs;
+  });
+}
+
+}  // extern "C"

@@ -185,6 +185,7 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 static thread_local bool t_prof = false;
+static thread_local std::string t_prof_only;  // record only this kernel name ("" = all)
 static thread_local std::vector<ProfRec> t_prof_recs;
 static thread_local std::vector<hipEvent_t> t_prof_free;
 
@@ -201,6 +202,7 @@ static hipEvent_t prof_event() {
 }
 ProfScope::ProfScope(const char* kernel) {
   if (!t_prof) return;
+  if (!t_prof_only.empty() && t_prof_only != kernel) return;
   ProfRec r{kernel, prof_event(), prof_event()};
   WS_CK(hipEventRecord(r.a, stream()));
   slot = (int)t_prof_recs.size();
@@ -244,6 +246,12 @@ void fill_i32(int32_t* p, int64_t n, int32_t v) {
   ProfScope ps("fill");
   hipLaunchKernelGGL(fill_k<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream(), p, n, v);
   check_launch("fill_i32");
+}
+
+// Plain streaming copy (16-byte vectors), used to calibrate byte counters against a known size.
+__global__ void copy_k(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 }  // namespace ws
@@ -393,9 +401,21 @@ void* ws_update_column(const void* old_col, const void* src, int64_t n, int w, i
   return nullptr;
 }
 
+int ws_copy_kernel(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) throw Error("copy_kernel: 16-byte multiples only", WS_ERR_INTERNAL);
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (!n16) return;
+    ProfScope ps("copy_kernel");
+    hipLaunchKernelGGL(copy_k, dim3(grid_for(n16)), dim3(256), 0, stream(), (const uint4*)src, (uint4*)dst, n16);
+    check_launch("copy_kernel");
+  });
+}
+
 // ------------------------------------------------------------------ C ABI: profiling
 
 void ws_prof_enable(int on) { t_prof = on != 0; }
+void ws_prof_only(const char* kernel) { t_prof_only = kernel ? kernel : ""; }
 void ws_prof_reset(void) {
   for (auto& r : t_prof_recs) {
     (void)hipEventSynchronize(r.b);

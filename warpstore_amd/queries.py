@@ -1,0 +1,95 @@
+"""ctypes mirror of libws_queries.so: the engine-side drivers of the measured query shapes,
+which call the operator library only through the drop-in gdb:: surface."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import PKG, check, lib
+from .abi import T_F64, T_I32, T_I64, T_DECIMAL  # noqa: F401
+
+_q = None
+
+
+def qlib():
+    global _q
+    if _q is None:
+        lib()  # load the operator library first (the drivers link against it)
+        path = os.path.join(PKG, "libws_queries.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with warpstore_amd/build.py")
+        Q = C.CDLL(path)
+        P, I, I64 = C.c_void_p, C.c_int, C.c_int64
+        Q.wsq_last_error.restype = C.c_char_p
+        Q.wsq_table_load.argtypes = [I, P, P, I64, P, C.POINTER(C.c_void_p)]
+        Q.wsq_table_load.restype = I
+        Q.wsq_table_free.argtypes = [P]
+        Q.wsq_c1.argtypes = [P, I, P, P, I64, C.POINTER(I64)]
+        Q.wsq_c1.restype = I
+        Q.wsq_c2.argtypes = [P, I64, P, C.POINTER(I64), P]
+        Q.wsq_c2.restype = I
+        Q.wsq_c3.argtypes = [P, P, P, C.POINTER(I64), P]
+        Q.wsq_c3.restype = I
+        _q = Q
+    return _q
+
+
+def _ck(rc: int) -> None:
+    if rc != 0:
+        from . import WsError
+        raise WsError(qlib().wsq_last_error().decode(errors="replace"), rc)
+
+
+class DeviceTable:
+    """A GPU-resident table cut into 2^20-row segments (optionally encoded columns)."""
+
+    def __init__(self, arrays: list, types: list, encode: list | None = None) -> None:
+        arrays = [np.ascontiguousarray(a) for a in arrays]
+        n = len(arrays[0])
+        t = (C.c_int * len(types))(*types)
+        e = (C.c_int * len(types))(*(encode or [0] * len(types)))
+        ptrs = (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+        h = C.c_void_p()
+        _ck(qlib().wsq_table_load(len(types), t, e, n, ptrs, C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def free(self) -> None:
+        if self.h:
+            qlib().wsq_table_free(self.h)
+            self.h = None
+
+    def __del__(self) -> None:
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def c1(t: DeviceTable, dense: bool = True):
+    """Rows (k, count, sum(v), min(v), max(v)) int64 and avg(p) double, in group order."""
+    oi = np.zeros((1000, 5), np.int64)
+    od = np.zeros(1000)
+    g = C.c_int64()
+    _ck(qlib().wsq_c1(t.h, 1 if dense else 0, oi.ctypes.data, od.ctypes.data, 1000, C.byref(g)))
+    return oi[:g.value], od[:g.value]
+
+
+def c2(t: DeviceTable, limit: int = 10):
+    """Top rows (user_id, count, sum(dur), count distinct url) and stats (pairs, groups, cap1, retries)."""
+    out = np.zeros((limit, 4), np.int64)
+    st = np.zeros(4, np.int64)
+    n = C.c_int64()
+    _ck(qlib().wsq_c2(t.h, limit, out.ctypes.data, C.byref(n), st.ctypes.data))
+    return out[:n.value], st
+
+
+def c3(fact: DeviceTable, dim: DeviceTable):
+    """Rows (flag, count, raw price sum, qty sum) and stats (dim rows, filtered fact rows, pairs)."""
+    out = np.zeros((64, 4), np.int64)
+    st = np.zeros(3, np.int64)
+    g = C.c_int64()
+    _ck(qlib().wsq_c3(fact.h, dim.h, out.ctypes.data, C.byref(g), st.ctypes.data))
+    return out[:g.value], st

@@ -114,7 +114,7 @@ inline void d2h(void* dst, const void* src, size_t bytes) { wsd::ck(ws_d2h(dst, 
 inline void d2d(void* dst, const void* src, size_t bytes) { wsd::ck(ws_d2d(dst, src, bytes)); }
 inline void gpuSync() { wsd::ck(ws_sync()); }
 
-// RAII temp device array from the stream-ordered pool (gpu.h:59-79 semantics).
+// RAII temp device array from the library's caching allocator (gpu.h:59-79 semantics).
 template <class T>
 struct DevVec {
   T* p = nullptr;

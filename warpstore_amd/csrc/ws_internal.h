@@ -60,7 +60,7 @@ inline unsigned grid_for(int64_t n, int block = 256, int per_cu = 8) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
-// ---- memory: stream-ordered pool (tmp) and pinned staging for small transfers
+// ---- memory: caching allocator (tmp, reused in stream order per thread) and pinned staging
 void* tmp_alloc(size_t bytes);
 void tmp_free(void* p);
 template <class T>
@@ -87,7 +87,7 @@ struct DevBuf {  // RAII tmp buffer
     return q;
   }
 };
-// Thread-local pinned buffer; waits for the previous asynchronous copy out of it first.
+// Thread-local ring of pinned buffers; a slot is reused once the copy out of it has run.
 uint8_t* pinned_stage(size_t bytes);
 void pinned_stage_done();  // records that copies issued from the stage are enqueued
 void h2d_sync(void* dst, const void* src, size_t bytes);

@@ -4,6 +4,8 @@
 #include <atomic>
 #include <cstdio>
 #include <mutex>
+#include <unordered_map>
+#include <vector>
 
 #include "ws_internal.h"
 
@@ -27,7 +29,6 @@ void hip_fail(hipError_t e, const char* what, const char* file, int line) {
 static std::once_flag g_init_once;
 static int g_cus = 256;
 static std::string g_arch = "gfx950";
-static bool g_pool_ok = false;
 static bool g_init_failed = false;
 static std::string g_init_error;
 
@@ -51,13 +52,6 @@ static void init_device() {
   }
   g_cus = prop.multiProcessorCount;
   g_arch = prop.gcnArchName;
-  // Query temporaries come from the stream-ordered pool, which keeps freed memory for the next
-  // queries (release threshold: never); a failed allocation trims the pool and retries.
-  hipMemPool_t pool;
-  if (hipDeviceGetDefaultMemPool(&pool, 0) == hipSuccess) {
-    uint64_t thresh = UINT64_MAX;
-    g_pool_ok = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thresh) == hipSuccess;
-  }
   (void)hipGetLastError();
 }
 
@@ -79,60 +73,143 @@ const std::string& arch_name() {
 static std::atomic<int64_t> g_vram_used{0};
 static int64_t g_vram_limit = INT64_MAX;
 
-void trim_pool() {
-  hipMemPool_t pool;
-  if (hipDeviceGetDefaultMemPool(&pool, 0) != hipSuccess) {
-    (void)hipGetLastError();
-    return;
+// Query temporaries come from a caching allocator. A freed block goes into the freeing thread's
+// cache, keyed by size class, and is handed out again only to that thread: all of a thread's work
+// is issued in order on its per-thread stream, so the block's earlier uses precede its next use
+// without any synchronization or driver call. hipMalloc runs only on a cache miss (warm-up), and
+// a miss that finds no free device memory releases the cached blocks and retries once. Blocks of
+// threads that exited are reused by others after a device synchronization.
+namespace {
+size_t size_class(size_t b) {
+  if (b <= 512) return 512;
+  if (b <= ((size_t)1 << 20)) {
+    size_t c = 1024;
+    while (c < b) c <<= 1;
+    return c;
   }
+  const int lg = 63 - __builtin_clzll((unsigned long long)b);
+  const size_t step = (size_t)1 << (lg - 3);  // 8 classes per octave: at most 12.5 % slack
+  return (b + step - 1) / step * step;
+}
+std::mutex g_mem_mu;
+std::unordered_map<void*, size_t> g_live;                   // every block handed out -> its class size
+std::unordered_map<size_t, std::vector<void*>> g_orphans;  // cached blocks of exited threads
+std::atomic<int64_t> g_orphan_count{0};
+struct BlockCache {
+  std::unordered_map<size_t, std::vector<void*>> free;
+  ~BlockCache() {  // thread exit: host bookkeeping only, no runtime calls
+    std::lock_guard<std::mutex> g(g_mem_mu);
+    for (auto& kv : free)
+      for (void* p : kv.second) {
+        g_orphans[kv.first].push_back(p);
+        ++g_orphan_count;
+      }
+  }
+};
+thread_local BlockCache t_cache;
+
+void release_cached() {  // frees this thread's cached blocks and the orphans
   (void)hipDeviceSynchronize();
-  (void)hipMemPoolTrimTo(pool, 0);
+  std::vector<void*> v;
+  {
+    std::lock_guard<std::mutex> g(g_mem_mu);
+    for (auto& kv : t_cache.free)
+      for (void* p : kv.second) v.push_back(p);
+    t_cache.free.clear();
+    for (auto& kv : g_orphans)
+      for (void* p : kv.second) v.push_back(p);
+    g_orphans.clear();
+    g_orphan_count = 0;
+    for (void* p : v) g_live.erase(p);
+  }
+  for (void* p : v) (void)hipFree(p);
   (void)hipGetLastError();
 }
+}  // namespace
+
+void trim_pool() { release_cached(); }
 
 void* tmp_alloc(size_t bytes) {
   ensure_init();
   if (!bytes) return nullptr;
-  void* p = nullptr;
-  if (g_pool_ok) {
-    if (hipMallocAsync(&p, bytes, stream()) != hipSuccess) {
-      (void)hipGetLastError();
-      trim_pool();
-      WS_CK(hipMallocAsync(&p, bytes, stream()));
-    }
-  } else {
-    WS_CK(hipMalloc(&p, bytes));
+  const size_t c = size_class(bytes);
+  auto it = t_cache.free.find(c);
+  if (it != t_cache.free.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    return p;
   }
+  if (g_orphan_count.load(std::memory_order_relaxed) > 0) {
+    void* p = nullptr;
+    {
+      std::lock_guard<std::mutex> g(g_mem_mu);
+      auto o = g_orphans.find(c);
+      if (o != g_orphans.end() && !o->second.empty()) {
+        p = o->second.back();
+        o->second.pop_back();
+        --g_orphan_count;
+      }
+    }
+    if (p) {
+      WS_CK(hipDeviceSynchronize());  // the exited thread's last work on the block has ended
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, c) != hipSuccess) {
+    (void)hipGetLastError();
+    release_cached();
+    WS_CK(hipMalloc(&p, c));
+  }
+  std::lock_guard<std::mutex> g(g_mem_mu);
+  g_live[p] = c;
   return p;
 }
 void tmp_free(void* p) {
   if (!p) return;
-  if (g_pool_ok) (void)hipFreeAsync(p, stream());
-  else {
-    (void)hipStreamSynchronize(stream());
-    (void)hipFree(p);
+  size_t c = 0;
+  {
+    std::lock_guard<std::mutex> g(g_mem_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) return;  // not a block of this allocator
+    c = it->second;
   }
+  t_cache.free[c].push_back(p);
 }
 
-struct Stage {
+// Pinned staging for small host-to-device uploads (launch descriptors, plans): a per-thread ring
+// of buffers, each with an event marking when the copy out of it has run. A slot is reused after
+// its event completes, which is normally long past since the ring holds kStageSlots uploads.
+constexpr int kStageSlots = 16;
+struct StageSlot {
   uint8_t* buf = nullptr;
   size_t cap = 0;
   hipEvent_t ev = nullptr;
   bool pending = false;
+};
+struct Stage {
+  StageSlot slot[kStageSlots];
+  int next = 0;
+  int cur = -1;
   ~Stage() {
-    // Thread exit: the buffer may still feed an in-flight copy; wait for it before freeing.
-    if (ev) {
-      if (pending) (void)hipEventSynchronize(ev);
-      (void)hipEventDestroy(ev);
+    // Thread exit: a buffer may still feed an in-flight copy; wait for it before freeing.
+    for (StageSlot& s : slot) {
+      if (s.ev) {
+        if (s.pending) (void)hipEventSynchronize(s.ev);
+        (void)hipEventDestroy(s.ev);
+      }
+      if (s.buf) (void)hipHostFree(s.buf);
     }
-    if (buf) (void)hipHostFree(buf);
   }
 };
 static thread_local Stage t_stage;
 
 uint8_t* pinned_stage(size_t bytes) {
   ensure_init();
-  Stage& s = t_stage;
+  Stage& st = t_stage;
+  StageSlot& s = st.slot[st.next];
+  st.cur = st.next;
+  st.next = (st.next + 1) % kStageSlots;
   if (!s.ev) WS_CK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
   if (s.pending) {
     WS_CK(hipEventSynchronize(s.ev));
@@ -147,8 +224,9 @@ uint8_t* pinned_stage(size_t bytes) {
   return s.buf;
 }
 void pinned_stage_done() {
-  WS_CK(hipEventRecord(t_stage.ev, stream()));
-  t_stage.pending = true;
+  StageSlot& s = t_stage.slot[t_stage.cur];
+  WS_CK(hipEventRecord(s.ev, stream()));
+  s.pending = true;
 }
 void h2d_sync(void* dst, const void* src, size_t bytes) {
   if (!bytes) return;

@@ -97,6 +97,17 @@ int ws_agg_sum_high(const ws_agg* a, int64_t* out);       /* sumHigh gpu.h:105 (
 int ws_agg_compact(ws_agg* a, double** keys, double** accs, int64_t* groups);
 int64_t ws_agg_capacity(const ws_agg* a);                  /* capacity   gpu.h:108 */
 int ws_agg_group_count(ws_agg* a, int64_t* out);           /* groupCount gpu.h:109 */
+/* Aggregation strategy of a hash GROUP BY (no reference counterpart: the hook through which a
+ * sizing model, or a test, chooses the physical plan; results do not depend on it).
+ *   WS_STRAT_AUTO: partitioned when the first run covers >= 2^20 rows and >= 2^16 groups are
+ *     expected; WS_STRAT_TABLE: the shared hash table (reference kernels.cuh:306-415);
+ *   WS_STRAT_PARTITIONED: count / scatter / merge through hash partitions, no shared table unless a
+ *     partition outgrows its LDS table.
+ * expected_groups > 0 replaces the capacity-derived estimate (capacity / 2) for sizing the
+ * partitions and the spill table. Call before the first run. */
+enum { WS_STRAT_AUTO = 0, WS_STRAT_TABLE = 1, WS_STRAT_PARTITIONED = 2 };
+int ws_agg_set_strategy(ws_agg* a, int strategy, int64_t expected_groups);
+int ws_agg_partitioned(const ws_agg* a); /* 1 when the last run produced a partitioned result */
 
 /* ------------------------------------------------------------------ selection / gathers */
 int ws_select(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, const uint32_t* del, int32_t* out_idx,

@@ -193,13 +193,22 @@ def env_with_batch(env: abi.ExprEnv, bound: Bound, b: int) -> abi.ExprEnv:
     return e
 
 
-def gpu_aggregate(plan, env, bound: Bound, cap: int = 1 << 16, dense=None, jit=None, fetch=True, windows=False):
+LAST_RUN = {"partitioned": False, "retries": 0}
+
+
+def gpu_aggregate(plan, env, bound: Bound, cap: int = 1 << 16, dense=None, jit=None, fetch=True, windows=False,
+                  strategy=None, expected=0):
     """Runs GpuAggregator through the C ABI, retrying bigger on overflow like the engine
-    (engine_select.cpp:1514-1522). Returns (keys g x nk, accs g x na, hi, cap used)."""
+    (engine_select.cpp:1514-1522). Returns (keys g x nk, accs g x na, hi, cap used).
+    strategy: None (library default) or ws_agg_set_strategy's value, with `expected` groups.
+    LAST_RUN records whether the final run was partitioned and how many retries it took."""
     L = ws.lib()
+    LAST_RUN["retries"] = 0
     while True:
         h = C.c_void_p()
         ws.check(L.ws_agg_create(C.byref(plan), cap, C.byref(dense) if dense is not None else None, C.byref(h)))
+        if strategy is not None:
+            ws.check(L.ws_agg_set_strategy(h, strategy, expected))
         try:
             for b in range(len(bound.t.sizes)):
                 e = env_with_batch(env, bound, b)
@@ -209,6 +218,7 @@ def gpu_aggregate(plan, env, bound: Bound, cap: int = 1 << 16, dense=None, jit=N
             ws.check(L.ws_agg_run(h, C.byref(env), jit))
             ov = C.c_int()
             ws.check(L.ws_agg_overflowed(h, C.byref(ov)))
+            LAST_RUN["partitioned"] = bool(L.ws_agg_partitioned(h))
             if ov.value:
                 gc = C.c_int64()
                 ws.check(L.ws_agg_group_count(h, C.byref(gc)))
@@ -216,6 +226,9 @@ def gpu_aggregate(plan, env, bound: Bound, cap: int = 1 << 16, dense=None, jit=N
                 cap = 1
                 while cap < want:
                     cap <<= 1
+                LAST_RUN["retries"] += 1
+                if not fetch:
+                    L.ws_agg_destroy(h)
                 continue
             if not fetch:
                 return h, cap

@@ -55,6 +55,7 @@ _SIGS = {
     "ws_agg_sum_bits_slots": (I, [P, P]), "ws_agg_sum_high": (I, [P, P]),
     "ws_agg_compact": (I, [P, PP, PP, C.POINTER(I64)]), "ws_agg_capacity": (I64, [P]),
     "ws_agg_group_count": (I, [P, C.POINTER(I64)]),
+    "ws_agg_set_strategy": (I, [P, I, I64]), "ws_agg_partitioned": (I, [P]),
     "ws_select": (I, [C.POINTER(ExprEnv), I, Prog, I64, P, P, P, C.POINTER(I64)]),
     "ws_select_multi": (I, [C.POINTER(ExprEnv), I, Prog, I, P, P, P, I, PP, P, P, C.POINTER(I64)]),
     "ws_gather_multi": (I, [P, P, I, I, I, P, I64, P]), "ws_remap_codes": (I, [P, I64, P, I64, P]),

@@ -41,24 +41,281 @@ __global__ void init_dense_k(double* acc, int64_t G, int na, ws_agg_plan plan) {
 __global__ void dense_reduce_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts, int nparts, double* gacc) {
   dense_reduce_body(plan, ds, parts, nparts, gacc);
 }
-// Compacted groups, row-major (fetch) or column-major (compact), keys from the table.
+// Compacted groups, row-major (fetch) or column-major (compact): rows [off, off + g) of a result
+// of G groups, taken from slots idx[i] of a row-major table (idx null: slot i).
 __global__ void gather_groups_k(const double* kv, int nk, const double* acc, int na, const int32_t* idx, int64_t g,
-                                double* keys_out, double* accs_out, int colmajor) {
+                                double* keys_out, double* accs_out, int colmajor, int64_t off, int64_t G) {
   const int64_t total = g * (nk + na);
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = j / (nk + na);
     const int c = (int)(j - i * (nk + na));
-    const int64_t s = idx[i];
+    const int64_t s = idx ? (int64_t)idx[i] : i;
+    const int64_t o = off + i;
     if (c < nk) {
       const double v = kv[s * nk + c];
-      if (colmajor) keys_out[(int64_t)c * g + i] = v;
-      else keys_out[i * nk + c] = v;
+      if (colmajor) keys_out[(int64_t)c * G + o] = v;
+      else keys_out[o * nk + c] = v;
     } else {
       const int a = c - nk;
       const double v = acc[s * na + a];
-      if (colmajor) accs_out[(int64_t)a * g + i] = v;
-      else accs_out[i * na + a] = v;
+      if (colmajor) accs_out[(int64_t)a * G + o] = v;
+      else accs_out[o * na + a] = v;
     }
+  }
+}
+
+// ------------------------------------------------------------------ partitioned GROUP BY (host-side passes)
+
+// part_off[c][p] = sum of part_cnt[c'][p] over c' < c, ptot[p] = sum over all chunks. A block covers
+// 64 partitions (one lane each) with 16 chunk ranges (one wave each); reads are coalesced over p.
+__global__ void __launch_bounds__(1024) part_offsets_k(const uint32_t* cnt, uint32_t* off, uint32_t* ptot, int64_t nchunks,
+                                                       int P) {
+  __shared__ uint32_t s_part[16][64];
+  const int pl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + pl;
+  const int64_t per = (nchunks + 15) / 16;
+  const int64_t c0 = g * per, c1 = c0 + per < nchunks ? c0 + per : nchunks;
+  uint32_t sum = 0;
+  if (p < P)
+    for (int64_t c = c0; c < c1; ++c) sum += cnt[c * P + p];
+  s_part[g][pl] = sum;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int j = 0; j < g; ++j) base += s_part[j][pl];
+  if (p < P) {
+    for (int64_t c = c0; c < c1; ++c) {
+      off[c * P + p] = base;
+      base += cnt[c * P + p];
+    }
+    if (g == 15) ptot[p] = base;
+  }
+}
+
+// pstart[p] = exclusive prefix of ptot over partitions, pstart[P] = total (one block, P <= 4096).
+__global__ void __launch_bounds__(1024) part_start_k(const uint32_t* ptot, uint32_t* pstart, int P) {
+  __shared__ uint32_t s_w[16];
+  const int per = (P + 1023) / 1024;
+  const int i0 = threadIdx.x * per;
+  uint32_t n = 0;
+  for (int i = i0; i < i0 + per && i < P; ++i) n += ptot[i];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t incl = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_w[wv] = incl;
+  __syncthreads();
+  uint32_t base = 0, total = 0;
+  for (int j = 0; j < 16; ++j) {
+    if (j < wv) base += s_w[j];
+    total += s_w[j];
+  }
+  uint32_t run = base + incl - n;
+  for (int i = i0; i < i0 + per && i < P; ++i) {
+    pstart[i] = run;
+    run += ptot[i];
+  }
+  if (threadIdx.x == 0) pstart[P] = total;
+}
+
+__device__ __forceinline__ double wave_combine_slot(bool bits, int op, double v) {
+  return bits ? bitsd(wave_combine_i(op, dbits(v))) : wave_combine_d(op, v);
+}
+
+struct PartMerge {
+  const u64* recs;          // [records][W]
+  const uint32_t* pstart;   // [P + 1]
+  int W, NK, NA, S;         // record words, keys, slots, LDS table slots
+  double* okv;              // [groups][NK] finished groups
+  double* oacc;             // [groups][NA]
+  u64* ocount;
+};
+
+// Inserts one record (held in registers) into the spill table; false when the table overflowed.
+template <int W>
+__device__ __forceinline__ bool spill_insert(const GTable& t, const ws_agg_plan& plan, int NK, const u64 (&rec)[W],
+                                             unsigned* new_groups) {
+  const int NA = W - 1 - NK;
+  const u64 h = rec[0];
+  u64 s = h & t.mask;
+  for (int probe = 0; probe < 256; ++probe) {
+    u64 prev = kEmpty;
+    const bool claimed = __hip_atomic_compare_exchange_strong(&t.keys[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+    if (claimed) {
+#pragma unroll
+      for (int j = 1; j < W; ++j)
+        if (j <= NK) t.kv[s * NK + (j - 1)] = bitsd((i64)rec[j]);
+      __hip_atomic_fetch_add(new_groups, 1u, __ATOMIC_RELAXED, WS_LDS);
+    }
+    if (claimed || prev == h) {
+#pragma unroll
+      for (int j = 1; j < W; ++j)
+        if (j > NK) {
+          const int a = j - 1 - NK;
+          atomic_combine_slot<WS_DEV>(plan.agg_bits[a] != 0, plan.ops[a], &t.acc[s * NA + a], bitsd((i64)rec[j]));
+        }
+      return true;
+    }
+    s = (s + 1) & t.mask;
+  }
+  t.flags[0] = 1;
+  return false;
+}
+
+// Merge pass, for records of W words: block p combines partition p's records in an LDS table of
+// m.S slots (linear probing from a multiply-shift index of the hash's low word; the partition is
+// its high bits). Each thread loads R records into registers before combining them, so a block
+// keeps many loads in flight. Lanes of a wave that hold the same group are folded into one lane
+// first (up to 3 groups per step): hot groups cost one LDS update per wave instead of one per
+// lane. A record that finds no LDS slot in 64 probes goes to the spill table (device-scope
+// atomics, as the plain path). At the end the block appends its groups, row-major, after one
+// atomic reservation.
+template <int W>
+__global__ void __launch_bounds__(512) part_merge_k(PartMerge m, ws_agg_plan plan, GTable spill) {
+  constexpr int R = W <= 3 ? 8 : W <= 6 ? 4 : W <= 10 ? 2 : 1;  // records per thread per step
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ unsigned s_new;
+  __shared__ uint32_t s_w[8];
+  __shared__ u64 s_base;
+  const int NK = m.NK, NA = W - 1 - NK, S = m.S;
+  u64* lk = (u64*)ws_smem;
+  double* lkv = (double*)(lk + S);
+  double* lacc = lkv + (int64_t)S * NK;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    lk[i] = kEmpty;
+    for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
+  }
+  if (threadIdx.x == 0) s_new = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t beg = m.pstart[blockIdx.x], end = m.pstart[blockIdx.x + 1];
+  for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * R) {
+    u64 rec[R][W];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = i0 + (int64_t)r * blockDim.x + threadIdx.x;
+#pragma unroll
+      for (int j = 0; j < W; ++j) rec[r][j] = i < end ? m.recs[i * W + j] : kEmpty;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const u64 h = rec[r][0];
+      bool live = h != kEmpty;
+      u64 todo = __ballot(live);
+      for (int round = 0; round < 3 && todo; ++round) {
+        const int L = __ffsll((unsigned long long)todo) - 1;
+        const u64 hl = (u64)__shfl((long long)h, L, 64);
+        const u64 same = __ballot(live && h == hl);
+        todo &= ~same;
+        if (__popcll(same) < 2) continue;
+        const bool in = (same >> lane) & 1;
+#pragma unroll
+        for (int j = 1; j < W; ++j) {
+          if (j <= NK) continue;
+          const int a = j - 1 - NK;
+          const bool bits = plan.agg_bits[a] != 0;
+          const int op = plan.ops[a];
+          const double v = wave_combine_slot(bits, op, in ? bitsd((i64)rec[r][j]) : slot_identity(bits, op));
+          if (lane == L) rec[r][j] = (u64)dbits(v);
+        }
+        if (in && lane != L) live = false;
+      }
+      if (!live) continue;
+      unsigned s = (unsigned)(((h & 0xffffffffull) * (u64)S) >> 32);
+      int slot = -1;
+      for (int probe = 0; probe < 64; ++probe) {
+        u64 prev = kEmpty;
+        const bool claimed = __hip_atomic_compare_exchange_strong(&lk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (claimed) {
+#pragma unroll
+          for (int j = 1; j < W; ++j)
+            if (j <= NK) lkv[(int64_t)s * NK + (j - 1)] = bitsd((i64)rec[r][j]);
+        }
+        if (claimed || prev == h) {
+          slot = (int)s;
+          break;
+        }
+        s = s + 1 == (unsigned)S ? 0 : s + 1;
+      }
+      if (slot >= 0) {
+#pragma unroll
+        for (int j = 1; j < W; ++j)
+          if (j > NK) {
+            const int a = j - 1 - NK;
+            atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, plan.ops[a], &lacc[(int64_t)slot * NA + a],
+                                        bitsd((i64)rec[r][j]));
+          }
+      } else {
+        spill_insert<W>(spill, plan, NK, rec[r], &s_new);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_new) {
+    const u64 total = atomicAdd((unsigned long long*)spill.count, (unsigned long long)s_new) + s_new;
+    if (total * 10 > (spill.mask + 1) * 7) spill.flags[0] = 1;  // spill load above 0.7: retry bigger
+  }
+  const int K = (S + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int s0 = threadIdx.x * K, s1 = s0 + K < S ? s0 + K : S;
+  uint32_t n = 0;
+  for (int q = s0; q < s1; ++q) n += lk[q] != kEmpty;
+  uint32_t incl = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 63) s_w[wv] = incl;
+  __syncthreads();
+  uint32_t wbase = 0, total = 0;
+  for (int j = 0; j < nw; ++j) {
+    if (j < wv) wbase += s_w[j];
+    total += s_w[j];
+  }
+  if (threadIdx.x == 0) s_base = total ? atomicAdd((unsigned long long*)m.ocount, (unsigned long long)total) : 0;
+  __syncthreads();
+  u64 o = s_base + wbase + incl - n;
+  for (int q = s0; q < s1; ++q) {
+    if (lk[q] == kEmpty) continue;
+    for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = lkv[(int64_t)q * NK + k];
+    for (int a = 0; a < NA; ++a) m.oacc[o * NA + a] = lacc[(int64_t)q * NA + a];
+    ++o;
+  }
+}
+
+// The merge kernel for records of W words (3..16).
+typedef void (*MergeKernel)(PartMerge, ws_agg_plan, GTable);
+static MergeKernel merge_kernel(int W) {
+  static const MergeKernel k[17] = {nullptr,           nullptr,           nullptr,           part_merge_k<3>,
+                                    part_merge_k<4>,   part_merge_k<5>,   part_merge_k<6>,   part_merge_k<7>,
+                                    part_merge_k<8>,   part_merge_k<9>,   part_merge_k<10>,  part_merge_k<11>,
+                                    part_merge_k<12>,  part_merge_k<13>,  part_merge_k<14>,  part_merge_k<15>,
+                                    part_merge_k<16>};
+  return W >= 3 && W <= 16 ? k[W] : nullptr;
+}
+
+// Moves finished groups (row-major kv / acc, or the occupied slots of a table when keys is set)
+// into a global table: the partitioned result becomes a plain table when more batches follow.
+__global__ void table_absorb_k(GTable dst, ws_agg_plan plan, const double* kv, const double* acc, const u64* keys,
+                               int64_t n) {
+  __shared__ unsigned s_new;
+  if (threadIdx.x == 0) s_new = 0;
+  __syncthreads();
+  const int NK = plan.nkeys, NA = plan.naggs;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (keys && keys[i] == kEmpty) continue;
+    u64 h = 0x9e3779b97f4a7c15ull;
+    for (int k = 0; k < NK; ++k) h = mix64(h ^ (u64)dbits(kv[i * NK + k])) + (u64)k;
+    if (h == kEmpty) h ^= 1;
+    global_insert(dst, plan, NK, NA, keys ? keys[i] : h, &kv[i * NK], &acc[i * NA], &s_new);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_new) {
+    const u64 total = atomicAdd((unsigned long long*)dst.count, (unsigned long long)s_new) + s_new;
+    if (total * 10 > (dst.mask + 1) * 7) dst.flags[0] = 1;
   }
 }
 // Dense groups: keys rebuilt from the group index (reference denseFinalize, gpu.cu:379-391).
@@ -229,27 +486,69 @@ struct ws_agg {
   ws_agg_plan plan{};
   ws_dense_spec dense{};
   bool is_dense = false;
-  int64_t cap = 1;
-  u64* keys = nullptr;
+  int64_t cap = 1;             // capacity the caller sized (slots of the plain table)
+  int64_t tcap = 0;            // slots of the allocated hash table (cap, or the spill table)
+  u64* keys = nullptr;         // hash table, allocated at the first run that needs it
   double* kv = nullptr;
-  double* acc = nullptr;
-  int* flags = nullptr;
-  u64* count = nullptr;
-  double* nk_out = nullptr;  // no GROUP BY: [2 * WS_MAX_AGGS] running result
+  double* acc = nullptr;       // hash table accumulators, or the dense groups
+  int* flags = nullptr;        // [0] overflow, [1] dense key out of range, [2] partition run overflow
+  u64* count = nullptr;        // groups inserted into the hash table
+  double* nk_out = nullptr;    // no GROUP BY: [2 * WS_MAX_AGGS] running result
   std::vector<int64_t> hi;
   int ncols = 0;
   std::vector<int64_t> bn;
   std::vector<const uint32_t*> bdel;
   std::vector<ws_colview> bcols;
   int64_t rows_done = 0;
-  int batches_done = 0, windows = 0;
+  int batches_done = 0, windows = 0, runs = 0;
+  // strategy: WS_STRAT_AUTO / WS_STRAT_TABLE / WS_STRAT_PARTITIONED, and the expected group count
+  int strategy = WS_STRAT_AUTO;
+  int64_t expected_groups = 0;
+  ws_expr_env last_env{};                  // the last run's environment and plan (for a rerun)
+  const ws_jit_program* last_jit = nullptr;
+  // partitioned result: finished groups row-major in okv / oacc (plus any spilled to the table)
+  bool part = false, settled = true;
+  int64_t part_groups = 0;
+  double* okv = nullptr;
+  double* oacc = nullptr;
+  u64* ocount = nullptr;
   // fetch state
   int64_t fetched = -1;
   DevBuf<double> fetch_buf;
+  void free_part() {
+    for (void* p : {(void*)okv, (void*)oacc, (void*)ocount}) tmp_free(p);
+    okv = oacc = nullptr;
+    ocount = nullptr;
+    part = false;
+    settled = true;
+    part_groups = 0;
+  }
+  void free_table() {
+    for (void* p : {(void*)keys, (void*)kv}) tmp_free(p);
+    if (!is_dense) tmp_free(acc), acc = nullptr;
+    keys = nullptr, kv = nullptr, tcap = 0;
+  }
   ~ws_agg() {
-    for (void* p : {(void*)keys, (void*)kv, (void*)acc, (void*)flags, (void*)count, (void*)nk_out}) tmp_free(p);
+    free_part();
+    free_table();
+    for (void* p : {(void*)acc, (void*)flags, (void*)count, (void*)nk_out}) tmp_free(p);
   }
 };
+
+// Allocates and clears the hash table (slots a power of two) and its group counter.
+static void ensure_table(ws_agg* A, int64_t slots) {
+  if (A->keys) return;
+  const int nk = A->plan.nkeys, na = std::max(A->plan.naggs, 1);
+  A->tcap = slots;
+  A->keys = (u64*)tmp_alloc(sizeof(u64) * (size_t)slots);
+  A->kv = (double*)tmp_alloc(sizeof(double) * (size_t)slots * nk);
+  A->acc = (double*)tmp_alloc(sizeof(double) * (size_t)slots * na);
+  fill_u64(A->count, 1, 0);
+  GTable t{A->keys, A->kv, A->acc, (u64)(slots - 1), A->flags, A->count};
+  hipLaunchKernelGGL(init_table_k, dim3(grid_for(slots)), dim3(256), 0, stream(), t, nk, A->plan.naggs, A->plan);
+  check_launch("init_table");
+}
+static GTable table_of(const ws_agg* A) { return GTable{A->keys, A->kv, A->acc, (u64)(A->tcap - 1), A->flags, A->count}; }
 
 // The lean interpreter serves plans whose programs use no scalar-function opcodes.
 static bool plan_is_simple(const ws_expr_env& env, const ws_agg_plan& plan) {
@@ -263,8 +562,182 @@ static bool plan_is_simple(const ws_expr_env& env, const ws_agg_plan& plan) {
 
 static double host_identity(uint8_t op) { return op == WS_AGG_MIN ? INFINITY : op == WS_AGG_MAX ? -INFINITY : 0.0; }
 
-static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit) {
+// Launches the grouped kernel (compiled, or the interpreter) of a plan with one descriptor.
+// pass: 0 the table kernel, 1 partition count, 2 partition scatter; jf: the compiled kernel of
+// that pass, or null for the interpreter.
+static void launch_grouped(void* jf, int pass, bool lean, ScanDesc* ddesc, GTable t, unsigned grid, size_t lds,
+                           const char* name, const char* jit_name, int block = kBlock) {
+  if (jf) {
+    void* args[] = {&ddesc, &t};
+    jit_launch_raw(jf, grid, (unsigned)block, lds, args, jit_name);
+    return;
+  }
+  ProfScope ps(name);
+  if (pass == 0)
+    hipLaunchKernelGGL(lean ? ws_l_grouped : ws_i_grouped, dim3(grid), dim3(block), lds, stream(), ddesc, t);
+  else if (pass == 1)
+    hipLaunchKernelGGL(lean ? ws_l_grouped_pcount : ws_i_grouped_pcount, dim3(grid), dim3(block), lds, stream(), ddesc, t);
+  else
+    hipLaunchKernelGGL(lean ? ws_l_grouped_pscatter : ws_i_grouped_pscatter, dim3(grid), dim3(block), lds, stream(), ddesc, t);
+  check_launch(name);
+}
+
+// ------------------------------------------------------------------ partitioned GROUP BY (host)
+
+constexpr int kPartMergeBlock = 512;
+constexpr int kPartScanBlock = 1024;          // count / scatter blocks (one chunk each)
+constexpr size_t kPartMergeLds = 60 * 1024;    // merge table per block: 2 blocks per CU
+constexpr size_t kPartScatterLds = 48 * 1024;  // cursors + pre-aggregation table per scatter block
+constexpr int kPartMaxBits = 12;
+constexpr int kChunksPerCu = 3;
+constexpr int64_t kPartAutoRows = 1 << 20;     // automatic choice: at least this many rows and
+constexpr int64_t kPartAutoGroups = 1 << 16;   // this many expected groups
+
+struct PartPlan {
+  int pbits = 0;  // 0: plain table path
+  int W = 0, S = 0, pre = 0, chunk_tiles = 0;
+  int64_t nchunks = 0, rows = 0;
+};
+
+// Partitioned aggregation pays when groups are many (a shared table then takes one device-scope
+// atomic per row and slot); the partition count keeps each partition's expected groups at about
+// half of the merge block's LDS table. Only a first run over all batches is partitioned.
+static PartPlan plan_partitions(const ws_agg* A, int64_t rows, int64_t ntiles) {
+  PartPlan pp;
   const ws_agg_plan& plan = A->plan;
+  if (A->strategy == WS_STRAT_TABLE || A->is_dense || plan.nkeys == 0 || A->runs > 0 || A->keys || A->part) return pp;
+  if (rows <= 0 || rows >= ((int64_t)1 << 31)) return pp;
+  const int W = 1 + plan.nkeys + plan.naggs;
+  if (W > 16) return pp;
+  const int64_t expect = A->expected_groups > 0 ? A->expected_groups : std::max<int64_t>(A->cap / 2, 1);
+  if (A->strategy == WS_STRAT_AUTO && (rows < kPartAutoRows || expect < kPartAutoGroups)) return pp;
+  pp.W = W;
+  pp.S = (int)(kPartMergeLds / (8 * (size_t)W));
+  int bits = 1;
+  while (bits < kPartMaxBits && ((int64_t)1 << bits) * (pp.S / 2) < expect) ++bits;
+  pp.pbits = bits;
+  const int P = 1 << bits;
+  int pre = 1024;
+  while (pre > 64 && 8 * (size_t)P + (size_t)pre * 8 * W > kPartScatterLds) pre >>= 1;
+  pp.pre = pre;
+  const int64_t want = std::max<int64_t>(1, (int64_t)cu_count() * kChunksPerCu);
+  pp.chunk_tiles = (int)std::max<int64_t>(1, (ntiles + want - 1) / want);
+  pp.nchunks = (ntiles + pp.chunk_tiles - 1) / pp.chunk_tiles;
+  pp.rows = rows;
+  return pp;
+}
+
+static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit, const ScanInput& in, ScanDesc d,
+                         const PartPlan& pp) {
+  const ws_agg_plan& plan = A->plan;
+  // Both scan passes must evaluate rows identically: compiled both, or interpreted both.
+  void* jf_count = jit ? jit_function(jit, kJitPartCount) : nullptr;
+  void* jf_scatter = jit ? jit_function(jit, kJitPartScatter) : nullptr;
+  if (!jf_count || !jf_scatter) jf_count = jf_scatter = nullptr;
+  const bool lean = !jf_count && plan_is_simple(env, plan);
+  const int NK = plan.nkeys, NA = plan.naggs, P = 1 << pp.pbits;
+  DevBuf<uint32_t> cnt(pp.nchunks * P), off(pp.nchunks * P), ptot(P), pstart(P + 1);
+  DevBuf<u64> recs(pp.rows * pp.W);
+  A->okv = (double*)tmp_alloc(sizeof(double) * (size_t)pp.rows * NK);
+  A->oacc = (double*)tmp_alloc(sizeof(double) * (size_t)pp.rows * std::max(NA, 1));
+  A->ocount = (u64*)tmp_alloc(sizeof(u64));
+  A->part = true;
+  A->settled = false;
+  fill_u64(A->ocount, 1, 0);
+  // spill table for partitions whose groups outgrow the merge block's LDS table
+  // (grows with the caller's capacity, so overflow retries converge even under a bad estimate)
+  int64_t spill = 4096;
+  const int64_t expect = A->expected_groups > 0 ? A->expected_groups : std::max<int64_t>(A->cap / 2, 1);
+  while (spill < A->cap && (spill < expect / 4 || spill < A->cap / 8)) spill <<= 1;
+  ensure_table(A, spill);
+  d.pbits = pp.pbits;
+  d.chunk_tiles = pp.chunk_tiles;
+  d.pre_slots = pp.pre;
+  d.rec_words = pp.W;
+  d.part_cnt = cnt.p;
+  d.part_off = off.p;
+  d.part_start = pstart.p;
+  d.recs = recs.p;
+  d.part_err = A->flags + 2;
+  GTable t = table_of(A);
+  {
+    DescUpload up;
+    build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
+    launch_grouped(jf_count, 1, lean, up.ddesc, t, (unsigned)pp.nchunks, (size_t)P * 4, "agg_part_count",
+                   "agg_part_count_jit", kPartScanBlock);
+  }
+  {
+    ProfScope ps("agg_part_offsets");
+    hipLaunchKernelGGL(part_offsets_k, dim3((unsigned)((P + 63) / 64)), dim3(1024), 0, stream(), (const uint32_t*)cnt.p, off.p,
+                       ptot.p, pp.nchunks, P);
+    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)ptot.p, pstart.p, P);
+    check_launch("agg_part_offsets");
+  }
+  {
+    DescUpload up;
+    build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
+    const size_t lds = (size_t)P * 8 + (size_t)pp.pre * 8 * pp.W;
+    launch_grouped(jf_scatter, 2, lean, up.ddesc, t, (unsigned)pp.nchunks, lds, "agg_part_scatter", "agg_part_scatter_jit",
+                   kPartScanBlock);
+  }
+  {
+    PartMerge m{recs.p, pstart.p, pp.W, NK, NA, pp.S, A->okv, A->oacc, A->ocount};
+    ProfScope ps("agg_part_merge");
+    hipLaunchKernelGGL(merge_kernel(pp.W), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m,
+                       plan, t);
+    check_launch("agg_part_merge");
+  }
+}
+
+static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit, bool allow_part);
+
+// Waits for a partitioned run and reads its group count. A chunk whose second pass found more rows
+// than its first (which the passes' identical evaluation rules out) is answered by redoing the
+// aggregation on the plain path, so the result never depends on it.
+static void settle(ws_agg* A) {
+  if (!A->part || A->settled) return;
+  int f[4];
+  u64 g = 0;
+  d2h_sync(f, A->flags, sizeof f);
+  d2h_sync(&g, A->ocount, sizeof g);
+  A->settled = true;
+  A->part_groups = (int64_t)g;
+  if (f[2]) {
+    A->free_part();
+    A->free_table();
+    fill_u32((uint32_t*)A->flags, 4, 0);
+    A->strategy = WS_STRAT_TABLE;
+    agg_run(A, A->last_env, A->last_jit, false);
+  }
+}
+
+// Turns a partitioned result into a plain table of the caller's capacity (more batches follow).
+static void part_to_table(ws_agg* A) {
+  settle(A);
+  if (!A->part) return;
+  u64 spilled = 0;
+  d2h_sync(&spilled, A->count, sizeof spilled);
+  u64* okeys = A->keys;
+  double *okv = A->kv, *oacc = A->acc;
+  const int64_t ocap = A->tcap;
+  A->keys = nullptr, A->kv = nullptr, A->acc = nullptr, A->tcap = 0;
+  ensure_table(A, A->cap);
+  const GTable t = table_of(A);
+  if (A->part_groups)
+    hipLaunchKernelGGL(table_absorb_k, dim3(grid_for(A->part_groups)), dim3(256), 0, stream(), t, A->plan, (const double*)A->okv,
+                       (const double*)A->oacc, (const u64*)nullptr, A->part_groups);
+  if (spilled)
+    hipLaunchKernelGGL(table_absorb_k, dim3(grid_for(ocap)), dim3(256), 0, stream(), t, A->plan, (const double*)okv,
+                       (const double*)oacc, (const u64*)okeys, ocap);
+  check_launch("table_absorb");
+  for (void* p : {(void*)okeys, (void*)okv, (void*)oacc}) tmp_free(p);
+  A->free_part();
+}
+
+static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit, bool allow_part) {
+  const ws_agg_plan& plan = A->plan;
+  if (&env != &A->last_env) A->last_env = env;
+  A->last_jit = jit;
   if (!jit && (plan.naggs > WS_INTERP_AGGS || plan.nkeys > WS_INTERP_KEYS))
     throw Error("this aggregation needs a compiled plan (more than 16 aggregates or 4 GROUP BY keys); codegen failed or is off",
                 WS_ERR_NEEDS_PLAN);
@@ -350,6 +823,15 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
     }
     return;
   }
+  int64_t rows = 0;
+  for (int64_t n : in.n) rows += n;
+  const PartPlan pp = allow_part ? plan_partitions(A, rows, ntiles) : PartPlan{};
+  if (pp.pbits) {
+    agg_run_part(A, env, jit, in, d, pp);
+    return;
+  }
+  if (A->part) part_to_table(A);
+  ensure_table(A, A->cap);
   // hash GROUP BY: an LDS table per block, sized to leave room for several blocks per CU
   const size_t per_slot = 8 + 8 * (size_t)plan.nkeys + 8 * (size_t)plan.naggs;
   int S = 64;
@@ -360,35 +842,48 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)cus * bpc));
   DescUpload up;
   build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
-  GTable t{A->keys, A->kv, A->acc, (u64)(A->cap - 1), A->flags, A->count};
-  if (jf) {
-    void* args[] = {&up.ddesc, &t};
-    jit_launch_raw(jf, grid, kBlock, lds, args, "agg_hash_jit");
-  } else {
-    ProfScope ps("agg_hash");
-    hipLaunchKernelGGL(lean ? ws_l_grouped : ws_i_grouped, dim3(grid), dim3(kBlock), lds, stream(), up.ddesc, t);
-    check_launch("agg_hash");
-  }
+  GTable t = table_of(A);
+  launch_grouped(jf, 0, lean, up.ddesc, t, grid, lds, "agg_hash", "agg_hash_jit");
 }
 
 // Compacts the groups into a tmp buffer: keys then accs, row-major (fetch) or column-major.
 static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor) {
   const int nk = A->plan.nkeys, na = A->plan.naggs;
-  DevBuf<int32_t> idx(A->cap);
-  const int64_t g = A->is_dense ? compact_dense(A->acc, na, A->dense.G, idx.p) : compact_occupied_hash(A->keys, A->cap, idx.p);
-  buf = DevBuf<double>(std::max<int64_t>(g * (nk + na), 1));
-  if (g) {
-    double* ko = buf.p;
-    double* ao = buf.p + g * nk;
-    ProfScope ps("gather_groups");
-    if (A->is_dense)
-      hipLaunchKernelGGL(gather_dense_k, dim3(grid_for(g * (nk + na))), dim3(256), 0, stream(), A->dense, A->plan, A->acc, nk,
-                         na, idx.p, g, ko, ao, colmajor ? 1 : 0);
-    else
-      hipLaunchKernelGGL(gather_groups_k, dim3(grid_for(g * (nk + na))), dim3(256), 0, stream(), A->kv, nk, A->acc, na, idx.p,
-                         g, ko, ao, colmajor ? 1 : 0);
-    check_launch("gather_groups");
+  settle(A);
+  if (!A->is_dense && !A->keys) {  // no run yet: no groups
+    buf = DevBuf<double>(1);
+    return 0;
   }
+  const int64_t slots = A->is_dense ? A->dense.G : A->tcap;
+  DevBuf<int32_t> idx(slots);
+  int64_t g1 = 0;  // finished groups of a partitioned run come first, then any spilled ones
+  bool table_groups = true;
+  if (A->part) {
+    u64 spilled = 0;
+    d2h_sync(&spilled, A->count, sizeof spilled);
+    g1 = A->part_groups;
+    table_groups = spilled != 0;
+  }
+  const int64_t g2 = !table_groups ? 0
+                     : A->is_dense ? compact_dense(A->acc, na, A->dense.G, idx.p)
+                                   : compact_occupied_hash(A->keys, A->tcap, idx.p);
+  const int64_t g = g1 + g2;
+  buf = DevBuf<double>(std::max<int64_t>(g * (nk + na), 1));
+  double* ko = buf.p;
+  double* ao = buf.p + g * nk;
+  ProfScope ps("gather_groups");
+  if (g1)
+    hipLaunchKernelGGL(gather_groups_k, dim3(grid_for(g1 * (nk + na))), dim3(256), 0, stream(), (const double*)A->okv, nk,
+                       (const double*)A->oacc, na, (const int32_t*)nullptr, g1, ko, ao, colmajor ? 1 : 0, (int64_t)0, g);
+  if (g2) {
+    if (A->is_dense)
+      hipLaunchKernelGGL(gather_dense_k, dim3(grid_for(g2 * (nk + na))), dim3(256), 0, stream(), A->dense, A->plan, A->acc, nk,
+                         na, idx.p, g2, ko, ao, colmajor ? 1 : 0);
+    else
+      hipLaunchKernelGGL(gather_groups_k, dim3(grid_for(g2 * (nk + na))), dim3(256), 0, stream(), (const double*)A->kv, nk,
+                         (const double*)A->acc, na, (const int32_t*)idx.p, g2, ko, ao, colmajor ? 1 : 0, g1, g);
+  }
+  check_launch("gather_groups");
   return g;
 }
 
@@ -409,21 +904,17 @@ int ws_agg_create(const ws_agg_plan* plan, int64_t capacity, const ws_dense_spec
     while (c < capacity) c <<= 1;
     A->cap = plan->nkeys ? (A->is_dense ? A->dense.G : c) : 1;
     if (plan->nkeys) {
-      const int nk = plan->nkeys, na = std::max(plan->naggs, 1);
+      const int na = std::max(plan->naggs, 1);
       A->flags = (int*)tmp_alloc(sizeof(int) * 4);
       A->count = (u64*)tmp_alloc(sizeof(u64));
       fill_u32((uint32_t*)A->flags, 4, 0);
       fill_u64(A->count, 1, 0);
-      A->acc = (double*)tmp_alloc(sizeof(double) * (size_t)A->cap * na);
+      // The hash table is allocated by the first run, which knows whether it is partitioned.
       if (A->is_dense) {
+        A->acc = (double*)tmp_alloc(sizeof(double) * (size_t)A->cap * na);
         hipLaunchKernelGGL(init_dense_k, dim3(grid_for(A->cap * na)), dim3(256), 0, stream(), A->acc, A->cap, plan->naggs, A->plan);
-      } else {
-        A->keys = (u64*)tmp_alloc(sizeof(u64) * (size_t)A->cap);
-        A->kv = (double*)tmp_alloc(sizeof(double) * (size_t)A->cap * nk);
-        GTable t{A->keys, A->kv, A->acc, (u64)(A->cap - 1), A->flags, A->count};
-        hipLaunchKernelGGL(init_table_k, dim3(grid_for(A->cap)), dim3(256), 0, stream(), t, nk, plan->naggs, A->plan);
+        check_launch("init_dense");
       }
-      check_launch("init_table");
     }
     *out = A.release();
   });
@@ -440,14 +931,16 @@ int ws_agg_add_batch(ws_agg* a, const ws_expr_env* env, int ncols, int64_t n, co
 }
 int ws_agg_run(ws_agg* a, const ws_expr_env* env, const ws_jit_program* jit) {
   return guard([&] {
-    agg_run(a, *env, jit);
+    agg_run(a, *env, jit, true);
+    ++a->runs;
     a->fetched = -1;
   });
 }
 int ws_agg_run_window(ws_agg* a, const ws_expr_env* env, const ws_jit_program* jit) {
   return guard([&] {
     if (a->bn.empty()) return;
-    agg_run(a, *env, jit);
+    agg_run(a, *env, jit, false);  // windows accumulate into the shared table
+    ++a->runs;
     ++a->windows;
     for (int64_t n : a->bn) a->rows_done += n;
     a->batches_done += (int)a->bn.size();
@@ -468,6 +961,7 @@ int ws_agg_overflowed(ws_agg* a, int* out) {
   return guard([&] {
     *out = 0;
     if (a->plan.nkeys == 0) return;
+    settle(a);
     int f[4];
     d2h_sync(f, a->flags, sizeof f);
     if (f[1]) throw Error("dense GROUP BY key outside the planned key range", WS_ERR_INTERNAL);
@@ -540,11 +1034,20 @@ int ws_agg_compact(ws_agg* a, double** keys, double** accs, int64_t* groups) {
 int64_t ws_agg_capacity(const ws_agg* a) { return a->cap; }
 int ws_agg_group_count(ws_agg* a, int64_t* out) {
   return guard([&] {
+    settle(a);
     u64 c = 0;
-    if (a->count) d2h_sync(&c, a->count, sizeof c);
-    *out = (int64_t)c;
+    if (a->count && (a->keys || a->is_dense)) d2h_sync(&c, a->count, sizeof c);
+    *out = (int64_t)c + (a->part ? a->part_groups : 0);
   });
 }
+int ws_agg_set_strategy(ws_agg* a, int strategy, int64_t expected_groups) {
+  return guard([&] {
+    if (strategy < WS_STRAT_AUTO || strategy > WS_STRAT_PARTITIONED) throw Error("agg: bad strategy", WS_ERR_INTERNAL);
+    a->strategy = strategy;
+    a->expected_groups = expected_groups;
+  });
+}
+int ws_agg_partitioned(const ws_agg* a) { return a->part ? 1 : 0; }
 
 // ------------------------------------------------------------------ selection
 

@@ -142,7 +142,8 @@ int64_t compact_flags(const uint8_t* flags, int64_t n, int32_t* out_idx);
 // ---- JIT (ws_jit.cpp)
 struct JitKernel;  // a compiled kernel ready to launch
 // Returns the compiled function for kernel kind fn of program p, or nullptr.
-void* jit_function(const ws_jit_program* p, int fn);
+void* jit_function(const ws_jit_program* p, int fn);  // fn: WS_JF_* or one of the kinds below
+constexpr int kJitPartCount = 100, kJitPartScatter = 101;  // partition passes of a grouped plan
 void jit_launch_raw(void* fn, unsigned grid, unsigned block, size_t shmem, void** args, const char* prof_name);
 int jit_rows(const ws_jit_program* p);
 

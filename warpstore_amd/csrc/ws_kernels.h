@@ -42,6 +42,16 @@ struct ScanDesc {
   int32_t lds_slots;             // hash: LDS table slots (power of two); dense: copies
   int32_t copy_stride;           // dense LDS copies: bytes per copy
   int32_t slot_off[WS_MAX_AGGS]; // dense LDS copies: byte offset of each slot's array in a copy
+  // Partitioned hash GROUP BY (the grouped plans' _pcount / _pscatter kernels).
+  int32_t pbits;                 // log2 of the partition count (1..12); partition = top bits of the hash
+  int32_t chunk_tiles;           // tiles per chunk; block c scans chunk c
+  int32_t pre_slots;             // scatter: LDS pre-aggregation slots (power of two)
+  int32_t rec_words;             // 1 + nkeys + naggs: words of one partial-group record
+  uint32_t* part_cnt;            // [nchunks][P] surviving rows per chunk and partition
+  const uint32_t* part_off;      // [nchunks][P] offset of the chunk's run inside its partition
+  const uint32_t* part_start;    // [P + 1] first record of each partition
+  u64* recs;                     // [rows][rec_words] records: hash, key values, slot values
+  int* part_err;                 // set when a run would exceed its counted length
 };
 
 // Global hash table of one grouped aggregation (row-major accumulators per slot).
@@ -646,6 +656,30 @@ __device__ __forceinline__ bool global_insert(const GTable& t, const ws_agg_plan
   return false;
 }
 
+// Key values of row r (integer keys as int64 bits, doubles normalised with +0.0) and the group
+// hash over them: h = fold(mix64(h ^ bits(key_k)) + k) from 0x9e3779b97f4a7c15, with the free-slot
+// marker mapped away (reference kernels.cuh:340-347).
+template <class Ev>
+__device__ __forceinline__ u64 row_keys(const Ev& ev, int NK, i64 r, double* kv) {
+  u64 h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+  for (int k = 0; k < Ev::kKeys; ++k) {
+    if (k >= NK) break;
+    kv[k] = ev.key_int(k) ? bitsd(ev.ikey(k, r)) : ev.key(k, r) + 0.0;
+    h = mix64(h ^ (u64)dbits(kv[k])) + (u64)k;
+  }
+  return h == kEmpty ? h ^ 1 : h;
+}
+// Slot values row r contributes (exact int64 bits for agg_bits slots, else doubles).
+template <class Ev>
+__device__ __forceinline__ void row_values(const Ev& ev, const ws_agg_plan& plan, int NA, i64 r, double* av) {
+#pragma unroll
+  for (int a = 0; a < Ev::kAggs; ++a) {
+    if (a >= NA) break;
+    av[a] = plan.agg_bits[a] ? bitsd(int_part(plan, a, ev.iagg(a, r))) : ev.agg(a, r);
+  }
+}
+
 // Hash filter + GROUP BY. Group identity is the 64-bit hash of the keys (as in the reference,
 // kernels.cuh:340-347): h = fold(mix64(h ^ bits(key_k)) + k) from 0x9e3779b97f4a7c15.
 // Each block keeps an open-addressing table of `lds_slots` groups in LDS for the whole launch
@@ -691,19 +725,8 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
       if (ev.has_where() && !ev.where(r)) continue;
       double kv[Ev::kKeys];
       double av[Ev::kAggs];
-      u64 h = 0x9e3779b97f4a7c15ull;
-#pragma unroll
-      for (int k = 0; k < Ev::kKeys; ++k) {
-        if (k >= NK) break;
-        kv[k] = ev.key_int(k) ? bitsd(ev.ikey(k, r)) : ev.key(k, r) + 0.0;
-        h = mix64(h ^ (u64)dbits(kv[k])) + (u64)k;
-      }
-      if (h == kEmpty) h ^= 1;
-#pragma unroll
-      for (int a = 0; a < Ev::kAggs; ++a) {
-        if (a >= NA) break;
-        av[a] = plan.agg_bits[a] ? bitsd(int_part(plan, a, ev.iagg(a, r))) : ev.agg(a, r);
-      }
+      const u64 h = row_keys(ev, NK, r, kv);
+      row_values(ev, plan, NA, r, av);
       unsigned s = (unsigned)(h & (u64)(S - 1));
       bool done = false;
       for (int probe = 0; probe < 8; ++probe) {
@@ -736,6 +759,149 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
     const u64 total = atomicAdd((unsigned long long*)gt.count, (unsigned long long)s_new) + s_new;
     if (total * 10 > (gt.mask + 1) * 7) gt.flags[0] = 1;  // load above 0.7: retry bigger
   }
+}
+
+// ------------------------------------------------------------------ partitioned hash GROUP BY
+
+// High-cardinality GROUP BY without global atomics on a shared table. Three passes over a plan:
+//   count   (NAME_pcount): block c scans chunk c (chunk_tiles tiles) and counts its surviving
+//           rows per partition (top pbits of the group hash) in LDS -> part_cnt[c][p];
+//   (host-side kernels turn the counts into part_start[p] and per-chunk offsets part_off[c][p]);
+//   scatter (NAME_pscatter): block c scans chunk c again and writes one record per row into its
+//           partition's run [part_start[p] + part_off[c][p], + part_cnt[c][p]). Groups that repeat
+//           inside the chunk are first combined in a small LDS table (first come, first served),
+//           which flushes one record per group at the end; the unused tail of each run is marked
+//           with the free hash. A run that would exceed its count raises part_err instead of
+//           writing outside it.
+//   merge   (part_agg_k): one block per partition combines its records in an LDS table and
+//           appends the finished groups to the output.
+// A record is [hash, key values (nk), slot values (na)], 8 bytes each.
+template <class Ev>
+__device__ void part_count_body(const ScanDesc& d) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  Ev ev(d);
+  const int NK = ev.nkeys();
+  const int P = 1 << d.pbits, shift = 64 - d.pbits;
+  uint32_t* hist = (uint32_t*)ws_smem;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) hist[i] = 0;
+  const i64 c = blockIdx.x;
+  const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
+  int cur = -1;
+  for (i64 t = t0; t < t1; ++t) {
+    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
+    if (b != cur) {
+      stage_batch(d, b, scols);  // also orders the histogram reset before the first update
+      ev.bind(scols);
+      cur = b;
+    }
+    const TileRows tr = tile_rows(d, t, b);
+    const uint32_t* del = d.del[b];
+    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+      const i64 r = tr.first + i;
+      if (is_deleted(del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      double kv[Ev::kKeys];
+      const u64 h = row_keys(ev, NK, r, kv);
+      __hip_atomic_fetch_add(&hist[h >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += blockDim.x) d.part_cnt[c * P + i] = hist[i];
+}
+
+__device__ __forceinline__ void put_record(u64* rec, u64 h, int NK, int NA, const double* kv, const double* av) {
+  rec[0] = h;
+  for (int k = 0; k < NK; ++k) rec[1 + k] = (u64)dbits(kv[k]);
+  for (int a = 0; a < NA; ++a) rec[1 + NK + a] = (u64)dbits(av[a]);
+}
+
+template <class Ev>
+__device__ void part_scatter_body(const ScanDesc& d) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  Ev ev(d);
+  const ws_agg_plan& plan = *d.plan;
+  const int NK = ev.nkeys(), NA = ev.naggs(), W = d.rec_words;
+  const int P = 1 << d.pbits, shift = 64 - d.pbits, S = d.pre_slots;
+  const i64 c = blockIdx.x;
+  uint32_t* cur_pos = (uint32_t*)ws_smem;  // [P] next record of this chunk's run
+  uint32_t* end_pos = cur_pos + P;         // [P] end of the run
+  u64* pk = (u64*)(end_pos + P);           // pre-aggregation table: hashes [S]
+  double* pkv = (double*)(pk + S);         // [S * NK]
+  double* pacc = pkv + (i64)S * NK;        // [S * NA]
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    const uint32_t b0 = d.part_start[p] + d.part_off[c * P + p];
+    cur_pos[p] = b0;
+    end_pos[p] = b0 + d.part_cnt[c * P + p];
+  }
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    pk[i] = kEmpty;
+    for (int a = 0; a < NA; ++a) pacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
+  }
+  const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
+  int cur = -1;
+  for (i64 t = t0; t < t1; ++t) {
+    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
+    if (b != cur) {
+      stage_batch(d, b, scols);  // also orders the table set-up before the first row
+      ev.bind(scols);
+      cur = b;
+    }
+    const TileRows tr = tile_rows(d, t, b);
+    const uint32_t* del = d.del[b];
+    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+      const i64 r = tr.first + i;
+      if (is_deleted(del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      double kv[Ev::kKeys];
+      double av[Ev::kAggs];
+      const u64 h = row_keys(ev, NK, r, kv);
+      row_values(ev, plan, NA, r, av);
+      unsigned s = (unsigned)(h & (u64)(S - 1));
+      bool done = false;
+      for (int probe = 0; probe < 4; ++probe) {
+        u64 prev = kEmpty;
+        const bool claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (claimed)
+          for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[k];
+        if (claimed || prev == h) {
+#pragma unroll
+          for (int a = 0; a < Ev::kAggs; ++a) {
+            if (a >= NA) break;
+            atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, ev.op(a), &pacc[(i64)s * NA + a], av[a]);
+          }
+          done = true;
+          break;
+        }
+        s = (s + 1) & (unsigned)(S - 1);
+      }
+      if (done) continue;
+      const unsigned p = (unsigned)(h >> shift);
+      const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
+      if (pos >= end_pos[p]) {
+        *d.part_err = 1;
+        continue;
+      }
+      put_record(d.recs + (i64)pos * W, h, NK, NA, kv, av);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const u64 h = pk[i];
+    if (h == kEmpty) continue;
+    const unsigned p = (unsigned)(h >> shift);
+    const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
+    if (pos >= end_pos[p]) {
+      *d.part_err = 1;
+      continue;
+    }
+    put_record(d.recs + (i64)pos * W, h, NK, NA, &pkv[(i64)i * NK], &pacc[(i64)i * NA]);
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x)
+    for (uint32_t pos = cur_pos[p]; pos < end_pos[p]; ++pos) d.recs[(i64)pos * W] = kEmpty;
 }
 
 // ------------------------------------------------------------------ filter + stream compaction
@@ -927,6 +1093,12 @@ __device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, i
 #define WS_KERNEL_GROUPED(EV, NAME)                                                                         \
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, ws::GTable gt) {            \
     ws::grouped_body<EV>(*d, gt);                                                                            \
+  }                                                                                                          \
+  extern "C" __global__ void __launch_bounds__(1024) NAME##_pcount(const ws::ScanDesc* d, ws::GTable) {     \
+    ws::part_count_body<EV>(*d);                                                                             \
+  }                                                                                                          \
+  extern "C" __global__ void __launch_bounds__(1024) NAME##_pscatter(const ws::ScanDesc* d, ws::GTable) {   \
+    ws::part_scatter_body<EV>(*d);                                                                           \
   }
 #define WS_KERNEL_SELECT(EV, NAME)                                                                          \
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, unsigned long long* state,  \

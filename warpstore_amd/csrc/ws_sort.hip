@@ -31,13 +31,20 @@ __global__ void sort_key_k(const int32_t* perm, const double* key, int64_t m, in
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
     out[j] = order_key(key[perm[j]], desc != 0);
 }
-// OR over (key ^ key[0]): the bits that vary between keys.
-__global__ void varying_bits_k(const u64* k, int64_t m, unsigned long long* out) {
+// OR over (key ^ key[0]): the bits that vary between keys. One atomic per block (a grid of about
+// one block per CU), since atomics on one address serialize.
+__global__ void __launch_bounds__(256) varying_bits_k(const u64* k, int64_t m, unsigned long long* out) {
+  __shared__ u64 s_or[4];
   const u64 k0 = k[0];
   u64 acc = 0;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) acc |= k[j] ^ k0;
   for (int o = 32; o > 0; o >>= 1) acc |= __shfl_xor(acc, o, 64);
-  if ((threadIdx.x & 63) == 0 && acc) atomicOr(out, (unsigned long long)acc);
+  if ((threadIdx.x & 63) == 0) s_or[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u64 v = s_or[0] | s_or[1] | s_or[2] | s_or[3];
+    if (v) atomicOr(out, (unsigned long long)v);
+  }
 }
 
 __global__ void __launch_bounds__(kSBlock) digit_hist_k(const u64* keys, int64_t m, int shift, int64_t ntiles, uint32_t* hist) {
@@ -151,7 +158,7 @@ static void scan_u32(const uint32_t* in, uint32_t* out, int64_t n) {
 static void radix_sort_pairs(DevBuf<u64>& keys, DevBuf<int32_t>& vals, DevBuf<u64>& k2, DevBuf<int32_t>& v2, int64_t m) {
   DevBuf<unsigned long long> vary(1);
   fill_u64((u64*)vary.p, 1, 0);
-  hipLaunchKernelGGL(varying_bits_k, dim3(grid_for(m)), dim3(256), 0, stream(), keys.p, m, vary.p);
+  hipLaunchKernelGGL(varying_bits_k, dim3(grid_for(m, 256, 1)), dim3(256), 0, stream(), keys.p, m, vary.p);
   check_launch("varying_bits");
   unsigned long long bits = 0;
   d2h_sync(&bits, vary.p, 8);

@@ -121,6 +121,8 @@ def main() -> int:
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a tuning knob of the physical plans (ws_tune_set) before running")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy kernel (known bytes, for counter calibration)")
     args = ap.parse_args()
@@ -134,6 +136,11 @@ def main() -> int:
     L = ws.lib()
     ws.check(L.ws_init())
     L.ws_jit_set_mode(abi.JIT_SYNC)
+    tuned = {}
+    for kv in args.tune:
+        key, val = kv.split("=", 1)
+        ws.check(L.ws_tune_set(key.encode(), int(val)))
+        tuned[key] = int(val)
     if args.calibrate:
         src, dst = ws.DevBuf(1 << 30), ws.DevBuf(1 << 30)
         ws.check(L.ws_copy_kernel(dst.p, src.p, 1 << 30))
@@ -199,6 +206,7 @@ def main() -> int:
             "rows": ROWS, "bytes_per_row": BYTES_PER_ROW, "segments_rows": 1 << 20, "table_copies": args.copies,
             "distinct_pairs": int(stats[0]), "groups": int(stats[1]), "phase1_capacity": int(stats[2]),
             "overflow_retries": int(stats[3]),
+            **({"tuning": tuned} if tuned else {}),
         },
         "pct_of_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 3),
         "queries_per_s": round(args.steps / elapsed, 3),

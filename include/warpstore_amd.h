@@ -108,6 +108,30 @@ int ws_agg_group_count(ws_agg* a, int64_t* out);           /* groupCount gpu.h:1
 enum { WS_STRAT_AUTO = 0, WS_STRAT_TABLE = 1, WS_STRAT_PARTITIONED = 2 };
 int ws_agg_set_strategy(ws_agg* a, int strategy, int64_t expected_groups);
 int ws_agg_partitioned(const ws_agg* a); /* 1 when the last run produced a partitioned result */
+/* Process-wide tuning knobs of the physical plans (no reference counterpart: the autotuner's
+ * interface; results do not depend on them). Keys: "part_merge_lds_kib" (8..152, default 60),
+ * "part_scatter_lds_kib" (16..120, default 48), "part_chunks_per_cu" (1..64, default 1),
+ * "part_auto_rows" (default 2^20), "part_auto_groups" (default 2^16). Unknown keys and values
+ * out of range return WS_ERR_INTERNAL. */
+int ws_tune_set(const char* key, int64_t value);
+int ws_tune_get(const char* key, int64_t* value);
+
+/* ------------------------------------------------------------------ column statistics (sizing)
+ * No reference counterpart: the segment statistics the sizing model reads (the reference engine
+ * keeps min / max per segment, engine_core.h:3869-3882; the HyperLogLog registers are new).
+ * Distinct identity: integer types by exact value, doubles by the bits of (v + 0.0); NULLs are
+ * counted in `nulls` and nowhere else. HyperLogLog with p = 11: register x >> 53 of
+ * x = mix64(value bits) (the splitmix64 finalizer), rank = leading zeros of (x << 11 | 1 << 10) + 1. */
+typedef struct ws_col_stats {
+  int64_t rows;       /* live (not deleted) rows */
+  int64_t nulls;      /* live NULL rows */
+  int64_t imin, imax; /* integer types: exact min / max (INT64_MAX / INT64_MIN when none) */
+  double dmin, dmax;  /* min / max as doubles (NaN when none) */
+  uint8_t hll[2048];  /* HyperLogLog registers */
+} ws_col_stats;
+int ws_column_stats(const ws_colview* col, int64_t n, const uint32_t* del, ws_col_stats* out); /* one batch */
+void ws_col_stats_merge(ws_col_stats* dst, const ws_col_stats* src);
+double ws_hll_estimate(const uint8_t* regs); /* distinct-count estimate of 2048 registers */
 
 /* ------------------------------------------------------------------ selection / gathers */
 int ws_select(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, const uint32_t* del, int32_t* out_idx,

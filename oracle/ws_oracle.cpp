@@ -15,6 +15,9 @@
 //   - inner / outer equi-join pair sets (NULL never matches)  src/gpu.cu:1089-1293
 //   - stable multi-key ORDER BY permutation (NULL -> -inf, -0 -> 0)  src/gpu.cu:1318-1356
 //   - exclusive sums of integer SUM halves recombined in 128 bits   src/engine_core.h:1972-1983
+//   - column statistics for sizing (no reference counterpart): live / NULL counts, min / max and
+//     HyperLogLog registers (Flajolet et al. 2007; p = 11, splitmix64 finalizer over the value's
+//     bits: exact int64 for integer types, bits of v + 0.0 for doubles)
 // Pinned by: golden vectors from the reference's own devtypes.h compiled on the host (the VM,
 // decoders, dates, checked arithmetic), brute force and closed forms in tests/test_oracle.py.
 #include <algorithm>
@@ -618,6 +621,62 @@ void or_remap_codes(const int32_t* in, int64_t n, const int32_t* map, int64_t ma
 // region < region_lt GROUP BY user_id: every group, ordered by count desc then user_id asc, as
 // rows (user_id, count, sum(dur), distinct urls). The two-phase plan of the engine
 // (engine_select.cpp:1403-1492) is restated as (user, url) pairs sorted and counted.
+// Column statistics of one batch, the fields of ws_col_stats in order: rows, nulls, imin, imax
+// (as int64), dmin, dmax (as doubles), then 2048 register bytes.
+void or_column_stats(const ws_colview* c, int64_t n, const uint32_t* del, int64_t* counts, double* minmax,
+                     uint8_t* regs) {
+  i64 rows = 0, nulls = 0, imin = I64MAX, imax = I64MIN;
+  double dmin = NaN(), dmax = NaN();
+  std::memset(regs, 0, 2048);
+  const bool is_int = c->t != WS_T_F64;
+  const bool nullable = c->t == WS_T_I32N || c->t == WS_T_I64N || c->t == WS_T_DECIMAL;
+  const i64 nul = c->t == WS_T_I32N ? (i64)WS_NULL_I32 : (i64)WS_NULL_I64;
+  for (i64 r = 0; r < n; ++r) {
+    if (del && ((del[r >> 5] >> (r & 31)) & 1u)) continue;
+    ++rows;
+    u64 bits;
+    double dv;
+    if (is_int) {
+      const i64 v = stored_int(*c, r);
+      if (nullable && v == nul) {
+        ++nulls;
+        continue;
+      }
+      bits = (u64)v;
+      imin = std::min(imin, v);
+      imax = std::max(imax, v);
+      dv = c->t == WS_T_DECIMAL ? (double)v / std::pow(10.0, c->scale) : (double)v;
+    } else {
+      dv = stored_dbl(*c, r);
+      if (std::isnan(dv)) {
+        ++nulls;
+        continue;
+      }
+      const double z = dv + 0.0;
+      std::memcpy(&bits, &z, 8);
+    }
+    dv += 0.0;
+    dmin = std::isnan(dmin) ? dv : std::min(dmin, dv);
+    dmax = std::isnan(dmax) ? dv : std::max(dmax, dv);
+    u64 x = bits;  // splitmix64 finalizer
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    const unsigned idx = (unsigned)(x >> 53);
+    u64 w = (x << 11) | (1ull << 10);
+    unsigned rank = 1;
+    while (!(w >> 63)) {
+      w <<= 1;
+      ++rank;
+    }
+    if (rank > regs[idx]) regs[idx] = (uint8_t)rank;
+  }
+  counts[0] = rows, counts[1] = nulls, counts[2] = imin, counts[3] = imax;
+  minmax[0] = dmin, minmax[1] = dmax;
+}
+
 int64_t or_c2_query(const int64_t* user, const int32_t* region, const int32_t* date, const int32_t* url, const int32_t* dur,
                     int64_t n, int32_t date_lt, int32_t region_lt, int64_t* out, int64_t cap) {
   struct R {

@@ -396,3 +396,28 @@ def gen_c3(n_fact: int, n_dim: int, seed: int = 3):
     flag = np.empty(n_fact, np.int32)
     g.wsg_c3_fact(seed, n_fact, n_dim, ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag))
     return fk, qty, price, flag, dkey, dattr
+
+
+# ------------------------------------------------------------------ column statistics
+
+def oracle_column_stats(view: abi.ColView, n: int, del_words=None) -> abi.ColStats:
+    """The oracle's column statistics of one host batch, as a ws_col_stats structure."""
+    O = oracle()
+    O.or_column_stats.restype = None
+    O.or_column_stats.argtypes = [C.POINTER(abi.ColView), I64, P, P, P, P]
+    counts = np.zeros(4, np.int64)
+    mm = np.zeros(2)
+    regs = np.zeros(2048, np.uint8)
+    O.or_column_stats(C.byref(view), n, ws.ptr(del_words) if del_words is not None else None, ws.ptr(counts), ws.ptr(mm),
+                      ws.ptr(regs))
+    st = abi.ColStats()
+    st.rows, st.nulls, st.imin, st.imax = (int(x) for x in counts)
+    st.dmin, st.dmax = float(mm[0]), float(mm[1])
+    C.memmove(st.hll, regs.ctypes.data, 2048)
+    return st
+
+
+def host_view(arr: np.ndarray, tcode: int, scale: int = 0) -> abi.ColView:
+    v = abi.ColView()
+    v.p, v.t, v.scale = arr.ctypes.data, tcode, scale
+    return v

@@ -119,3 +119,18 @@ def test_no_gpu_means_loud_failure():
     plan = abi.AggPlan()
     plan.naggs = 1
     assert L.ws_agg_create(C.byref(plan), 1, None, C.byref(h)) == ws.WS_ERR_INTERNAL
+
+
+def test_tuning_knobs_roundtrip():
+    """ws_tune_set / ws_tune_get: known keys round-trip, unknown keys and out-of-range values fail."""
+    import ctypes as C
+    L = ws.lib()
+    v = C.c_int64()
+    assert L.ws_tune_get(b"part_chunks_per_cu", C.byref(v)) == 0
+    old = v.value
+    assert L.ws_tune_set(b"part_chunks_per_cu", 5) == 0
+    assert L.ws_tune_get(b"part_chunks_per_cu", C.byref(v)) == 0 and v.value == 5
+    assert L.ws_tune_set(b"part_chunks_per_cu", 0) != 0
+    assert L.ws_tune_set(b"no_such_knob", 1) != 0
+    assert b"unknown" in L.ws_last_error()
+    assert L.ws_tune_set(b"part_chunks_per_cu", old) == 0

@@ -56,6 +56,9 @@ _SIGS = {
     "ws_agg_compact": (I, [P, PP, PP, C.POINTER(I64)]), "ws_agg_capacity": (I64, [P]),
     "ws_agg_group_count": (I, [P, C.POINTER(I64)]),
     "ws_agg_set_strategy": (I, [P, I, I64]), "ws_agg_partitioned": (I, [P]),
+    "ws_tune_set": (I, [C.c_char_p, I64]), "ws_tune_get": (I, [C.c_char_p, C.POINTER(I64)]),
+    "ws_column_stats": (I, [P, I64, P, P]), "ws_col_stats_merge": (None, [P, P]),
+    "ws_hll_estimate": (C.c_double, [P]),
     "ws_select": (I, [C.POINTER(ExprEnv), I, Prog, I64, P, P, P, C.POINTER(I64)]),
     "ws_select_multi": (I, [C.POINTER(ExprEnv), I, Prog, I, P, P, P, I, PP, P, P, C.POINTER(I64)]),
     "ws_gather_multi": (I, [P, P, I, I, I, P, I64, P]), "ws_remap_codes": (I, [P, I64, P, I64, P]),

@@ -77,6 +77,11 @@ class TempColSpec(C.Structure):
                 ("mul", C.c_double)]
 
 
+class ColStats(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("nulls", C.c_int64), ("imin", C.c_int64), ("imax", C.c_int64),
+                ("dmin", C.c_double), ("dmax", C.c_double), ("hll", C.c_uint8 * 2048)]
+
+
 class ProfRow(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int64), ("total_ms", C.c_double)]
 

@@ -1,7 +1,9 @@
 // GpuAggregator and row selection over batches (reference src/gpu.cu:393-755), planned for gfx950.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <memory>
 
 #include "ws_internal.h"
@@ -127,15 +129,58 @@ struct PartMerge {
   const u64* recs;          // [records][W]
   const uint32_t* pstart;   // [P + 1]
   int W, NK, NA, S;         // record words, keys, slots, LDS table slots
-  double* okv;              // [groups][NK] finished groups
-  double* oacc;             // [groups][NA]
-  u64* ocount;
+  u64 kinds;                // slot kinds (slot_kind)
+  double* okv;              // [records][NK] finished groups: partition p's at rows pstart[p] + rank
+  double* oacc;             // [records][NA]
+  uint32_t* gcnt;           // [P] groups finished per partition
 };
+
+// Row of the partitioned result that holds group i of the compacted order (groups of partition p
+// are i in [goff[p], goff[p + 1]), stored from row pstart[p]).
+__device__ __forceinline__ int64_t part_row(const uint32_t* goff, const uint32_t* pstart, int P, int64_t i) {
+  int lo = 0, hi = P - 1;  // last p with goff[p] <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int64_t)goff[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return (int64_t)pstart[lo] + (i - (int64_t)goff[lo]);
+}
+
+// Compacted groups of a partitioned result: rows [0, g) of a result of G groups.
+__global__ void gather_part_k(const double* kv, int nk, const double* acc, int na, const uint32_t* goff,
+                              const uint32_t* pstart, int P, int64_t g, double* keys_out, double* accs_out, int colmajor,
+                              int64_t G) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < g; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = part_row(goff, pstart, P, i);
+    for (int c = 0; c < nk; ++c) {
+      const double v = kv[s * nk + c];
+      if (colmajor) keys_out[(int64_t)c * G + i] = v;
+      else keys_out[i * nk + c] = v;
+    }
+    for (int a = 0; a < na; ++a) {
+      const double v = acc[s * na + a];
+      if (colmajor) accs_out[(int64_t)a * G + i] = v;
+      else accs_out[i * na + a] = v;
+    }
+  }
+}
+
+// Slot kinds of a plan packed 3 bits per slot (op in bits 0-1, int64-bits flag in bit 2), so
+// kernels branch on a uniform scalar instead of loading the plan per record.
+__host__ __device__ __forceinline__ int slot_kind(u64 kinds, int a) { return (int)((kinds >> (3 * a)) & 7); }
+__device__ __forceinline__ double kind_identity(int k) { return slot_identity((k & 4) != 0, k & 3); }
+template <int SCOPE>
+__device__ __forceinline__ void atomic_combine_kind(int k, double* addr, double v) {
+  atomic_combine_slot<SCOPE>((k & 4) != 0, k & 3, addr, v);
+}
+__device__ __forceinline__ double wave_combine_kind(int k, double v) {
+  return (k & 4) ? bitsd(wave_combine_i(k & 3, dbits(v))) : wave_combine_d(k & 3, v);
+}
 
 // Inserts one record (held in registers) into the spill table; false when the table overflowed.
 template <int W>
-__device__ __forceinline__ bool spill_insert(const GTable& t, const ws_agg_plan& plan, int NK, const u64 (&rec)[W],
-                                             unsigned* new_groups) {
+__device__ __forceinline__ bool spill_insert(const GTable& t, u64 kinds, int NK, const u64 (&rec)[W], unsigned* new_groups) {
   const int NA = W - 1 - NK;
   const u64 h = rec[0];
   u64 s = h & t.mask;
@@ -154,7 +199,7 @@ __device__ __forceinline__ bool spill_insert(const GTable& t, const ws_agg_plan&
       for (int j = 1; j < W; ++j)
         if (j > NK) {
           const int a = j - 1 - NK;
-          atomic_combine_slot<WS_DEV>(plan.agg_bits[a] != 0, plan.ops[a], &t.acc[s * NA + a], bitsd((i64)rec[j]));
+          atomic_combine_kind<WS_DEV>(slot_kind(kinds, a), &t.acc[s * NA + a], bitsd((i64)rec[j]));
         }
       return true;
     }
@@ -166,61 +211,67 @@ __device__ __forceinline__ bool spill_insert(const GTable& t, const ws_agg_plan&
 
 // Merge pass, for records of W words: block p combines partition p's records in an LDS table of
 // m.S slots (linear probing from a multiply-shift index of the hash's low word; the partition is
-// its high bits). Each thread loads R records into registers before combining them, so a block
-// keeps many loads in flight. Lanes of a wave that hold the same group are folded into one lane
-// first (up to 3 groups per step): hot groups cost one LDS update per wave instead of one per
-// lane. A record that finds no LDS slot in 64 probes goes to the spill table (device-scope
-// atomics, as the plain path). At the end the block appends its groups, row-major, after one
-// atomic reservation.
+// its high bits). Each thread loads R records into registers before combining them one by one
+// (a register queue shifted per record), so a block keeps many loads in flight with one copy of
+// the combining code. Lanes of a wave holding the same group as the wave's first live lane fold
+// into that lane first: a hot group costs one LDS update per wave instead of one per lane. A
+// record that finds no LDS slot in 64 probes goes to the spill table (device-scope atomics, as
+// the plain path). At the end the block writes its groups inside the partition's own record range
+// and its group count to gcnt[p]; no block waits on another or on a shared counter.
 template <int W>
-__global__ void __launch_bounds__(512) part_merge_k(PartMerge m, ws_agg_plan plan, GTable spill) {
+__global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   constexpr int R = W <= 3 ? 8 : W <= 6 ? 4 : W <= 10 ? 2 : 1;  // records per thread per step
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
   __shared__ unsigned s_new;
   __shared__ uint32_t s_w[8];
-  __shared__ u64 s_base;
   const int NK = m.NK, NA = W - 1 - NK, S = m.S;
+  const u64 kinds = m.kinds;
   u64* lk = (u64*)ws_smem;
   double* lkv = (double*)(lk + S);
   double* lacc = lkv + (int64_t)S * NK;
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     lk[i] = kEmpty;
-    for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
+    for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = kind_identity(slot_kind(kinds, a));
   }
   if (threadIdx.x == 0) s_new = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t beg = m.pstart[blockIdx.x], end = m.pstart[blockIdx.x + 1];
   for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * R) {
-    u64 rec[R][W];
+    u64 q[R][W];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int64_t i = i0 + (int64_t)r * blockDim.x + threadIdx.x;
 #pragma unroll
-      for (int j = 0; j < W; ++j) rec[r][j] = i < end ? m.recs[i * W + j] : kEmpty;
+      for (int j = 0; j < W; ++j) q[r][j] = i < end ? m.recs[i * W + j] : kEmpty;
     }
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < R; ++r) {
-      const u64 h = rec[r][0];
+      u64 rec[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) rec[j] = q[0][j];
+#pragma unroll
+      for (int r2 = 0; r2 + 1 < R; ++r2)
+#pragma unroll
+        for (int j = 0; j < W; ++j) q[r2][j] = q[r2 + 1][j];
+      const u64 h = rec[0];
       bool live = h != kEmpty;
-      u64 todo = __ballot(live);
-      for (int round = 0; round < 3 && todo; ++round) {
-        const int L = __ffsll((unsigned long long)todo) - 1;
+      const u64 lv = __ballot(live);
+      if (lv) {
+        const int L = __ffsll((unsigned long long)lv) - 1;
         const u64 hl = (u64)__shfl((long long)h, L, 64);
         const u64 same = __ballot(live && h == hl);
-        todo &= ~same;
-        if (__popcll(same) < 2) continue;
-        const bool in = (same >> lane) & 1;
+        if (__popcll(same) > 1) {
+          const bool in = (same >> lane) & 1;
 #pragma unroll
-        for (int j = 1; j < W; ++j) {
-          if (j <= NK) continue;
-          const int a = j - 1 - NK;
-          const bool bits = plan.agg_bits[a] != 0;
-          const int op = plan.ops[a];
-          const double v = wave_combine_slot(bits, op, in ? bitsd((i64)rec[r][j]) : slot_identity(bits, op));
-          if (lane == L) rec[r][j] = (u64)dbits(v);
+          for (int j = 1; j < W; ++j) {
+            if (j <= NK) continue;
+            const int k = slot_kind(kinds, j - 1 - NK);
+            const double v = wave_combine_kind(k, in ? bitsd((i64)rec[j]) : kind_identity(k));
+            if (lane == L) rec[j] = (u64)dbits(v);
+          }
+          if (in && lane != L) live = false;
         }
-        if (in && lane != L) live = false;
       }
       if (!live) continue;
       unsigned s = (unsigned)(((h & 0xffffffffull) * (u64)S) >> 32);
@@ -232,7 +283,7 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, ws_agg_plan pla
         if (claimed) {
 #pragma unroll
           for (int j = 1; j < W; ++j)
-            if (j <= NK) lkv[(int64_t)s * NK + (j - 1)] = bitsd((i64)rec[r][j]);
+            if (j <= NK) lkv[(int64_t)s * NK + (j - 1)] = bitsd((i64)rec[j]);
         }
         if (claimed || prev == h) {
           slot = (int)s;
@@ -245,11 +296,10 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, ws_agg_plan pla
         for (int j = 1; j < W; ++j)
           if (j > NK) {
             const int a = j - 1 - NK;
-            atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, plan.ops[a], &lacc[(int64_t)slot * NA + a],
-                                        bitsd((i64)rec[r][j]));
+            atomic_combine_kind<WS_LDS>(slot_kind(kinds, a), &lacc[(int64_t)slot * NA + a], bitsd((i64)rec[j]));
           }
       } else {
-        spill_insert<W>(spill, plan, NK, rec[r], &s_new);
+        spill_insert<W>(spill, kinds, NK, rec, &s_new);
       }
     }
   }
@@ -261,7 +311,7 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, ws_agg_plan pla
   const int K = (S + (int)blockDim.x - 1) / (int)blockDim.x;
   const int s0 = threadIdx.x * K, s1 = s0 + K < S ? s0 + K : S;
   uint32_t n = 0;
-  for (int q = s0; q < s1; ++q) n += lk[q] != kEmpty;
+  for (int qi = s0; qi < s1; ++qi) n += lk[qi] != kEmpty;
   uint32_t incl = n;
   for (int o = 1; o < 64; o <<= 1) {
     const uint32_t y = __shfl_up(incl, o, 64);
@@ -275,19 +325,20 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, ws_agg_plan pla
     if (j < wv) wbase += s_w[j];
     total += s_w[j];
   }
-  if (threadIdx.x == 0) s_base = total ? atomicAdd((unsigned long long*)m.ocount, (unsigned long long)total) : 0;
-  __syncthreads();
-  u64 o = s_base + wbase + incl - n;
-  for (int q = s0; q < s1; ++q) {
-    if (lk[q] == kEmpty) continue;
-    for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = lkv[(int64_t)q * NK + k];
-    for (int a = 0; a < NA; ++a) m.oacc[o * NA + a] = lacc[(int64_t)q * NA + a];
+  // Groups stay inside the partition's own record range (a partition has at least as many
+  // records as groups), so no block waits on another or on a shared counter.
+  if (threadIdx.x == 0) m.gcnt[blockIdx.x] = total;
+  u64 o = (u64)m.pstart[blockIdx.x] + wbase + incl - n;
+  for (int qi = s0; qi < s1; ++qi) {
+    if (lk[qi] == kEmpty) continue;
+    for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = lkv[(int64_t)qi * NK + k];
+    for (int a = 0; a < NA; ++a) m.oacc[o * NA + a] = lacc[(int64_t)qi * NA + a];
     ++o;
   }
 }
 
 // The merge kernel for records of W words (3..16).
-typedef void (*MergeKernel)(PartMerge, ws_agg_plan, GTable);
+typedef void (*MergeKernel)(PartMerge, GTable);
 static MergeKernel merge_kernel(int W) {
   static const MergeKernel k[17] = {nullptr,           nullptr,           nullptr,           part_merge_k<3>,
                                     part_merge_k<4>,   part_merge_k<5>,   part_merge_k<6>,   part_merge_k<7>,
@@ -297,15 +348,17 @@ static MergeKernel merge_kernel(int W) {
   return W >= 3 && W <= 16 ? k[W] : nullptr;
 }
 
-// Moves finished groups (row-major kv / acc, or the occupied slots of a table when keys is set)
-// into a global table: the partitioned result becomes a plain table when more batches follow.
+// Moves finished groups into a global table: rows of a partitioned result (goff set), or the
+// occupied slots of a table (keys set). A partitioned result becomes a plain table this way when
+// more batches follow.
 __global__ void table_absorb_k(GTable dst, ws_agg_plan plan, const double* kv, const double* acc, const u64* keys,
-                               int64_t n) {
+                               const uint32_t* goff, const uint32_t* pstart, int P, int64_t n) {
   __shared__ unsigned s_new;
   if (threadIdx.x == 0) s_new = 0;
   __syncthreads();
   const int NK = plan.nkeys, NA = plan.naggs;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = goff ? part_row(goff, pstart, P, j) : j;
     if (keys && keys[i] == kEmpty) continue;
     u64 h = 0x9e3779b97f4a7c15ull;
     for (int k = 0; k < NK; ++k) h = mix64(h ^ (u64)dbits(kv[i * NK + k])) + (u64)k;
@@ -509,16 +562,19 @@ struct ws_agg {
   // partitioned result: finished groups row-major in okv / oacc (plus any spilled to the table)
   bool part = false, settled = true;
   int64_t part_groups = 0;
-  double* okv = nullptr;
-  double* oacc = nullptr;
-  u64* ocount = nullptr;
+  int P = 0;                    // partitions of the partitioned result
+  double* okv = nullptr;        // [records][nk] groups of partition p from row pstart[p]
+  double* oacc = nullptr;       // [records][na]
+  uint32_t* pstart = nullptr;   // [P + 1] first record row of each partition
+  uint32_t* goff = nullptr;     // [P + 1] first compacted group of each partition; goff[P] = groups
   // fetch state
   int64_t fetched = -1;
   DevBuf<double> fetch_buf;
   void free_part() {
-    for (void* p : {(void*)okv, (void*)oacc, (void*)ocount}) tmp_free(p);
+    for (void* p : {(void*)okv, (void*)oacc, (void*)pstart, (void*)goff}) tmp_free(p);
     okv = oacc = nullptr;
-    ocount = nullptr;
+    pstart = goff = nullptr;
+    P = 0;
     part = false;
     settled = true;
     part_groups = 0;
@@ -586,12 +642,23 @@ static void launch_grouped(void* jf, int pass, bool lean, ScanDesc* ddesc, GTabl
 
 constexpr int kPartMergeBlock = 512;
 constexpr int kPartScanBlock = 1024;          // count / scatter blocks (one chunk each)
-constexpr size_t kPartMergeLds = 60 * 1024;    // merge table per block: 2 blocks per CU
-constexpr size_t kPartScatterLds = 48 * 1024;  // cursors + pre-aggregation table per scatter block
 constexpr int kPartMaxBits = 12;
-constexpr int kChunksPerCu = 3;
-constexpr int64_t kPartAutoRows = 1 << 20;     // automatic choice: at least this many rows and
-constexpr int64_t kPartAutoGroups = 1 << 16;   // this many expected groups
+
+// Tuning knobs of the partitioned plan (set through ws_tune_set; the autotuner's interface).
+struct TuneKnob {
+  const char* name;
+  std::atomic<int64_t> value;
+  int64_t lo, hi;
+};
+TuneKnob g_tune[] = {
+    {"part_merge_lds_kib", {60}, 8, 152},     // merge table per block (KiB): 2 blocks per CU at 60
+    {"part_scatter_lds_kib", {48}, 16, 120},  // cursors + pre-aggregation table per scatter block (KiB)
+    {"part_chunks_per_cu", {1}, 1, 64},       // count / scatter blocks (chunks) per CU
+    {"part_auto_rows", {1 << 20}, 0, (int64_t)1 << 40},   // automatic choice: at least this many rows
+    {"part_auto_groups", {1 << 16}, 0, (int64_t)1 << 40}, // and this many expected groups
+};
+static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
+enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups };
 
 struct PartPlan {
   int pbits = 0;  // 0: plain table path
@@ -610,17 +677,17 @@ static PartPlan plan_partitions(const ws_agg* A, int64_t rows, int64_t ntiles) {
   const int W = 1 + plan.nkeys + plan.naggs;
   if (W > 16) return pp;
   const int64_t expect = A->expected_groups > 0 ? A->expected_groups : std::max<int64_t>(A->cap / 2, 1);
-  if (A->strategy == WS_STRAT_AUTO && (rows < kPartAutoRows || expect < kPartAutoGroups)) return pp;
+  if (A->strategy == WS_STRAT_AUTO && (rows < tune(kTuneAutoRows) || expect < tune(kTuneAutoGroups))) return pp;
   pp.W = W;
-  pp.S = (int)(kPartMergeLds / (8 * (size_t)W));
+  pp.S = (int)((size_t)tune(kTuneMergeLds) * 1024 / (8 * (size_t)W));
   int bits = 1;
   while (bits < kPartMaxBits && ((int64_t)1 << bits) * (pp.S / 2) < expect) ++bits;
   pp.pbits = bits;
   const int P = 1 << bits;
   int pre = 1024;
-  while (pre > 64 && 8 * (size_t)P + (size_t)pre * 8 * W > kPartScatterLds) pre >>= 1;
+  while (pre > 64 && 8 * (size_t)P + (size_t)pre * 8 * W > (size_t)tune(kTuneScatterLds) * 1024) pre >>= 1;
   pp.pre = pre;
-  const int64_t want = std::max<int64_t>(1, (int64_t)cu_count() * kChunksPerCu);
+  const int64_t want = std::max<int64_t>(1, (int64_t)cu_count() * tune(kTuneChunks));
   pp.chunk_tiles = (int)std::max<int64_t>(1, (ntiles + want - 1) / want);
   pp.nchunks = (ntiles + pp.chunk_tiles - 1) / pp.chunk_tiles;
   pp.rows = rows;
@@ -636,14 +703,16 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   if (!jf_count || !jf_scatter) jf_count = jf_scatter = nullptr;
   const bool lean = !jf_count && plan_is_simple(env, plan);
   const int NK = plan.nkeys, NA = plan.naggs, P = 1 << pp.pbits;
-  DevBuf<uint32_t> cnt(pp.nchunks * P), off(pp.nchunks * P), ptot(P), pstart(P + 1);
+  DevBuf<uint32_t> cnt(pp.nchunks * P), off(pp.nchunks * P), ptot(P), gcnt(P);
   DevBuf<u64> recs(pp.rows * pp.W);
   A->okv = (double*)tmp_alloc(sizeof(double) * (size_t)pp.rows * NK);
   A->oacc = (double*)tmp_alloc(sizeof(double) * (size_t)pp.rows * std::max(NA, 1));
-  A->ocount = (u64*)tmp_alloc(sizeof(u64));
+  A->pstart = (uint32_t*)tmp_alloc(sizeof(uint32_t) * (size_t)(P + 1));
+  A->goff = (uint32_t*)tmp_alloc(sizeof(uint32_t) * (size_t)(P + 1));
+  A->P = P;
   A->part = true;
   A->settled = false;
-  fill_u64(A->ocount, 1, 0);
+  uint32_t* pstart = A->pstart;
   // spill table for partitions whose groups outgrow the merge block's LDS table
   // (grows with the caller's capacity, so overflow retries converge even under a bad estimate)
   int64_t spill = 4096;
@@ -656,7 +725,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   d.rec_words = pp.W;
   d.part_cnt = cnt.p;
   d.part_off = off.p;
-  d.part_start = pstart.p;
+  d.part_start = pstart;
   d.recs = recs.p;
   d.part_err = A->flags + 2;
   GTable t = table_of(A);
@@ -670,7 +739,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     ProfScope ps("agg_part_offsets");
     hipLaunchKernelGGL(part_offsets_k, dim3((unsigned)((P + 63) / 64)), dim3(1024), 0, stream(), (const uint32_t*)cnt.p, off.p,
                        ptot.p, pp.nchunks, P);
-    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)ptot.p, pstart.p, P);
+    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)ptot.p, pstart, P);
     check_launch("agg_part_offsets");
   }
   {
@@ -681,10 +750,12 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
                    kPartScanBlock);
   }
   {
-    PartMerge m{recs.p, pstart.p, pp.W, NK, NA, pp.S, A->okv, A->oacc, A->ocount};
+    u64 kinds = 0;
+    for (int a = 0; a < NA; ++a) kinds |= (u64)((plan.ops[a] & 3) | (plan.agg_bits[a] ? 4 : 0)) << (3 * a);
+    PartMerge m{recs.p, pstart, pp.W, NK, NA, pp.S, kinds, A->okv, A->oacc, gcnt.p};
     ProfScope ps("agg_part_merge");
-    hipLaunchKernelGGL(merge_kernel(pp.W), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m,
-                       plan, t);
+    hipLaunchKernelGGL(merge_kernel(pp.W), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m, t);
+    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)gcnt.p, A->goff, P);
     check_launch("agg_part_merge");
   }
 }
@@ -697,9 +768,9 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
 static void settle(ws_agg* A) {
   if (!A->part || A->settled) return;
   int f[4];
-  u64 g = 0;
+  uint32_t g = 0;
   d2h_sync(f, A->flags, sizeof f);
-  d2h_sync(&g, A->ocount, sizeof g);
+  d2h_sync(&g, A->goff + A->P, sizeof g);
   A->settled = true;
   A->part_groups = (int64_t)g;
   if (f[2]) {
@@ -725,10 +796,11 @@ static void part_to_table(ws_agg* A) {
   const GTable t = table_of(A);
   if (A->part_groups)
     hipLaunchKernelGGL(table_absorb_k, dim3(grid_for(A->part_groups)), dim3(256), 0, stream(), t, A->plan, (const double*)A->okv,
-                       (const double*)A->oacc, (const u64*)nullptr, A->part_groups);
+                       (const double*)A->oacc, (const u64*)nullptr, (const uint32_t*)A->goff, (const uint32_t*)A->pstart, A->P,
+                       A->part_groups);
   if (spilled)
     hipLaunchKernelGGL(table_absorb_k, dim3(grid_for(ocap)), dim3(256), 0, stream(), t, A->plan, (const double*)okv,
-                       (const double*)oacc, (const u64*)okeys, ocap);
+                       (const double*)oacc, (const u64*)okeys, (const uint32_t*)nullptr, (const uint32_t*)nullptr, 0, ocap);
   check_launch("table_absorb");
   for (void* p : {(void*)okeys, (void*)okv, (void*)oacc}) tmp_free(p);
   A->free_part();
@@ -873,8 +945,8 @@ static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor) {
   double* ao = buf.p + g * nk;
   ProfScope ps("gather_groups");
   if (g1)
-    hipLaunchKernelGGL(gather_groups_k, dim3(grid_for(g1 * (nk + na))), dim3(256), 0, stream(), (const double*)A->okv, nk,
-                       (const double*)A->oacc, na, (const int32_t*)nullptr, g1, ko, ao, colmajor ? 1 : 0, (int64_t)0, g);
+    hipLaunchKernelGGL(gather_part_k, dim3(grid_for(g1)), dim3(256), 0, stream(), (const double*)A->okv, nk, (const double*)A->oacc,
+                       na, (const uint32_t*)A->goff, (const uint32_t*)A->pstart, A->P, g1, ko, ao, colmajor ? 1 : 0, g);
   if (g2) {
     if (A->is_dense)
       hipLaunchKernelGGL(gather_dense_k, dim3(grid_for(g2 * (nk + na))), dim3(256), 0, stream(), A->dense, A->plan, A->acc, nk,
@@ -1048,6 +1120,27 @@ int ws_agg_set_strategy(ws_agg* a, int strategy, int64_t expected_groups) {
   });
 }
 int ws_agg_partitioned(const ws_agg* a) { return a->part ? 1 : 0; }
+int ws_tune_set(const char* key, int64_t value) {
+  return guard([&] {
+    for (TuneKnob& k : g_tune)
+      if (key && std::strcmp(key, k.name) == 0) {
+        if (value < k.lo || value > k.hi) throw Error(std::string("tune: value out of range for ") + key, WS_ERR_INTERNAL);
+        k.value.store(value, std::memory_order_relaxed);
+        return;
+      }
+    throw Error(std::string("tune: unknown knob ") + (key ? key : "(null)"), WS_ERR_INTERNAL);
+  });
+}
+int ws_tune_get(const char* key, int64_t* value) {
+  return guard([&] {
+    for (TuneKnob& k : g_tune)
+      if (key && std::strcmp(key, k.name) == 0) {
+        *value = k.value.load(std::memory_order_relaxed);
+        return;
+      }
+    throw Error(std::string("tune: unknown knob ") + (key ? key : "(null)"), WS_ERR_INTERNAL);
+  });
+}
 
 // ------------------------------------------------------------------ selection
 

@@ -776,6 +776,8 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
 //   merge   (part_agg_k): one block per partition combines its records in an LDS table and
 //           appends the finished groups to the output.
 // A record is [hash, key values (nk), slot values (na)], 8 bytes each.
+constexpr int kPartRows = 4;  // count pass of a compiled plan: rows per thread per step
+
 template <class Ev>
 __device__ void part_count_body(const ScanDesc& d) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
@@ -797,13 +799,24 @@ __device__ void part_count_body(const ScanDesc& d) {
     }
     const TileRows tr = tile_rows(d, t, b);
     const uint32_t* del = d.del[b];
-    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
-      const i64 r = tr.first + i;
-      if (is_deleted(del, r)) continue;
-      if (ev.has_where() && !ev.where(r)) continue;
-      double kv[Ev::kKeys];
-      const u64 h = row_keys(ev, NK, r, kv);
-      __hip_atomic_fetch_add(&hist[h >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
+    // U rows per thread per step, evaluated before any LDS update so their loads overlap (the
+    // interpreter's out-of-line VM calls gain nothing from it: one row there)
+    constexpr int U = Ev::kRows > 1 ? kPartRows : 1;
+    for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
+      u64 h[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 ii = i + (i64)u * blockDim.x;
+        const i64 r = tr.first + ii;
+        h[u] = kEmpty;
+        if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
+          double kv[Ev::kKeys];
+          h[u] = row_keys(ev, NK, r, kv);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (h[u] != kEmpty) __hip_atomic_fetch_add(&hist[h[u] >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
     }
   }
   __syncthreads();
@@ -850,41 +863,54 @@ __device__ void part_scatter_body(const ScanDesc& d) {
     }
     const TileRows tr = tile_rows(d, t, b);
     const uint32_t* del = d.del[b];
-    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
-      const i64 r = tr.first + i;
-      if (is_deleted(del, r)) continue;
-      if (ev.has_where() && !ev.where(r)) continue;
-      double kv[Ev::kKeys];
-      double av[Ev::kAggs];
-      const u64 h = row_keys(ev, NK, r, kv);
-      row_values(ev, plan, NA, r, av);
-      unsigned s = (unsigned)(h & (u64)(S - 1));
-      bool done = false;
-      for (int probe = 0; probe < 4; ++probe) {
-        u64 prev = kEmpty;
-        const bool claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (claimed)
-          for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[k];
-        if (claimed || prev == h) {
+    // Ev::kRows rows per thread per step: all are evaluated (their loads overlap) before the LDS
+    // table and the record writes see them.
+    constexpr int U = Ev::kRows;
+    for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
+      double kv[U][Ev::kKeys];
+      double av[U][Ev::kAggs];
+      u64 h[U];
 #pragma unroll
-          for (int a = 0; a < Ev::kAggs; ++a) {
-            if (a >= NA) break;
-            atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, ev.op(a), &pacc[(i64)s * NA + a], av[a]);
-          }
-          done = true;
-          break;
+      for (int u = 0; u < U; ++u) {
+        const i64 ii = i + (i64)u * blockDim.x;
+        const i64 r = tr.first + ii;
+        h[u] = kEmpty;
+        if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
+          h[u] = row_keys(ev, NK, r, kv[u]);
+          row_values(ev, plan, NA, r, av[u]);
         }
-        s = (s + 1) & (unsigned)(S - 1);
       }
-      if (done) continue;
-      const unsigned p = (unsigned)(h >> shift);
-      const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
-      if (pos >= end_pos[p]) {
-        *d.part_err = 1;
-        continue;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (h[u] == kEmpty) continue;
+        unsigned s = (unsigned)(h[u] & (u64)(S - 1));
+        bool done = false;
+        for (int probe = 0; probe < 4; ++probe) {
+          u64 prev = kEmpty;
+          const bool claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h[u], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (claimed)
+            for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[u][k];
+          if (claimed || prev == h[u]) {
+#pragma unroll
+            for (int a = 0; a < Ev::kAggs; ++a) {
+              if (a >= NA) break;
+              atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, ev.op(a), &pacc[(i64)s * NA + a], av[u][a]);
+            }
+            done = true;
+            break;
+          }
+          s = (s + 1) & (unsigned)(S - 1);
+        }
+        if (done) continue;
+        const unsigned p = (unsigned)(h[u] >> shift);
+        const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
+        if (pos >= end_pos[p]) {
+          *d.part_err = 1;
+          continue;
+        }
+        put_record(d.recs + (i64)pos * W, h[u], NK, NA, kv[u], av[u]);
       }
-      put_record(d.recs + (i64)pos * W, h, NK, NA, kv, av);
     }
   }
   __syncthreads();

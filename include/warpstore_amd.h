@@ -97,6 +97,14 @@ int ws_agg_sum_high(const ws_agg* a, int64_t* out);       /* sumHigh gpu.h:105 (
 int ws_agg_compact(ws_agg* a, double** keys, double** accs, int64_t* groups);
 int64_t ws_agg_capacity(const ws_agg* a);                  /* capacity   gpu.h:108 */
 int ws_agg_group_count(ws_agg* a, int64_t* out);           /* groupCount gpu.h:109 */
+/* Batched form (no reference counterpart; the reference runs one aggregator at a time, with at
+ * most 4 concurrent queries, engine.cpp:1262-1337): many dense aggregations of one plan shape,
+ * each with its own batches, environment and accumulators, in one launch; and their groups
+ * fetched in one pass (counts[q] groups of aggs[q], keys / accs row-major query after query,
+ * ascending group index; fails with *total set when *total > cap). jit: a compiled dense plan of
+ * the shape, or NULL for the interpreter. Results equal separate runs (double sums up to order). */
+int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit);
+int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs);
 /* Aggregation strategy of a hash GROUP BY (no reference counterpart: the hook through which a
  * sizing model, or a test, chooses the physical plan; results do not depend on it).
  *   WS_STRAT_AUTO: partitioned when the first run covers >= 2^20 rows and >= 2^16 groups are
@@ -111,8 +119,9 @@ int ws_agg_partitioned(const ws_agg* a); /* 1 when the last run produced a parti
 /* Process-wide tuning knobs of the physical plans (no reference counterpart: the autotuner's
  * interface; results do not depend on them). Keys: "part_merge_lds_kib" (8..152, default 60),
  * "part_scatter_lds_kib" (16..120, default 48), "part_chunks_per_cu" (1..64, default 1),
- * "part_auto_rows" (default 2^20), "part_auto_groups" (default 2^16). Unknown keys and values
- * out of range return WS_ERR_INTERNAL. */
+ * "part_auto_rows" (default 2^20), "part_auto_groups" (default 2^16), "batch_tile_rows" (256..2^20,
+ * default 4096), "batch_lds_slots" (64..8192, default 1024), "batch_blocks_per_cu" (1..32,
+ * default 4). Unknown keys and values out of range return WS_ERR_INTERNAL. */
 int ws_tune_set(const char* key, int64_t value);
 int ws_tune_get(const char* key, int64_t* value);
 
@@ -132,6 +141,30 @@ typedef struct ws_col_stats {
 int ws_column_stats(const ws_colview* col, int64_t n, const uint32_t* del, ws_col_stats* out); /* one batch */
 void ws_col_stats_merge(ws_col_stats* dst, const ws_col_stats* src);
 double ws_hll_estimate(const uint8_t* regs); /* distinct-count estimate of 2048 registers */
+
+/* ------------------------------------------------------------------ learned sizing
+ * No reference counterpart (the reference sizes from key ranges, engine_select.cpp:1154-1235): an
+ * MLP 16 -> 32 -> 32 -> 1 (ReLU, fp32) predicting log2(groups + 1) / 32 from ws_sizing_features,
+ * trained on the GPU (forward, backward and Adam kernels) on exact counts the aggregations log.
+ * Parameter layout: W1[32][16], b1[32], W2[32][32], b2[32], w3[32], b3. */
+#define WS_SIZER_INPUTS 16
+#define WS_SIZER_PARAMS 1633
+typedef struct ws_sizer ws_sizer;
+int ws_sizer_create(uint64_t seed, ws_sizer** out); /* He-uniform weights from the seed */
+void ws_sizer_destroy(ws_sizer* s);
+int ws_sizer_get_params(const ws_sizer* s, float* out);
+int ws_sizer_set_params(ws_sizer* s, const float* in);
+int ws_sizer_predict(ws_sizer* s, const float* x, int64_t n, float* y);     /* x: n x 16, host */
+int ws_sizer_loss_grad(ws_sizer* s, const float* x, const float* t, int64_t n, float* loss, float* grad); /* MSE, dL/dparams */
+int ws_sizer_train(ws_sizer* s, const float* x, const float* t, int64_t n, int epochs, int batch, float lr, uint64_t seed,
+                   float* final_loss);
+int ws_sizer_groups(ws_sizer* s, const float* features, int64_t* groups);  /* predicted group count */
+int ws_sizing_features(const ws_col_stats* keys, int nkeys, const int64_t* dict_sizes, int64_t rows, double selectivity,
+                       float* out);                                           /* 16 features of a GROUP BY */
+int ws_agg_set_features(ws_agg* a, const float* features); /* log (features, exact groups) after the run */
+int64_t ws_sizing_log_size(void);
+int64_t ws_sizing_log_copy(float* x, float* t, int64_t cap); /* x: cap x 16; t: log2(groups + 1) / 32 */
+void ws_sizing_log_clear(void);
 
 /* ------------------------------------------------------------------ selection / gathers */
 int ws_select(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, const uint32_t* del, int32_t* out_idx,

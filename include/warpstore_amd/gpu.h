@@ -168,6 +168,9 @@ class GpuAggregator {
   void runWindow(const ExprEnv& env, const JitProgram* jit) {
     wsd::ck(ws_agg_run_window(h_, wsd::env(env), wsd::handle(jit)));
   }
+  // The C ABI handle (for the batched form, ws_agg_run_batch / ws_agg_fetch_many; no reference
+  // counterpart).
+  ws_agg* handle() const { return h_; }
   int batches() const { return ws_agg_batches(h_); }
   int windows() const { return ws_agg_windows(h_); }
   int64_t rowsIn() const { return ws_agg_rows_in(h_); }

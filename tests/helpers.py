@@ -197,7 +197,7 @@ LAST_RUN = {"partitioned": False, "retries": 0}
 
 
 def gpu_aggregate(plan, env, bound: Bound, cap: int = 1 << 16, dense=None, jit=None, fetch=True, windows=False,
-                  strategy=None, expected=0):
+                  strategy=None, expected=0, features=None):
     """Runs GpuAggregator through the C ABI, retrying bigger on overflow like the engine
     (engine_select.cpp:1514-1522). Returns (keys g x nk, accs g x na, hi, cap used).
     strategy: None (library default) or ws_agg_set_strategy's value, with `expected` groups.
@@ -209,6 +209,8 @@ def gpu_aggregate(plan, env, bound: Bound, cap: int = 1 << 16, dense=None, jit=N
         ws.check(L.ws_agg_create(C.byref(plan), cap, C.byref(dense) if dense is not None else None, C.byref(h)))
         if strategy is not None:
             ws.check(L.ws_agg_set_strategy(h, strategy, expected))
+        if features is not None:
+            ws.check(L.ws_agg_set_features(h, ws.ptr(features)))
         try:
             for b in range(len(bound.t.sizes)):
                 e = env_with_batch(env, bound, b)

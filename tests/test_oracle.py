@@ -375,3 +375,60 @@ def test_column_stats_merge_nulls_signed_zero():
     szn = _stats(np.array([-0.0]), abi.T_F64)
     assert bytes(sz.hll) == bytes(szn.hll)
     assert round(L.ws_hll_estimate(sd.hll)) == 3  # {0, 1.5, -2}
+
+
+# ------------------------------------------------------------------ sizing model (MLP) oracle
+
+def _mlp():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mlp_oracle", os.path.join(H.ROOT, "oracle", "mlp_oracle.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The oracle's analytic gradient equals central finite differences of its loss (float64),
+    for every one of the 1633 parameters; a dropped ReLU mask, a transposed weight or a missing
+    1/B would fail it."""
+    M = _mlp()
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.3, M.N_PARAMS)
+    x = rng.random((24, M.N_IN))
+    t = rng.random(24)
+    g = M.grad(p, x, t)
+    eps = 1e-6
+    fd = np.empty_like(p)
+    for i in range(M.N_PARAMS):
+        q = p.copy()
+        q[i] += eps
+        lp = M.loss(q, x, t)
+        q[i] -= 2 * eps
+        lm = M.loss(q, x, t)
+        fd[i] = (lp - lm) / (2 * eps)
+    assert np.allclose(g, fd, rtol=1e-5, atol=1e-8), np.abs(g - fd).max()
+    assert M.N_PARAMS == 1633
+
+
+def test_mlp_oracle_adam_closed_form():
+    """First Adam step with bias correction moves each parameter by lr * sign(g) (|g| >> eps)."""
+    M = _mlp()
+    rng = np.random.default_rng(4)
+    p = rng.normal(0, 1, 50)
+    g = rng.normal(0, 1, 50)
+    q, m, v = M.adam(p, g, np.zeros(50), np.zeros(50), 1, 0.01)
+    assert np.allclose(p - q, 0.01 * np.sign(g), atol=1e-6)
+
+
+def test_sizing_features_basic():
+    """Feature builder: selectivity, key count and distinct-count features from known stats."""
+    L = ws.lib()
+    vals = np.arange(5000, dtype=np.int64) * 3 + 100
+    st = H.oracle_column_stats(H.host_view(vals, abi.T_I64), len(vals))
+    f = np.zeros(16, np.float32)
+    ws.check(L.ws_sizing_features(C.byref(st), 1, None, 1 << 20, 0.25, ws.ptr(f)))
+    assert abs(f[0] - np.log2(2 ** 20 + 1) / 32) < 1e-6 and abs(f[1] - 0.25) < 1e-7
+    assert abs(f[3] * 32 - np.log2(5001)) < 0.1  # HyperLogLog within a few %
+    assert abs(f[4] * 32 - np.log2(3 * 4999 + 2)) < 1e-4  # integer span
+    assert f[11] == 0.25 and f[5] == 0 and f[6] == 0
+    assert ws.lib().ws_sizing_features(C.byref(st), 5, None, 10, 1.0, ws.ptr(f)) != 0

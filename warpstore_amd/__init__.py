@@ -59,6 +59,14 @@ _SIGS = {
     "ws_tune_set": (I, [C.c_char_p, I64]), "ws_tune_get": (I, [C.c_char_p, C.POINTER(I64)]),
     "ws_column_stats": (I, [P, I64, P, P]), "ws_col_stats_merge": (None, [P, P]),
     "ws_hll_estimate": (C.c_double, [P]),
+    "ws_sizer_create": (I, [C.c_uint64, PP]), "ws_sizer_destroy": (None, [P]), "ws_sizer_get_params": (I, [P, P]),
+    "ws_sizer_set_params": (I, [P, P]), "ws_sizer_predict": (I, [P, P, I64, P]),
+    "ws_sizer_loss_grad": (I, [P, P, P, I64, P, P]),
+    "ws_sizer_train": (I, [P, P, P, I64, I, I, C.c_float, C.c_uint64, P]),
+    "ws_sizer_groups": (I, [P, P, C.POINTER(I64)]), "ws_sizing_features": (I, [P, I, P, I64, C.c_double, P]),
+    "ws_agg_set_features": (I, [P, P]), "ws_agg_run_batch": (I, [I, P, P, P]),
+    "ws_agg_fetch_many": (I, [I, P, P, C.POINTER(I64), I64, P, P]), "ws_sizing_log_size": (I64, []), "ws_sizing_log_copy": (I64, [P, P, I64]),
+    "ws_sizing_log_clear": (None, []),
     "ws_select": (I, [C.POINTER(ExprEnv), I, Prog, I64, P, P, P, C.POINTER(I64)]),
     "ws_select_multi": (I, [C.POINTER(ExprEnv), I, Prog, I, P, P, P, I, PP, P, P, C.POINTER(I64)]),
     "ws_gather_multi": (I, [P, P, I, I, I, P, I64, P]), "ws_remap_codes": (I, [P, I64, P, I64, P]),

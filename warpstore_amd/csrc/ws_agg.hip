@@ -10,7 +10,7 @@
 
 // Ahead-of-time kernels: the shared bodies with the bytecode interpreter.
 WS_KERNEL_NOKEYS(ws::InterpEval, ws_i_nokeys)
-WS_KERNEL_DENSE(ws::InterpEval, ws_i_dense)
+WS_KERNEL_DENSE(ws::InterpEval, ws_i_dense)  // also ws_i_dense_batch
 WS_KERNEL_GROUPED(ws::InterpEval, ws_i_grouped)
 WS_KERNEL_SELECT(ws::InterpEval, ws_i_select)
 WS_KERNEL_NOKEYS(ws::InterpEvalLean, ws_l_nokeys)
@@ -63,6 +63,62 @@ __global__ void gather_groups_k(const double* kv, int nk, const double* acc, int
       if (colmajor) accs_out[(int64_t)a * G + o] = v;
       else accs_out[o * na + a] = v;
     }
+  }
+}
+
+// ------------------------------------------------------------------ batched dense fetch
+
+struct FetchQuery {
+  const double* acc;    // [G][NA]
+  ws_dense_spec ds;     // keys are rebuilt from the group index
+  int64_t G;
+};
+// Block q: number of groups of query q with rows (slot 0 = COUNT(*) > 0).
+__global__ void __launch_bounds__(256) dense_count_many_k(const FetchQuery* qs, int na, uint32_t* counts) {
+  __shared__ uint32_t s[256];
+  const FetchQuery& Q = qs[blockIdx.x];
+  uint32_t c = 0;
+  for (int64_t g = threadIdx.x; g < Q.G; g += blockDim.x) c += Q.acc[g * na] > 0;
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = s[0];
+}
+// Block q: its groups with rows, ascending group index, row-major keys | accs from row offs[q].
+__global__ void __launch_bounds__(256) dense_pack_many_k(const FetchQuery* qs, ws_agg_plan plan, int nk, int na,
+                                                         const uint32_t* offs, double* out) {
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_base;
+  const FetchQuery& Q = qs[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_base = offs[blockIdx.x];
+  __syncthreads();
+  for (int64_t g0 = 0; g0 < Q.G; g0 += blockDim.x) {
+    const int64_t g = g0 + threadIdx.x;
+    const bool has = g < Q.G && Q.acc[g * na] > 0;
+    const u64 m = __ballot(has);
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    if (lane == 0) s_w[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int j = 0; j < 4; ++j) {
+      if (j < wv) before += s_w[j];
+      total += s_w[j];
+    }
+    if (has) {
+      double* row = out + (int64_t)(s_base + before + rank) * (nk + na);
+      for (int k = 0; k < nk; ++k) {
+        const i64 v = Q.ds.kmin[k] + (g / Q.ds.stride[k]) % Q.ds.range[k];
+        row[k] = plan.key_int[k] ? bitsd(v) : (double)v;
+      }
+      for (int a = 0; a < na; ++a) row[nk + a] = Q.acc[g * na + a];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base += total;
+    __syncthreads();
   }
 }
 
@@ -557,6 +613,8 @@ struct ws_agg {
   // strategy: WS_STRAT_AUTO / WS_STRAT_TABLE / WS_STRAT_PARTITIONED, and the expected group count
   int strategy = WS_STRAT_AUTO;
   int64_t expected_groups = 0;
+  float features[WS_SIZER_INPUTS] = {};     // sizing features; logged with the exact group count
+  bool has_features = false, logged = false;
   ws_expr_env last_env{};                  // the last run's environment and plan (for a rerun)
   const ws_jit_program* last_jit = nullptr;
   // partitioned result: finished groups row-major in okv / oacc (plus any spilled to the table)
@@ -656,9 +714,13 @@ TuneKnob g_tune[] = {
     {"part_chunks_per_cu", {1}, 1, 64},       // count / scatter blocks (chunks) per CU
     {"part_auto_rows", {1 << 20}, 0, (int64_t)1 << 40},   // automatic choice: at least this many rows
     {"part_auto_groups", {1 << 16}, 0, (int64_t)1 << 40}, // and this many expected groups
+    {"batch_tile_rows", {4096}, 256, 1 << 20},  // batched dense run: rows per tile
+    {"batch_lds_slots", {1024}, 64, 8192},      // its LDS table slots per block
+    {"batch_blocks_per_cu", {4}, 1, 32},        // its grid, per CU
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
-enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups };
+enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
+       kTuneBatchBlocksPerCu };
 
 struct PartPlan {
   int pbits = 0;  // 0: plain table path
@@ -1038,7 +1100,18 @@ int ws_agg_overflowed(ws_agg* a, int* out) {
     d2h_sync(f, a->flags, sizeof f);
     if (f[1]) throw Error("dense GROUP BY key outside the planned key range", WS_ERR_INTERNAL);
     if (!a->is_dense) *out = f[0] != 0;
+    if (!*out && a->has_features && !a->logged && !a->is_dense) {  // the exact count, for the sizing model
+      u64 c = 0;
+      if (a->keys) d2h_sync(&c, a->count, sizeof c);
+      sizing_log(a->features, (int64_t)c + (a->part ? a->part_groups : 0));
+      a->logged = true;
+    }
   });
+}
+int ws_agg_set_features(ws_agg* a, const float* features) {
+  std::memcpy(a->features, features, sizeof a->features);
+  a->has_features = true;
+  return WS_OK;
 }
 int ws_agg_fetch_begin(ws_agg* a, int64_t* groups) {
   return guard([&] {
@@ -1103,6 +1176,131 @@ int ws_agg_compact(ws_agg* a, double** keys, double** accs, int64_t* groups) {
     *groups = g;
   });
 }
+// Many dense aggregations of one plan shape in one launch (see batch_dense_body). Each aggregator
+// keeps its own batches, environment (constants, columns) and accumulators.
+int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit) {
+  return guard([&] {
+    if (n <= 0) return;
+    const ws_agg* A0 = aggs[0];
+    const int nk = A0->plan.nkeys, na = A0->plan.naggs;
+    bool simple = true;
+    for (int q = 0; q < n; ++q) {
+      const ws_agg* A = aggs[q];
+      if (!A->is_dense || A->plan.nkeys != nk || A->plan.naggs != na)
+        throw Error("batched run: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
+      simple = simple && plan_is_simple(*envs[q], A->plan);
+    }
+    if (!jit && (na > WS_INTERP_AGGS || nk > WS_INTERP_KEYS))
+      throw Error("this aggregation needs a compiled plan (more than 16 aggregates or 4 GROUP BY keys); codegen failed or is off",
+                  WS_ERR_NEEDS_PLAN);
+    void* jf = jit ? jit_function(jit, kJitBatchDense) : nullptr;
+    const bool lean = !jf && simple;
+    const int tile_rows = (int)tune(kTuneBatchTile);
+    // blob: [BatchQuery n][BatchTile T] then per query plan, dense spec, env, n[], del[], cols[], tile0[], row0[]
+    std::vector<BatchTile> tiles;
+    for (int q = 0; q < n; ++q)
+      for (size_t b = 0; b < aggs[q]->bn.size(); ++b)
+        for (int64_t f = 0; f < aggs[q]->bn[b]; f += tile_rows)
+          tiles.push_back(BatchTile{q, (int32_t)b, (int32_t)std::min<int64_t>(tile_rows, aggs[q]->bn[b] - f), 0, f});
+    if (tiles.empty()) return;
+    Blob blob;
+    const size_t o_q = blob.reserve(sizeof(BatchQuery) * n);
+    const size_t o_t = blob.put(tiles.data(), sizeof(BatchTile) * tiles.size());
+    std::vector<size_t> o_plan(n), o_ds(n), o_env(n), o_n(n), o_del(n), o_cols(n), o_t0(n);
+    for (int q = 0; q < n; ++q) {
+      const ws_agg* A = aggs[q];
+      o_plan[q] = blob.put(&A->plan, sizeof A->plan);
+      o_ds[q] = blob.put(&A->dense, sizeof A->dense);
+      o_env[q] = blob.put(envs[q], sizeof(ws_expr_env));
+      o_n[q] = blob.put(A->bn.data(), 8 * A->bn.size());
+      o_del[q] = blob.put(A->bdel.data(), 8 * A->bdel.size());
+      o_cols[q] = blob.put(A->bcols.data(), sizeof(ws_colview) * A->bcols.size());
+      std::vector<int64_t> t0(A->bn.size() + 1, 0);
+      o_t0[q] = blob.put(t0.data(), 8 * t0.size());
+    }
+    uint8_t* dev = (uint8_t*)tmp_alloc(blob.host.size());
+    for (int q = 0; q < n; ++q) {
+      ws_agg* A = aggs[q];
+      BatchQuery bq{};
+      ScanDesc& d = bq.d;
+      d.nb = (int32_t)A->bn.size();
+      d.ncols = A->ncols;
+      d.tile_rows = tile_rows;
+      d.n = (const int64_t*)(dev + o_n[q]);
+      d.del = (const uint32_t* const*)(dev + o_del[q]);
+      d.cols = (const ws_colview*)(dev + o_cols[q]);
+      d.tile0 = (const int64_t*)(dev + o_t0[q]);
+      d.row0 = d.tile0;
+      d.env = (const ws_expr_env*)(dev + o_env[q]);
+      d.plan = (const ws_agg_plan*)(dev + o_plan[q]);
+      d.ds = (const ws_dense_spec*)(dev + o_ds[q]);
+      bq.gacc = A->acc;
+      bq.flags = A->flags;
+      *blob.at<BatchQuery>(o_q + sizeof(BatchQuery) * q) = bq;
+      A->fetched = -1;
+      ++A->runs;
+    }
+    h2d_async_staged(dev, blob.host.data(), blob.host.size());
+    const int S = (int)tune(kTuneBatchSlots);
+    const size_t lds = (((size_t)S * 4 + 15) & ~(size_t)15) + (size_t)S * na * 8;
+    const unsigned grid =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (int64_t)cu_count() * tune(kTuneBatchBlocksPerCu)));
+    const BatchQuery* dq = (const BatchQuery*)(dev + o_q);
+    const BatchTile* dt = (const BatchTile*)(dev + o_t);
+    long long nt = (long long)tiles.size();
+    int sl = S;
+    if (jf) {
+      void* args[] = {&dq, &dt, &nt, &sl};
+      jit_launch_raw(jf, grid, kBlock, lds, args, "agg_batch_dense_jit");
+    } else {
+      ProfScope ps("agg_batch_dense");
+      hipLaunchKernelGGL(lean ? ws_l_dense_batch : ws_i_dense_batch, dim3(grid), dim3(kBlock), lds, stream(), dq, dt, nt, sl);
+      check_launch("agg_batch_dense");
+    }
+    tmp_free(dev);
+  });
+}
+
+// Groups of many dense aggregations in one pass: counts[q] = groups of aggs[q]; keys (row-major,
+// total x nkeys) and accs (total x naggs) hold them query after query, ascending group index.
+// Fails (with counts set and *total the size needed) when the total exceeds cap.
+int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs) {
+  return guard([&] {
+    *total = 0;
+    if (n <= 0) return;
+    const int nk = aggs[0]->plan.nkeys, na = aggs[0]->plan.naggs;
+    std::vector<FetchQuery> fq(n);
+    for (int q = 0; q < n; ++q) {
+      const ws_agg* A = aggs[q];
+      if (!A->is_dense || A->plan.nkeys != nk || A->plan.naggs != na)
+        throw Error("batched fetch: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
+      fq[q] = FetchQuery{A->acc, A->dense, A->dense.G};
+    }
+    DevBuf<FetchQuery> dq(n);
+    h2d_async_staged(dq.p, fq.data(), sizeof(FetchQuery) * n);
+    DevBuf<uint32_t> cnt(n), offs(n + 1);
+    hipLaunchKernelGGL(dense_count_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)dq.p, na, cnt.p);
+    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)cnt.p, offs.p, n);
+    check_launch("fetch_many_count");
+    std::vector<uint32_t> h(n + 1);
+    d2h_sync(h.data(), offs.p, sizeof(uint32_t) * (n + 1));
+    for (int q = 0; q < n; ++q) counts[q] = (int64_t)(h[q + 1] - h[q]);
+    *total = h[n];
+    if (*total > cap) throw Error("batched fetch: more groups than the buffers hold", WS_ERR_INTERNAL);
+    if (!*total) return;
+    DevBuf<double> packed(*total * (nk + na));
+    hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)dq.p, aggs[0]->plan, nk,
+                       na, (const uint32_t*)offs.p, packed.p);
+    check_launch("fetch_many_pack");
+    std::vector<double> hp(*total * (nk + na));
+    d2h_sync(hp.data(), packed.p, sizeof(double) * hp.size());
+    for (int64_t i = 0; i < *total; ++i) {
+      std::memcpy(keys + i * nk, &hp[i * (nk + na)], sizeof(double) * nk);
+      std::memcpy(accs + i * na, &hp[i * (nk + na) + nk], sizeof(double) * na);
+    }
+  });
+}
+
 int64_t ws_agg_capacity(const ws_agg* a) { return a->cap; }
 int ws_agg_group_count(ws_agg* a, int64_t* out) {
   return guard([&] {

@@ -30,6 +30,7 @@ struct ws_jit_program {
   hipModule_t module = nullptr;
   hipFunction_t fn = nullptr;
   hipFunction_t fn_part[2] = {nullptr, nullptr};  // grouped plans: partition count / scatter kernels
+  hipFunction_t fn_batch = nullptr;               // dense plans: the batched (many queries) kernel
   std::atomic<int> state{0};  // 0 compiling, 1 ready, 2 failed
   bool from_disk = false;
   int rows = 1;
@@ -404,6 +405,7 @@ void load_image(ws_jit_program& p, const std::string& image) {
   hipFunction_t f;
   WS_CK(hipModuleGetFunction(&f, mod, kKernelName[p.fn_kind]));
   p.fn = f;
+  if (p.fn_kind == WS_JF_DENSE) WS_CK(hipModuleGetFunction(&p.fn_batch, mod, (std::string(kKernelName[p.fn_kind]) + "_batch").c_str()));
   if (p.fn_kind == WS_JF_GROUPED) {
     const std::string base = kKernelName[p.fn_kind];
     WS_CK(hipModuleGetFunction(&p.fn_part[0], mod, (base + "_pcount").c_str()));
@@ -490,6 +492,7 @@ void* jit_function(const ws_jit_program* p, int fn) {
   if (!p || p->state.load(std::memory_order_acquire) != 1) return nullptr;
   if (fn == kJitPartCount || fn == kJitPartScatter)
     return p->fn_kind == WS_JF_GROUPED ? (void*)p->fn_part[fn == kJitPartScatter ? 1 : 0] : nullptr;
+  if (fn == kJitBatchDense) return p->fn_kind == WS_JF_DENSE ? (void*)p->fn_batch : nullptr;
   const bool sel = (fn == WS_JF_FLAG || fn == WS_JF_FLAGM) && (p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM);
   return (p->fn_kind == fn || sel) ? (void*)p->fn : nullptr;
 }

@@ -761,6 +761,105 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
   }
 }
 
+// ------------------------------------------------------------------ batched dense GROUP BY
+
+// Many dense aggregations (one plan shape, each with its own batches, constants and table) in one
+// launch: the host lists every tile of every query; blocks take tiles in a grid-stride loop (no
+// queue counter) and pre-aggregate each tile in an LDS table keyed by the dense group index, then
+// fold it into the query's accumulators with device-scope atomics. Values are combined in the
+// final slot encoding of the shared-table dense layout (int64 bits for agg_bits slots, doubles
+// otherwise), so the results equal a plain run of each query (up to double summation order).
+struct BatchQuery {
+  ScanDesc d;    // batches, column views, env, plan and dense spec of the query
+  double* gacc;  // [G][NA] accumulators
+  int* flags;    // [1] dense key out of range
+};
+struct BatchTile {
+  int32_t q, b;   // query, batch of the query
+  int32_t count;  // rows
+  int32_t pad;
+  int64_t first;  // first row in the batch
+};
+
+template <int SCOPE>
+__device__ __forceinline__ void dense_fold(const ws_agg_plan& plan, uint8_t k, int a, double* dst, i64 iv, double dv) {
+  if (plan.agg_bits[a]) atomic_combine_i<SCOPE>(dk_op(k), (i64*)dst, iv);
+  else if (dk_int(k)) atomic_combine_d<SCOPE>(a == 0 ? WS_AGG_SUM : dk_op(k), dst, (double)iv);
+  else atomic_combine_d<SCOPE>(dk_op(k), dst, dv);
+}
+
+template <class Ev>
+__device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i64 ntiles, int S) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  int* lg = (int*)ws_smem;                                    // [S] group index, -1 = free
+  double* lacc = (double*)(ws_smem + (((i64)S * 4 + 15) & ~(i64)15));  // [S][NA]
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const BatchTile bt = tiles[t];
+    const BatchQuery& Q = qs[bt.q];
+    const ScanDesc& d = Q.d;
+    const ws_agg_plan& plan = *d.plan;
+    const ws_dense_spec& ds = *d.ds;
+    Ev ev(d);
+    const int NK = ev.nkeys(), NA = ev.naggs();
+    __syncthreads();  // the previous tile's fold has read the table
+    for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)bt.b * d.ncols + i];
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+      lg[i] = -1;
+      for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, dk_op(ds.kind[a]));
+    }
+    __syncthreads();
+    ev.bind(scols);
+    const uint32_t* del = d.del[bt.b];
+    bool bad = false;
+    for (int i = threadIdx.x; i < bt.count; i += blockDim.x) {
+      const i64 r = bt.first + i;
+      if (is_deleted(del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      const i64 g = dense_index(ev, ds, NK, r);
+      if (g < 0) {
+        bad = true;
+        continue;
+      }
+      int s = (int)(g % S), slot = -1;
+      for (int probe = 0; probe < 8; ++probe) {
+        int prev = __hip_atomic_load(&lg[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prev == -1 && __hip_atomic_compare_exchange_strong(&lg[s], &prev, (int)g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+          prev = (int)g;
+        if (prev == (int)g) {
+          slot = s;
+          break;
+        }
+        s = s + 1 == S ? 0 : s + 1;
+      }
+      double* dst = slot >= 0 ? lacc + (i64)slot * NA : Q.gacc + g * NA;
+#pragma unroll
+      for (int a = 0; a < Ev::kAggs; ++a) {
+        if (a >= NA) break;
+        const uint8_t k = ev.dkind(a, ds);
+        i64 iv = 0;
+        double dv = 0;
+        dense_value(ev, plan, k, a, r, iv, dv);
+        if (slot >= 0) dense_fold<WS_LDS>(plan, k, a, &dst[a], iv, dv);
+        else dense_fold<WS_DEV>(plan, k, a, &dst[a], iv, dv);
+      }
+    }
+    if (bad) Q.flags[1] = 1;
+    __syncthreads();
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+      const int g = lg[i];
+      if (g < 0) continue;
+      for (int a = 0; a < NA; ++a) {
+        const double v = lacc[(i64)i * NA + a];
+        const uint8_t k = ds.kind[a];
+        if (plan.agg_bits[a]) atomic_combine_i<WS_DEV>(dk_op(k), (i64*)&Q.gacc[(i64)g * NA + a], dbits(v));
+        else atomic_combine_d<WS_DEV>(a == 0 ? WS_AGG_SUM : dk_op(k), &Q.gacc[(i64)g * NA + a], v);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ partitioned hash GROUP BY
 
 // High-cardinality GROUP BY without global atomics on a shared table. Three passes over a plan:
@@ -883,14 +982,18 @@ __device__ void part_scatter_body(const ScanDesc& d) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (h[u] == kEmpty) continue;
+        // Two probes; a plain LDS read first, so a full table costs reads, not compare-and-swaps.
         unsigned s = (unsigned)(h[u] & (u64)(S - 1));
         bool done = false;
-        for (int probe = 0; probe < 4; ++probe) {
-          u64 prev = kEmpty;
-          const bool claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h[u], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (claimed)
-            for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[u][k];
+        for (int probe = 0; probe < 2; ++probe) {
+          u64 prev = __hip_atomic_load(&pk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          bool claimed = false;
+          if (prev == kEmpty) {
+            claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h[u], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (claimed)
+              for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[u][k];
+          }
           if (claimed || prev == h[u]) {
 #pragma unroll
             for (int a = 0; a < Ev::kAggs; ++a) {
@@ -1106,6 +1209,11 @@ __device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, i
 // ------------------------------------------------------------------ kernel entry points
 // The AOT build instantiates every kernel with InterpEval; a compiled plan instantiates only the one
 // it needs with its generated evaluator (extern "C" so the module symbol has a fixed name).
+#define WS_KERNEL_BATCH_DENSE(EV, NAME)                                                                     \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const ws::BatchQuery* qs, const ws::BatchTile* tiles,  \
+                                                         long long ntiles, int lds_slots) {                  \
+    ws::batch_dense_body<EV>(qs, tiles, ntiles, lds_slots);                                                 \
+  }
 #define WS_KERNEL_NOKEYS(EV, NAME)                                                                          \
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, double* out,               \
                                                          ws::NoKeysPartial* parts, unsigned* done) {        \
@@ -1115,7 +1223,8 @@ __device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, i
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, double* gacc, long long* parts, \
                                                          int* flags) {                                       \
     ws::dense_body<EV>(*d, gacc, parts, flags);                                                              \
-  }
+  }                                                                                                          \
+  WS_KERNEL_BATCH_DENSE(EV, NAME##_batch)
 #define WS_KERNEL_GROUPED(EV, NAME)                                                                         \
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, ws::GTable gt) {            \
     ws::grouped_body<EV>(*d, gt);                                                                            \

@@ -13,6 +13,11 @@
 #define WARPSTORE_AMD_STANDALONE 1
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -24,6 +29,10 @@
 #include "warpstore_amd/jit.h"
 
 extern "C" int wsg_encode_int(const void* x, int i64, int64_t n, uint8_t* out);
+extern "C" void wsg_c4_shapes(uint64_t seed, int64_t nq, int64_t* rows, int64_t* distinct, int64_t* theta);
+extern "C" void wsg_c4_data(uint64_t seed, int64_t q, int64_t rows, int64_t distinct, int32_t* key, int64_t* val);
+extern "C" int wsq_table_load(int ncols, const int* types, const int* encode, int64_t n, const void* const* host, void** out);
+extern "C" const char* wsq_last_error(void);
 
 namespace {
 
@@ -548,6 +557,333 @@ int guard(F&& f) {
   }
 }
 
+// ------------------------------------------------------------------ C4: many small aggregations
+
+// SELECT key, COUNT(*), SUM(val), MIN(val), MAX(val) FROM seg_q WHERE val > theta_q GROUP BY key
+// over its own table per query (columns 0 key INT, 1 val BIGINT). Every query has the same plan
+// shape (theta is a constant of the environment), so one compiled plan serves all of them.
+struct C4Query {
+  const DevTable* t = nullptr;
+  int64_t theta = 0;
+};
+struct C4Plan {
+  Comp c;
+  AggPlan plan;
+  DenseSpec ds;
+  bool dense = false;
+  int64_t cap = 1024;
+};
+const std::vector<int> kC4Slots = {0, 1};
+
+C4Plan c4_plan(const C4Query& q) {
+  C4Plan p;
+  const int sk = p.c.slot(WS_T_I32), sv = p.c.slot(WS_T_I64);
+  std::memset(&p.plan, 0, sizeof p.plan);
+  p.plan.has_where = 1;
+  p.plan.where = p.c.prog({p.c.col(sv), p.c.ilit(q.theta), Comp::op(WS_OP_GT)});
+  p.plan.nkeys = 1;
+  p.plan.keys[0] = p.c.prog({p.c.col(sk)});
+  p.plan.key_int[0] = 1;
+  const uint8_t ops[] = {WS_AGG_COUNT, WS_AGG_SUM, WS_AGG_SUM, WS_AGG_MIN, WS_AGG_MAX};
+  p.plan.naggs = 5;
+  for (int a = 0; a < 5; ++a) p.plan.ops[a] = ops[a];
+  p.plan.agg_int[0] = 1;
+  for (int a = 1; a < 5; ++a) {
+    p.plan.aggs[a] = p.c.prog({p.c.col(sv)});
+    p.plan.agg_int[a] = p.plan.agg_bits[a] = 1;
+  }
+  p.plan.agg_part[1] = 1, p.plan.agg_part[2] = 2;
+  const DevColumn& k = q.t->cols[0];
+  p.dense = plan_dense(p.plan, {{k.mn, k.mx}}, q.t->rows, p.ds);
+  p.plan.dense_priv = p.dense ? (p.ds.priv ? 1 : 2) : 0;
+  const double est = std::min<double>((double)(k.mx - k.mn + 1), (double)std::max<int64_t>(q.t->rows, 1));
+  while (p.cap < est * 2) p.cap <<= 1;
+  return p;
+}
+
+// Order-independent digest of one query's result rows (key, count, sum, min, max): equal digests
+// mean equal row sets.
+uint64_t c4_digest(const std::vector<double>& keys, const std::vector<double>& accs, int64_t g) {
+  uint64_t d = 0x12345678ull;
+  for (int64_t i = 0; i < g; ++i) {
+    int64_t key, lo, hi, mn, mx;
+    std::memcpy(&key, &keys[i], 8);
+    std::memcpy(&lo, &accs[i * 5 + 1], 8);
+    std::memcpy(&hi, &accs[i * 5 + 2], 8);
+    std::memcpy(&mn, &accs[i * 5 + 3], 8);
+    std::memcpy(&mx, &accs[i * 5 + 4], 8);
+    const int64_t cnt = (int64_t)accs[i * 5];
+    const int64_t sum = hi * (int64_t)4294967296ll + lo;
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    for (int64_t v : {key, cnt, sum, mn, mx}) {
+      h ^= (uint64_t)v;
+      h *= 0xbf58476d1ce4e5b9ull;
+      h ^= h >> 31;
+    }
+    d += h;  // sum of row hashes: independent of row order
+  }
+  return d + (uint64_t)g;
+}
+
+// One query alone through the plain per-query path (the engine's aggregation driver).
+uint64_t c4_alone(const C4Query& q, int64_t* groups) {
+  C4Plan p = c4_plan(q);
+  const JitProgram* jit = jitGet(p.c.env, p.c.ncols, p.plan, p.dense ? JF_DENSE : JF_GROUPED, "aggregate");
+  std::vector<double> keys, accs;
+  int64_t cap = p.cap, g = 0;
+  for (;;) {
+    GpuAggregator agg(p.plan, cap, p.dense ? &p.ds : nullptr);
+    for (size_t s = 0; s < q.t->seg_rows.size(); ++s) {
+      ExprEnv e = bind_segment(p.c, *q.t, kC4Slots, s);
+      agg.addBatch(e, p.c.ncols, q.t->seg_rows[s], nullptr);
+    }
+    agg.run(p.c.env, jit);
+    if (agg.overflowed()) {
+      cap = pow2_at_least(std::max<int64_t>(cap * 4, agg.groupCount() * 4), 1);
+      continue;
+    }
+    g = agg.fetch(keys, accs);
+    break;
+  }
+  if (groups) *groups = g;
+  return c4_digest(keys, accs, g);
+}
+
+// Dense queries of one shape together: one batched launch and one batched fetch.
+void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>& digests, std::vector<int64_t>& groups) {
+  const size_t n = qs.size();
+  digests.assign(n, 0);
+  groups.assign(n, 0);
+  if (!n) return;
+  std::vector<C4Plan> plans;
+  plans.reserve(n);
+  std::vector<size_t> dense_idx;
+  for (size_t i = 0; i < n; ++i) {
+    plans.push_back(c4_plan(*qs[i]));
+    if (plans.back().dense) dense_idx.push_back(i);
+    else digests[i] = c4_alone(*qs[i], &groups[i]);  // not dense: plain path
+  }
+  if (dense_idx.empty()) return;
+  const C4Plan& p0 = plans[dense_idx[0]];
+  const JitProgram* jit = jitGet(p0.c.env, p0.c.ncols, p0.plan, JF_DENSE, "aggregate (batched)");
+  std::vector<std::unique_ptr<GpuAggregator>> aggs;
+  std::vector<ws_agg*> handles;
+  std::vector<const ws_expr_env*> envs;
+  int64_t cap = 0;
+  for (size_t i : dense_idx) {
+    const C4Plan& p = plans[i];
+    aggs.emplace_back(new GpuAggregator(p.plan, p.cap, &p.ds));
+    for (size_t s = 0; s < qs[i]->t->seg_rows.size(); ++s) {
+      ExprEnv e = bind_segment(p.c, *qs[i]->t, kC4Slots, s);
+      aggs.back()->addBatch(e, p.c.ncols, qs[i]->t->seg_rows[s], nullptr);
+    }
+    handles.push_back(aggs.back()->handle());
+    envs.push_back(reinterpret_cast<const ws_expr_env*>(&p.c.env));
+    cap += p.ds.G;
+  }
+  const int m = (int)handles.size();
+  wsd::ck(ws_agg_run_batch(m, handles.data(), envs.data(), jit ? jit->handle : nullptr));
+  std::vector<int64_t> counts((size_t)m);
+  int64_t total = 0;
+  std::vector<double> keys((size_t)std::max<int64_t>(cap, 1)), accs((size_t)std::max<int64_t>(cap * 5, 1));
+  wsd::ck(ws_agg_fetch_many(m, handles.data(), counts.data(), &total, cap, keys.data(), accs.data()));
+  int64_t off = 0;
+  for (int j = 0; j < m; ++j) {
+    const size_t i = dense_idx[(size_t)j];
+    const int64_t g = counts[(size_t)j];
+    std::vector<double> k(keys.begin() + off, keys.begin() + off + g), a(accs.begin() + off * 5, accs.begin() + (off + g) * 5);
+    digests[i] = c4_digest(k, a, g);
+    groups[i] = g;
+    off += g;
+  }
+}
+
+// ------------------------------------------------------------------ serving loop (open-loop stream)
+
+// A request of the mixed stream: a small aggregate over one of the pool's segments, the web
+// query, or the star join.
+enum { kReqSmall = 0, kReqWeb = 1, kReqJoin = 2 };
+struct Request {
+  int type = kReqSmall;
+  int64_t pick = 0;       // small: pool index
+  int64_t theta = 0;      // small: filter constant
+  double arrive = 0;      // seconds from the start
+  double done = 0;
+  uint64_t digest = 0;
+};
+struct ServeSet {
+  std::vector<const DevTable*> pool;  // small segments (columns 0 key INT, 1 val BIGINT)
+  const DevTable* web = nullptr;
+  const DevTable* fact = nullptr;
+  const DevTable* dim = nullptr;
+};
+
+double u01(uint64_t seed, uint64_t i) {
+  uint64_t x = seed * 0x9e3779b97f4a7c15ull + i * 0xd1b54a32d192ed03ull + 1;
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// The stream: Poisson arrivals at `rate` per second, 70 % small aggregates, 25 % web queries,
+// 5 % star joins (when the set has them).
+std::vector<Request> make_stream(const ServeSet& ss, double rate, int64_t count, uint64_t seed) {
+  std::vector<Request> rs((size_t)count);
+  double t = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    Request& r = rs[(size_t)i];
+    t += -std::log(1.0 - u01(seed, 4 * i)) / rate;
+    r.arrive = t;
+    const double x = u01(seed, 4 * i + 1);
+    r.type = x < 0.70 || !ss.web ? kReqSmall : (x < 0.95 || !ss.fact ? kReqWeb : kReqJoin);
+    r.pick = (int64_t)(u01(seed, 4 * i + 2) * (double)ss.pool.size());
+    r.theta = 100000 + (int64_t)(u01(seed, 4 * i + 3) * 800000.0);
+  }
+  return rs;
+}
+
+uint64_t run_heavy(const ServeSet& ss, const Request& r) {
+  if (r.type == kReqWeb) {
+    int64_t out[40];
+    C2Stats st;
+    const int64_t n = query_c2(*ss.web, out, 10, &st);
+    uint64_t d = (uint64_t)st.groups * 1000003ull + (uint64_t)st.pairs;
+    for (int64_t i = 0; i < n; ++i) d = d * 31 + (uint64_t)out[i * 4 + 1];  // the top-10 counts (ties may reorder keys)
+    return d;
+  }
+  int64_t out[4 * 64];
+  C3Stats st;
+  const int64_t g = query_c3(*ss.fact, *ss.dim, out, &st);
+  uint64_t d = (uint64_t)g;
+  for (int64_t i = 0; i < g * 4; ++i) d = d * 1099511628211ull + (uint64_t)out[i];
+  return d;
+}
+
+uint64_t run_alone(const ServeSet& ss, const Request& r) {
+  if (r.type != kReqSmall) return run_heavy(ss, r);
+  C4Query q{ss.pool[(size_t)r.pick], r.theta};
+  return c4_alone(q, nullptr);
+}
+
+}  // namespace
+
+namespace {
+
+// Mode 0 (the reference's behaviour, engine.cpp:1262-1337): up to `workers` requests at a time,
+// each run alone through the per-query path on its own thread and stream, in arrival order.
+// Mode 1 (continuous batching): one thread serves the small aggregates, each time taking every
+// small request that has arrived (up to `max_batch`) into one batched launch and fetch; `workers`
+// threads serve the web / join queries at the same time, so their work overlaps on the device.
+// Requests that arrive while others run join the next batch. Latency = completion - arrival.
+void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, int max_batch) {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<size_t> small_q, heavy_q, all_q;
+  bool closing = false;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  std::vector<std::thread> th;
+  std::exception_ptr err;
+  auto guard_thread = [&](auto fn) {
+    return [&, fn] {
+      try {
+        fn();
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!err) err = std::current_exception();
+        closing = true;
+        cv.notify_all();
+      }
+    };
+  };
+  if (mode == 0) {
+    for (int w = 0; w < workers; ++w)
+      th.emplace_back(guard_thread([&] {
+        for (;;) {
+          size_t i;
+          {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&] { return closing || !all_q.empty(); });
+            if (all_q.empty()) return;
+            i = all_q.front();
+            all_q.pop_front();
+          }
+          rs[i].digest = run_alone(ss, rs[i]);
+          rs[i].done = now();
+        }
+      }));
+  } else {
+    th.emplace_back(guard_thread([&] {
+      for (;;) {
+        std::vector<size_t> batch;
+        {
+          std::unique_lock<std::mutex> g(mu);
+          cv.wait(g, [&] { return closing || !small_q.empty(); });
+          if (small_q.empty()) return;
+          while (!small_q.empty() && (int)batch.size() < max_batch) {
+            batch.push_back(small_q.front());
+            small_q.pop_front();
+          }
+        }
+        std::vector<C4Query> qs(batch.size());
+        std::vector<const C4Query*> qp(batch.size());
+        for (size_t j = 0; j < batch.size(); ++j) {
+          qs[j] = C4Query{ss.pool[(size_t)rs[batch[j]].pick], rs[batch[j]].theta};
+          qp[j] = &qs[j];
+        }
+        std::vector<uint64_t> dg;
+        std::vector<int64_t> gr;
+        c4_batched(qp, dg, gr);
+        const double d = now();
+        for (size_t j = 0; j < batch.size(); ++j) rs[batch[j]].digest = dg[j], rs[batch[j]].done = d;
+      }
+    }));
+    for (int w = 0; w < workers; ++w)
+      th.emplace_back(guard_thread([&] {
+        for (;;) {
+          size_t i;
+          {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&] { return closing || !heavy_q.empty(); });
+            if (heavy_q.empty()) return;
+            i = heavy_q.front();
+            heavy_q.pop_front();
+          }
+          rs[i].digest = run_heavy(ss, rs[i]);
+          rs[i].done = now();
+        }
+      }));
+  }
+  // arrivals (open loop: requests are released at their arrival times whatever the backlog)
+  for (size_t i = 0; i < rs.size(); ++i) {
+    const double wait = rs[i].arrive - now();
+    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+    std::lock_guard<std::mutex> g(mu);
+    if (closing) break;
+    if (mode == 0) all_q.push_back(i);
+    else if (rs[i].type == kReqSmall) small_q.push_back(i);
+    else heavy_q.push_back(i);
+    cv.notify_all();
+  }
+  {
+    std::lock_guard<std::mutex> g(mu);
+    closing = true;
+    cv.notify_all();
+  }
+  for (auto& t : th) t.join();
+  if (err) std::rethrow_exception(err);
+}
+
+double pct(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  const size_t k = (size_t)std::min<double>((double)v.size() - 1, std::floor(p * (double)(v.size() - 1) + 0.5));
+  return v[k];
+}
+
 }  // namespace
 
 extern "C" {
@@ -613,6 +949,108 @@ int wsq_c3(void* fact, void* dim, int64_t* out, int64_t* groups, int64_t* stats)
     C3Stats st;
     *groups = query_c3(*(DevTable*)fact, *(DevTable*)dim, out, &st);
     if (stats) stats[0] = st.dim_rows, stats[1] = st.fact_rows, stats[2] = st.pairs;
+  });
+}
+
+// C4: the generated segment set of the many-small-aggregations workload (seed; nq queries with
+// rows lognormal between 4096 and 4M, distinct keys 10^U(1,5), theta uniform in [1e5, 9e5)).
+int wsq_c4_load(uint64_t seed, int64_t nq, int64_t rows_override, void** out, int64_t* total_rows) {
+  return guard([&] {
+    auto set = new std::vector<std::pair<DevTable*, int64_t>>();
+    std::vector<int64_t> rows((size_t)nq), distinct((size_t)nq), theta((size_t)nq);
+    wsg_c4_shapes(seed, nq, rows.data(), distinct.data(), theta.data());
+    int64_t tot = 0;
+    try {
+      for (int64_t q = 0; q < nq; ++q) {
+        const int64_t n = rows_override > 0 ? rows_override : rows[(size_t)q];
+        std::vector<int32_t> key((size_t)n);
+        std::vector<int64_t> val((size_t)n);
+        wsg_c4_data(seed, q, n, distinct[(size_t)q], key.data(), val.data());
+        const int types[2] = {WS_T_I32, WS_T_I64};
+        const void* host[2] = {key.data(), val.data()};
+        void* t = nullptr;
+        const int rc = wsq_table_load(2, types, nullptr, n, host, &t);
+        if (rc) throw SqlError(wsq_last_error(), rc);
+        set->push_back({(DevTable*)t, theta[(size_t)q]});
+        tot += n;
+      }
+    } catch (...) {
+      for (auto& e : *set) delete e.first;
+      delete set;
+      throw;
+    }
+    *out = set;
+    if (total_rows) *total_rows = tot;
+  });
+}
+void wsq_c4_free(void* set) {
+  auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)set;
+  for (auto& e : *v) delete e.first;
+  delete v;
+}
+// Runs every query of the set: batched (one launch + one fetch) or one by one (plain path).
+// digests[q] / groups[q] per query.
+int wsq_c4_run(void* set, int batched, uint64_t* digests, int64_t* groups) {
+  return guard([&] {
+    auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)set;
+    std::vector<C4Query> qs;
+    for (auto& e : *v) qs.push_back(C4Query{e.first, e.second});
+    if (batched) {
+      std::vector<const C4Query*> qp;
+      for (auto& q : qs) qp.push_back(&q);
+      std::vector<uint64_t> d;
+      std::vector<int64_t> g;
+      c4_batched(qp, d, g);
+      for (size_t i = 0; i < qs.size(); ++i) digests[i] = d[i], groups[i] = g[i];
+    } else {
+      for (size_t i = 0; i < qs.size(); ++i) digests[i] = c4_alone(qs[i], &groups[i]);
+    }
+  });
+}
+
+// The serving loop over a mixed open-loop stream (see serve()). pool: a C4 set whose tables serve
+// the small aggregates; web / fact / dim may be null (the mix then shifts to what exists).
+// stats: [0] completed, [1] wall seconds, [2] queries/s, [3] p50 ms, [4] p99 ms, [5] small p50,
+// [6] small p99, [7] heavy p50, [8] heavy p99 (ms). digests / types per request (may be null).
+int wsq_serve(void* pool, void* web, void* fact, void* dim, double rate, int64_t count, int mode, int workers, int max_batch,
+              uint64_t seed, double* stats, uint64_t* digests, int32_t* types) {
+  return guard([&] {
+    auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)pool;
+    ServeSet ss;
+    for (auto& e : *v) ss.pool.push_back(e.first);
+    ss.web = (const DevTable*)web;
+    ss.fact = (const DevTable*)fact;
+    ss.dim = (const DevTable*)dim;
+    std::vector<Request> rs = make_stream(ss, rate, count, seed);
+    serve(ss, rs, mode, workers, max_batch);
+    std::vector<double> all, sm, hv;
+    double end = 0;
+    for (size_t i = 0; i < rs.size(); ++i) {
+      const double ms = (rs[i].done - rs[i].arrive) * 1e3;
+      all.push_back(ms);
+      (rs[i].type == kReqSmall ? sm : hv).push_back(ms);
+      end = std::max(end, rs[i].done);
+      if (digests) digests[i] = rs[i].digest;
+      if (types) types[i] = rs[i].type;
+    }
+    const double wall = end - (rs.empty() ? 0 : rs[0].arrive);
+    const double out[9] = {(double)rs.size(), wall, wall > 0 ? (double)rs.size() / wall : 0, pct(all, 0.5), pct(all, 0.99),
+                           pct(sm, 0.5), pct(sm, 0.99), pct(hv, 0.5), pct(hv, 0.99)};
+    std::memcpy(stats, out, sizeof out);
+  });
+}
+// One request of the stream (same seed, rate and index) run alone through the plain path.
+int wsq_serve_alone(void* pool, void* web, void* fact, void* dim, double rate, int64_t count, uint64_t seed, int64_t index,
+                    uint64_t* digest) {
+  return guard([&] {
+    auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)pool;
+    ServeSet ss;
+    for (auto& e : *v) ss.pool.push_back(e.first);
+    ss.web = (const DevTable*)web;
+    ss.fact = (const DevTable*)fact;
+    ss.dim = (const DevTable*)dim;
+    std::vector<Request> rs = make_stream(ss, rate, count, seed);
+    *digest = run_alone(ss, rs[(size_t)index]);
   });
 }
 

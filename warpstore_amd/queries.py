@@ -32,6 +32,15 @@ def qlib():
         Q.wsq_c2.restype = I
         Q.wsq_c3.argtypes = [P, P, P, C.POINTER(I64), P]
         Q.wsq_c3.restype = I
+        Q.wsq_c4_load.argtypes = [C.c_uint64, I64, I64, C.POINTER(C.c_void_p), C.POINTER(I64)]
+        Q.wsq_c4_load.restype = I
+        Q.wsq_c4_free.argtypes = [P]
+        Q.wsq_c4_run.argtypes = [P, I, P, P]
+        Q.wsq_c4_run.restype = I
+        Q.wsq_serve.argtypes = [P, P, P, P, C.c_double, I64, I, I, I, C.c_uint64, P, P, P]
+        Q.wsq_serve.restype = I
+        Q.wsq_serve_alone.argtypes = [P, P, P, P, C.c_double, I64, C.c_uint64, I64, C.POINTER(C.c_uint64)]
+        Q.wsq_serve_alone.restype = I
         _q = Q
     return _q
 
@@ -93,3 +102,45 @@ def c3(fact: DeviceTable, dim: DeviceTable):
     g = C.c_int64()
     _ck(qlib().wsq_c3(fact.h, dim.h, out.ctypes.data, C.byref(g), st.ctypes.data))
     return out[:g.value], st
+
+
+class C4Set:
+    """The many-small-aggregations workload: nq generated segments (rows lognormal in [4096, 4M],
+    or rows_override each), one filter + GROUP BY query per segment."""
+
+    def __init__(self, seed: int, nq: int, rows_override: int = 0) -> None:
+        h = C.c_void_p()
+        tot = C.c_int64()
+        _ck(qlib().wsq_c4_load(seed, nq, rows_override, C.byref(h), C.byref(tot)))
+        self.h, self.nq, self.rows = h, nq, tot.value
+
+    def run(self, batched: bool):
+        d = np.zeros(self.nq, np.uint64)
+        g = np.zeros(self.nq, np.int64)
+        _ck(qlib().wsq_c4_run(self.h, 1 if batched else 0, d.ctypes.data, g.ctypes.data))
+        return d, g
+
+    def free(self) -> None:
+        if self.h:
+            qlib().wsq_c4_free(self.h)
+            self.h = None
+
+
+def serve(pool: C4Set, web=None, fact=None, dim=None, rate: float = 100.0, count: int = 200, mode: int = 1,
+          workers: int = 2, max_batch: int = 256, seed: int = 5):
+    """Runs the open-loop mixed stream; returns (stats dict, digests, types)."""
+    st = np.zeros(9)
+    d = np.zeros(count, np.uint64)
+    ty = np.zeros(count, np.int32)
+    _ck(qlib().wsq_serve(pool.h, web.h if web else None, fact.h if fact else None, dim.h if dim else None, rate, count, mode,
+                         workers, max_batch, seed, st.ctypes.data, d.ctypes.data, ty.ctypes.data))
+    keys = ["completed", "wall_s", "qps", "p50_ms", "p99_ms", "small_p50_ms", "small_p99_ms", "heavy_p50_ms", "heavy_p99_ms"]
+    return dict(zip(keys, st.tolist())), d, ty
+
+
+def serve_alone(pool: C4Set, web=None, fact=None, dim=None, rate: float = 100.0, count: int = 200, seed: int = 5,
+                index: int = 0) -> int:
+    out = C.c_uint64()
+    _ck(qlib().wsq_serve_alone(pool.h, web.h if web else None, fact.h if fact else None, dim.h if dim else None, rate, count,
+                               seed, index, C.byref(out)))
+    return out.value

@@ -125,11 +125,19 @@ def main() -> int:
                     help="set a tuning knob of the physical plans (ws_tune_set) before running")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy kernel (known bytes, for counter calibration)")
+    ap.add_argument("--workload", choices=["web", "batch", "serve"], default="web",
+                    help="web: the headline 10M-row web-analytics query (default); batch: 256 small GROUP BY "
+                         "queries, one batched launch vs per-query; serve: the open-loop mixed stream, "
+                         "continuous batching vs per-query serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path's multi-GPU sharding is a design note, no collective runs here)", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import warpstore_amd as ws
     from warpstore_amd import abi, queries as Q
@@ -226,6 +234,86 @@ def main() -> int:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
     return 0 if parity else 1
+
+
+def bench_batch(args) -> int:
+    """256 filter + GROUP BY queries (seed 4): rows lognormal in [4096, 4M], keys with 10 to 1e5
+    distinct values, WHERE val > theta_q; 12 B/row scanned. One step = all 256 queries, run as one
+    batched launch + one batched fetch, and (separately timed) one by one through the per-query path."""
+    import warpstore_amd as ws
+    from warpstore_amd import abi, queries as Q
+    L = ws.lib()
+    ws.check(L.ws_init())
+    L.ws_jit_set_mode(abi.JIT_SYNC)
+    for kv in args.tune:
+        key, val = kv.split("=", 1)
+        ws.check(L.ws_tune_set(key.encode(), int(val)))
+    s = Q.C4Set(seed=4, nq=256)
+    res = {}
+    for mode in ("batched", "per_query"):
+        for _ in range(max(args.warmup, 1)):
+            d, g = s.run(batched=mode == "batched")
+        L.ws_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d, g = s.run(batched=mode == "batched")
+        L.ws_sync()
+        dt = (time.perf_counter() - t0) / args.steps
+        res[mode] = (dt, d, g)
+    same = bool(np.array_equal(res["batched"][1], res["per_query"][1]))
+    bt, pt = res["batched"][0], res["per_query"][0]
+    out = {
+        "metric": "GB/s scanned by 256 batched filter+GROUP BY queries", "value": round(s.rows * 12 / bt / 1e9, 3),
+        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(bt * 1e3, 4),
+        "higher_is_better": True, "dtype": "int64", "data": "synthetic (seeded generator, seed 4)",
+        "config": {"workload": "256 filter + GROUP BY queries, one batched launch", "rows": s.rows, "bytes_per_row": 12},
+        "queries_per_s": round(256 / bt, 1), "per_query_ms_per_step": round(pt * 1e3, 4),
+        "per_query_queries_per_s": round(256 / pt, 1), "speedup_vs_per_query": round(pt / bt, 3),
+        "results_equal_per_query": same, "groups": int(res["batched"][2].sum()),
+    }
+    s.free()
+    print(json.dumps(out), flush=True)
+    return 0 if same else 1
+
+
+def bench_serve(args) -> int:
+    """The open-loop mixed stream (seed 5): Poisson arrivals, 70 % small aggregates on 64k-row
+    segments, 25 % web-analytics queries on the 10M-row table, 5 % star joins (fact 60M, dim 1.5M).
+    Per offered rate, continuous batching (one thread batches the small aggregates, 2 threads run
+    the others concurrently) against per-query serving (4 concurrent queries, each alone); reports
+    queries/s and p50 / p99 latency."""
+    import warpstore_amd as ws
+    from warpstore_amd import abi, queries as Q
+    L = ws.lib()
+    ws.check(L.ws_init())
+    L.ws_jit_set_mode(abi.JIT_SYNC)
+    pool = Q.C4Set(seed=5, nq=256, rows_override=65536)
+    cols = gen_c2(ROWS)
+    web = Q.DeviceTable(list(cols), [abi.T_I64, abi.T_I32, abi.T_I32, abi.T_I32, abi.T_I32], [0, 1, 0, 0, 0])
+    g = ws.datagen()
+    nf, nd = 60_000_000, 1_500_000
+    dkey, dattr = np.empty(nd, np.int32), np.empty(nd, np.int32)
+    g.wsg_c3_dim(3, nd, ws.ptr(dkey), ws.ptr(dattr))
+    fk, qty, price, flag = np.empty(nf, np.int32), np.empty(nf, np.int64), np.empty(nf, np.int64), np.empty(nf, np.int32)
+    g.wsg_c3_fact(3, nf, nd, ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag))
+    fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
+    dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
+    del fk, qty, price, flag
+    Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=1, seed=1)  # warm-up: compile plans
+    Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=0, seed=1)
+    rows = []
+    for rate in (100.0, 200.0, 400.0, 800.0, 1600.0):
+        count = int(min(2000, max(200, rate * 2)))
+        for mode, name in ((1, "continuous_batching"), (0, "per_query")):
+            st, _, _ = Q.serve(pool, web, fact, dim, rate=rate, count=count, mode=mode, workers=2 if mode else 4,
+                               max_batch=256, seed=5)
+            st.update({"offered_rate": rate, "mode": name})
+            rows.append(st)
+            print(json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}), flush=True)
+    for t in (web, fact, dim):
+        t.free()
+    pool.free()
+    return 0
 
 
 if __name__ == "__main__":

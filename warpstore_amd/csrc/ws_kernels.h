@@ -6,6 +6,7 @@
 //   - at run time by hipRTC with a typed evaluator generated from the same bytecode (ws_jit.cpp).
 // An evaluator supplies:
 //   static constexpr int kKeys, kAggs, kRows;   // loop bounds, rows per thread per tile
+//   static constexpr bool kPureKeys, kPureVals;  // keys / slot values have no side effect
 //   Ev(const ScanDesc&);  void bind(const ws_colview* cols);   // per batch
 //   int nkeys(), naggs(), op(a); bool has_where(), agg_int(a), key_int(k);
 //   bool where(r); double key(k, r); i64 ikey(k, r); double agg(a, r); i64 iagg(a, r); uint8_t dkind(a);
@@ -222,6 +223,7 @@ __device__ __noinline__ double eval_prog_ni(const ws_expr_env* env, const ws_col
 template <bool kFull>
 struct InterpEvalT {
   static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 1;
+  static constexpr bool kPureKeys = false, kPureVals = false;  // (see the compiled evaluator)
   const ws_expr_env* env;
   const ws_agg_plan* plan;
   const ws_colview* cols = nullptr;
@@ -908,7 +910,13 @@ __device__ void part_count_body(const ScanDesc& d) {
         const i64 ii = i + (i64)u * blockDim.x;
         const i64 r = tr.first + ii;
         h[u] = kEmpty;
-        if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
+        if (Ev::kPureKeys) {  // key loads issued with the WHERE loads, not after them
+          if (ii < tr.count) {
+            double kv[Ev::kKeys];
+            const u64 hk = row_keys(ev, NK, r, kv);
+            if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
+          }
+        } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
           double kv[Ev::kKeys];
           h[u] = row_keys(ev, NK, r, kv);
         }
@@ -974,7 +982,13 @@ __device__ void part_scatter_body(const ScanDesc& d) {
         const i64 ii = i + (i64)u * blockDim.x;
         const i64 r = tr.first + ii;
         h[u] = kEmpty;
-        if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
+        if (Ev::kPureKeys && Ev::kPureVals) {  // all loads of the row issued together
+          if (ii < tr.count) {
+            const u64 hk = row_keys(ev, NK, r, kv[u]);
+            row_values(ev, plan, NA, r, av[u]);
+            if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
+          }
+        } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
           h[u] = row_keys(ev, NK, r, kv[u]);
           row_values(ev, plan, NA, r, av[u]);
         }

@@ -12,6 +12,7 @@
 // The host side above the C ABI: it owns no kernels and calls only the public surface.
 #define WARPSTORE_AMD_STANDALONE 1
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -49,6 +50,7 @@ struct DevColumn {
   int t = WS_T_I32;  // type code of the column's views
   std::vector<SegCol> segs;
   int64_t mn = INT64_MAX, mx = INT64_MIN;  // min / max of integer columns (colRange)
+  ws_col_stats stats{};                    // merged segment statistics (computed at load)
 };
 struct DevTable {
   std::vector<int64_t> seg_rows;
@@ -249,6 +251,29 @@ int64_t query_c1(const DevTable& t, int64_t* out_i, double* out_d, int64_t cap_r
   return g;
 }
 
+// ------------------------------------------------------------------ learned sizing hook
+
+// The sizing model the drivers consult (null: the engine's range rule alone). Every aggregation
+// the drivers run carries its features, so its exact group count is logged for training.
+std::atomic<ws_sizer*> g_sizer{nullptr};
+
+// Fraction of an integer column's range below `lt` (uniform within min / max).
+double frac_below(const DevColumn& c, int64_t lt) {
+  if (c.mn > c.mx) return 1.0;
+  return std::min(1.0, std::max(0.0, (double)(lt - c.mn) / (double)(c.mx - c.mn + 1)));
+}
+// Features of a GROUP BY over columns `keys` of t (or over `rows` input rows); expected groups
+// from the model when one is set (0 otherwise).
+int64_t size_groupby(const std::vector<const DevColumn*>& keys, int64_t rows, double sel, float* f) {
+  std::vector<ws_col_stats> st;
+  for (const DevColumn* k : keys) st.push_back(k->stats);
+  wsd::ck(ws_sizing_features(st.data(), (int)st.size(), nullptr, rows, sel, f));
+  ws_sizer* m = g_sizer.load();
+  int64_t g = 0;
+  if (m) wsd::ck(ws_sizer_groups(m, f, &g));
+  return g;
+}
+
 // ------------------------------------------------------------------ C2: COUNT DISTINCT + top 10
 
 // SELECT user_id, COUNT(*) c, SUM(dur), COUNT(DISTINCT url) FROM h WHERE event_date < 19723+300 AND
@@ -288,10 +313,16 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
   plan1.dense_priv = 0;
   const JitProgram* jit1 = jitGet(c.env, c.ncols, plan1, JF_GROUPED, "aggregate (distinct pairs)");
   int64_t cap1 = pow2_at_least(std::min<int64_t>(std::max<int64_t>(t.rows, 1), 1 << 22), 1024);
+  float f1[WS_SIZER_INPUTS];
+  const double sel = frac_below(t.cols[2], 19723 + 300) * frac_below(t.cols[1], 224);
+  const int64_t pred1 = size_groupby({&t.cols[0], &t.cols[3]}, t.rows, sel, f1);
+  if (pred1 > 0) cap1 = pow2_at_least(pred1 * 5 / 2, 1024);  // 2x headroom + 25 % for the error
   DevVec<double> dk, da;
   int64_t g1 = 0;
   for (;;) {
     GpuAggregator a1(plan1, cap1, nullptr);
+    ws_agg_set_features(a1.handle(), f1);
+    if (pred1 > 0) wsd::ck(ws_agg_set_strategy(a1.handle(), WS_STRAT_AUTO, pred1 + pred1 / 4));
     for (size_t s = 0; s < t.seg_rows.size(); ++s) {
       ExprEnv e = bind_segment(c, t, slot_col, s);
       a1.addBatch(e, c.ncols, t.seg_rows[s], nullptr);
@@ -343,14 +374,27 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
   }
   plan2.ops[na] = WS_AGG_SUM;
   plan2.naggs = na + 1;
-  const int64_t cap2 = pow2_at_least(2 * g1, 1024);
-  GpuAggregator agg2(plan2, cap2, nullptr);
-  agg2.addBatch(env2, nk + 1 + na, g1, nullptr);
-  agg2.run(env2, nullptr);
-  if (agg2.overflowed()) throw SqlError("distinct phase 2 overflowed", 1105);
+  float f2[WS_SIZER_INPUTS];
+  const int64_t pred2 = size_groupby({&t.cols[0]}, g1, 1.0, f2);
+  int64_t cap2 = pow2_at_least(2 * g1, 1024);
+  if (pred2 > 0) cap2 = pow2_at_least(std::min<int64_t>(g1, pred2 + pred2 / 4) * 2, 1024);
+  // (without a model cap2 >= 2 * g1 cannot overflow; a model's under-estimate retries bigger)
+  std::unique_ptr<GpuAggregator> agg2;
+  int64_t expect2 = pred2 > 0 ? std::min<int64_t>(g1, pred2 + pred2 / 4) : 0;
+  for (;;) {
+    agg2.reset(new GpuAggregator(plan2, cap2, nullptr));
+    ws_agg_set_features(agg2->handle(), f2);
+    if (expect2 > 0) wsd::ck(ws_agg_set_strategy(agg2->handle(), WS_STRAT_AUTO, expect2));
+    agg2->addBatch(env2, nk + 1 + na, g1, nullptr);
+    agg2->run(env2, nullptr);
+    if (!agg2->overflowed()) break;
+    cap2 = pow2_at_least(std::max<int64_t>(cap2 * 4, agg2->groupCount() * 4), 1);
+    expect2 = std::max<int64_t>(expect2 * 4, agg2->groupCount() * 2);
+    if (st) ++st->retries;
+  }
   // post-aggregation: ORDER BY c DESC LIMIT n on the GPU over the compacted groups
   DevVec<double> gk, ga;
-  const int64_t g = agg2.compact(gk, ga);
+  const int64_t g = agg2->compact(gk, ga);
   ExprEnv penv;
   std::memset(&penv, 0, sizeof penv);
   std::vector<DevVec<double>> conv;
@@ -924,6 +968,16 @@ int wsq_table_load(int ncols, const int* types, const int* encode, int64_t n, co
           h2d(sc.dev, src, (size_t)(rows * w));
         }
         col.segs.push_back(sc);
+        {  // segment statistics, once at load (the sizing model's inputs)
+          ws_colview v{};
+          v.p = sc.dev;
+          v.t = (uint8_t)col.t;
+          v.enc = sc.enc;
+          ws_col_stats st;
+          wsd::ck(ws_column_stats(&v, rows, nullptr, &st));
+          if (col.segs.size() == 1) col.stats = st;
+          else ws_col_stats_merge(&col.stats, &st);
+        }
         r0 += rows;
       }
     }
@@ -951,6 +1005,9 @@ int wsq_c3(void* fact, void* dim, int64_t* out, int64_t* groups, int64_t* stats)
     if (stats) stats[0] = st.dim_rows, stats[1] = st.fact_rows, stats[2] = st.pairs;
   });
 }
+
+// The sizing model the drivers consult (null to go back to the range rule alone). Not owned.
+void wsq_set_sizer(ws_sizer* m) { g_sizer.store(m); }
 
 // C4: the generated segment set of the many-small-aggregations workload (seed; nq queries with
 // rows lognormal between 4096 and 4M, distinct keys 10^U(1,5), theta uniform in [1e5, 9e5)).

@@ -41,6 +41,8 @@ def qlib():
         Q.wsq_serve.restype = I
         Q.wsq_serve_alone.argtypes = [P, P, P, P, C.c_double, I64, C.c_uint64, I64, C.POINTER(C.c_uint64)]
         Q.wsq_serve_alone.restype = I
+        Q.wsq_set_sizer.argtypes = [P]
+        Q.wsq_set_sizer.restype = None
         _q = Q
     return _q
 

@@ -302,8 +302,8 @@ def bench_serve(args) -> int:
     Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=1, seed=1)  # warm-up: compile plans
     Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=0, seed=1)
     rows = []
-    for rate in (100.0, 200.0, 400.0, 800.0, 1600.0):
-        count = int(min(2000, max(200, rate * 2)))
+    for rate in (400.0, 1600.0, 3200.0, 6400.0):
+        count = int(min(4000, max(400, rate * 1.5)))
         for mode, name in ((1, "continuous_batching"), (0, "per_query")):
             st, _, _ = Q.serve(pool, web, fact, dim, rate=rate, count=count, mode=mode, workers=2 if mode else 4,
                                max_batch=256, seed=5)

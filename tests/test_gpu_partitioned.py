@@ -118,9 +118,11 @@ def test_partitioned_nulls_deleted_double_keys(gpu):
         H.assert_groups_equal(plan, gk, ga, ok, oa)
 
 
-def test_partitioned_heavy_skew(gpu):
-    """90 % of the rows in one group (pre-aggregation and wave folding), the rest spread."""
-    H.say("case partitioned heavy skew")
+@pytest.mark.parametrize("mode", ["interp", "jit"])
+def test_partitioned_heavy_skew(gpu, mode):
+    """90 % of the rows in one group (pre-aggregation and wave folding), the rest spread; the
+    interpreted run takes the column-only evaluator (every program is one column)."""
+    H.say(f"case partitioned heavy skew {mode}")
     n = 1_500_000
     rng = np.random.default_rng(5)
     key = rng.integers(0, 200_000, n).astype(np.int64)
@@ -143,11 +145,13 @@ def test_partitioned_heavy_skew(gpu):
     slots = [("key", abi.T_I64, 0), ("val", abi.T_I64, 0), ("p", abi.T_F64, 0)]
     dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
     ok, oa, _ = H.oracle_aggregate(plan, b.env, host)
-    jit = jit_for(gpu, b.env, len(slots), plan)
+    jit = jit_for(gpu, b.env, len(slots), plan) if mode == "jit" else None
     gk, ga, _, _ = H.gpu_aggregate(plan, b.env, dev, cap=1 << 19, jit=jit, strategy=STRAT_PART)
     assert H.LAST_RUN["partitioned"]
     H.assert_groups_equal(plan, gk, ga, ok, oa)
     assert int(ga[:, 0].sum()) == n
+    tk, ta, _, _ = H.gpu_aggregate(plan, b.env, dev, cap=1 << 19, jit=jit, strategy=STRAT_TABLE)
+    H.assert_groups_equal(plan, tk, ta, ok, oa)
 
 
 def test_partitioned_spill_and_retry(gpu):

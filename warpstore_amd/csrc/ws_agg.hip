@@ -17,6 +17,15 @@ WS_KERNEL_NOKEYS(ws::InterpEvalLean, ws_l_nokeys)
 WS_KERNEL_DENSE(ws::InterpEvalLean, ws_l_dense)
 WS_KERNEL_GROUPED(ws::InterpEvalLean, ws_l_grouped)
 WS_KERNEL_SELECT(ws::InterpEvalLean, ws_l_select)
+WS_KERNEL_NOKEYS(ws::InterpEvalCols, ws_c_nokeys)
+WS_KERNEL_DENSE(ws::InterpEvalCols, ws_c_dense)
+WS_KERNEL_GROUPED(ws::InterpEvalCols, ws_c_grouped)
+WS_KERNEL_SELECT(ws::InterpEvalCols, ws_c_select)
+
+// Interpreter variant of a plan: the full VM, the lean VM (no scalar-function opcodes), or the
+// column-only evaluator (every program one column or literal). IV_PICK names the kernel.
+enum { kIvFull = 0, kIvLean = 1, kIvCols = 2 };
+#define IV_PICK(iv, stem) ((iv) == kIvCols ? ws_c_##stem : (iv) == kIvLean ? ws_l_##stem : ws_i_##stem)
 
 namespace ws {
 
@@ -673,13 +682,23 @@ static bool plan_is_simple(const ws_expr_env& env, const ws_agg_plan& plan) {
     if (plan.ops[a] != WS_AGG_COUNT && !prog_is_simple(env, plan.aggs[a])) return false;
   return true;
 }
+static bool prog_is_leaf(const ws_expr_env& env, ws_prog p) {
+  return p.len == 1 && p.start >= 0 && p.start < WS_MAX_CODE &&
+         (env.code[p.start].op == WS_OP_COL || env.code[p.start].op == WS_OP_CONST);
+}
+static int interp_of(const ws_expr_env& env, const ws_agg_plan& plan) {
+  bool leaf = !plan.has_where || prog_is_leaf(env, plan.where);
+  for (int k = 0; k < plan.nkeys; ++k) leaf = leaf && prog_is_leaf(env, plan.keys[k]);
+  for (int a = 0; a < plan.naggs; ++a) leaf = leaf && (plan.ops[a] == WS_AGG_COUNT || prog_is_leaf(env, plan.aggs[a]));
+  return leaf ? kIvCols : plan_is_simple(env, plan) ? kIvLean : kIvFull;
+}
 
 static double host_identity(uint8_t op) { return op == WS_AGG_MIN ? INFINITY : op == WS_AGG_MAX ? -INFINITY : 0.0; }
 
 // Launches the grouped kernel (compiled, or the interpreter) of a plan with one descriptor.
 // pass: 0 the table kernel, 1 partition count, 2 partition scatter; jf: the compiled kernel of
 // that pass, or null for the interpreter.
-static void launch_grouped(void* jf, int pass, bool lean, ScanDesc* ddesc, GTable t, unsigned grid, size_t lds,
+static void launch_grouped(void* jf, int pass, int iv, ScanDesc* ddesc, GTable t, unsigned grid, size_t lds,
                            const char* name, const char* jit_name, int block = kBlock) {
   if (jf) {
     void* args[] = {&ddesc, &t};
@@ -688,11 +707,11 @@ static void launch_grouped(void* jf, int pass, bool lean, ScanDesc* ddesc, GTabl
   }
   ProfScope ps(name);
   if (pass == 0)
-    hipLaunchKernelGGL(lean ? ws_l_grouped : ws_i_grouped, dim3(grid), dim3(block), lds, stream(), ddesc, t);
+    hipLaunchKernelGGL(IV_PICK(iv, grouped), dim3(grid), dim3(block), lds, stream(), ddesc, t);
   else if (pass == 1)
-    hipLaunchKernelGGL(lean ? ws_l_grouped_pcount : ws_i_grouped_pcount, dim3(grid), dim3(block), lds, stream(), ddesc, t);
+    hipLaunchKernelGGL(IV_PICK(iv, grouped_pcount), dim3(grid), dim3(block), lds, stream(), ddesc, t);
   else
-    hipLaunchKernelGGL(lean ? ws_l_grouped_pscatter : ws_i_grouped_pscatter, dim3(grid), dim3(block), lds, stream(), ddesc, t);
+    hipLaunchKernelGGL(IV_PICK(iv, grouped_pscatter), dim3(grid), dim3(block), lds, stream(), ddesc, t);
   check_launch(name);
 }
 
@@ -763,7 +782,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   void* jf_count = jit ? jit_function(jit, kJitPartCount) : nullptr;
   void* jf_scatter = jit ? jit_function(jit, kJitPartScatter) : nullptr;
   if (!jf_count || !jf_scatter) jf_count = jf_scatter = nullptr;
-  const bool lean = !jf_count && plan_is_simple(env, plan);
+  const int iv = interp_of(env, plan);
   const int NK = plan.nkeys, NA = plan.naggs, P = 1 << pp.pbits;
   DevBuf<uint32_t> cnt(pp.nchunks * P), off(pp.nchunks * P), ptot(P), gcnt(P);
   DevBuf<u64> recs(pp.rows * pp.W);
@@ -794,7 +813,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   {
     DescUpload up;
     build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
-    launch_grouped(jf_count, 1, lean, up.ddesc, t, (unsigned)pp.nchunks, (size_t)P * 4, "agg_part_count",
+    launch_grouped(jf_count, 1, iv, up.ddesc, t, (unsigned)pp.nchunks, (size_t)P * 4, "agg_part_count",
                    "agg_part_count_jit", kPartScanBlock);
   }
   {
@@ -808,7 +827,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     DescUpload up;
     build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
     const size_t lds = (size_t)P * 8 + (size_t)pp.pre * 8 * pp.W;
-    launch_grouped(jf_scatter, 2, lean, up.ddesc, t, (unsigned)pp.nchunks, lds, "agg_part_scatter", "agg_part_scatter_jit",
+    launch_grouped(jf_scatter, 2, iv, up.ddesc, t, (unsigned)pp.nchunks, lds, "agg_part_scatter", "agg_part_scatter_jit",
                    kPartScanBlock);
   }
   {
@@ -902,7 +921,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
   const int fn = plan.nkeys == 0 ? WS_JF_NOKEYS : A->is_dense ? WS_JF_DENSE : WS_JF_GROUPED;
   void* jf = jit ? jit_function(jit, fn) : nullptr;
-  const bool lean = !jf && plan_is_simple(env, plan);
+  const int iv = interp_of(env, plan);
   const int cus = cu_count();
   if (plan.nkeys == 0) {
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)cus * 8));
@@ -917,7 +936,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
       jit_launch_raw(jf, grid, kBlock, 0, args, "agg_nokeys_jit");
     } else {
       ProfScope ps("agg_nokeys");
-      hipLaunchKernelGGL(lean ? ws_l_nokeys : ws_i_nokeys, dim3(grid), dim3(kBlock), 0, stream(), up.ddesc, out, parts.p, done);
+      hipLaunchKernelGGL(IV_PICK(iv, nokeys), dim3(grid), dim3(kBlock), 0, stream(), up.ddesc, out, parts.p, done);
       check_launch("agg_nokeys");
     }
     return;
@@ -945,7 +964,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
       jit_launch_raw(jf, (unsigned)grid, kBlock, dp.lds, args, "agg_dense_jit");
     } else {
       ProfScope ps("agg_dense");
-      hipLaunchKernelGGL(lean ? ws_l_dense : ws_i_dense, dim3((unsigned)grid), dim3(kBlock), dp.lds, stream(), up.ddesc, gacc, pp,
+      hipLaunchKernelGGL(IV_PICK(iv, dense), dim3((unsigned)grid), dim3(kBlock), dp.lds, stream(), up.ddesc, gacc, pp,
                          flags);
       check_launch("agg_dense");
     }
@@ -977,7 +996,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   DescUpload up;
   build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
   GTable t = table_of(A);
-  launch_grouped(jf, 0, lean, up.ddesc, t, grid, lds, "agg_hash", "agg_hash_jit");
+  launch_grouped(jf, 0, iv, up.ddesc, t, grid, lds, "agg_hash", "agg_hash_jit");
 }
 
 // Compacts the groups into a tmp buffer: keys then accs, row-major (fetch) or column-major.
@@ -1183,18 +1202,17 @@ int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs,
     if (n <= 0) return;
     const ws_agg* A0 = aggs[0];
     const int nk = A0->plan.nkeys, na = A0->plan.naggs;
-    bool simple = true;
+    int iv = kIvCols;
     for (int q = 0; q < n; ++q) {
       const ws_agg* A = aggs[q];
       if (!A->is_dense || A->plan.nkeys != nk || A->plan.naggs != na)
         throw Error("batched run: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
-      simple = simple && plan_is_simple(*envs[q], A->plan);
+      iv = std::min(iv, interp_of(*envs[q], A->plan));
     }
     if (!jit && (na > WS_INTERP_AGGS || nk > WS_INTERP_KEYS))
       throw Error("this aggregation needs a compiled plan (more than 16 aggregates or 4 GROUP BY keys); codegen failed or is off",
                   WS_ERR_NEEDS_PLAN);
     void* jf = jit ? jit_function(jit, kJitBatchDense) : nullptr;
-    const bool lean = !jf && simple;
     const int tile_rows = (int)tune(kTuneBatchTile);
     // blob: [BatchQuery n][BatchTile T] then per query plan, dense spec, env, n[], del[], cols[], tile0[], row0[]
     std::vector<BatchTile> tiles;
@@ -1254,7 +1272,7 @@ int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs,
       jit_launch_raw(jf, grid, kBlock, lds, args, "agg_batch_dense_jit");
     } else {
       ProfScope ps("agg_batch_dense");
-      hipLaunchKernelGGL(lean ? ws_l_dense_batch : ws_i_dense_batch, dim3(grid), dim3(kBlock), lds, stream(), dq, dt, nt, sl);
+      hipLaunchKernelGGL(IV_PICK(iv, dense_batch), dim3(grid), dim3(kBlock), lds, stream(), dq, dt, nt, sl);
       check_launch("agg_batch_dense");
     }
     tmp_free(dev);
@@ -1370,8 +1388,8 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
     jit_launch_raw(jf, (unsigned)ntiles, kBlock, 0, args, "select_jit");
   } else {
     ProfScope ps("select");
-    const bool lean = !has_where || prog_is_simple(env, where);
-    hipLaunchKernelGGL(lean ? ws_l_select : ws_i_select, dim3((unsigned)ntiles), dim3(kBlock), 0, stream(), up.ddesc, st.p, ctr,
+    const int iv = !has_where || prog_is_leaf(env, where) ? kIvCols : prog_is_simple(env, where) ? kIvLean : kIvFull;
+    hipLaunchKernelGGL(IV_PICK(iv, select), dim3((unsigned)ntiles), dim3(kBlock), 0, stream(), up.ddesc, st.p, ctr,
                        out_idx, pb, total, err);
     check_launch("select");
   }

@@ -265,6 +265,40 @@ struct InterpEvalT {
 typedef InterpEvalT<true> InterpEval;
 typedef InterpEvalT<false> InterpEvalLean;
 
+// Interpreter for plans whose every program (WHERE, keys, aggregate arguments) is a single column
+// or literal: no VM call at all, so rows can be unrolled and their loads issued together.
+struct InterpEvalCols {
+  static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 4;
+  static constexpr bool kPureKeys = true, kPureVals = true;
+  const ws_expr_env* env;
+  const ws_agg_plan* plan;
+  const ws_colview* cols = nullptr;
+  __device__ InterpEvalCols(const ScanDesc& d) : env(d.env), plan(d.plan) {}
+  __device__ void bind(const ws_colview* c) { cols = c; }
+  __device__ int nkeys() const { return plan->nkeys; }
+  __device__ int naggs() const { return plan->naggs; }
+  __device__ int op(int a) const { return plan->ops[a]; }
+  __device__ bool agg_int(int a) const { return plan->agg_int[a] != 0; }
+  __device__ bool key_int(int k) const { return plan->key_int[k] != 0; }
+  __device__ bool has_where() const { return plan->has_where != 0; }
+  __device__ uint8_t dkind(int a, const ws_dense_spec& ds) const { return ds.kind[a]; }
+  __device__ double run(ws_prog p, i64 r) const {
+    const ws_instr in = env->code[p.start];
+    if (in.op == WS_OP_COL) return load_col(cols[in.arg], r);
+    return in.is_int ? (double)dbits(env->consts[in.arg]) : env->consts[in.arg];
+  }
+  __device__ i64 irun(ws_prog p, i64 r) const {
+    const ws_instr in = env->code[p.start];
+    if (in.op == WS_OP_COL && cols[in.arg].t != WS_T_F64) return col_int(cols[in.arg], r);
+    return (i64)run(p, r);
+  }
+  __device__ bool where(i64 r) const { return !plan->has_where || truthy(run(plan->where, r)); }
+  __device__ double key(int k, i64 r) const { return run(plan->keys[k], r); }
+  __device__ i64 ikey(int k, i64 r) const { return irun(plan->keys[k], r); }
+  __device__ double agg(int a, i64 r) const { return plan->ops[a] == WS_AGG_COUNT ? 1.0 : run(plan->aggs[a], r); }
+  __device__ i64 iagg(int a, i64 r) const { return plan->ops[a] == WS_AGG_COUNT ? 1 : irun(plan->aggs[a], r); }
+};
+
 // ------------------------------------------------------------------ no GROUP BY
 
 // Per-block partial of a no-keys aggregation: per slot a 128-bit (lo, hi) pair for integer sums,
@@ -972,7 +1006,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
     const uint32_t* del = d.del[b];
     // Ev::kRows rows per thread per step: all are evaluated (their loads overlap) before the LDS
     // table and the record writes see them.
-    constexpr int U = Ev::kRows;
+    constexpr int U = Ev::kRows * (Ev::kKeys + Ev::kAggs) <= 24 ? Ev::kRows : 1;  // register budget
     for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
       double kv[U][Ev::kKeys];
       double av[U][Ev::kAggs];

@@ -43,7 +43,8 @@ __global__ void stat_init_k(StatAcc* a) {
   }
 }
 
-__global__ void __launch_bounds__(256) col_stats_k(ws_colview c, int64_t n, const uint32_t* del, StatAcc* acc) {
+__global__ void __launch_bounds__(256) col_stats_k(ws_colview c, int64_t n, const uint32_t* del, StatAcc* acc,
+                                                   uint8_t* part_regs) {
   __shared__ unsigned regs[kHllRegs];
   for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) regs[i] = 0;
   __syncthreads();
@@ -107,8 +108,19 @@ __global__ void __launch_bounds__(256) col_stats_k(ws_colview c, int64_t n, cons
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x)
-    if (regs[i]) atomicMax(&acc->regs[i], regs[i]);
+  // registers: one partial per block (plain stores), folded by stat_regs_k
+  uint8_t* mine = part_regs + (int64_t)blockIdx.x * kHllRegs;
+  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) mine[i] = (uint8_t)regs[i];
+}
+__global__ void stat_regs_k(const uint8_t* part_regs, int nblocks, StatAcc* acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kHllRegs) return;
+  unsigned m = 0;
+  for (int b = 0; b < nblocks; ++b) {
+    const unsigned v = part_regs[(int64_t)b * kHllRegs + i];
+    m = v > m ? v : m;
+  }
+  acc->regs[i] = m;
 }
 
 }  // namespace ws
@@ -128,9 +140,11 @@ int ws_column_stats(const ws_colview* col, int64_t n, const uint32_t* del, ws_co
     if (col->t == WS_T_VEC) throw Error("column statistics: vector columns have none", WS_ERR_INTERNAL);
     DevBuf<StatAcc> acc(1);
     hipLaunchKernelGGL(stat_init_k, dim3(1), dim3(256), 0, stream(), acc.p);
-    // about one block per CU: each block flushes 2048 registers with global atomics
+    // about one block per CU; each block leaves its 2048 registers for one folding kernel
     const unsigned grid = grid_for(n, 256, 1);
-    hipLaunchKernelGGL(col_stats_k, dim3(grid), dim3(256), 0, stream(), *col, n, del, acc.p);
+    DevBuf<uint8_t> parts((int64_t)grid * kHllRegs);
+    hipLaunchKernelGGL(col_stats_k, dim3(grid), dim3(256), 0, stream(), *col, n, del, acc.p, parts.p);
+    hipLaunchKernelGGL(stat_regs_k, dim3(kHllRegs / 256), dim3(256), 0, stream(), (const uint8_t*)parts.p, (int)grid, acc.p);
     check_launch("column_stats");
     StatAcc h;
     d2h_sync(&h, acc.p, sizeof h);

@@ -634,6 +634,14 @@ struct ws_agg {
   double* oacc = nullptr;       // [records][na]
   uint32_t* pstart = nullptr;   // [P + 1] first record row of each partition
   uint32_t* goff = nullptr;     // [P + 1] first compacted group of each partition; goff[P] = groups
+  // flags, table count and partitioned group count as last read back (one round trip); valid
+  // until the next launch that can change them
+  struct Status {
+    int flags[4];
+    u64 count;
+    uint32_t groups;
+  } st{};
+  bool st_valid = false;
   // fetch state
   int64_t fetched = -1;
   DevBuf<double> fetch_buf;
@@ -661,6 +669,7 @@ struct ws_agg {
 // Allocates and clears the hash table (slots a power of two) and its group counter.
 static void ensure_table(ws_agg* A, int64_t slots) {
   if (A->keys) return;
+  A->st_valid = false;
   const int nk = A->plan.nkeys, na = std::max(A->plan.naggs, 1);
   A->tcap = slots;
   A->keys = (u64*)tmp_alloc(sizeof(u64) * (size_t)slots);
@@ -670,6 +679,19 @@ static void ensure_table(ws_agg* A, int64_t slots) {
   GTable t{A->keys, A->kv, A->acc, (u64)(slots - 1), A->flags, A->count};
   hipLaunchKernelGGL(init_table_k, dim3(grid_for(slots)), dim3(256), 0, stream(), t, nk, A->plan.naggs, A->plan);
   check_launch("init_table");
+}
+// Reads the aggregation's small results in one round trip (cached until the next launch).
+static const ws_agg::Status& status(ws_agg* A) {
+  if (A->st_valid) return A->st;
+  A->st = ws_agg::Status{};
+  D2HPiece pc[3];
+  int n = 0;
+  if (A->flags) pc[n++] = D2HPiece{A->st.flags, A->flags, sizeof A->st.flags};
+  if (A->count) pc[n++] = D2HPiece{&A->st.count, A->count, sizeof(u64)};
+  if (A->part && A->goff) pc[n++] = D2HPiece{&A->st.groups, A->goff + A->P, sizeof(uint32_t)};
+  d2h_gather_sync(pc, n);
+  A->st_valid = true;
+  return A->st;
 }
 static GTable table_of(const ws_agg* A) { return GTable{A->keys, A->kv, A->acc, (u64)(A->tcap - 1), A->flags, A->count}; }
 
@@ -848,16 +870,14 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
 // aggregation on the plain path, so the result never depends on it.
 static void settle(ws_agg* A) {
   if (!A->part || A->settled) return;
-  int f[4];
-  uint32_t g = 0;
-  d2h_sync(f, A->flags, sizeof f);
-  d2h_sync(&g, A->goff + A->P, sizeof g);
+  const ws_agg::Status& s = status(A);  // flags, spilled groups and finished groups together
   A->settled = true;
-  A->part_groups = (int64_t)g;
-  if (f[2]) {
+  A->part_groups = (int64_t)s.groups;
+  if (s.flags[2]) {
     A->free_part();
     A->free_table();
     fill_u32((uint32_t*)A->flags, 4, 0);
+    A->st_valid = false;
     A->strategy = WS_STRAT_TABLE;
     agg_run(A, A->last_env, A->last_jit, false);
   }
@@ -867,8 +887,8 @@ static void settle(ws_agg* A) {
 static void part_to_table(ws_agg* A) {
   settle(A);
   if (!A->part) return;
-  u64 spilled = 0;
-  d2h_sync(&spilled, A->count, sizeof spilled);
+  const u64 spilled = status(A).count;
+  A->st_valid = false;
   u64* okeys = A->keys;
   double *okv = A->kv, *oacc = A->acc;
   const int64_t ocap = A->tcap;
@@ -891,6 +911,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   const ws_agg_plan& plan = A->plan;
   if (&env != &A->last_env) A->last_env = env;
   A->last_jit = jit;
+  A->st_valid = false;
   if (!jit && (plan.naggs > WS_INTERP_AGGS || plan.nkeys > WS_INTERP_KEYS))
     throw Error("this aggregation needs a compiled plan (more than 16 aggregates or 4 GROUP BY keys); codegen failed or is off",
                 WS_ERR_NEEDS_PLAN);
@@ -999,12 +1020,14 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   launch_grouped(jf, 0, iv, up.ddesc, t, grid, lds, "agg_hash", "agg_hash_jit");
 }
 
-// Compacts the groups into a tmp buffer: keys then accs, row-major (fetch) or column-major.
-static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor) {
+// Compacts the groups into tmp buffers: keys then accs in `buf`, row-major (fetch) or
+// column-major; with `abuf` the accs go there instead (two separately owned buffers).
+static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor, DevBuf<double>* abuf = nullptr) {
   const int nk = A->plan.nkeys, na = A->plan.naggs;
   settle(A);
   if (!A->is_dense && !A->keys) {  // no run yet: no groups
     buf = DevBuf<double>(1);
+    if (abuf) *abuf = DevBuf<double>(1);
     return 0;
   }
   const int64_t slots = A->is_dense ? A->dense.G : A->tcap;
@@ -1012,8 +1035,7 @@ static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor) {
   int64_t g1 = 0;  // finished groups of a partitioned run come first, then any spilled ones
   bool table_groups = true;
   if (A->part) {
-    u64 spilled = 0;
-    d2h_sync(&spilled, A->count, sizeof spilled);
+    const u64 spilled = status(A).count;
     g1 = A->part_groups;
     table_groups = spilled != 0;
   }
@@ -1021,9 +1043,17 @@ static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor) {
                      : A->is_dense ? compact_dense(A->acc, na, A->dense.G, idx.p)
                                    : compact_occupied_hash(A->keys, A->tcap, idx.p);
   const int64_t g = g1 + g2;
-  buf = DevBuf<double>(std::max<int64_t>(g * (nk + na), 1));
-  double* ko = buf.p;
-  double* ao = buf.p + g * nk;
+  double *ko, *ao;
+  if (abuf) {
+    buf = DevBuf<double>(std::max<int64_t>(g * nk, 1));
+    *abuf = DevBuf<double>(std::max<int64_t>(g * na, 1));
+    ko = buf.p;
+    ao = abuf->p;
+  } else {
+    buf = DevBuf<double>(std::max<int64_t>(g * (nk + na), 1));
+    ko = buf.p;
+    ao = buf.p + g * nk;
+  }
   ProfScope ps("gather_groups");
   if (g1)
     hipLaunchKernelGGL(gather_part_k, dim3(grid_for(g1)), dim3(256), 0, stream(), (const double*)A->okv, nk, (const double*)A->oacc,
@@ -1115,13 +1145,11 @@ int ws_agg_overflowed(ws_agg* a, int* out) {
     *out = 0;
     if (a->plan.nkeys == 0) return;
     settle(a);
-    int f[4];
-    d2h_sync(f, a->flags, sizeof f);
-    if (f[1]) throw Error("dense GROUP BY key outside the planned key range", WS_ERR_INTERNAL);
-    if (!a->is_dense) *out = f[0] != 0;
+    const ws_agg::Status& s = status(a);
+    if (s.flags[1]) throw Error("dense GROUP BY key outside the planned key range", WS_ERR_INTERNAL);
+    if (!a->is_dense) *out = s.flags[0] != 0;
     if (!*out && a->has_features && !a->logged && !a->is_dense) {  // the exact count, for the sizing model
-      u64 c = 0;
-      if (a->keys) d2h_sync(&c, a->count, sizeof c);
+      const u64 c = a->keys ? s.count : 0;
       sizing_log(a->features, (int64_t)c + (a->part ? a->part_groups : 0));
       a->logged = true;
     }
@@ -1181,15 +1209,8 @@ int ws_agg_sum_high(const ws_agg* a, int64_t* out) {
 }
 int ws_agg_compact(ws_agg* a, double** keys, double** accs, int64_t* groups) {
   return guard([&] {
-    DevBuf<double> buf;
-    const int64_t g = agg_compact_into(a, buf, true);
-    const int nk = a->plan.nkeys, na = a->plan.naggs;
-    // Two separately owned buffers (the caller frees each): split the column-major result.
-    DevBuf<double> k(std::max<int64_t>(g * nk, 1)), ac(std::max<int64_t>(g * na, 1));
-    if (g) {
-      WS_CK(hipMemcpyAsync(k.p, buf.p, sizeof(double) * g * nk, hipMemcpyDeviceToDevice, stream()));
-      WS_CK(hipMemcpyAsync(ac.p, buf.p + g * nk, sizeof(double) * g * na, hipMemcpyDeviceToDevice, stream()));
-    }
+    DevBuf<double> k, ac;  // two separately owned buffers (the caller frees each)
+    const int64_t g = agg_compact_into(a, k, true, &ac);
     *keys = k.release();
     *accs = ac.release();
     *groups = g;
@@ -1256,6 +1277,7 @@ int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs,
       bq.flags = A->flags;
       *blob.at<BatchQuery>(o_q + sizeof(BatchQuery) * q) = bq;
       A->fetched = -1;
+      A->st_valid = false;
       ++A->runs;
     }
     h2d_async_staged(dev, blob.host.data(), blob.host.size());
@@ -1323,8 +1345,7 @@ int64_t ws_agg_capacity(const ws_agg* a) { return a->cap; }
 int ws_agg_group_count(ws_agg* a, int64_t* out) {
   return guard([&] {
     settle(a);
-    u64 c = 0;
-    if (a->count && (a->keys || a->is_dense)) d2h_sync(&c, a->count, sizeof c);
+    const u64 c = a->keys || a->is_dense ? status(a).count : 0;
     *out = (int64_t)c + (a->part ? a->part_groups : 0);
   });
 }

@@ -92,6 +92,14 @@ uint8_t* pinned_stage(size_t bytes);
 void pinned_stage_done();  // records that copies issued from the stage are enqueued
 void h2d_sync(void* dst, const void* src, size_t bytes);
 void d2h_sync(void* dst, const void* src, size_t bytes);
+// Several small device values (4-byte multiples, 64 KiB in all) read back in one round trip.
+constexpr int kD2HMaxPieces = 8;
+struct D2HPiece {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+void d2h_gather_sync(const D2HPiece* pieces, int n);
 void h2d_async_staged(void* dst, const void* src, size_t bytes);  // via the pinned stage
 
 // Byte blob assembled on the host and uploaded in one copy; offsets are resolved to device

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -233,10 +234,70 @@ void h2d_sync(void* dst, const void* src, size_t bytes) {
   WS_CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
   WS_CK(hipStreamSynchronize(stream()));
 }
+// Pinned per-thread landing buffer for small device-to-host reads: a copy into pageable memory
+// goes through the runtime's own staging and costs an extra wait.
+constexpr size_t kLandBytes = 64 << 10;
+struct Landing {
+  uint8_t* buf = nullptr;
+  uint32_t* dev = nullptr;  // the same memory as a kernel sees it
+  ~Landing() {
+    if (buf) (void)hipHostFree(buf);  // every read through it ended with a stream sync
+  }
+};
+static thread_local Landing t_land;
+static Landing& landing() {
+  ensure_init();
+  if (!t_land.buf) {
+    WS_CK(hipHostMalloc((void**)&t_land.buf, kLandBytes, hipHostMallocDefault));
+    WS_CK(hipHostGetDevicePointer((void**)&t_land.dev, t_land.buf, 0));
+  }
+  return t_land;
+}
 void d2h_sync(void* dst, const void* src, size_t bytes) {
   if (!bytes) return;
+  if (bytes <= kLandBytes) {
+    Landing& l = landing();
+    WS_CK(hipMemcpyAsync(l.buf, src, bytes, hipMemcpyDeviceToHost, stream()));
+    WS_CK(hipStreamSynchronize(stream()));
+    std::memcpy(dst, l.buf, bytes);
+    return;
+  }
   WS_CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
   WS_CK(hipStreamSynchronize(stream()));
+}
+
+// One kernel copies every piece into the landing buffer, then one stream sync: a single round trip
+// for the handful of counters and flags a query step reads back.
+struct D2HList {
+  const uint32_t* src[kD2HMaxPieces];
+  uint32_t words[kD2HMaxPieces];
+  uint32_t off[kD2HMaxPieces];
+  int n;
+};
+__global__ void d2h_gather_k(D2HList l, uint32_t* dst) {
+  for (int i = 0; i < l.n; ++i)
+    for (uint32_t w = threadIdx.x; w < l.words[i]; w += blockDim.x) dst[l.off[i] + w] = l.src[i][w];
+}
+void d2h_gather_sync(const D2HPiece* pieces, int n) {
+  if (n <= 0) return;
+  if (n > kD2HMaxPieces) throw Error("d2h_gather_sync: too many pieces", WS_ERR_INTERNAL);
+  D2HList l{};
+  uint32_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    if (pieces[i].bytes % 4 || (uintptr_t)pieces[i].src % 4)
+      throw Error("d2h_gather_sync: pieces must be 4-byte multiples", WS_ERR_INTERNAL);
+    l.src[i] = (const uint32_t*)pieces[i].src;
+    l.words[i] = (uint32_t)(pieces[i].bytes / 4);
+    l.off[i] = off;
+    off += l.words[i];
+  }
+  if ((size_t)off * 4 > kLandBytes) throw Error("d2h_gather_sync: pieces too large", WS_ERR_INTERNAL);
+  l.n = n;
+  Landing& land = landing();
+  hipLaunchKernelGGL(d2h_gather_k, dim3(1), dim3(256), 0, stream(), l, land.dev);
+  check_launch("d2h_gather");
+  WS_CK(hipStreamSynchronize(stream()));
+  for (int i = 0; i < n; ++i) std::memcpy(pieces[i].dst, land.buf + (size_t)l.off[i] * 4, pieces[i].bytes);
 }
 void h2d_async_staged(void* dst, const void* src, size_t bytes) {
   if (!bytes) return;

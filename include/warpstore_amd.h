@@ -214,6 +214,10 @@ int ws_cross_join(int64_t na, int64_t nb, int32_t** out_a, int32_t** out_b, int6
 /* gpuSortPerm gpu.h:197: stable multi-key sort (keys[0] most significant, desc[i] != 0 descending),
  * *perm is a tmp buffer of m int32. */
 int ws_sort_perm(const double* const* keys, const uint8_t* desc, int nkeys, int64_t m, int32_t** perm);
+/* ORDER BY ... LIMIT k (engine_select.cpp:1320-1342 sorts all m rows and keeps perm[0, k)):
+ * *perm (a tmp buffer of min(k, m) int32) is exactly the first min(k, m) entries of ws_sort_perm's
+ * permutation, found by selection for up to 4 keys and k <= 256 (else by the full sort). */
+int ws_topk_perm(const double* const* keys, const uint8_t* desc, int nkeys, int64_t m, int64_t k, int32_t** perm);
 int ws_map_rank(double* keys, const int32_t* rank, int64_t m);                      /* gpuMapRank gpu.h:199 */
 int ws_sort_perm_host(const void* host_col, int ptype, int64_t n, int32_t* out);    /* gpuSortPermHost gpu.h:201 */
 

@@ -351,6 +351,15 @@ inline DevVec<int32_t> gpuSortPerm(const std::vector<std::pair<const double*, bo
   wsd::ck(ws_sort_perm(k.data(), desc.data(), (int)k.size(), m, &perm));
   return DevVec<int32_t>::adopt(perm, m);
 }
+// First min(k, m) entries of gpuSortPerm(keys, m) (ORDER BY ... LIMIT k).
+inline DevVec<int32_t> gpuTopKPerm(const std::vector<std::pair<const double*, bool>>& keys, int64_t m, int64_t k) {
+  std::vector<const double*> kp;
+  std::vector<uint8_t> desc;
+  for (auto& x : keys) kp.push_back(x.first), desc.push_back(x.second ? 1 : 0);
+  int32_t* perm = nullptr;
+  wsd::ck(ws_topk_perm(kp.data(), desc.data(), (int)kp.size(), m, k, &perm));
+  return DevVec<int32_t>::adopt(perm, std::min(k, m));
+}
 inline void gpuMapRank(double* keys, const int32_t* rank, int64_t m) { wsd::ck(ws_map_rank(keys, rank, m)); }
 inline std::vector<int32_t> gpuSortPermHost(const void* hostCol, PType t, int64_t n) {
   std::vector<int32_t> out((size_t)n);

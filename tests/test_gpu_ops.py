@@ -171,6 +171,40 @@ def test_sort_perm(gpu, m):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("m,k,nk", [(476_610, 10, 1), (2_000_000, 16, 1), (100_000, 1, 3), (50_000, 256, 2),
+                                    (300_001, 257, 1), (70_000, 100, 4), (70_000, 12, 5), (9_000, 1_000, 1),
+                                    (5_000, 7, 2), (3, 10, 1), (1_000_003, 64, 1)])
+def test_topk_perm_is_sort_prefix(gpu, m, k, nk):
+    """ORDER BY ... LIMIT k: the first k of the stable multi-key sort, ties (few distinct values),
+    NULLs (NaN), -0.0 and descending keys included; selection (up to 4 keys, k <= 256) and the
+    full-sort fallbacks (k > 256, 5 keys, k not much below m)."""
+    H.say(f"case topk m={m} k={k} nk={nk}")
+    rng = np.random.default_rng(m + k + nk)
+    keys = []
+    for j in range(nk):
+        if j == 0:
+            v = rng.integers(0, 40, m).astype(np.float64)  # heavy ties on the first key
+        elif j % 2:
+            v = rng.standard_normal(m)
+            v[rng.random(m) < 0.05] = -0.0
+            v[rng.random(m) < 0.05] = 0.0
+        else:
+            v = np.floor(rng.random(m) * 3)
+        v[rng.random(m) < 0.03] = np.nan
+        keys.append(v)
+    desc = np.array([(1 + j) % 2 for j in range(nk)], np.uint8)
+    dks = [ws.DevBuf.from_array(v) for v in keys]
+    kp = (C.c_void_p * nk)(*[d.p for d in dks])
+    perm = C.c_void_p()
+    ws.check(gpu.ws_topk_perm(kp, ws.ptr(desc), nk, m, k, C.byref(perm)))
+    kk = min(k, m)
+    got = ws.DevBuf(kk * 4, tmp=True, adopt=perm.value).to_array(np.int32, kk)
+    exp = np.zeros(m, np.int32)
+    hp = (C.c_void_p * nk)(*[v.ctypes.data for v in keys])
+    H.oracle().or_sort_perm(hp, ws.ptr(desc), nk, m, ws.ptr(exp))
+    assert np.array_equal(got, exp[:kk])
+
+
 def test_gathers_and_codes(gpu):
     H.say("case gathers")
     rng = np.random.default_rng(19)

@@ -82,7 +82,7 @@ _SIGS = {
     "ws_hash_join_parts": (I, [ColView, P, I, ColView, P, I, PP, PP, P, C.POINTER(I64)]),
     "ws_hash_join_outer": (I, [ColView, I64, ColView, I64, PP, PP, I, C.POINTER(I64)]),
     "ws_cross_join": (I, [I64, I64, PP, PP, C.POINTER(I64)]),
-    "ws_sort_perm": (I, [P, P, I, I64, PP]), "ws_map_rank": (I, [P, P, I64]),
+    "ws_sort_perm": (I, [P, P, I, I64, PP]), "ws_topk_perm": (I, [P, P, I, I64, I64, PP]), "ws_map_rank": (I, [P, P, I64]),
     "ws_sort_perm_host": (I, [P, I, I64, P]),
     "ws_jit_enabled": (I, []), "ws_jit_mode": (I, []), "ws_jit_set_mode": (None, [I]),
     "ws_jit_set_default_mode": (None, [I]), "ws_jit_set_cache_dir": (None, [C.c_char_p]),

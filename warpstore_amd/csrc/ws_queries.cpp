@@ -417,7 +417,7 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
   const int64_t mc = std::min<int64_t>(g, limit);
   DevVec<double> okey(std::max<int64_t>(g, 1));
   gpuEvalAt(penv, order, nullptr, g, okey.p);
-  DevVec<int32_t> perm = gpuSortPerm({{okey.p, true}}, g);
+  DevVec<int32_t> perm = gpuTopKPerm({{okey.p, true}}, g, mc);
   const int W = nk + plan2.naggs;
   DevVec<double> tmp(std::max<int64_t>(mc * W, 1));
   for (int col = 0; col < W; ++col)

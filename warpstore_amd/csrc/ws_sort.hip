@@ -196,6 +196,180 @@ static DevBuf<int32_t> sort_perm(const std::vector<std::pair<const double*, bool
   return perm;
 }
 
+// ------------------------------------------------------------------ ORDER BY ... LIMIT k
+// The first k entries of the stable sort permutation without sorting all m rows: rows rank by
+// (order keys, row index), a total order whose first k are exactly the stable sort's. A block
+// keeps its best k so far in LDS, with the k-th as a threshold. A step offers 256 rows (one per
+// thread); those that rank before the threshold are appended after the list, and every entry of
+// the union computes its new rank in parallel (a list entry: its position plus the appended
+// entries before it; an appended entry: a binary search in the list plus the appended entries
+// before it) and moves to that slot of the other buffer if the rank is below k. After the first
+// steps few rows pass the threshold, so a step costs little more than reading its rows. A second
+// launch of one block selects the final k from the blocks' lists.
+constexpr int kTopMaxK = 256, kTopMaxKeys = 4, kTopThreads = 256, kTopSlots = kTopMaxK + kTopThreads;
+
+struct TopIn {
+  const double* key[kTopMaxKeys];  // first pass: the key columns
+  int desc[kTopMaxKeys];
+  const u64* cok;     // second pass: candidates' order keys [n][nk]
+  const int32_t* ci;  // and row indexes [n] (INT32_MAX: none)
+};
+
+template <int NK>
+struct TopRow {
+  u64 k[NK];
+  int32_t i;  // INT32_MAX: no row
+};
+template <int NK>
+__device__ __forceinline__ bool top_before(const TopRow<NK>& a, const TopRow<NK>& b) {
+#pragma unroll
+  for (int j = 0; j < NK; ++j)
+    if (a.k[j] != b.k[j]) return a.k[j] < b.k[j];
+  return a.i < b.i;
+}
+
+template <int NK, bool kFirst>
+__device__ __forceinline__ TopRow<NK> top_load(const TopIn& in, int64_t r, int64_t r1) {
+  TopRow<NK> e;
+  e.i = INT32_MAX;
+  if (r < r1) {
+    if (kFirst) {
+#pragma unroll
+      for (int j = 0; j < NK; ++j) e.k[j] = order_key(in.key[j][r], in.desc[j] != 0);
+      e.i = (int32_t)r;
+    } else {
+      e.i = in.ci[r];
+#pragma unroll
+      for (int j = 0; j < NK; ++j) e.k[j] = in.cok[r * NK + j];
+    }
+  }
+  return e;
+}
+
+// LDS candidate buffer: key j of slot x at k[j][x] (a broadcast read per key), index at i[x].
+template <int NK>
+struct TopSlots {
+  u64 k[NK][kTopSlots];
+  int32_t i[kTopSlots];
+  __device__ TopRow<NK> get(int x) const {
+    TopRow<NK> e;
+#pragma unroll
+    for (int j = 0; j < NK; ++j) e.k[j] = k[j][x];
+    e.i = i[x];
+    return e;
+  }
+  __device__ void put(int x, const TopRow<NK>& e) {
+#pragma unroll
+    for (int j = 0; j < NK; ++j) k[j][x] = e.k[j];
+    i[x] = e.i;
+  }
+};
+
+// New rank of slot x of the union of the sorted list [0, cnt) and the new entries [cnt, n):
+// an old entry moves up by the new entries before it; a new entry's rank is its lower bound in
+// the old list plus the new entries before it.
+template <int NK>
+__device__ __forceinline__ int top_rank(const TopSlots<NK>& b, int x, int cnt, int n) {
+  const TopRow<NK> me = b.get(x);
+  int rank;
+  if (x < cnt) {
+    rank = x;
+  } else {
+    int lo = 0, hi = cnt;  // first old entry not before me
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (top_before(b.get(mid), me)) lo = mid + 1;
+      else hi = mid;
+    }
+    rank = lo;
+  }
+#pragma unroll 8
+  for (int y = cnt; y < n; ++y) rank += top_before(b.get(y), me);
+  return rank;
+}
+
+template <int NK, bool kFirst>
+__global__ void __launch_bounds__(kTopThreads) topk_k(TopIn in, int64_t m, int64_t per_block, int k, u64* out_ok,
+                                                      int32_t* out_idx) {
+  __shared__ TopSlots<NK> s_b[2];
+  __shared__ int s_c;
+  const int t = threadIdx.x;
+  if (t == 0) s_c = 0;
+  int cur = 0, cnt = 0;  // the list: entries [0, cnt) of buffer cur, ascending
+  TopRow<NK> thr;        // its k-th entry, once cnt == k
+  const int64_t r0 = (int64_t)blockIdx.x * per_block, r1 = r0 + per_block < m ? r0 + per_block : m;
+  TopRow<NK> nxt = top_load<NK, kFirst>(in, r0 + t, r1);
+  __syncthreads();
+  for (int64_t base = r0; base < r1; base += kTopThreads) {
+    const TopRow<NK> e = nxt;
+    nxt = top_load<NK, kFirst>(in, base + kTopThreads + t, r1);  // next step's row, in flight meanwhile
+    if (e.i != INT32_MAX && (cnt < k || top_before(e, thr)))
+      s_b[cur].put(cnt + __hip_atomic_fetch_add(&s_c, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), e);
+    __syncthreads();
+    const int c = s_c;
+    __syncthreads();  // every thread has read the count before it is reset or appended to again
+    if (c == 0) continue;  // (uniform)
+    if (t == 0) s_c = 0;
+    const int n = cnt + c, o = cur ^ 1;
+    for (int x = t; x < n; x += kTopThreads) {
+      const int rank = top_rank(s_b[cur], x, cnt, n);
+      if (rank < k) s_b[o].put(rank, s_b[cur].get(x));
+    }
+    __syncthreads();
+    cur = o;
+    cnt = n < k ? n : k;
+    if (cnt == k) thr = s_b[cur].get(k - 1);
+    // (the next step appends from slot cnt >= k on, past the threshold entry read here)
+  }
+  for (int x = t; x < k; x += kTopThreads) {
+    const int64_t ob = (int64_t)blockIdx.x * k + x;
+    const TopRow<NK> e = s_b[cur].get(x < cnt ? x : 0);
+#pragma unroll
+    for (int j = 0; j < NK; ++j) out_ok[ob * NK + j] = x < cnt ? e.k[j] : ~0ull;
+    out_idx[ob] = x < cnt ? e.i : INT32_MAX;
+  }
+}
+
+template <int NK>
+static void topk_launch(TopIn in, int64_t m, int64_t kk, int32_t* out) {
+  // first pass: about 8 steps per block, at most one block per CU; second pass: one block
+  const int64_t want = std::max<int64_t>(1, std::min<int64_t>((m + 8 * kTopThreads - 1) / (8 * kTopThreads), cu_count()));
+  const int64_t per = (m + want - 1) / want;
+  const int64_t nb = (m + per - 1) / per;
+  DevBuf<u64> cok(nb * kk * NK), fok(kk * NK);
+  DevBuf<int32_t> ci(nb * kk);
+  hipLaunchKernelGGL((topk_k<NK, true>), dim3((unsigned)nb), dim3(kTopThreads), 0, stream(), in, m, per, (int)kk, cok.p, ci.p);
+  in.cok = cok.p;
+  in.ci = ci.p;
+  hipLaunchKernelGGL((topk_k<NK, false>), dim3(1), dim3(kTopThreads), 0, stream(), in, nb * kk, nb * kk, (int)kk, fok.p, out);
+  check_launch("topk");
+}
+
+// First min(k, m) entries of sort_perm(keys, m), by selection when k and the key count allow.
+static DevBuf<int32_t> topk_perm(const std::vector<std::pair<const double*, bool>>& keys, int64_t m, int64_t k) {
+  const int64_t kk = std::min(k, m);
+  const int nk = (int)keys.size();
+  if (kk <= 0 || nk == 0) return DevBuf<int32_t>(1);
+  if (kk > kTopMaxK || nk > kTopMaxKeys || kk * 16 >= m) {  // a full sort is as cheap
+    DevBuf<int32_t> perm = sort_perm(keys, m);
+    if (kk == m) return perm;
+    DevBuf<int32_t> out(kk);
+    WS_CK(hipMemcpyAsync(out.p, perm.p, 4 * (size_t)kk, hipMemcpyDeviceToDevice, stream()));
+    return out;
+  }
+  ProfScope ps("topk_perm");
+  TopIn in{};
+  for (int j = 0; j < nk; ++j) in.key[j] = keys[j].first, in.desc[j] = keys[j].second ? 1 : 0;
+  DevBuf<int32_t> out(kk);
+  switch (nk) {
+    case 1: topk_launch<1>(in, m, kk, out.p); break;
+    case 2: topk_launch<2>(in, m, kk, out.p); break;
+    case 3: topk_launch<3>(in, m, kk, out.p); break;
+    default: topk_launch<4>(in, m, kk, out.p); break;
+  }
+  return out;
+}
+
 template <class T>
 __global__ void to_double_k(const T* src, int64_t n, double* dst) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) dst[j] = (double)src[j];
@@ -214,6 +388,16 @@ int ws_sort_perm(const double* const* keys, const uint8_t* desc, int nkeys, int6
     for (int i = 0; i < nkeys; ++i) k.push_back({keys[i], desc && desc[i]});
     if (m > INT32_MAX) throw Error("sort: too many rows", WS_ERR_INTERNAL);
     *perm = sort_perm(k, m).release();
+  });
+}
+int ws_topk_perm(const double* const* keys, const uint8_t* desc, int nkeys, int64_t m, int64_t k, int32_t** perm) {
+  return guard([&] {
+    *perm = nullptr;
+    std::vector<std::pair<const double*, bool>> kv;
+    for (int i = 0; i < nkeys; ++i) kv.push_back({keys[i], desc && desc[i]});
+    if (m > INT32_MAX - 1) throw Error("sort: too many rows", WS_ERR_INTERNAL);
+    if (k < 0) throw Error("top-k: negative k", WS_ERR_INTERNAL);
+    *perm = topk_perm(kv, m, k).release();
   });
 }
 int ws_sort_perm_host(const void* host_col, int ptype, int64_t n, int32_t* out) {

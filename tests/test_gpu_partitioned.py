@@ -295,3 +295,58 @@ def test_partitioned_run_twice_migrates(gpu):
     finally:
         L.ws_agg_destroy(h)
     H.assert_groups_equal(plan, keys.reshape(-1, 2), accs.reshape(-1, 2), ok, oa * 2)
+
+
+@pytest.mark.parametrize("case", ["narrow", "negative_wide", "full_range", "nullable"])
+def test_partitioned_packed_records(gpu, case):
+    """Integer keys: packed partial-group records (the keys' ranges fit 63 bits: narrow, negative
+    and wide) or the full layout (full-range int64 pairs, a nullable key whose NULL sentinel spans
+    the range), each against the oracle and against the run with packing switched off."""
+    H.say(f"case partitioned packed records {case}")
+    n = 500_000
+    rng = np.random.default_rng(len(case))
+    if case == "narrow":
+        k1, k2 = rng.integers(0, 3000, n), rng.integers(0, 200, n)
+    elif case == "negative_wide":
+        k1 = rng.integers(-(1 << 40), 1 << 40, n) // 100_003 * 100_003  # ~22M distinct over 41 bits
+        k1[rng.random(n) < 0.5] = -(1 << 40)
+        k2 = rng.integers(-7, 9, n)
+    elif case == "full_range":
+        k1 = rng.integers(-(1 << 62), 1 << 62, n) * 2
+        k1[:100] = np.iinfo(np.int64).min
+        k1[100:200] = np.iinfo(np.int64).max
+        k2 = rng.integers(-(1 << 62), 1 << 62, n)
+        k1[rng.random(n) < 0.7] = 12345  # repeats, so groups merge
+        k2[rng.random(n) < 0.7] = -9
+    else:
+        k1, k2 = rng.integers(0, 50_000, n), rng.integers(0, 10, n)
+        k1[rng.random(n) < 0.05] = abi.NULL_I64
+    k1, k2 = k1.astype(np.int64), k2.astype(np.int64)
+    val = rng.integers(-1000, 1000, n).astype(np.int64)
+    t = H.Table({"k1": k1, "k2": k2, "val": val}, [200_000, 300_000])
+    b = abi.EnvBuilder()
+    t1 = abi.T_I64N if case == "nullable" else abi.T_I64
+    s1, s2, sv = b.slot(t1), b.slot(abi.T_I64), b.slot(abi.T_I64)
+    plan = abi.AggPlan()
+    plan.nkeys = 2
+    plan.keys[0], plan.keys[1] = b.col(s1), b.col(s2)
+    plan.key_int[0] = plan.key_int[1] = 1
+    plan.naggs = 3
+    plan.ops[0], plan.ops[1], plan.ops[2] = abi.AGG_COUNT, abi.AGG_SUM, abi.AGG_MIN
+    plan.agg_int[0] = plan.agg_int[1] = plan.agg_int[2] = 1
+    plan.aggs[1] = plan.aggs[2] = b.col(sv)
+    plan.agg_bits[1] = plan.agg_bits[2] = 1
+    slots = [("k1", t1, 0), ("k2", abi.T_I64, 0), ("val", abi.T_I64, 0)]
+    dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, host)
+    L = ws.lib()
+    for pack in (1, 0):
+        assert L.ws_tune_set(b"part_pack", pack) == 0
+        try:
+            for mode in ("interp", "jit"):
+                jit = jit_for(gpu, b.env, len(slots), plan) if mode == "jit" else None
+                gk, ga, _, _ = H.gpu_aggregate(plan, b.env, dev, cap=1 << 20, jit=jit, strategy=STRAT_PART)
+                assert H.LAST_RUN["partitioned"]
+                H.assert_groups_equal(plan, gk, ga, ok, oa)
+        finally:
+            assert L.ws_tune_set(b"part_pack", 1) == 0

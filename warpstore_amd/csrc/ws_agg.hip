@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <utility>
 
 #include "ws_internal.h"
 
@@ -158,8 +159,59 @@ __global__ void __launch_bounds__(1024) part_offsets_k(const uint32_t* cnt, uint
   }
 }
 
+// Chooses the packed record layout from the integer keys' ranges (thread 0 of part_start_k;
+// range = [min, max] per key over the whole input).
+__device__ void pack_plan(const long long* range, int nk, PackPlan* pk) {
+  PackPlan q{};
+  int bits = 0;
+  bool fits = nk <= kPackKeys;
+  for (int k = 0; k < nk && fits; ++k) {
+    const i64 lo = range[k], hi = range[kPackKeys + k];
+    int w = 0;
+    if (lo <= hi) {
+      const u64 span = (u64)hi - (u64)lo;  // hi - lo without overflow
+      w = span ? 64 - __clzll((long long)span) : 0;
+    }
+    q.shift[k] = bits;
+    q.mask[k] = w >= 64 ? ~0ull : (1ull << w) - 1;
+    q.kmin[k] = lo <= hi ? lo : 0;
+    bits += w;
+    fits = bits <= 63;
+  }
+  q.ok = fits ? 1 : 0;
+  *pk = q;
+}
+
 // pstart[p] = exclusive prefix of ptot over partitions, pstart[P] = total (one block, P <= 4096).
-__global__ void __launch_bounds__(1024) part_start_k(const uint32_t* ptot, uint32_t* pstart, int P) {
+// With a pack plan to fill, the block also folds the count pass's per-chunk key ranges
+// ([nchunks][2 * nk]) and thread 0 chooses the record layout (the scatter pass, next, reads it).
+__global__ void __launch_bounds__(1024) part_start_k(const uint32_t* ptot, uint32_t* pstart, int P,
+                                                     const int64_t* key_range = nullptr, int64_t nchunks = 0, int nk = 0,
+                                                     PackPlan* pack = nullptr) {
+  __shared__ long long s_rng[2 * kPackKeys];
+  if (pack) {
+    if (threadIdx.x < 2 * kPackKeys) s_rng[threadIdx.x] = threadIdx.x < kPackKeys ? kI64Max : kI64Min;
+    __syncthreads();
+    for (int k = 0; k < nk && k < kPackKeys; ++k) {
+      i64 lo = kI64Max, hi = kI64Min;
+      for (int64_t c = threadIdx.x; c < nchunks; c += blockDim.x) {
+        const i64 a = key_range[c * 2 * nk + k], b = key_range[c * 2 * nk + nk + k];
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const i64 a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+      }
+      if ((threadIdx.x & 63) == 0) {
+        atomicMin(&s_rng[k], (long long)lo);
+        atomicMax(&s_rng[kPackKeys + k], (long long)hi);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) pack_plan(s_rng, nk, pack);
+  }
   __shared__ uint32_t s_w[16];
   const int per = (P + 1023) / 1024;
   const int i0 = threadIdx.x * per;
@@ -193,11 +245,12 @@ __device__ __forceinline__ double wave_combine_slot(bool bits, int op, double v)
 struct PartMerge {
   const u64* recs;          // [records][W]
   const uint32_t* pstart;   // [P + 1]
-  int W, NK, NA, S;         // record words, keys, slots, LDS table slots
+  int W, NK, NA, S;         // record words (full layout), keys, slots, LDS table slots
   u64 kinds;                // slot kinds (slot_kind)
   double* okv;              // [records][NK] finished groups: partition p's at rows pstart[p] + rank
   double* oacc;             // [records][NA]
   uint32_t* gcnt;           // [P] groups finished per partition
+  const PackPlan* pack;     // the layout the scatter pass used (null: full records)
 };
 
 // Row of the partitioned result that holds group i of the compacted order (groups of partition p
@@ -243,29 +296,24 @@ __device__ __forceinline__ double wave_combine_kind(int k, double v) {
   return (k & 4) ? bitsd(wave_combine_i(k & 3, dbits(v))) : wave_combine_d(k & 3, v);
 }
 
-// Inserts one record (held in registers) into the spill table; false when the table overflowed.
-template <int W>
-__device__ __forceinline__ bool spill_insert(const GTable& t, u64 kinds, int NK, const u64 (&rec)[W], unsigned* new_groups) {
-  const int NA = W - 1 - NK;
-  const u64 h = rec[0];
+// Inserts one group (hash, key bits, slot values in registers) into the spill table; false when
+// the table overflowed.
+template <int NA_MAX>
+__device__ __forceinline__ bool spill_insert(const GTable& t, u64 kinds, int NK, int NA, u64 h, const u64* key,
+                                             const u64 (&val)[NA_MAX], unsigned* new_groups) {
   u64 s = h & t.mask;
   for (int probe = 0; probe < 256; ++probe) {
     u64 prev = kEmpty;
     const bool claimed = __hip_atomic_compare_exchange_strong(&t.keys[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT);
     if (claimed) {
-#pragma unroll
-      for (int j = 1; j < W; ++j)
-        if (j <= NK) t.kv[s * NK + (j - 1)] = bitsd((i64)rec[j]);
+      for (int k = 0; k < NK; ++k) t.kv[s * NK + k] = bitsd((i64)key[k]);
       __hip_atomic_fetch_add(new_groups, 1u, __ATOMIC_RELAXED, WS_LDS);
     }
     if (claimed || prev == h) {
 #pragma unroll
-      for (int j = 1; j < W; ++j)
-        if (j > NK) {
-          const int a = j - 1 - NK;
-          atomic_combine_kind<WS_DEV>(slot_kind(kinds, a), &t.acc[s * NA + a], bitsd((i64)rec[j]));
-        }
+      for (int a = 0; a < NA_MAX; ++a)
+        if (a < NA) atomic_combine_kind<WS_DEV>(slot_kind(kinds, a), &t.acc[s * NA + a], bitsd((i64)val[a]));
       return true;
     }
     s = (s + 1) & t.mask;
@@ -274,26 +322,50 @@ __device__ __forceinline__ bool spill_insert(const GTable& t, u64 kinds, int NK,
   return false;
 }
 
+// Key hash of the unpacked keys of a packed record: the formula of row_keys over integer keys.
+__device__ __forceinline__ u64 packed_keys(const PackPlan& pk, int NK, u64 w, u64* key) {
+  u64 h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+  for (int k = 0; k < kPackKeys; ++k) {
+    if (k >= NK) break;
+    key[k] = (u64)((i64)((w >> pk.shift[k]) & pk.mask[k]) + pk.kmin[k]);
+    h = mix64(h ^ key[k]) + (u64)k;
+  }
+  return h == kEmpty ? h ^ 1 : h;
+}
+
 // Merge pass, for records of W words: block p combines partition p's records in an LDS table of
 // m.S slots (linear probing from a multiply-shift index of the hash's low word; the partition is
-// its high bits). Each thread loads R records into registers before combining them one by one
-// (a register queue shifted per record), so a block keeps many loads in flight with one copy of
-// the combining code. Lanes of a wave holding the same group as the wave's first live lane fold
-// into that lane first: a hot group costs one LDS update per wave instead of one per lane. A
-// record that finds no LDS slot in 64 probes goes to the spill table (device-scope atomics, as
-// the plain path). At the end the block writes its groups inside the partition's own record range
-// and its group count to gcnt[p]; no block waits on another or on a shared counter.
-template <int W>
+// its high bits). Full records are [hash, key values, slot values]; packed ones (PK) are [packed
+// keys, slot values] and the hash is recomputed from the unpacked keys, so a group has the same
+// identity either way; the LDS table then keeps the packed word instead of the key values. Each
+// thread loads R records into registers before combining them one by one (a register queue
+// shifted per record), so a block keeps many loads in flight with one copy of the combining code.
+// Lanes of a wave holding the same group as the wave's first live lane fold into that lane first:
+// a hot group costs one LDS update per wave instead of one per lane. A record that finds no LDS
+// slot in 64 probes goes to the spill table (device-scope atomics, as the plain path). At the end
+// the block writes its groups inside the partition's own record range and its group count to
+// gcnt[p]; no block waits on another or on a shared counter. Launched for both layouts; the one
+// that does not match the scatter pass's choice (m.pack) returns at once.
+template <int W, bool PK>
 __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   constexpr int R = W <= 3 ? 8 : W <= 6 ? 4 : W <= 10 ? 2 : 1;  // records per thread per step
+  const bool packed = m.pack && m.pack->ok;
+  if (packed != PK) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
   __shared__ unsigned s_new;
   __shared__ uint32_t s_w[8];
-  const int NK = m.NK, NA = W - 1 - NK, S = m.S;
+  const int NK = m.NK, S = m.S;
+  const int KW = PK ? 1 : NK;             // key words per record / LDS slot
+  const int NA = W - 1 - (PK ? 0 : NK);   // slot values per record
+  constexpr int VA = PK ? W - 1 : W - 2;  // register bound on slot values (NK >= 1)
+  constexpr int KB = PK ? kPackKeys : (W > 1 ? W - 1 : 1);  // register bound on key words
+  PackPlan pk{};
+  if (PK) pk = *m.pack;
   const u64 kinds = m.kinds;
   u64* lk = (u64*)ws_smem;
-  double* lkv = (double*)(lk + S);
-  double* lacc = lkv + (int64_t)S * NK;
+  u64* lkey = lk + S;                                 // [S][KW]: key bits, or the packed word
+  double* lacc = (double*)(lkey + (int64_t)S * KW);  // [S][NA]
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     lk[i] = kEmpty;
     for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = kind_identity(slot_kind(kinds, a));
@@ -319,8 +391,19 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
       for (int r2 = 0; r2 + 1 < R; ++r2)
 #pragma unroll
         for (int j = 0; j < W; ++j) q[r2][j] = q[r2 + 1][j];
-      const u64 h = rec[0];
-      bool live = h != kEmpty;
+      bool live = rec[0] != kEmpty;
+      u64 key[KB] = {};  // key bits (unpacked, or copied for a spill)
+      const u64 h = !live ? kEmpty : PK ? packed_keys(pk, NK, rec[0], key) : rec[0];
+      u64 val[VA > 0 ? VA : 1];
+#pragma unroll
+      for (int a = 0; a < (VA > 0 ? VA : 1); ++a) {
+        const int j = 1 + (PK ? 0 : NK) + a;
+        u64 v = 0;
+#pragma unroll
+        for (int jj = 1; jj < W; ++jj)
+          if (jj == j) v = rec[jj];
+        val[a] = v;
+      }
       const u64 lv = __ballot(live);
       if (lv) {
         const int L = __ffsll((unsigned long long)lv) - 1;
@@ -329,11 +412,11 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
         if (__popcll(same) > 1) {
           const bool in = (same >> lane) & 1;
 #pragma unroll
-          for (int j = 1; j < W; ++j) {
-            if (j <= NK) continue;
-            const int k = slot_kind(kinds, j - 1 - NK);
-            const double v = wave_combine_kind(k, in ? bitsd((i64)rec[j]) : kind_identity(k));
-            if (lane == L) rec[j] = (u64)dbits(v);
+          for (int a = 0; a < (VA > 0 ? VA : 1); ++a) {
+            if (a >= NA) break;
+            const int k = slot_kind(kinds, a);
+            const double v = wave_combine_kind(k, in ? bitsd((i64)val[a]) : kind_identity(k));
+            if (lane == L) val[a] = (u64)dbits(v);
           }
           if (in && lane != L) live = false;
         }
@@ -346,9 +429,13 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
         const bool claimed = __hip_atomic_compare_exchange_strong(&lk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
         if (claimed) {
+          if (PK) {
+            lkey[s] = rec[0];
+          } else {
 #pragma unroll
-          for (int j = 1; j < W; ++j)
-            if (j <= NK) lkv[(int64_t)s * NK + (j - 1)] = bitsd((i64)rec[j]);
+            for (int j = 1; j < W; ++j)
+              if (j <= NK) lkey[(int64_t)s * NK + (j - 1)] = rec[j];
+          }
         }
         if (claimed || prev == h) {
           slot = (int)s;
@@ -358,13 +445,14 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
       }
       if (slot >= 0) {
 #pragma unroll
-        for (int j = 1; j < W; ++j)
-          if (j > NK) {
-            const int a = j - 1 - NK;
-            atomic_combine_kind<WS_LDS>(slot_kind(kinds, a), &lacc[(int64_t)slot * NA + a], bitsd((i64)rec[j]));
-          }
+        for (int a = 0; a < (VA > 0 ? VA : 1); ++a)
+          if (a < NA) atomic_combine_kind<WS_LDS>(slot_kind(kinds, a), &lacc[(int64_t)slot * NA + a], bitsd((i64)val[a]));
       } else {
-        spill_insert<W>(spill, kinds, NK, rec, &s_new);
+        if (!PK)
+#pragma unroll
+          for (int k = 0; k < KB; ++k)
+            if (k < NK) key[k] = rec[1 + k < W ? 1 + k : 0];
+        spill_insert(spill, kinds, NK, NA, h, key, val, &s_new);
       }
     }
   }
@@ -396,21 +484,29 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   u64 o = (u64)m.pstart[blockIdx.x] + wbase + incl - n;
   for (int qi = s0; qi < s1; ++qi) {
     if (lk[qi] == kEmpty) continue;
-    for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = lkv[(int64_t)qi * NK + k];
+    if (PK) {
+      u64 kb[kPackKeys];
+      packed_keys(pk, NK, lkey[qi], kb);
+      for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = bitsd((i64)kb[k]);
+    } else {
+      for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = bitsd((i64)lkey[(int64_t)qi * NK + k]);
+    }
     for (int a = 0; a < NA; ++a) m.oacc[o * NA + a] = lacc[(int64_t)qi * NA + a];
     ++o;
   }
 }
 
-// The merge kernel for records of W words (3..16).
+// The merge kernels for records of W words: full layout (3..16 words) and packed (1..16).
 typedef void (*MergeKernel)(PartMerge, GTable);
-static MergeKernel merge_kernel(int W) {
-  static const MergeKernel k[17] = {nullptr,           nullptr,           nullptr,           part_merge_k<3>,
-                                    part_merge_k<4>,   part_merge_k<5>,   part_merge_k<6>,   part_merge_k<7>,
-                                    part_merge_k<8>,   part_merge_k<9>,   part_merge_k<10>,  part_merge_k<11>,
-                                    part_merge_k<12>,  part_merge_k<13>,  part_merge_k<14>,  part_merge_k<15>,
-                                    part_merge_k<16>};
-  return W >= 3 && W <= 16 ? k[W] : nullptr;
+template <bool PK, int Base, int... Ws>
+static MergeKernel merge_pick(int W, std::integer_sequence<int, Ws...>) {
+  MergeKernel k = nullptr;
+  ((k = W == Base + Ws ? part_merge_k<Base + Ws, PK> : k), ...);
+  return k;
+}
+static MergeKernel merge_kernel(int W, bool packed) {
+  return packed ? merge_pick<true, 1>(W, std::make_integer_sequence<int, 16>{})
+                : merge_pick<false, 3>(W, std::make_integer_sequence<int, 14>{});
 }
 
 // Moves finished groups into a global table: rows of a partitioned result (goff set), or the
@@ -758,10 +854,11 @@ TuneKnob g_tune[] = {
     {"batch_tile_rows", {4096}, 256, 1 << 20},  // batched dense run: rows per tile
     {"batch_lds_slots", {1024}, 64, 8192},      // its LDS table slots per block
     {"batch_blocks_per_cu", {4}, 1, 32},        // its grid, per CU
+    {"part_pack", {1}, 0, 1},                   // packed partial-group records allowed (integer keys)
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
 enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
-       kTuneBatchBlocksPerCu };
+       kTuneBatchBlocksPerCu, kTunePack };
 
 struct PartPlan {
   int pbits = 0;  // 0: plain table path
@@ -778,7 +875,7 @@ static PartPlan plan_partitions(const ws_agg* A, int64_t rows, int64_t ntiles) {
   if (A->strategy == WS_STRAT_TABLE || A->is_dense || plan.nkeys == 0 || A->runs > 0 || A->keys || A->part) return pp;
   if (rows <= 0 || rows >= ((int64_t)1 << 31)) return pp;
   const int W = 1 + plan.nkeys + plan.naggs;
-  if (W > 16) return pp;
+  if (W < 3 || W > 16) return pp;  // the merge kernels' record widths
   const int64_t expect = A->expected_groups > 0 ? A->expected_groups : std::max<int64_t>(A->cap / 2, 1);
   if (A->strategy == WS_STRAT_AUTO && (rows < tune(kTuneAutoRows) || expect < tune(kTuneAutoGroups))) return pp;
   pp.W = W;
@@ -808,6 +905,12 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   const int NK = plan.nkeys, NA = plan.naggs, P = 1 << pp.pbits;
   DevBuf<uint32_t> cnt(pp.nchunks * P), off(pp.nchunks * P), ptot(P), gcnt(P);
   DevBuf<u64> recs(pp.rows * pp.W);
+  // Integer keys (at most kPackKeys): the count pass measures their ranges and, when they fit 63
+  // bits together, the scatter pass writes packed records (fewer bytes to write and to merge).
+  bool packable = NK <= kPackKeys && tune(kTunePack) != 0;
+  for (int k = 0; k < NK; ++k) packable = packable && plan.key_int[k];
+  DevBuf<int64_t> krange(packable ? pp.nchunks * 2 * NK : 0);  // every chunk writes its partial
+  DevBuf<PackPlan> pack(packable ? 1 : 0);
   A->okv = (double*)tmp_alloc(sizeof(double) * (size_t)pp.rows * NK);
   A->oacc = (double*)tmp_alloc(sizeof(double) * (size_t)pp.rows * std::max(NA, 1));
   A->pstart = (uint32_t*)tmp_alloc(sizeof(uint32_t) * (size_t)(P + 1));
@@ -831,6 +934,8 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   d.part_start = pstart;
   d.recs = recs.p;
   d.part_err = A->flags + 2;
+  d.key_range = packable ? krange.p : nullptr;
+  d.pack = packable ? pack.p : nullptr;
   GTable t = table_of(A);
   {
     DescUpload up;
@@ -842,7 +947,8 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     ProfScope ps("agg_part_offsets");
     hipLaunchKernelGGL(part_offsets_k, dim3((unsigned)((P + 63) / 64)), dim3(1024), 0, stream(), (const uint32_t*)cnt.p, off.p,
                        ptot.p, pp.nchunks, P);
-    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)ptot.p, pstart, P);
+    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)ptot.p, pstart, P,
+                       (const int64_t*)krange.p, pp.nchunks, NK, packable ? pack.p : nullptr);
     check_launch("agg_part_offsets");
   }
   {
@@ -855,9 +961,14 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   {
     u64 kinds = 0;
     for (int a = 0; a < NA; ++a) kinds |= (u64)((plan.ops[a] & 3) | (plan.agg_bits[a] ? 4 : 0)) << (3 * a);
-    PartMerge m{recs.p, pstart, pp.W, NK, NA, pp.S, kinds, A->okv, A->oacc, gcnt.p};
+    PartMerge m{recs.p, pstart, pp.W, NK, NA, pp.S, kinds, A->okv, A->oacc, gcnt.p, packable ? pack.p : nullptr};
     ProfScope ps("agg_part_merge");
-    hipLaunchKernelGGL(merge_kernel(pp.W), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m, t);
+    // both layouts are launched; the one the scatter pass did not choose returns at once
+    hipLaunchKernelGGL(merge_kernel(pp.W, false), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m,
+                       t);
+    if (packable)
+      hipLaunchKernelGGL(merge_kernel(1 + NA, true), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(),
+                         m, t);
     hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)gcnt.p, A->goff, P);
     check_launch("agg_part_merge");
   }

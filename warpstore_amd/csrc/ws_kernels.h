@@ -53,6 +53,21 @@ struct ScanDesc {
   const uint32_t* part_start;    // [P + 1] first record of each partition
   u64* recs;                     // [rows][rec_words] records: hash, key values, slot values
   int* part_err;                 // set when a run would exceed its counted length
+  int64_t* key_range;            // count pass: [chunk][2 * nkeys] min then max of the integer keys, or null
+  const struct PackPlan* pack;   // scatter pass: the packed record layout chosen from them, or null
+};
+
+// Packed partial-group records: when every GROUP BY key is an integer (at most kPackKeys of them)
+// and their ranges over the surviving rows fit 63 bits together, a record is [packed keys, slot
+// values] instead of [hash, key values, slot values]: key k contributes (v - kmin[k]) << shift[k].
+// The merge recomputes the hash from the unpacked keys, so groups are identified exactly as on the
+// plain path. A packed word is below 2^63, so the free marker (all ones) stays distinct.
+constexpr int kPackKeys = 4;
+struct PackPlan {
+  int32_t ok;  // 1: records are packed (1 + naggs words each)
+  int32_t shift[kPackKeys];
+  u64 mask[kPackKeys];  // (1 << width) - 1
+  int64_t kmin[kPackKeys];
 };
 
 // Global hash table of one grouped aggregation (row-major accumulators per slot).
@@ -922,6 +937,14 @@ __device__ void part_count_body(const ScanDesc& d) {
   const int P = 1 << d.pbits, shift = 64 - d.pbits;
   uint32_t* hist = (uint32_t*)ws_smem;
   for (int i = threadIdx.x; i < P; i += blockDim.x) hist[i] = 0;
+  // key ranges for the packed record layout (integer keys only; the host sets key_range then):
+  // per thread, then per block in LDS, one partial per chunk
+  __shared__ long long s_rng[2 * kPackKeys];
+  const bool ranged = d.key_range != nullptr;
+  if (threadIdx.x < 2 * kPackKeys) s_rng[threadIdx.x] = threadIdx.x < kPackKeys ? kI64Max : kI64Min;
+  i64 kmin[kPackKeys], kmax[kPackKeys];
+#pragma unroll
+  for (int k = 0; k < kPackKeys; ++k) kmin[k] = kI64Max, kmax[k] = kI64Min;
   const i64 c = blockIdx.x;
   const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
   int cur = -1;
@@ -944,15 +967,23 @@ __device__ void part_count_body(const ScanDesc& d) {
         const i64 ii = i + (i64)u * blockDim.x;
         const i64 r = tr.first + ii;
         h[u] = kEmpty;
+        double kv[Ev::kKeys];
         if (Ev::kPureKeys) {  // key loads issued with the WHERE loads, not after them
           if (ii < tr.count) {
-            double kv[Ev::kKeys];
             const u64 hk = row_keys(ev, NK, r, kv);
             if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
           }
         } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
-          double kv[Ev::kKeys];
           h[u] = row_keys(ev, NK, r, kv);
+        }
+        if (ranged && h[u] != kEmpty) {
+#pragma unroll
+          for (int k = 0; k < kPackKeys; ++k) {
+            if (k >= NK || k >= Ev::kKeys) break;
+            const i64 v = dbits(kv[k]);
+            kmin[k] = v < kmin[k] ? v : kmin[k];
+            kmax[k] = v > kmax[k] ? v : kmax[k];
+          }
         }
       }
 #pragma unroll
@@ -960,11 +991,59 @@ __device__ void part_count_body(const ScanDesc& d) {
         if (h[u] != kEmpty) __hip_atomic_fetch_add(&hist[h[u] >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
     }
   }
+  if (ranged) {  // one LDS atomic per wave and bound
+#pragma unroll
+    for (int k = 0; k < kPackKeys; ++k) {
+      if (k >= NK || k >= Ev::kKeys) break;
+      i64 lo = kmin[k], hi = kmax[k];
+      for (int o = 32; o > 0; o >>= 1) {
+        const i64 a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+      }
+      if ((threadIdx.x & 63) == 0 && lo <= hi) {
+        atomicMin(&s_rng[k], (long long)lo);
+        atomicMax(&s_rng[kPackKeys + k], (long long)hi);
+      }
+    }
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < P; i += blockDim.x) d.part_cnt[c * P + i] = hist[i];
+  if (ranged && threadIdx.x < 2 * NK) {
+    const int k = threadIdx.x < NK ? threadIdx.x : threadIdx.x - NK;
+    d.key_range[c * 2 * NK + threadIdx.x] = s_rng[(threadIdx.x < NK ? 0 : kPackKeys) + k];
+  }
 }
 
-__device__ __forceinline__ void put_record(u64* rec, u64 h, int NK, int NA, const double* kv, const double* av) {
+// Writes one record: [hash, key values, slot values], or with a packing [packed keys, slot values].
+struct RecLayout {
+  bool packed;
+  int words;  // record stride
+  int32_t shift[kPackKeys];
+  int64_t kmin[kPackKeys];
+};
+__device__ __forceinline__ RecLayout rec_layout(const ScanDesc& d, int NK, int NA) {
+  RecLayout L;
+  L.packed = d.pack != nullptr && d.pack->ok;
+  L.words = L.packed ? 1 + NA : d.rec_words;
+#pragma unroll
+  for (int k = 0; k < kPackKeys; ++k) {
+    L.shift[k] = L.packed ? d.pack->shift[k] : 0;
+    L.kmin[k] = L.packed ? d.pack->kmin[k] : 0;
+  }
+  return L;
+}
+__device__ __forceinline__ void put_record(const RecLayout& L, u64* rec, u64 h, int NK, int NA, const double* kv,
+                                           const double* av) {
+  if (L.packed) {
+    u64 pk = 0;
+#pragma unroll
+    for (int k = 0; k < kPackKeys; ++k)
+      if (k < NK) pk |= (u64)(dbits(kv[k]) - L.kmin[k]) << L.shift[k];
+    rec[0] = pk;
+    for (int a = 0; a < NA; ++a) rec[1 + a] = (u64)dbits(av[a]);
+    return;
+  }
   rec[0] = h;
   for (int k = 0; k < NK; ++k) rec[1 + k] = (u64)dbits(kv[k]);
   for (int a = 0; a < NA; ++a) rec[1 + NK + a] = (u64)dbits(av[a]);
@@ -976,7 +1055,9 @@ __device__ void part_scatter_body(const ScanDesc& d) {
   __shared__ ws_colview scols[WS_MAX_COLS];
   Ev ev(d);
   const ws_agg_plan& plan = *d.plan;
-  const int NK = ev.nkeys(), NA = ev.naggs(), W = d.rec_words;
+  const int NK = ev.nkeys(), NA = ev.naggs();
+  const RecLayout L = rec_layout(d, NK, NA);
+  const int W = L.words;
   const int P = 1 << d.pbits, shift = 64 - d.pbits, S = d.pre_slots;
   const i64 c = blockIdx.x;
   uint32_t* cur_pos = (uint32_t*)ws_smem;  // [P] next record of this chunk's run
@@ -1060,7 +1141,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
           *d.part_err = 1;
           continue;
         }
-        put_record(d.recs + (i64)pos * W, h[u], NK, NA, kv[u], av[u]);
+        put_record(L, d.recs + (i64)pos * W, h[u], NK, NA, kv[u], av[u]);
       }
     }
   }
@@ -1074,7 +1155,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
       *d.part_err = 1;
       continue;
     }
-    put_record(d.recs + (i64)pos * W, h, NK, NA, &pkv[(i64)i * NK], &pacc[(i64)i * NA]);
+    put_record(L, d.recs + (i64)pos * W, h, NK, NA, &pkv[(i64)i * NK], &pacc[(i64)i * NA]);
   }
   __syncthreads();
   for (int p = threadIdx.x; p < P; p += blockDim.x)

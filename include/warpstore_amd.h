@@ -118,7 +118,7 @@ int ws_agg_set_strategy(ws_agg* a, int strategy, int64_t expected_groups);
 int ws_agg_partitioned(const ws_agg* a); /* 1 when the last run produced a partitioned result */
 /* Process-wide tuning knobs of the physical plans (no reference counterpart: the autotuner's
  * interface; results do not depend on them). Keys: "part_merge_lds_kib" (8..152, default 60),
- * "part_scatter_lds_kib" (16..120, default 48), "part_chunks_per_cu" (1..64, default 1),
+ * "part_scatter_lds_kib" (16..152, default 120), "part_chunks_per_cu" (1..64, default 1),
  * "part_auto_rows" (default 2^20), "part_auto_groups" (default 2^16), "batch_tile_rows" (256..2^20,
  * default 4096), "batch_lds_slots" (64..8192, default 1024), "batch_blocks_per_cu" (1..32,
  * default 4), "part_pack" (0 or 1, default 1: packed partial-group records for integer keys).

@@ -847,7 +847,7 @@ struct TuneKnob {
 };
 TuneKnob g_tune[] = {
     {"part_merge_lds_kib", {60}, 8, 152},     // merge table per block (KiB): 2 blocks per CU at 60
-    {"part_scatter_lds_kib", {48}, 16, 120},  // cursors + pre-aggregation table per scatter block (KiB)
+    {"part_scatter_lds_kib", {120}, 16, 152}, // cursors + pre-aggregation table per scatter block (KiB)
     {"part_chunks_per_cu", {1}, 1, 64},       // count / scatter blocks (chunks) per CU
     {"part_auto_rows", {1 << 20}, 0, (int64_t)1 << 40},   // automatic choice: at least this many rows
     {"part_auto_groups", {1 << 16}, 0, (int64_t)1 << 40}, // and this many expected groups
@@ -884,7 +884,7 @@ static PartPlan plan_partitions(const ws_agg* A, int64_t rows, int64_t ntiles) {
   while (bits < kPartMaxBits && ((int64_t)1 << bits) * (pp.S / 2) < expect) ++bits;
   pp.pbits = bits;
   const int P = 1 << bits;
-  int pre = 1024;
+  int pre = 8192;
   while (pre > 64 && 8 * (size_t)P + (size_t)pre * 8 * W > (size_t)tune(kTuneScatterLds) * 1024) pre >>= 1;
   pp.pre = pre;
   const int64_t want = std::max<int64_t>(1, (int64_t)cu_count() * tune(kTuneChunks));

@@ -211,8 +211,13 @@ struct Translator {
         }
         case WS_OP_AND: case WS_OP_OR: {
           Node b = pop(), a = pop();
+          // Both operands are evaluated (bitwise & / |) unless one can raise the overflow flag:
+          // their column loads then issue together instead of one memory round trip after the
+          // other. Operands that can set env->ovf keep the short circuit.
+          const bool fx = a.e.find("env->ovf") != std::string::npos || b.e.find("env->ovf") != std::string::npos;
+          const char* join = in.op == WS_OP_AND ? (fx ? "&&" : "&") : (fx ? "||" : "|");
           if (a.is_int && b.is_int)
-            st.push_back({"((long long)((" + a.e + "!=0)" + (in.op == WS_OP_AND ? "&&" : "||") + "(" + b.e + "!=0)))", true});
+            st.push_back({"((long long)(((" + a.e + ")!=0)" + join + "((" + b.e + ")!=0)))", true});
           else st.push_back({std::string(in.op == WS_OP_AND ? "ws::and3(" : "ws::or3(") + as_d(a) + "," + as_d(b) + ")", false});
           break;
         }

@@ -265,21 +265,26 @@ __device__ __forceinline__ int64_t part_row(const uint32_t* goff, const uint32_t
   return (int64_t)pstart[lo] + (i - (int64_t)goff[lo]);
 }
 
-// Compacted groups of a partitioned result: rows [0, g) of a result of G groups.
+// Compacted groups of a partitioned result: rows [0, g) of a result of G groups. Block b copies
+// partitions b, b + gridDim.x, ...: partition p's groups are rows pstart[p] + [0, goff[p + 1] -
+// goff[p]) and become compacted groups goff[p] + rank (contiguous reads and writes, no search).
 __global__ void gather_part_k(const double* kv, int nk, const double* acc, int na, const uint32_t* goff,
                               const uint32_t* pstart, int P, int64_t g, double* keys_out, double* accs_out, int colmajor,
                               int64_t G) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < g; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = part_row(goff, pstart, P, i);
-    for (int c = 0; c < nk; ++c) {
-      const double v = kv[s * nk + c];
-      if (colmajor) keys_out[(int64_t)c * G + i] = v;
-      else keys_out[i * nk + c] = v;
-    }
-    for (int a = 0; a < na; ++a) {
-      const double v = acc[s * na + a];
-      if (colmajor) accs_out[(int64_t)a * G + i] = v;
-      else accs_out[i * na + a] = v;
+  for (int p = blockIdx.x; p < P; p += gridDim.x) {
+    const int64_t o0 = goff[p], cnt = (int64_t)goff[p + 1] - o0, s0 = pstart[p];
+    for (int64_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+      const int64_t s = s0 + j, i = o0 + j;
+      for (int c = 0; c < nk; ++c) {
+        const double v = kv[s * nk + c];
+        if (colmajor) keys_out[(int64_t)c * G + i] = v;
+        else keys_out[i * nk + c] = v;
+      }
+      for (int a = 0; a < na; ++a) {
+        const double v = acc[s * na + a];
+        if (colmajor) accs_out[(int64_t)a * G + i] = v;
+        else accs_out[i * na + a] = v;
+      }
     }
   }
 }
@@ -1167,7 +1172,7 @@ static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor, D
   }
   ProfScope ps("gather_groups");
   if (g1)
-    hipLaunchKernelGGL(gather_part_k, dim3(grid_for(g1)), dim3(256), 0, stream(), (const double*)A->okv, nk, (const double*)A->oacc,
+    hipLaunchKernelGGL(gather_part_k, dim3((unsigned)A->P), dim3(256), 0, stream(), (const double*)A->okv, nk, (const double*)A->oacc,
                        na, (const uint32_t*)A->goff, (const uint32_t*)A->pstart, A->P, g1, ko, ao, colmajor ? 1 : 0, g);
   if (g2) {
     if (A->is_dense)

@@ -122,11 +122,12 @@ std::string as_i(const Node& n) { return n.is_int ? n.e : "((long long)" + n.e +
 struct Translator {
   const ws_expr_env& env;
   int ncols;
-  std::set<int> cols, iconsts, dconsts;
+  std::set<int> cols, iconsts, dconsts, encs;  // encs: encoded integer columns read
 
   std::string raw_int(int c) {
     const ws_colview& v = env.cols[c];
-    return v.enc ? "ws::col_int(v" + std::to_string(c) + ",r)" : "((long long)c" + std::to_string(c) + "[r])";
+    if (v.enc) encs.insert(c);
+    return v.enc ? "ws::enc_get(e" + std::to_string(c) + ",r)" : "((long long)c" + std::to_string(c) + "[r])";
   }
   Node run(ws_prog p) {
     std::vector<Node> st;
@@ -303,9 +304,11 @@ struct Translator {
   }
 };
 
-const char* ctype_of(int t) {
-  return (t == WS_T_I32 || t == WS_T_I32N) ? "const int*" : (t == WS_T_F64) ? "const double*" : t == WS_T_VEC ? "const float*"
-                                                                                                : "const long long*";
+const char* ctype_of(int t) {  // global-memory pointers (loads issue as global, not flat)
+  return (t == WS_T_I32 || t == WS_T_I32N) ? "const int WS_GLOBAL*"
+         : (t == WS_T_F64)                 ? "const double WS_GLOBAL*"
+         : t == WS_T_VEC                   ? "const float WS_GLOBAL*"
+                                           : "const long long WS_GLOBAL*";
 }
 
 int dense_kind_of(int op, bool is_int) {
@@ -339,15 +342,19 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   o << "  static constexpr int kKeys = " << nk << ", kAggs = " << na << ", kRows = " << rows << ";\n";
   o << "  static constexpr bool kPureKeys = " << (pure_keys ? "true" : "false") << ", kPureVals = "
     << (pure_vals ? "true" : "false") << ";\n";
+  o << "  static constexpr bool kGlobalCols = true;\n";
   o << "  const ws_expr_env* env;\n";
   for (int c : tr.cols) o << "  " << ctype_of(env.cols[c].t) << " c" << c << ";\n  ws_colview v" << c << ";\n";
+  for (int c : tr.encs) o << "  ws::EncView e" << c << ";\n";
   for (int k : tr.iconsts) o << "  long long ik" << k << ";\n";
   for (int k : tr.dconsts) o << "  double k" << k << ";\n";
   o << "  __device__ QEval(const ws::ScanDesc& d) : env(d.env) {\n";
   for (int k : tr.iconsts) o << "    ik" << k << " = ws::dbits(env->consts[" << k << "]);\n";
   for (int k : tr.dconsts) o << "    k" << k << " = env->consts[" << k << "];\n";
   o << "  }\n  __device__ void bind(const ws_colview* c) {\n";
-  for (int c : tr.cols) o << "    v" << c << " = c[" << c << "]; c" << c << " = (" << ctype_of(env.cols[c].t) << ")v" << c << ".p;\n";
+  for (int c : tr.cols)
+    o << "    v" << c << " = ws::uniform_view(c[" << c << "]); c" << c << " = (" << ctype_of(env.cols[c].t) << ")v" << c << ".p;\n";
+  for (int c : tr.encs) o << "    e" << c << " = ws::enc_view(v" << c << ");\n";
   o << "  }\n";
   o << "  __device__ int nkeys() const { return " << plan.nkeys << "; }\n";
   o << "  __device__ int naggs() const { return " << plan.naggs << "; }\n";
@@ -379,7 +386,7 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   for (size_t a = 0; a < aggs.size(); ++a) {
     // A nullable int column: its raw value, NULL sentinel included (the kernel skips it).
     std::string e = as_i(aggs[a]);
-    if (plan.agg_null[a]) e = "ws::col_int(v" + std::to_string(env.code[plan.aggs[a].start].arg) + ",r)";
+    if (plan.agg_null[a]) e = tr.raw_int(env.code[plan.aggs[a].start].arg);
     o << "\n    case " << a << ": return " << e << ";";
   }
   o << " } return 0; }\n};\n";

@@ -7,6 +7,9 @@
 // An evaluator supplies:
 //   static constexpr int kKeys, kAggs, kRows;   // loop bounds, rows per thread per tile
 //   static constexpr bool kPureKeys, kPureVals;  // keys / slot values have no side effect
+//   static constexpr bool kGlobalCols;            // bind() takes the batch's column views in global
+//                                                 // memory (uniform loads: SGPR column bases) rather
+//                                                 // than the block's LDS copy
 //   Ev(const ScanDesc&);  void bind(const ws_colview* cols);   // per batch
 //   int nkeys(), naggs(), op(a); bool has_where(), agg_int(a), key_int(k);
 //   bool where(r); double key(k, r); i64 ikey(k, r); double agg(a, r); i64 iagg(a, r); uint8_t dkind(a);
@@ -81,6 +84,51 @@ struct GTable {
 };
 
 // ------------------------------------------------------------------ small helpers
+
+// Column access for compiled evaluators. A batch's column view is the same for every lane, so it
+// is moved into scalar registers (readfirstlane per dword): column bases become SGPRs, loads use
+// the global (not flat) instructions with a 32-bit lane offset, and the encoding switch is a
+// scalar branch.
+#define WS_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ ws_colview uniform_view(const ws_colview& v) {
+  static_assert(sizeof(ws_colview) == 16, "column view layout");
+  unsigned w[4];
+  __builtin_memcpy(w, &v, 16);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) w[i] = (unsigned)__builtin_amdgcn_readfirstlane((int)w[i]);
+  ws_colview u;
+  __builtin_memcpy(&u, w, 16);
+  return u;
+}
+// Frame-of-reference integer column (encodings 1-3): codes from `data` (offset applied), value =
+// base + code, the top code standing for the type's NULL.
+struct EncView {
+  const unsigned char WS_GLOBAL* data;
+  i64 base, nul;
+  int enc;
+};
+__device__ __forceinline__ EncView enc_view(const ws_colview& v) {
+  EncView e;
+  e.enc = v.enc;
+  e.nul = (v.t == WS_T_I32 || v.t == WS_T_I32N) ? (i64)WS_NULL_I32 : (i64)WS_NULL_I64;
+  e.base = v.enc ? *(const i64 WS_GLOBAL*)v.p : 0;
+  e.data = (const unsigned char WS_GLOBAL*)v.p + WS_ENC_HEADER + (i64)v.off * enc_width(v.enc);
+  return e;
+}
+__device__ __forceinline__ i64 enc_get(const EncView& e, i64 r) {
+  unsigned code, top;
+  if (e.enc == 1) {
+    code = e.data[r];
+    top = 0xFFu;
+  } else if (e.enc == 2) {
+    code = ((const unsigned short WS_GLOBAL*)e.data)[r];
+    top = 0xFFFFu;
+  } else {
+    code = ((const unsigned WS_GLOBAL*)e.data)[r];
+    top = 0xFFFFFFFFu;
+  }
+  return code == top ? e.nul : e.base + (i64)code;
+}
 
 __device__ __forceinline__ u64 mix64(u64 x) {  // splitmix64 finalizer
   x ^= x >> 30;
@@ -193,10 +241,22 @@ struct TileRows {
   int b;
   i64 first, count;
 };
+// Tiles are block-uniform, so their batch, row range and delete bitmap are too; readfirstlane
+// tells the compiler (scalar registers, scalar branches, and column loads of the batch from a
+// scalar base) instead of it treating values loaded per lane as divergent.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ i64 uni(i64 x) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((u64)x >> 32));
+  return (i64)(((u64)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p) { return (T*)uni((i64)p); }
+
 __device__ __forceinline__ int batch_of_tile(const ScanDesc& d, i64 t, int hint) {
   int b = hint < 0 ? 0 : hint;
   while (b + 1 < d.nb && d.tile0[b + 1] <= t) ++b;
-  return b;
+  return uni(b);
 }
 __device__ __forceinline__ int batch_of_tile_bsearch(const ScanDesc& d, i64 t) {
   int lo = 0, hi = d.nb - 1;  // last batch with tile0 <= t
@@ -205,14 +265,14 @@ __device__ __forceinline__ int batch_of_tile_bsearch(const ScanDesc& d, i64 t) {
     if (d.tile0[mid] <= t) lo = mid;
     else hi = mid - 1;
   }
-  return lo;
+  return uni(lo);
 }
 __device__ __forceinline__ TileRows tile_rows(const ScanDesc& d, i64 t, int b) {
   TileRows tr;
   tr.b = b;
-  tr.first = (t - d.tile0[b]) * (i64)d.tile_rows;
+  tr.first = uni((t - d.tile0[b]) * (i64)d.tile_rows);
   const i64 left = d.n[b] - tr.first;
-  tr.count = left < d.tile_rows ? left : d.tile_rows;
+  tr.count = uni(left < d.tile_rows ? left : (i64)d.tile_rows);
   return tr;
 }
 // Block-wide: stage batch b's column views in LDS (all threads call).
@@ -239,6 +299,7 @@ template <bool kFull>
 struct InterpEvalT {
   static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 1;
   static constexpr bool kPureKeys = false, kPureVals = false;  // (see the compiled evaluator)
+  static constexpr bool kGlobalCols = false;
   const ws_expr_env* env;
   const ws_agg_plan* plan;
   const ws_colview* cols = nullptr;
@@ -285,6 +346,7 @@ typedef InterpEvalT<false> InterpEvalLean;
 struct InterpEvalCols {
   static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 4;
   static constexpr bool kPureKeys = true, kPureVals = true;
+  static constexpr bool kGlobalCols = false;
   const ws_expr_env* env;
   const ws_agg_plan* plan;
   const ws_colview* cols = nullptr;
@@ -353,11 +415,11 @@ __device__ void nokeys_body(const ScanDesc& d, double* out, NoKeysPartial* parts
     const int b = batch_of_tile(d, t, cur);
     if (b != cur) {
       stage_batch(d, b, scols);
-      ev.bind(scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
     }
     const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = d.del[b];
+    const uint32_t* del = uni(d.del[b]);
 #pragma unroll
     for (int j = 0; j < Ev::kRows; ++j) {
       for (i64 i = threadIdx.x + (i64)j * blockDim.x; i < tr.count; i += (i64)Ev::kRows * blockDim.x) {
@@ -539,11 +601,11 @@ __device__ void dense_body(const ScanDesc& d, double* gacc, i64* parts, int* fla
     const int b = batch_of_tile(d, t, cur);
     if (b != cur) {
       stage_batch(d, b, scols);
-      ev.bind(scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
     }
     const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = d.del[b];
+    const uint32_t* del = uni(d.del[b]);
     for (i64 i = tid; i < tr.count; i += T) {
       const i64 r = tr.first + i;
       if (is_deleted(del, r)) continue;
@@ -765,11 +827,11 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
     const int b = batch_of_tile(d, t, cur);
     if (b != cur) {
       stage_batch(d, b, scols);
-      ev.bind(scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
     }
     const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = d.del[b];
+    const uint32_t* del = uni(d.del[b]);
     for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
       const i64 r = tr.first + i;
       if (is_deleted(del, r)) continue;
@@ -860,8 +922,8 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
       for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, dk_op(ds.kind[a]));
     }
     __syncthreads();
-    ev.bind(scols);
-    const uint32_t* del = d.del[bt.b];
+    ev.bind(Ev::kGlobalCols ? d.cols + (i64)bt.b * d.ncols : scols);
+    const uint32_t* del = uni(d.del[bt.b]);
     bool bad = false;
     for (int i = threadIdx.x; i < bt.count; i += blockDim.x) {
       const i64 r = bt.first + i;
@@ -926,7 +988,11 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
 //   merge   (part_agg_k): one block per partition combines its records in an LDS table and
 //           appends the finished groups to the output.
 // A record is [hash, key values (nk), slot values (na)], 8 bytes each.
-constexpr int kPartRows = 4;  // count pass of a compiled plan: rows per thread per step
+// Rows per thread per step of the partition passes: loads of several rows overlap, but the
+// 1024-thread blocks leave each thread at most 128 VGPRs, and past that the passes spill to
+// scratch memory (at 4 rows, the web table's compiled plan did), which costs more than the
+// overlap gains.
+constexpr int kPartRows = 4;
 
 template <class Ev>
 __device__ void part_count_body(const ScanDesc& d) {
@@ -952,11 +1018,11 @@ __device__ void part_count_body(const ScanDesc& d) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
     if (b != cur) {
       stage_batch(d, b, scols);  // also orders the histogram reset before the first update
-      ev.bind(scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
     }
     const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = d.del[b];
+    const uint32_t* del = uni(d.del[b]);
     // U rows per thread per step, evaluated before any LDS update so their loads overlap (the
     // interpreter's out-of-line VM calls gain nothing from it: one row there)
     constexpr int U = Ev::kRows > 1 ? kPartRows : 1;
@@ -1080,14 +1146,14 @@ __device__ void part_scatter_body(const ScanDesc& d) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
     if (b != cur) {
       stage_batch(d, b, scols);  // also orders the table set-up before the first row
-      ev.bind(scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
     }
     const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = d.del[b];
+    const uint32_t* del = uni(d.del[b]);
     // Ev::kRows rows per thread per step: all are evaluated (their loads overlap) before the LDS
     // table and the record writes see them.
-    constexpr int U = Ev::kRows * (Ev::kKeys + Ev::kAggs) <= 24 ? Ev::kRows : 1;  // register budget
+    constexpr int U = Ev::kRows > 1 && Ev::kKeys + Ev::kAggs <= 6 ? kPartRows : 1;  // register budget
     for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
       double kv[U][Ev::kKeys];
       double av[U][Ev::kAggs];
@@ -1277,9 +1343,9 @@ __device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, i
   for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)b * d.ncols + i];
   __syncthreads();
   Ev ev(d);
-  ev.bind(scols);
+  ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
   const TileRows tr = tile_rows(d, t, b);
-  const uint32_t* del = d.del[b];
+  const uint32_t* del = uni(d.del[b]);
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
   // Each thread owns kRows consecutive chunks of blockDim rows: row i = j * blockDim + tid.
   bool pass[kSelRows];

@@ -115,6 +115,12 @@ __global__ void eval_at_k(const ws_expr_env* env, ws_prog prog, const int32_t* i
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
     out[j] = eval_prog(*env, env->cols, prog, idx ? idx[j] : j);
 }
+// A program that is one column (ORDER BY an aggregate): the VM's column load, without the VM's
+// stack (which spills to scratch memory).
+__global__ void eval_col_at_k(ws_colview c, const int32_t* idx, int64_t m, double* out) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x)
+    out[j] = load_col(c, idx ? idx[j] : j);
+}
 // Exact int64 evaluation of integer columns, integer literals, + - * and negation (NULLs propagate,
 // overflow raises env->ovf); false for anything else (then the double VM). Reference gpu.cu:826-866.
 __device__ bool eval_int_exact(const ws_expr_env& env, ws_prog p, i64 r, i64& out) {
@@ -385,6 +391,14 @@ int ws_map_rank(double* keys, const int32_t* rank, int64_t m) {
 int ws_eval_at(const ws_expr_env* env, ws_prog prog, const int32_t* idx, int64_t m, double* out) {
   return guard([&] {
     if (m <= 0) return;
+    if (prog.len == 1 && prog.start >= 0 && prog.start < WS_MAX_CODE && env->code[prog.start].op == WS_OP_COL &&
+        env->code[prog.start].arg < WS_MAX_COLS) {
+      ProfScope ps("eval_at");
+      hipLaunchKernelGGL(eval_col_at_k, dim3(grid_for(m)), dim3(256), 0, stream(), env->cols[env->code[prog.start].arg], idx, m,
+                         out);
+      check_launch("eval_at");
+      return;
+    }
     DevBuf<ws_expr_env> d = env_on_device(*env);
     ProfScope ps("eval_at");
     hipLaunchKernelGGL(eval_at_k, dim3(grid_for(m)), dim3(256), 0, stream(), (const ws_expr_env*)d.p, prog, idx, m, out);

@@ -366,7 +366,15 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   constexpr int VA = PK ? W - 1 : W - 2;  // register bound on slot values (NK >= 1)
   constexpr int KB = PK ? kPackKeys : (W > 1 ? W - 1 : 1);  // register bound on key words
   PackPlan pk{};
-  if (PK) pk = *m.pack;
+  if (PK) {
+    pk = *m.pack;
+#pragma unroll
+    for (int k = 0; k < kPackKeys; ++k) {  // the same for every lane: scalar registers
+      pk.shift[k] = uni(pk.shift[k]);
+      pk.mask[k] = (u64)uni((i64)pk.mask[k]);
+      pk.kmin[k] = uni((i64)pk.kmin[k]);
+    }
+  }
   const u64 kinds = m.kinds;
   u64* lk = (u64*)ws_smem;
   u64* lkey = lk + S;                                 // [S][KW]: key bits, or the packed word
@@ -378,14 +386,15 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   if (threadIdx.x == 0) s_new = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int64_t beg = m.pstart[blockIdx.x], end = m.pstart[blockIdx.x + 1];
+  const int64_t beg = uni((i64)m.pstart[blockIdx.x]), end = uni((i64)m.pstart[blockIdx.x + 1]);
+  const u64 WS_GLOBAL* recs = (const u64 WS_GLOBAL*)m.recs;
   for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * R) {
     u64 q[R][W];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int64_t i = i0 + (int64_t)r * blockDim.x + threadIdx.x;
 #pragma unroll
-      for (int j = 0; j < W; ++j) q[r][j] = i < end ? m.recs[i * W + j] : kEmpty;
+      for (int j = 0; j < W; ++j) q[r][j] = i < end ? recs[i * W + j] : kEmpty;
     }
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {

@@ -365,6 +365,15 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   o << "  __device__ bool agg_int(int a) const { switch (a) {";
   for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (plan.agg_int[a] ? "true" : "false") << ";";
   o << " } return false; }\n";
+  o << "  __device__ bool agg_bits(int a) const { switch (a) {";
+  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (plan.agg_bits[a] ? "true" : "false") << ";";
+  o << " } return false; }\n";
+  o << "  __device__ int agg_part(int a) const { switch (a) {";
+  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (int)plan.agg_part[a] << ";";
+  o << " } return 0; }\n";
+  o << "  __device__ int agg_null(int a) const { switch (a) {";
+  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (int)plan.agg_null[a] << ";";
+  o << " } return 0; }\n";
   o << "  __device__ bool key_int(int k) const { switch (k) {";
   for (int k = 0; k < plan.nkeys; ++k) o << " case " << k << ": return " << (plan.key_int[k] ? "true" : "false") << ";";
   o << " } return false; }\n";

@@ -11,7 +11,7 @@
 //                                                 // memory (uniform loads: SGPR column bases) rather
 //                                                 // than the block's LDS copy
 //   Ev(const ScanDesc&);  void bind(const ws_colview* cols);   // per batch
-//   int nkeys(), naggs(), op(a); bool has_where(), agg_int(a), key_int(k);
+//   int nkeys(), naggs(), op(a), agg_part(a), agg_null(a); bool has_where(), agg_int(a), agg_bits(a), key_int(k);
 //   bool where(r); double key(k, r); i64 ikey(k, r); double agg(a, r); i64 iagg(a, r); uint8_t dkind(a);
 // Semantics follow the reference kernels (src/kernels.cuh): NULL-skipping combines, exact int64 /
 // int128 integer accumulation, the split BIGINT sum, and 64-bit hash group identity.
@@ -161,11 +161,14 @@ __device__ __forceinline__ double combine_slot(bool bits, int op, double a, doub
 }
 // Integer aggregate argument as the slot accumulates it (reference kernels.cuh:65-69): a NULL of a
 // nullable int column becomes the neutral value; a split BIGINT SUM keeps its low / high half.
-__device__ __forceinline__ i64 int_part(const ws_agg_plan& p, int a, i64 v) {
-  if (p.agg_null[a] && v == (p.agg_null[a] == WS_T_I32N ? (i64)WS_NULL_I32 : (i64)WS_NULL_I64))
-    return int_identity(p.ops[a]);
-  if (!p.agg_bits[a]) return v;
-  return p.agg_part[a] == 1 ? (v & 0xffffffffll) : p.agg_part[a] == 2 ? (v >> 32) : v;
+// (The slot properties come from the evaluator: constants in a compiled plan, so the hot loops
+// load nothing from the plan per row.)
+template <class Ev>
+__device__ __forceinline__ i64 int_part(const Ev& ev, int a, i64 v) {
+  const int nul = ev.agg_null(a);
+  if (nul && v == (nul == WS_T_I32N ? (i64)WS_NULL_I32 : (i64)WS_NULL_I64)) return int_identity(ev.op(a));
+  if (!ev.agg_bits(a)) return v;
+  return ev.agg_part(a) == 1 ? (v & 0xffffffffll) : ev.agg_part(a) == 2 ? (v >> 32) : v;
 }
 
 // Atomic combines into a slot. SCOPE is __HIP_MEMORY_SCOPE_WORKGROUP for LDS tables and
@@ -309,6 +312,9 @@ struct InterpEvalT {
   __device__ int naggs() const { return plan->naggs; }
   __device__ int op(int a) const { return plan->ops[a]; }
   __device__ bool agg_int(int a) const { return plan->agg_int[a] != 0; }
+  __device__ bool agg_bits(int a) const { return plan->agg_bits[a] != 0; }
+  __device__ int agg_part(int a) const { return plan->agg_part[a]; }
+  __device__ int agg_null(int a) const { return plan->agg_null[a]; }
   __device__ bool key_int(int k) const { return plan->key_int[k] != 0; }
   __device__ bool has_where() const { return plan->has_where != 0; }
   __device__ uint8_t dkind(int a, const ws_dense_spec& ds) const { return ds.kind[a]; }
@@ -356,6 +362,9 @@ struct InterpEvalCols {
   __device__ int naggs() const { return plan->naggs; }
   __device__ int op(int a) const { return plan->ops[a]; }
   __device__ bool agg_int(int a) const { return plan->agg_int[a] != 0; }
+  __device__ bool agg_bits(int a) const { return plan->agg_bits[a] != 0; }
+  __device__ int agg_part(int a) const { return plan->agg_part[a]; }
+  __device__ int agg_null(int a) const { return plan->agg_null[a]; }
   __device__ bool key_int(int k) const { return plan->key_int[k] != 0; }
   __device__ bool has_where() const { return plan->has_where != 0; }
   __device__ uint8_t dkind(int a, const ws_dense_spec& ds) const { return ds.kind[a]; }
@@ -433,7 +442,7 @@ __device__ void nokeys_body(const ScanDesc& d, double* out, NoKeysPartial* parts
           const int op = ev.op(a);
           if (op == WS_AGG_COUNT) continue;
           if (ev.agg_int(a)) {
-            const i64 v = int_part(plan, a, ev.iagg(a, r));
+            const i64 v = int_part(ev, a, ev.iagg(a, r));
             if (op == WS_AGG_SUM) add128(lo[a], hi[a], (u64)v, v < 0 ? -1 : 0);
             else lo[a] = (u64)combine_i(op, (i64)lo[a], v);
           } else {
@@ -544,7 +553,7 @@ __device__ __forceinline__ bool dk_int(uint8_t k) {
 template <class Ev>
 __device__ __forceinline__ void dense_value(const Ev& ev, const ws_agg_plan& plan, uint8_t k, int a, i64 r, i64& iv, double& dv) {
   if (k == WS_DK_COUNT) iv = 1;
-  else if (dk_int(k)) iv = int_part(plan, a, ev.iagg(a, r));
+  else if (dk_int(k)) iv = int_part(ev, a, ev.iagg(a, r));
   else dv = ev.agg(a, r);
 }
 // Block partial of one dense slot -> the value written to the partials (or combined globally):
@@ -651,7 +660,7 @@ __device__ void dense_body(const ScanDesc& d, double* gacc, i64* parts, int* fla
           i64 iv = 0;
           double dv = 0;
           dense_value(ev, plan, k, a, r, iv, dv);
-          if (plan.agg_bits[a]) atomic_combine_i<WS_DEV>(dk_op(k), (i64*)&row[a], iv);
+          if (ev.agg_bits(a)) atomic_combine_i<WS_DEV>(dk_op(k), (i64*)&row[a], iv);
           else if (dk_int(k)) atomic_combine_d<WS_DEV>(a == 0 ? WS_AGG_SUM : dk_op(k), &row[a], (double)iv);
           else atomic_combine_d<WS_DEV>(dk_op(k), &row[a], dv);
         }
@@ -789,7 +798,7 @@ __device__ __forceinline__ void row_values(const Ev& ev, const ws_agg_plan& plan
 #pragma unroll
   for (int a = 0; a < Ev::kAggs; ++a) {
     if (a >= NA) break;
-    av[a] = plan.agg_bits[a] ? bitsd(int_part(plan, a, ev.iagg(a, r))) : ev.agg(a, r);
+    av[a] = ev.agg_bits(a) ? bitsd(int_part(ev, a, ev.iagg(a, r))) : ev.agg(a, r);
   }
 }
 
@@ -814,7 +823,7 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
   double* lacc = lkv + (i64)S * NK;
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     lk[i] = kEmpty;
-    for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
+    for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(ev.agg_bits(a), ev.op(a));
   }
   if (threadIdx.x == 0) s_new = 0;
   int cur = -1;
@@ -852,7 +861,7 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
 #pragma unroll
           for (int a = 0; a < Ev::kAggs; ++a) {
             if (a >= NA) break;
-            atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, ev.op(a), &lacc[(i64)s * NA + a], av[a]);
+            atomic_combine_slot<WS_LDS>(ev.agg_bits(a), ev.op(a), &lacc[(i64)s * NA + a], av[a]);
           }
           done = true;
           break;
@@ -1090,16 +1099,16 @@ struct RecLayout {
 };
 __device__ __forceinline__ RecLayout rec_layout(const ScanDesc& d, int NK, int NA) {
   RecLayout L;
-  L.packed = d.pack != nullptr && d.pack->ok;
+  L.packed = uni(d.pack != nullptr && d.pack->ok ? 1 : 0) != 0;
   L.words = L.packed ? 1 + NA : d.rec_words;
 #pragma unroll
   for (int k = 0; k < kPackKeys; ++k) {
-    L.shift[k] = L.packed ? d.pack->shift[k] : 0;
-    L.kmin[k] = L.packed ? d.pack->kmin[k] : 0;
+    L.shift[k] = uni(L.packed ? d.pack->shift[k] : 0);
+    L.kmin[k] = uni(L.packed ? (i64)d.pack->kmin[k] : (i64)0);
   }
   return L;
 }
-__device__ __forceinline__ void put_record(const RecLayout& L, u64* rec, u64 h, int NK, int NA, const double* kv,
+__device__ __forceinline__ void put_record(const RecLayout& L, u64 WS_GLOBAL* rec, u64 h, int NK, int NA, const double* kv,
                                            const double* av) {
   if (L.packed) {
     u64 pk = 0;
@@ -1124,6 +1133,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
   const int NK = ev.nkeys(), NA = ev.naggs();
   const RecLayout L = rec_layout(d, NK, NA);
   const int W = L.words;
+  u64 WS_GLOBAL* recs = (u64 WS_GLOBAL*)d.recs;
   const int P = 1 << d.pbits, shift = 64 - d.pbits, S = d.pre_slots;
   const i64 c = blockIdx.x;
   uint32_t* cur_pos = (uint32_t*)ws_smem;  // [P] next record of this chunk's run
@@ -1138,7 +1148,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
   }
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     pk[i] = kEmpty;
-    for (int a = 0; a < NA; ++a) pacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
+    for (int a = 0; a < NA; ++a) pacc[(i64)i * NA + a] = slot_identity(ev.agg_bits(a), ev.op(a));
   }
   const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
   int cur = -1;
@@ -1193,7 +1203,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
 #pragma unroll
             for (int a = 0; a < Ev::kAggs; ++a) {
               if (a >= NA) break;
-              atomic_combine_slot<WS_LDS>(plan.agg_bits[a] != 0, ev.op(a), &pacc[(i64)s * NA + a], av[u][a]);
+              atomic_combine_slot<WS_LDS>(ev.agg_bits(a), ev.op(a), &pacc[(i64)s * NA + a], av[u][a]);
             }
             done = true;
             break;
@@ -1207,7 +1217,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
           *d.part_err = 1;
           continue;
         }
-        put_record(L, d.recs + (i64)pos * W, h[u], NK, NA, kv[u], av[u]);
+        put_record(L, recs + (i64)pos * W, h[u], NK, NA, kv[u], av[u]);
       }
     }
   }
@@ -1221,11 +1231,11 @@ __device__ void part_scatter_body(const ScanDesc& d) {
       *d.part_err = 1;
       continue;
     }
-    put_record(L, d.recs + (i64)pos * W, h, NK, NA, &pkv[(i64)i * NK], &pacc[(i64)i * NA]);
+    put_record(L, recs + (i64)pos * W, h, NK, NA, &pkv[(i64)i * NK], &pacc[(i64)i * NA]);
   }
   __syncthreads();
   for (int p = threadIdx.x; p < P; p += blockDim.x)
-    for (uint32_t pos = cur_pos[p]; pos < end_pos[p]; ++pos) d.recs[(i64)pos * W] = kEmpty;
+    for (uint32_t pos = cur_pos[p]; pos < end_pos[p]; ++pos) recs[(i64)pos * W] = kEmpty;
 }
 
 // ------------------------------------------------------------------ filter + stream compaction

@@ -374,6 +374,9 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
   }
   plan2.ops[na] = WS_AGG_SUM;
   plan2.naggs = na + 1;
+  // The engine interprets this phase; a compiled plan gives the same groups (the plan cache keeps
+  // it across queries: the environment's column pointers are bound per run, not compiled in).
+  const JitProgram* jit2 = jitGet(env2, nk + 1 + na, plan2, JF_GROUPED, "aggregate (distinct users)");
   float f2[WS_SIZER_INPUTS];
   const int64_t pred2 = size_groupby({&t.cols[0]}, g1, 1.0, f2);
   int64_t cap2 = pow2_at_least(2 * g1, 1024);
@@ -386,7 +389,7 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
     ws_agg_set_features(agg2->handle(), f2);
     if (expect2 > 0) wsd::ck(ws_agg_set_strategy(agg2->handle(), WS_STRAT_AUTO, expect2));
     agg2->addBatch(env2, nk + 1 + na, g1, nullptr);
-    agg2->run(env2, nullptr);
+    agg2->run(env2, jit2);
     if (!agg2->overflowed()) break;
     cap2 = pow2_at_least(std::max<int64_t>(cap2 * 4, agg2->groupCount() * 4), 1);
     expect2 = std::max<int64_t>(expect2 * 4, agg2->groupCount() * 2);

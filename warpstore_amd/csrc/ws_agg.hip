@@ -391,13 +391,17 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * R) {
     u64 q[R][W];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < R; ++r) {  // (the record buffer is padded: loads past the end stay inside it)
       const int64_t i = i0 + (int64_t)r * blockDim.x + threadIdx.x;
 #pragma unroll
-      for (int j = 0; j < W; ++j) q[r][j] = i < end ? recs[i * W + j] : kEmpty;
+      for (int j = 0; j < W; ++j) {
+        const u64 v = recs[i * W + j];
+        q[r][j] = i < end ? v : kEmpty;
+      }
     }
 #pragma unroll 1
     for (int r = 0; r < R; ++r) {
+      if (i0 + (int64_t)r * blockDim.x >= end) break;  // (block-uniform) the rest of the step is past the partition
       u64 rec[W];
 #pragma unroll
       for (int j = 0; j < W; ++j) rec[j] = q[0][j];
@@ -918,7 +922,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
   const int iv = interp_of(env, plan);
   const int NK = plan.nkeys, NA = plan.naggs, P = 1 << pp.pbits;
   DevBuf<uint32_t> cnt(pp.nchunks * P), off(pp.nchunks * P), ptot(P), gcnt(P);
-  DevBuf<u64> recs(pp.rows * pp.W);
+  DevBuf<u64> recs((pp.rows + 8 * (int64_t)kPartMergeBlock) * pp.W);  // padded for the merge's unguarded loads
   // Integer keys (at most kPackKeys): the count pass measures their ranges and, when they fit 63
   // bits together, the scatter pass writes packed records (fewer bytes to write and to merge).
   bool packable = NK <= kPackKeys && tune(kTunePack) != 0;

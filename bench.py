@@ -94,11 +94,12 @@ def cpu_baseline(cols, target_s: float):
 
 # Scan kernels of the phase-1 aggregation that can dominate, with their algorithmic bytes per launch
 # and the kernel symbol they launch.
+PHASE1 = "[aggregate (distinct pairs)]"  # the plan-cache kind of the phase-1 plan: its launches' profile tag
 ALGO = {
-    "agg_hash_jit": (22 * ROWS, "ws_q_grouped", "compiled filter + shared-table hash GROUP BY (user_id, url)"),
-    "agg_part_scatter_jit": (22 * ROWS, "ws_q_grouped_pscatter",
-                             "compiled filter + partition scatter of (user_id, url) partial groups"),
-    "agg_part_count_jit": (18 * ROWS, "ws_q_grouped_pcount", "compiled filter + partition count"),
+    "agg_hash_jit" + PHASE1: (22 * ROWS, "ws_q_grouped", "compiled filter + shared-table hash GROUP BY (user_id, url)"),
+    "agg_part_scatter_jit" + PHASE1: (22 * ROWS, "ws_q_grouped_pscatter",
+                                      "compiled filter + partition scatter of (user_id, url) partial groups"),
+    "agg_part_count_jit" + PHASE1: (18 * ROWS, "ws_q_grouped_pcount", "compiled filter + partition count"),
 }
 
 
@@ -169,7 +170,7 @@ def main() -> int:
     L.ws_sync()
     warm = ws.prof_rows()
     L.ws_prof_enable(0)
-    dom = max((n for n in ALGO if n in warm), key=lambda n: warm[n][1], default="agg_hash_jit")
+    dom = max((n for n in ALGO if n in warm), key=lambda n: warm[n][1], default="agg_hash_jit" + PHASE1)
     algo_bytes, symbol, what = ALGO[dom]
 
     L.ws_prof_reset()

@@ -270,7 +270,7 @@ void ws_prof_only(const char* kernel); /* restrict recording to one kernel name 
 void ws_prof_reset(void);
 /* Totals per kernel name: returns the number of kernels (at most cap). */
 typedef struct ws_prof_row {
-  char name[48];
+  char name[96];
   int64_t launches;
   double total_ms;
 } ws_prof_row;

@@ -391,11 +391,15 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   for (int64_t i0 = beg; i0 < end; i0 += (int64_t)blockDim.x * R) {
     u64 q[R][W];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {  // (the record buffer is padded: loads past the end stay inside it)
+    for (int r = 0; r < R; ++r) {
+      // A slice of the step that starts past the partition end loads nothing (block-uniform test);
+      // inside a slice the loads are unguarded (the record buffer is padded past its last record).
       const int64_t i = i0 + (int64_t)r * blockDim.x + threadIdx.x;
+      const bool slice = i0 + (int64_t)r * blockDim.x < end;
 #pragma unroll
       for (int j = 0; j < W; ++j) {
-        const u64 v = recs[i * W + j];
+        u64 v = kEmpty;
+        if (slice) v = recs[i * W + j];
         q[r][j] = i < end ? v : kEmpty;
       }
     }
@@ -959,7 +963,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     DescUpload up;
     build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
     launch_grouped(jf_count, 1, iv, up.ddesc, t, (unsigned)pp.nchunks, (size_t)P * 4, "agg_part_count",
-                   "agg_part_count_jit", kPartScanBlock);
+                   jit_prof_name(jit, "agg_part_count_jit"), kPartScanBlock);
   }
   {
     ProfScope ps("agg_part_offsets");
@@ -973,7 +977,7 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     DescUpload up;
     build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
     const size_t lds = (size_t)P * 8 + (size_t)pp.pre * 8 * pp.W;
-    launch_grouped(jf_scatter, 2, iv, up.ddesc, t, (unsigned)pp.nchunks, lds, "agg_part_scatter", "agg_part_scatter_jit",
+    launch_grouped(jf_scatter, 2, iv, up.ddesc, t, (unsigned)pp.nchunks, lds, "agg_part_scatter", jit_prof_name(jit, "agg_part_scatter_jit"),
                    kPartScanBlock);
   }
   {
@@ -1146,7 +1150,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   DescUpload up;
   build_desc(up, d, in, plan, nullptr, env, false, 0, nullptr);
   GTable t = table_of(A);
-  launch_grouped(jf, 0, iv, up.ddesc, t, grid, lds, "agg_hash", "agg_hash_jit");
+  launch_grouped(jf, 0, iv, up.ddesc, t, grid, lds, "agg_hash", jit_prof_name(jit, "agg_hash_jit"));
 }
 
 // Compacts the groups into tmp buffers: keys then accs in `buf`, row-major (fetch) or

@@ -156,6 +156,9 @@ constexpr int kJitBatchDense = 102;                         // batched kernel of
 // ---- learned sizing (ws_sizing.hip): appends one (features, exact groups) sample to the log
 void sizing_log(const float* features, int64_t groups);
 void jit_launch_raw(void* fn, unsigned grid, unsigned block, size_t shmem, void** args, const char* prof_name);
+// Profile name of a compiled plan's launch: base + "[" + the plan's kind + "]" (interned; base
+// alone without a plan), so stages of one query that run the same kernel kind are told apart.
+const char* jit_prof_name(const ws_jit_program* p, const char* base);
 int jit_rows(const ws_jit_program* p);
 
 }  // namespace ws

@@ -528,6 +528,13 @@ void* jit_function(const ws_jit_program* p, int fn) {
   return (p->fn_kind == fn || sel) ? (void*)p->fn : nullptr;
 }
 int jit_rows(const ws_jit_program* p) { return p ? p->rows : 1; }
+const char* jit_prof_name(const ws_jit_program* p, const char* base) {
+  if (!p || p->kind.empty()) return base;
+  static std::mutex mu;
+  static std::set<std::string> names;  // kept for the process: profile slots hold the pointer
+  std::lock_guard<std::mutex> g(mu);
+  return names.insert(std::string(base) + "[" + p->kind + "]").first->c_str();
+}
 void jit_launch_raw(void* fn, unsigned grid, unsigned block, size_t shmem, void** args, const char* prof_name) {
   ProfScope ps(prof_name);
   WS_CK(hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, block, 1, 1, (unsigned)shmem, stream(), args, nullptr));

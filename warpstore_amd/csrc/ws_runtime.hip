@@ -571,7 +571,7 @@ int ws_prof_collect(ws_prof_row* rows, int cap) {
       (void)hipGetLastError();
       continue;
     }
-    auto it = std::find_if(agg.begin(), agg.end(), [&](const ws_prof_row& x) { return strcmp(x.name, r.name) == 0; });
+    auto it = std::find_if(agg.begin(), agg.end(), [&](const ws_prof_row& x) { return strncmp(x.name, r.name, sizeof x.name - 1) == 0; });
     if (it == agg.end()) {
       ws_prof_row x{};
       snprintf(x.name, sizeof x.name, "%s", r.name);

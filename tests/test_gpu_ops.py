@@ -64,6 +64,48 @@ def test_select_multi(gpu, mode):
     assert np.array_equal(per, eper)
 
 
+@pytest.mark.parametrize("mode", ["interp", "jit"])
+def test_select_multi_chunked(gpu, mode):
+    """Inputs above 4 tiles per CU take the chunked selection kernel (one look-back per chunk of
+    tiles): 2.6M rows in ragged batches with deletes, against the oracle."""
+    H.say(f"case select_multi chunked {mode}")
+    n = 2_600_000
+    rng = np.random.default_rng(71)
+    a = rng.integers(-1000, 1000, n).astype(np.int32)
+    x = rng.random(n)
+    x[rng.random(n) < 0.03] = np.nan
+    deleted = rng.random(n) < 0.1
+    t = H.Table({"a": a, "x": x}, [1_048_576, 3, 0, 700_001, 851_420], deleted)
+    b = abi.EnvBuilder()
+    sa, sx = b.slot(abi.T_I32), b.slot(abi.T_F64)
+    where = b.prog([(abi.OP_COL, sa), b.ilit(990), (abi.OP_GT,), (abi.OP_COL, sx), b.dlit(0.002), (abi.OP_LT,), (abi.OP_OR,)])
+    slots = [("a", abi.T_I32, 0), ("x", abi.T_F64, 0)]
+    dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
+    nb = len(t.sizes)
+    bn = np.array(t.sizes, np.int64)
+    jit = None
+    if mode == "jit":
+        gpu.ws_jit_set_mode(abi.JIT_SYNC)
+        fp = abi.AggPlan()
+        fp.has_where, fp.where = 1, where
+        jit = C.c_void_p()
+        ws.check(gpu.ws_jit_get(C.byref(b.env), 2, C.byref(fp), abi.JF_FLAGM, b"filter", C.byref(jit)))
+        assert jit.value
+    out = C.c_void_p()
+    per = np.zeros(nb, np.int64)
+    cnt = C.c_int64()
+    ws.check(gpu.ws_select_multi(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(dev.dels, C.c_void_p),
+                                 C.cast(dev.views, C.c_void_p), 2, C.byref(out), ws.ptr(per), jit, C.byref(cnt)))
+    got = ws.DevBuf(cnt.value * 4, tmp=True, adopt=out.value).to_array(np.int32, cnt.value)
+    exp = np.zeros(n, np.int32)
+    eper = np.zeros(nb, np.int64)
+    m = H.oracle().or_select(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(host.dels, C.c_void_p),
+                             C.cast(host.views, C.c_void_p), 2, ws.ptr(exp), ws.ptr(eper))
+    assert cnt.value == m and m > 1000
+    assert np.array_equal(got, exp[:m])
+    assert np.array_equal(per, eper)
+
+
 def test_select_single_env_cols(gpu):
     H.say("case select single")
     n = 100_003

@@ -53,6 +53,10 @@ __global__ void init_dense_k(double* acc, int64_t G, int na, ws_agg_plan plan) {
 __global__ void dense_reduce_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts, int nparts, double* gacc) {
   dense_reduce_body(plan, ds, parts, nparts, gacc);
 }
+__global__ void __launch_bounds__(256) dense_reduce_slot_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts, int nparts,
+                                                           double* gacc) {
+  dense_reduce_slot_body(plan, ds, parts, nparts, gacc);
+}
 // Compacted groups, row-major (fetch) or column-major (compact): rows [off, off + g) of a result
 // of G groups, taken from slots idx[i] of a row-major table (idx null: slot i).
 __global__ void gather_groups_k(const double* kv, int nk, const double* acc, int na, const int32_t* idx, int64_t g,
@@ -877,10 +881,11 @@ TuneKnob g_tune[] = {
     {"batch_lds_slots", {1024}, 64, 8192},      // its LDS table slots per block
     {"batch_blocks_per_cu", {4}, 1, 32},        // its grid, per CU
     {"part_pack", {1}, 0, 1},                   // packed partial-group records allowed (integer keys)
+    {"select_chunks_per_cu", {8}, 1, 64},       // chunked selection: chunks of tiles per CU
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
 enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
-       kTuneBatchBlocksPerCu, kTunePack };
+       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks };
 
 struct PartPlan {
   int pbits = 0;  // 0: plain table path
@@ -1124,8 +1129,12 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
     }
     if (dp.mode != 0) {
       ProfScope ps("agg_dense_reduce");
-      hipLaunchKernelGGL(dense_reduce_k, dim3(grid_for(S)), dim3(kBlock), 0, stream(), plan, ds, (const i64*)pp, (int)grid,
-                         gacc);
+      if (S <= 16384 && grid >= 64)  // few slots, many partials: a block per slot
+        hipLaunchKernelGGL(dense_reduce_slot_k, dim3((unsigned)S), dim3(256), 0, stream(), plan, ds, (const i64*)pp, (int)grid,
+                           gacc);
+      else
+        hipLaunchKernelGGL(dense_reduce_k, dim3(grid_for(S)), dim3(kBlock), 0, stream(), plan, ds, (const i64*)pp, (int)grid,
+                           gacc);
       check_launch("agg_dense_reduce");
     }
     return;
@@ -1527,24 +1536,43 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
     if (per_batch) per_batch->assign(in.nb, 0);
     return 0;
   }
+  // Long inputs take the chunked kernel: a chunk of consecutive tiles per block (select_chunks_per_cu
+  // chunks per CU, at most 170 tiles so its ballots fit 32 KiB of LDS), one look-back per chunk.
+  const int per_tile = kSelChunkRows * (kBlock / 64);  // ballots per chunked tile (2048 rows)
+  const int64_t want = (int64_t)cu_count() * tune(kTuneSelectChunks);
+  const bool chunked = ntiles > 2 * want;
+  int64_t nunits = ntiles;
+  size_t lds = 0;
+  if (chunked) {
+    d.tile_rows = kSelChunkRows * kBlock;
+    ntiles = 0;
+    for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
+    d.chunk_tiles = (int32_t)std::min<int64_t>(85, (ntiles + want - 1) / want);  // ballots within 32 KiB
+    nunits = (ntiles + d.chunk_tiles - 1) / d.chunk_tiles;
+    lds = (size_t)d.chunk_tiles * per_tile * 12;
+  }
   DescUpload up;
   build_desc(up, d, in, fp, nullptr, env, cols_from_env, 0, nullptr);
-  // [tile states][counter][total][err][per-batch counts]
+  // [tile (or chunk) states][counter][total][err][per-batch counts]
   DevBuf<u64> st(ntiles + 3 + in.nb);
   fill_u64(st.p, st.n, 0);
   unsigned* ctr = (unsigned*)(st.p + ntiles);
   u64* total = st.p + ntiles + 1;
   int* err = (int*)(st.p + ntiles + 2);
   u64* pb = st.p + ntiles + 3;
-  void* jf = jit ? jit_function(jit, fn) : nullptr;
+  void* jf = jit ? jit_function(jit, chunked ? kJitSelectChunk : fn) : nullptr;
   if (jf) {
     void* args[] = {&up.ddesc, &st.p, &ctr, &out_idx, &pb, &total, &err};
-    jit_launch_raw(jf, (unsigned)ntiles, kBlock, 0, args, "select_jit");
+    jit_launch_raw(jf, (unsigned)nunits, kBlock, lds, args, jit_prof_name(jit, "select_jit"));
   } else {
     ProfScope ps("select");
     const int iv = !has_where || prog_is_leaf(env, where) ? kIvCols : prog_is_simple(env, where) ? kIvLean : kIvFull;
-    hipLaunchKernelGGL(IV_PICK(iv, select), dim3((unsigned)ntiles), dim3(kBlock), 0, stream(), up.ddesc, st.p, ctr,
-                       out_idx, pb, total, err);
+    if (chunked)
+      hipLaunchKernelGGL(IV_PICK(iv, select_chunk), dim3((unsigned)nunits), dim3(kBlock), lds, stream(), up.ddesc, st.p, ctr,
+                         out_idx, pb, total, err);
+    else
+      hipLaunchKernelGGL(IV_PICK(iv, select), dim3((unsigned)ntiles), dim3(kBlock), 0, stream(), up.ddesc, st.p, ctr,
+                         out_idx, pb, total, err);
     check_launch("select");
   }
   std::vector<u64> h(3 + in.nb);

@@ -153,6 +153,7 @@ struct JitKernel;  // a compiled kernel ready to launch
 void* jit_function(const ws_jit_program* p, int fn);  // fn: WS_JF_* or one of the kinds below
 constexpr int kJitPartCount = 100, kJitPartScatter = 101;  // partition passes of a grouped plan
 constexpr int kJitBatchDense = 102;                         // batched kernel of a dense plan
+constexpr int kJitSelectChunk = 103;                        // chunked kernel of a selection plan
 // ---- learned sizing (ws_sizing.hip): appends one (features, exact groups) sample to the log
 void sizing_log(const float* features, int64_t groups);
 void jit_launch_raw(void* fn, unsigned grid, unsigned block, size_t shmem, void** args, const char* prof_name);

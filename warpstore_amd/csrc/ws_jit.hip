@@ -31,6 +31,7 @@ struct ws_jit_program {
   hipFunction_t fn = nullptr;
   hipFunction_t fn_part[2] = {nullptr, nullptr};  // grouped plans: partition count / scatter kernels
   hipFunction_t fn_batch = nullptr;               // dense plans: the batched (many queries) kernel
+  hipFunction_t fn_chunk = nullptr;               // selection plans: the chunked kernel
   std::atomic<int> state{0};  // 0 compiling, 1 ready, 2 failed
   bool from_disk = false;
   int rows = 1;
@@ -122,7 +123,7 @@ std::string as_i(const Node& n) { return n.is_int ? n.e : "((long long)" + n.e +
 struct Translator {
   const ws_expr_env& env;
   int ncols;
-  std::set<int> cols, iconsts, dconsts, encs;  // encs: encoded integer columns read
+  std::set<int> cols, iconsts, dconsts, encs, sets;  // encs: encoded integer columns read; sets: INSET bitmaps
 
   std::string raw_int(int c) {
     const ws_colview& v = env.cols[c];
@@ -179,7 +180,7 @@ struct Translator {
           Node b = pop(), a = pop();
           if (a.is_int && b.is_int) {
             const char* f = in.op == WS_OP_ADD ? "ws::add_chk(" : in.op == WS_OP_SUB ? "ws::sub_chk(" : "ws::mul_chk(";
-            st.push_back({std::string(f) + a.e + "," + b.e + ",env->ovf)", true});
+            st.push_back({std::string(f) + a.e + "," + b.e + ",&ovf_)", true});
           } else {
             const char* o = in.op == WS_OP_ADD ? "+" : in.op == WS_OP_SUB ? "-" : "*";
             st.push_back({"(" + as_d(a) + o + as_d(b) + ")", false});
@@ -215,7 +216,7 @@ struct Translator {
           // Both operands are evaluated (bitwise & / |) unless one can raise the overflow flag:
           // their column loads then issue together instead of one memory round trip after the
           // other. Operands that can set env->ovf keep the short circuit.
-          const bool fx = a.e.find("env->ovf") != std::string::npos || b.e.find("env->ovf") != std::string::npos;
+          const bool fx = a.e.find("&ovf_") != std::string::npos || b.e.find("&ovf_") != std::string::npos;
           const char* join = in.op == WS_OP_AND ? (fx ? "&&" : "&") : (fx ? "||" : "|");
           if (a.is_int && b.is_int)
             st.push_back({"((long long)(((" + a.e + ")!=0)" + join + "((" + b.e + ")!=0)))", true});
@@ -231,8 +232,10 @@ struct Translator {
         case WS_OP_INSET: {
           Node a = pop();
           const std::string s = std::to_string(in.arg);
-          if (a.is_int) st.push_back({"ws::inset_i(env->sets[" + s + "],env->set_size[" + s + "]," + a.e + ")", true});
-          else st.push_back({"ws::inset_d(env->sets[" + s + "],env->set_size[" + s + "]," + a.e + ")", false});
+          if (in.arg >= WS_MAX_SETS) throw Error("codegen: set out of range", WS_ERR_INTERNAL);
+          sets.insert(in.arg);
+          if (a.is_int) st.push_back({"ws::inset_g(s" + s + ",sn" + s + "," + a.e + ")", true});
+          else st.push_back({"ws::inset_gd(s" + s + ",sn" + s + "," + a.e + ")", false});
           break;
         }
         case WS_OP_SELECT: {
@@ -343,14 +346,26 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   o << "  static constexpr bool kPureKeys = " << (pure_keys ? "true" : "false") << ", kPureVals = "
     << (pure_vals ? "true" : "false") << ";\n";
   o << "  static constexpr bool kGlobalCols = true;\n";
+  o << "  static constexpr bool kPureWhere = " << (where.e.find("&ovf_") == std::string::npos ? "true" : "false") << ";\n";
   o << "  const ws_expr_env* env;\n";
+  // Overflow of checked integer arithmetic: a per-thread flag, stored to env->ovf once when the
+  // evaluator goes out of scope (a possible store in the row loop would pin every later load
+  // behind it).
+  o << "  mutable int ovf_ = 0;\n";
+  o << "  __device__ ~QEval() { if (ovf_ && env->ovf) *env->ovf = 1; }\n";
   for (int c : tr.cols) o << "  " << ctype_of(env.cols[c].t) << " c" << c << ";\n  ws_colview v" << c << ";\n";
   for (int c : tr.encs) o << "  ws::EncView e" << c << ";\n";
   for (int k : tr.iconsts) o << "  long long ik" << k << ";\n";
   for (int k : tr.dconsts) o << "  double k" << k << ";\n";
+  for (int k : tr.sets) o << "  const unsigned WS_GLOBAL* s" << k << ";\n  long long sn" << k << ";\n";
   o << "  __device__ QEval(const ws::ScanDesc& d) : env(d.env) {\n";
-  for (int k : tr.iconsts) o << "    ik" << k << " = ws::dbits(env->consts[" << k << "]);\n";
-  for (int k : tr.dconsts) o << "    k" << k << " = env->consts[" << k << "];\n";
+  // literals and set bitmaps are the same for every lane: scalar registers
+  for (int k : tr.iconsts) o << "    ik" << k << " = ws::uni((long long)ws::dbits(env->consts[" << k << "]));\n";
+  for (int k : tr.dconsts) o << "    k" << k << " = ws::bitsd(ws::uni((long long)ws::dbits(env->consts[" << k << "])));\n";
+  // an empty set reads its own size word (always readable; every bit is masked off)
+  for (int k : tr.sets)
+    o << "    sn" << k << " = ws::uni((long long)env->set_size[" << k << "]); s" << k << " = (const unsigned WS_GLOBAL*)ws::uni(sn" << k
+      << " > 0 ? env->sets[" << k << "] : (const unsigned*)&env->set_size[" << k << "]);\n";
   o << "  }\n  __device__ void bind(const ws_colview* c) {\n";
   for (int c : tr.cols)
     o << "    v" << c << " = ws::uniform_view(c[" << c << "]); c" << c << " = (" << ctype_of(env.cols[c].t) << ")v" << c << ".p;\n";
@@ -437,6 +452,8 @@ void load_image(ws_jit_program& p, const std::string& image) {
   WS_CK(hipModuleGetFunction(&f, mod, kKernelName[p.fn_kind]));
   p.fn = f;
   if (p.fn_kind == WS_JF_DENSE) WS_CK(hipModuleGetFunction(&p.fn_batch, mod, (std::string(kKernelName[p.fn_kind]) + "_batch").c_str()));
+  if (p.fn_kind == WS_JF_FLAG || p.fn_kind == WS_JF_FLAGM)
+    WS_CK(hipModuleGetFunction(&p.fn_chunk, mod, (std::string(kKernelName[p.fn_kind]) + "_chunk").c_str()));
   if (p.fn_kind == WS_JF_GROUPED) {
     const std::string base = kKernelName[p.fn_kind];
     WS_CK(hipModuleGetFunction(&p.fn_part[0], mod, (base + "_pcount").c_str()));
@@ -524,6 +541,7 @@ void* jit_function(const ws_jit_program* p, int fn) {
   if (fn == kJitPartCount || fn == kJitPartScatter)
     return p->fn_kind == WS_JF_GROUPED ? (void*)p->fn_part[fn == kJitPartScatter ? 1 : 0] : nullptr;
   if (fn == kJitBatchDense) return p->fn_kind == WS_JF_DENSE ? (void*)p->fn_batch : nullptr;
+  if (fn == kJitSelectChunk) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_chunk : nullptr;
   const bool sel = (fn == WS_JF_FLAG || fn == WS_JF_FLAGM) && (p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM);
   return (p->fn_kind == fn || sel) ? (void*)p->fn : nullptr;
 }

@@ -105,6 +105,37 @@ __device__ __forceinline__ int probe_row(const ProbeDesc& d, const JEntry* tab, 
   return c;
 }
 
+// Match counts of a thread's kJRows rows of a tile (rows r0 + jj * kJBlock + tid): every row's key
+// and first table slot are loaded before any chain is walked, so the rows' memory round trips
+// overlap instead of running one after another.
+__device__ __forceinline__ void probe_counts(const ProbeDesc& d, const JEntry* tab, u64 mask, i64 r0, i64 r1, int (&m)[kJRows]) {
+  i64 k[kJRows];
+  u64 s[kJRows];
+  bool ok[kJRows];
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) {
+    const i64 j = r0 + (i64)jj * kJBlock + threadIdx.x;
+    ok[jj] = tab && j < r1;
+    const i64 jc = ok[jj] ? j : r0;  // (a valid row; its result is dropped)
+    ok[jj] = ok[jj] && !jkey_null(d.key, jc);
+    k[jj] = jkey_at(d.key, jc);
+    s[jj] = mix64((u64)k[jj]) & mask;
+  }
+  JEntry e[kJRows];
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) e[jj] = ok[jj] ? tab[s[jj]] : JEntry{kJEmpty, 0};
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) {
+    int c = 0;
+    for (u64 step = 0; step <= mask && e[jj].key != kJEmpty; ++step) {
+      c += e[jj].key == k[jj];
+      s[jj] = (s[jj] + 1) & mask;
+      e[jj] = tab[s[jj]];
+    }
+    m[jj] = c;
+  }
+}
+
 // Pass 1: tile match counts -> exclusive output offsets by look-back.
 __global__ void __launch_bounds__(kJBlock) join_count_k(ProbeDesc d, u64* state, unsigned* ctr, u64* tile_off, u64* total,
                                                         int* err) {
@@ -119,10 +150,12 @@ __global__ void __launch_bounds__(kJBlock) join_count_k(ProbeDesc d, u64* state,
   const i64 r0 = d.p_off[p] + (t - d.tile0[p]) * kJTile;
   const i64 r1 = min((long long)(r0 + kJTile), (long long)d.p_off[p + 1]);
   u64 c = 0;
-  for (i64 j = r0 + threadIdx.x; j < r1; j += kJBlock) {
-    int m = tab ? probe_row(d, tab, mask, j, [](int32_t) {}) : 0;
-    if (d.outer && m == 0) m = 1;
-    c += (u64)m;
+  int mc[kJRows];
+  probe_counts(d, tab, mask, r0, r1, mc);
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) {
+    const i64 j = r0 + (i64)jj * kJBlock + threadIdx.x;
+    c += (u64)(d.outer && j < r1 && mc[jj] == 0 ? 1 : mc[jj]);
   }
   c = wave_sum_u64(c);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -151,9 +184,11 @@ __global__ void __launch_bounds__(kJBlock) join_write_k(ProbeDesc d, const u64* 
   const i64 r1 = min((long long)(r0 + kJTile), (long long)d.p_off[p + 1]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   u64 carry = tile_off[t];
+  int mc[kJRows];
+  probe_counts(d, tab, mask, r0, r1, mc);
   for (int jj = 0; jj < kJRows; ++jj) {
     const i64 j = r0 + (i64)jj * kJBlock + threadIdx.x;
-    int m = (j < r1 && tab) ? probe_row(d, tab, mask, j, [](int32_t) {}) : 0;
+    int m = j < r1 ? mc[jj] : 0;
     const bool fill = d.outer && j < r1 && m == 0;
     const u64 cnt = (u64)(fill ? 1 : m);
     // inclusive wave scan, then wave totals through LDS

@@ -7,6 +7,7 @@
 // An evaluator supplies:
 //   static constexpr int kKeys, kAggs, kRows;   // loop bounds, rows per thread per tile
 //   static constexpr bool kPureKeys, kPureVals;  // keys / slot values have no side effect
+//   static constexpr bool kPureWhere;             // neither has the WHERE (no overflow flag)
 //   static constexpr bool kGlobalCols;            // bind() takes the batch's column views in global
 //                                                 // memory (uniform loads: SGPR column bases) rather
 //                                                 // than the block's LDS copy
@@ -256,6 +257,20 @@ __device__ __forceinline__ i64 uni(i64 x) {
 template <class T>
 __device__ __forceinline__ T* uni(T* p) { return (T*)uni((i64)p); }
 
+// Set membership against a bitmap in global memory (compiled plans: the bitmap base is scalar).
+// Branch-free: the load is unconditional (index clamped to 0 outside the set) and its bit is
+// masked, so a row's evaluation stays one basic block and the loads of a thread's rows can issue
+// together. `bits` must be readable at word 0 even for an empty set (size <= 0 masks every bit).
+__device__ __forceinline__ i64 inset_g(const unsigned WS_GLOBAL* bits, i64 size, i64 c) {
+  const unsigned in = (unsigned)(c >= 0) & (unsigned)(c < size);
+  const unsigned w = bits[(in ? c : 0) >> 5];
+  return (i64)((w >> (c & 31)) & in);
+}
+__device__ __forceinline__ double inset_gd(const unsigned WS_GLOBAL* bits, i64 size, double x) {
+  if (x != x) return x;
+  return (double)inset_g(bits, size, (i64)x);
+}
+
 __device__ __forceinline__ int batch_of_tile(const ScanDesc& d, i64 t, int hint) {
   int b = hint < 0 ? 0 : hint;
   while (b + 1 < d.nb && d.tile0[b + 1] <= t) ++b;
@@ -302,6 +317,7 @@ template <bool kFull>
 struct InterpEvalT {
   static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 1;
   static constexpr bool kPureKeys = false, kPureVals = false;  // (see the compiled evaluator)
+  static constexpr bool kPureWhere = false;
   static constexpr bool kGlobalCols = false;
   const ws_expr_env* env;
   const ws_agg_plan* plan;
@@ -351,7 +367,7 @@ typedef InterpEvalT<false> InterpEvalLean;
 // or literal: no VM call at all, so rows can be unrolled and their loads issued together.
 struct InterpEvalCols {
   static constexpr int kKeys = WS_INTERP_KEYS, kAggs = WS_INTERP_AGGS, kRows = 4;
-  static constexpr bool kPureKeys = true, kPureVals = true;
+  static constexpr bool kPureKeys = true, kPureVals = true, kPureWhere = true;
   static constexpr bool kGlobalCols = false;
   const ws_expr_env* env;
   const ws_agg_plan* plan;
@@ -750,6 +766,53 @@ __device__ void dense_reduce_body(const ws_agg_plan& plan, const ws_dense_spec& 
       if (!any) continue;
       gacc[s] = combine_d(op, gacc[s], x);
     }
+  }
+}
+
+// The same fold with one block per slot (few slots, many partials): the block's threads combine
+// strided partials, then a fixed-order tree in LDS (deterministic for doubles) finishes the slot.
+__device__ void dense_reduce_slot_body(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int nparts,
+                                       double* gacc) {
+  __shared__ i64 s_v[256];
+  __shared__ int s_any[256];
+  const int NA = plan.naggs;
+  const i64 S = ds.G * NA;
+  const i64 s = blockIdx.x;
+  const i64 g = s / NA;
+  const int a = (int)(s - g * NA);
+  const uint8_t k = ds.kind[a];
+  const int op = dk_op(k);
+  const bool isint = dk_int(k);
+  i64 xi = op == WS_AGG_SUM ? 0 : int_identity(op);
+  double xd = agg_identity(op);
+  bool any = false;
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+    if (parts[(i64)p * S + g * NA] == 0) continue;  // the block saw no row of g
+    any = true;
+    if (isint) xi = combine_i(op, xi, parts[(i64)p * S + s]);
+    else xd = combine_d(op, xd, bitsd(parts[(i64)p * S + s]));
+  }
+  s_v[threadIdx.x] = isint ? xi : dbits(xd);
+  s_any[threadIdx.x] = any ? 1 : 0;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s_any[threadIdx.x + w]) {
+      const i64 o = s_v[threadIdx.x + w];
+      s_v[threadIdx.x] = !s_any[threadIdx.x] ? o
+                         : isint         ? combine_i(op, s_v[threadIdx.x], o)
+                                         : dbits(combine_d(op, bitsd(s_v[threadIdx.x]), bitsd(o)));
+      s_any[threadIdx.x] = 1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0 || !s_any[0]) return;
+  const bool bits = plan.agg_bits[a] != 0;
+  if (isint) {
+    const i64 x = s_v[0];
+    if (bits) gacc[s] = bitsd(combine_i(op, dbits(gacc[s]), x));
+    else gacc[s] = combine_d(a == 0 ? WS_AGG_SUM : op, gacc[s], (double)x);
+  } else {
+    gacc[s] = combine_d(op, gacc[s], bitsd(s_v[0]));
   }
 }
 
@@ -1244,6 +1307,7 @@ __device__ void part_scatter_body(const ScanDesc& d) {
 // prefix; [61:0] the value.
 constexpr u64 kLbAgg = 1ull << 62, kLbInc = 2ull << 62, kLbMask = (1ull << 62) - 1;
 constexpr int kSelRows = 4;  // rows per thread of a select tile: tile = 4 * blockDim rows
+constexpr int kSelChunkRows = 8;  // rows per thread of a chunked select's tile: tile = 8 * blockDim rows
 
 // Wave-wide decoupled look-back, called by one full wave of the block that owns tile t: publishes
 // the tile's count, then sums the counts of earlier tiles (64 per step) until an inclusive prefix is
@@ -1409,6 +1473,125 @@ __device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, i
   }
 }
 
+// Chunked single-pass WHERE + compaction for long inputs (the tile-per-block form above pays an
+// atomic tile fetch, a look-back and four barriers per 1024 rows). A block takes the next chunk
+// (chunk_tiles consecutive tiles) from a counter, evaluates every row of it keeping only the
+// wave ballots in LDS (8 B per 64 rows), looks back once over earlier chunks for its output
+// offset, then replays the ballots in (tile, step, wave) order and writes ascending global row
+// indices. Chunks are handed out in order, so every predecessor a block waits for is running.
+template <class Ev>
+__device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk_ctr, int32_t* out_idx, u64* per_batch,
+                                  u64* total, int* err) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  __shared__ unsigned s_chunk;
+  __shared__ u64 s_excl;
+  __shared__ uint32_t s_w[16];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  if (threadIdx.x == 0) s_chunk = atomicAdd(chunk_ctr, 1u);
+  __syncthreads();
+  const i64 c = s_chunk;
+  const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
+  if (t0 >= d.ntiles) return;
+  const int per_tile = kSelChunkRows * nw;             // ballots per tile
+  u64* s_mask = (u64*)ws_smem;                         // [tiles][kSelRows][nw]
+  uint32_t* s_pre = (uint32_t*)(s_mask + (t1 - t0) * per_tile);  // exclusive prefix of their counts
+  Ev ev(d);
+  int cur = -1;
+  u64 bcount = 0;  // this wave's passing rows in batch cur, added to per_batch when the batch ends
+  for (i64 t = t0; t < t1; ++t) {
+    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
+    if (b != cur) {
+      if (cur >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+      bcount = 0;
+      stage_batch(d, b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
+      cur = b;
+    }
+    const TileRows tr = tile_rows(d, t, b);
+    const uint32_t* del = uni(d.del[b]);
+    bool pass[kSelChunkRows];
+    if (Ev::kPureWhere || !del) {  // (block-uniform)
+      // Every row's WHERE is evaluated (rows past the tile re-read its first row), so the loads of
+      // all the thread's rows issue before any result is needed. Safe when the WHERE has no side
+      // effect, or when no row is deleted: then each evaluated row is a live row anyway.
+      bool w[kSelChunkRows];
+#pragma unroll
+      for (int j = 0; j < kSelChunkRows; ++j) {
+        const i64 i = (i64)j * blockDim.x + threadIdx.x;
+        w[j] = !ev.has_where() || ev.where(tr.first + (i < tr.count ? i : 0));
+      }
+#pragma unroll
+      for (int j = 0; j < kSelChunkRows; ++j) {
+        const i64 i = (i64)j * blockDim.x + threadIdx.x;
+        pass[j] = i < tr.count && w[j] && !is_deleted(del, tr.first + i);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kSelChunkRows; ++j) {
+        const i64 i = (i64)j * blockDim.x + threadIdx.x;
+        const i64 r = tr.first + i;
+        pass[j] = i < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSelChunkRows; ++j) {
+      const u64 m = __ballot(pass[j]);
+      if (lane == 0) s_mask[(t - t0) * per_tile + j * nw + wave] = m;
+      bcount += (u64)__popcll(m);
+    }
+  }
+  if (lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+  __syncthreads();
+  // Exclusive prefix of the ballots' counts in (tile, step, wave) order: per thread a run of
+  // consecutive ballots, then a block scan of the run totals.
+  const int nm = (int)(t1 - t0) * per_tile;
+  const int K = (nm + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int m0 = threadIdx.x * K, m1 = m0 + K < nm ? m0 + K : nm;
+  uint32_t run = 0;
+  for (int q = m0; q < m1; ++q) run += (uint32_t)__popcll(s_mask[q]);
+  uint32_t incl = run;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += y;
+  }
+  if (lane == kWave - 1) s_w[wave] = incl;
+  __syncthreads();
+  uint32_t wbase = 0, count = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wave) wbase += s_w[w];
+    count += s_w[w];
+  }
+  uint32_t pre = wbase + incl - run;
+  for (int q = m0; q < m1; ++q) {
+    s_pre[q] = pre;
+    pre += (uint32_t)__popcll(s_mask[q]);
+  }
+  if (wave == 0) {
+    const u64 excl = lookback_exclusive(state, c, count, err);
+    if (lane == 0) {
+      s_excl = excl;
+      if (t1 == d.ntiles) *total = excl + count;
+    }
+  }
+  __syncthreads();
+  const u64 base = s_excl;
+  cur = -1;
+  for (i64 t = t0; t < t1; ++t) {
+    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
+    cur = b;
+    const TileRows tr = tile_rows(d, t, b);
+    const i64 g0 = d.row0[b] + tr.first;
+#pragma unroll
+    for (int j = 0; j < kSelChunkRows; ++j) {
+      const int q = (int)(t - t0) * per_tile + j * nw + wave;
+      const u64 m = s_mask[q];
+      if ((m >> lane) & 1)
+        out_idx[base + s_pre[q] + (u64)__popcll(m & ((1ull << lane) - 1))] = (int32_t)(g0 + (i64)j * blockDim.x + threadIdx.x);
+    }
+  }
+}
+
 }  // namespace ws
 
 // ------------------------------------------------------------------ kernel entry points
@@ -1446,4 +1629,10 @@ __device__ void select_body(const ScanDesc& d, u64* state, unsigned* tile_ctr, i
                                                          unsigned long long* per_batch,                     \
                                                          unsigned long long* total, int* err) {             \
     ws::select_body<EV>(*d, state, tile_ctr, out_idx, per_batch, total, err);                               \
+  }                                                                                                         \
+  extern "C" __global__ void __launch_bounds__(256) NAME##_chunk(const ws::ScanDesc* d, unsigned long long* state, \
+                                                                 unsigned* chunk_ctr, int* out_idx,         \
+                                                                 unsigned long long* per_batch,             \
+                                                                 unsigned long long* total, int* err) {     \
+    ws::select_chunk_body<EV>(*d, state, chunk_ctr, out_idx, per_batch, total, err);                         \
   }

@@ -450,14 +450,14 @@ struct RelCol {
 // materialize(table, WHERE) of the needed columns (engine_core.h:4345-4487): one fused filter
 // launch over all segments, then one gather per needed column.
 int64_t materialize(const DevTable& t, Comp& c, bool has_where, Prog where, const std::vector<int>& slot_col,
-                    const std::vector<int>& need, std::vector<RelCol>& out) {
+                    const std::vector<int>& need, std::vector<RelCol>& out, const char* kind = "filter") {
   const JitProgram* jit = nullptr;
   if (has_where) {
     AggPlan fp;
     std::memset(&fp, 0, sizeof fp);
     fp.has_where = 1;
     fp.where = where;
-    jit = jitGet(c.env, c.ncols, fp, JF_FLAGM, "filter");
+    jit = jitGet(c.env, c.ncols, fp, JF_FLAGM, kind);
   }
   std::vector<int64_t> bn;
   std::vector<const uint32_t*> bdel;
@@ -502,7 +502,7 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
   const int d_attr = cd.slot(WS_T_I32);
   const Prog dwhere = cd.prog({cd.col(d_attr), cd.ilit(50), Comp::op(WS_OP_LT)});
   std::vector<RelCol> drel;
-  const int64_t nd = materialize(dim, cd, true, dwhere, {1}, {0}, drel);
+  const int64_t nd = materialize(dim, cd, true, dwhere, {1}, {0}, drel, "filter (dimension)");
   // join filter: the dim keys were reduced below half their table, fact is large: exact bitmap
   // of the keys over the dim key range (engine_select.cpp:1015-1041)
   const int64_t lo = dim.cols[0].mn, nbits = dim.cols[0].mx - dim.cols[0].mn + 1;
@@ -518,7 +518,7 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
   cf.env.set_size[0] = (int32_t)nbits;
   const Prog fwhere = cf.prog({cf.col(f_fk), cf.ilit(lo), Comp::op(WS_OP_SUB), Comp::I{WS_OP_INSET, 0, 0}});
   std::vector<RelCol> frel;
-  const int64_t nf = materialize(fact, cf, true, fwhere, {0}, {0, 2, 1, 3}, frel);
+  const int64_t nf = materialize(fact, cf, true, fwhere, {0}, {0, 2, 1, 3}, frel, "filter (fact by join keys)");
   // hash join: build on the smaller side (dim), probe the filtered fact rows
   DevVec<int32_t> ib, ip;
   ColView bk{}, pk{};

@@ -321,7 +321,6 @@ def bench_join(args) -> int:
     g.wsg_c3_fact(3, JOIN_FACT, JOIN_DIM, ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag))
     fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
     dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
-    fact_filter = "select_jit[filter (fact by join keys)]"
     L.ws_prof_reset()
     L.ws_prof_only(b"")
     L.ws_prof_enable(1)
@@ -329,6 +328,8 @@ def bench_join(args) -> int:
         rows, st = Q.c3(fact, dim)
     L.ws_sync()
     warm = ws.prof_rows()
+    # the fact filter's launch: the persistent (sets staged in LDS) or the chunked selection kernel
+    fact_filter = next((k for k in warm if k.endswith("[filter (fact by join keys)]")), "select_jit[filter (fact by join keys)]")
     L.ws_prof_enable(0)
     L.ws_prof_reset()
     L.ws_prof_only(fact_filter.encode())
@@ -368,7 +369,7 @@ def bench_join(args) -> int:
         "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 4),
         "parity_vs_oracle": parity,
         "stage_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(warm.items(), key=lambda kv: -kv[1][1])[:8]},
-        "roofline": {"kernel": "ws_q_select (compiled fact filter: fk - lo IN dimension-key bitmap)", "bound": "hbm",
+        "roofline": {"kernel": fact_filter + " (compiled fact filter: fk - lo IN dimension-key bitmap)", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": algo,
                      "mean_launch_ms": round(kern_ms, 4), "launches": int(launches)},

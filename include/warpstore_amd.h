@@ -122,7 +122,9 @@ int ws_agg_partitioned(const ws_agg* a); /* 1 when the last run produced a parti
  * "part_auto_rows" (default 2^20), "part_auto_groups" (default 2^16), "batch_tile_rows" (256..2^20,
  * default 4096), "batch_lds_slots" (64..8192, default 1024), "batch_blocks_per_cu" (1..32,
  * default 4), "part_pack" (0 or 1, default 1: packed partial-group records for integer keys),
- * "select_chunks_per_cu" (1..64, default 8: chunked selection of long inputs).
+ * "select_chunks_per_cu" (1..64, default 8: chunked selection of long inputs),
+ * "select_mask_min_rows" (default 2^16: compiled selections of at least this many rows take the
+ * ballot pass with set bitmaps staged in LDS, then a count scan and an expansion).
  * Unknown keys and values out of range return WS_ERR_INTERNAL. */
 int ws_tune_set(const char* key, int64_t value);
 int ws_tune_get(const char* key, int64_t* value);

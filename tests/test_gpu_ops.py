@@ -106,6 +106,70 @@ def test_select_multi_chunked(gpu, mode):
     assert np.array_equal(per, eper)
 
 
+@pytest.mark.parametrize("case", ["partial", "full", "empty"])
+def test_select_staged_sets(gpu, case):
+    """Compiled selections with set bitmaps: the ballot pass stages the sets' leading words in LDS
+    and reads the rest from global memory. A 2^21-bit set only partly fits ("partial"), a 5000-bit
+    one fully ("full"), an empty set matches nothing ("empty"); 3M rows in ragged batches with
+    deletes, against the oracle and the chunked single-pass kernel."""
+    H.say(f"case select staged sets {case}")
+    n = 3_000_000
+    rng = np.random.default_rng(72)
+    big = (1 << 21) + 37
+    k = rng.integers(-50, big + 50, n).astype(np.int32)
+    a = rng.integers(-10, 6000, n).astype(np.int32)
+    deleted = rng.random(n) < 0.05
+    t = H.Table({"k": k, "a": a}, [1_048_576, 5, 0, 1_200_001, 751_418], deleted)
+    sizes = {"partial": (big, 5000), "full": (0, 5000), "empty": (0, 0)}[case]
+    bits = [rng.random((s + 31) // 32 * 32) < 0.3 for s in sizes]
+    words = [np.packbits(bb, bitorder="little").view(np.uint32).copy() if bb.size else np.zeros(1, np.uint32)
+             for bb in bits]
+    dsets = [ws.DevBuf.from_array(w) for w in words]
+    b = abi.EnvBuilder()
+    sk, sa = b.slot(abi.T_I32), b.slot(abi.T_I32)
+    s0 = b.add_set(dsets[0].p if sizes[0] else 0, sizes[0])
+    s1 = b.add_set(dsets[1].p if sizes[1] else 0, sizes[1])
+    where = b.prog([(abi.OP_COL, sk), b.ilit(3), (abi.OP_SUB,), (abi.OP_INSET, s0), (abi.OP_COL, sa), (abi.OP_INSET, s1),
+                    (abi.OP_OR,)])
+    slots = [("k", abi.T_I32, 0), ("a", abi.T_I32, 0)]
+    dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
+    nb = len(t.sizes)
+    bn = np.array(t.sizes, np.int64)
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    fp = abi.AggPlan()
+    fp.has_where, fp.where = 1, where
+    jit = C.c_void_p()
+    ws.check(gpu.ws_jit_get(C.byref(b.env), 2, C.byref(fp), abi.JF_FLAGM, b"filter", C.byref(jit)))
+    assert jit.value
+
+    def run():
+        out = C.c_void_p()
+        per = np.zeros(nb, np.int64)
+        cnt = C.c_int64()
+        ws.check(gpu.ws_select_multi(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(dev.dels, C.c_void_p),
+                                     C.cast(dev.views, C.c_void_p), 2, C.byref(out), ws.ptr(per), jit, C.byref(cnt)))
+        return ws.DevBuf(cnt.value * 4, tmp=True, adopt=out.value).to_array(np.int32, cnt.value), per
+
+    v = C.c_int64()
+    ws.check(gpu.ws_tune_get(b"select_mask_min_rows", C.byref(v)))
+    try:
+        ws.check(gpu.ws_tune_set(b"select_mask_min_rows", 1 << 40))  # the chunked single-pass kernel
+        plain, plain_per = run()
+        ws.check(gpu.ws_tune_set(b"select_mask_min_rows", 0))  # the ballot pass with staged sets
+        got, per = run()
+    finally:
+        ws.check(gpu.ws_tune_set(b"select_mask_min_rows", v.value))
+    hsets = [w.ctypes.data for w in words]
+    b.env.sets[s0], b.env.sets[s1] = (hsets[0] if sizes[0] else 0), (hsets[1] if sizes[1] else 0)
+    exp = np.zeros(n, np.int32)
+    eper = np.zeros(nb, np.int64)
+    m = H.oracle().or_select(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(host.dels, C.c_void_p),
+                             C.cast(host.views, C.c_void_p), 2, ws.ptr(exp), ws.ptr(eper))
+    assert len(got) == m and (m > 1000 or case == "empty")
+    assert np.array_equal(got, exp[:m]) and np.array_equal(per, eper)
+    assert np.array_equal(plain, exp[:m]) and np.array_equal(plain_per, eper)
+
+
 def test_select_single_env_cols(gpu):
     H.say("case select single")
     n = 100_003

@@ -38,6 +38,11 @@ constexpr size_t kStaticLds = 2 * 1024 + 512;  // staged column views + small sh
 
 // ------------------------------------------------------------------ small kernels
 
+__global__ void __launch_bounds__(1024) scan_counts_k(uint32_t* cnt, int64_t n, u64* total) { scan_counts_body(cnt, n, total); }
+__global__ void __launch_bounds__(256) select_expand_k(const ScanDesc* d, const u64* masks, const uint32_t* tile_off,
+                                                        int32_t* out_idx) {
+  select_expand_body(*d, masks, tile_off, out_idx);
+}
 __global__ void init_table_k(GTable t, int nk, int na, ws_agg_plan plan) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= (int64_t)t.mask; i += (int64_t)gridDim.x * blockDim.x) {
     t.keys[i] = kEmpty;
@@ -882,10 +887,11 @@ TuneKnob g_tune[] = {
     {"batch_blocks_per_cu", {4}, 1, 32},        // its grid, per CU
     {"part_pack", {1}, 0, 1},                   // packed partial-group records allowed (integer keys)
     {"select_chunks_per_cu", {8}, 1, 64},       // chunked selection: chunks of tiles per CU
+    {"select_mask_min_rows", {1 << 16}, 0, (int64_t)1 << 40},  // compiled selection: ballot pass from this many rows
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
 enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
-       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks };
+       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks, kTuneMaskMinRows };
 
 struct PartPlan {
   int pbits = 0;  // 0: plain table path
@@ -1543,7 +1549,26 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
   const bool chunked = ntiles > 2 * want;
   int64_t nunits = ntiles;
   size_t lds = 0;
-  if (chunked) {
+  // Compiled plans over all but short inputs take the ballot pass: one block of kStagedBlock threads
+  // per CU with the plan's set bitmaps (their leading words) staged in LDS, since a lookup in a
+  // bitmap beyond L1 is otherwise one L2 request per row; then a scan of the per-tile counts and an
+  // expansion of the ballots. No look-back: its ready frontier advances about one wave of tiles per
+  // L2 round trip, which bounds the single-pass kernels on inputs of many tiles.
+  int64_t set_words = 0, rows = 0;
+  for (int k = 0; k < WS_MAX_SETS; ++k)
+    if (env.sets[k] && env.set_size[k] > 0) set_words += ((int64_t)env.set_size[k] + 31) / 32;
+  for (int64_t n : in.n) rows += n;
+  constexpr int kStagedBlock = 1024;
+  const bool mask_pass = jit && rows >= tune(kTuneMaskMinRows) && jit_function(jit, kJitSelectStaged) != nullptr;
+  if (mask_pass) {
+    d.tile_rows = kSelMaskRows * kStagedBlock;
+    ntiles = 0;
+    for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
+    nunits = ntiles;
+    const int64_t room = ((int64_t)kLdsPerCu - jit_staged_static_lds(jit)) / 4;
+    d.set_words = (int32_t)std::max<int64_t>(0, std::min<int64_t>(set_words, room));
+    lds = 4 * (size_t)d.set_words;
+  } else if (chunked) {
     d.tile_rows = kSelChunkRows * kBlock;
     ntiles = 0;
     for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
@@ -1560,8 +1585,21 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
   u64* total = st.p + ntiles + 1;
   int* err = (int*)(st.p + ntiles + 2);
   u64* pb = st.p + ntiles + 3;
-  void* jf = jit ? jit_function(jit, chunked ? kJitSelectChunk : fn) : nullptr;
-  if (jf) {
+  void* jf = jit ? jit_function(jit, mask_pass ? kJitSelectStaged : chunked ? kJitSelectChunk : fn) : nullptr;
+  if (mask_pass) {
+    const int words = d.tile_rows / 64;  // one expansion thread per ballot word
+    DevBuf<u64> masks(ntiles * words);
+    DevBuf<uint32_t> cnt(ntiles);  // per-tile counts, then their exclusive prefix
+    fill_u32(cnt.p, ntiles, 0);
+    void* args[] = {&up.ddesc, &masks.p, &cnt.p, &pb};
+    jit_launch_raw(jf, (unsigned)std::min<int64_t>(ntiles, cu_count()), kStagedBlock, lds, args,
+                   jit_prof_name(jit, "select_mask"));
+    ProfScope ps("select_expand");
+    hipLaunchKernelGGL(scan_counts_k, dim3(1), dim3(1024), 0, stream(), cnt.p, ntiles, total);
+    hipLaunchKernelGGL(select_expand_k, dim3((unsigned)ntiles), dim3(words), 0, stream(), (const ScanDesc*)up.ddesc,
+                       (const u64*)masks.p, (const uint32_t*)cnt.p, out_idx);
+    check_launch("select_expand");
+  } else if (jf) {
     void* args[] = {&up.ddesc, &st.p, &ctr, &out_idx, &pb, &total, &err};
     jit_launch_raw(jf, (unsigned)nunits, kBlock, lds, args, jit_prof_name(jit, "select_jit"));
   } else {

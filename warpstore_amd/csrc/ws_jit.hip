@@ -32,6 +32,8 @@ struct ws_jit_program {
   hipFunction_t fn_part[2] = {nullptr, nullptr};  // grouped plans: partition count / scatter kernels
   hipFunction_t fn_batch = nullptr;               // dense plans: the batched (many queries) kernel
   hipFunction_t fn_chunk = nullptr;               // selection plans: the chunked kernel
+  hipFunction_t fn_staged = nullptr;              // selection plans: the persistent chunked kernel (sets in LDS)
+  int staged_static_lds = 0;                      // its static LDS bytes
   std::atomic<int> state{0};  // 0 compiling, 1 ready, 2 failed
   bool from_disk = false;
   int rows = 1;
@@ -52,7 +54,8 @@ thread_local std::string t_sql;
 thread_local ws_jit_query_info t_info;
 thread_local std::vector<ws_jit_program*> t_used;
 
-const char* kKernelMacro[] = {"WS_KERNEL_NOKEYS", "WS_KERNEL_GROUPED", "WS_KERNEL_DENSE", "WS_KERNEL_SELECT", "WS_KERNEL_SELECT"};
+const char* kKernelMacro[] = {"WS_KERNEL_NOKEYS", "WS_KERNEL_GROUPED", "WS_KERNEL_DENSE", "WS_KERNEL_SELECT_JIT",
+                              "WS_KERNEL_SELECT_JIT"};
 const char* kKernelName[] = {"ws_q_nokeys", "ws_q_grouped", "ws_q_dense", "ws_q_select", "ws_q_select"};
 
 // ---------------------------------------------------------------- persistent plan cache
@@ -234,8 +237,9 @@ struct Translator {
           const std::string s = std::to_string(in.arg);
           if (in.arg >= WS_MAX_SETS) throw Error("codegen: set out of range", WS_ERR_INTERNAL);
           sets.insert(in.arg);
-          if (a.is_int) st.push_back({"ws::inset_g(s" + s + ",sn" + s + "," + a.e + ")", true});
-          else st.push_back({"ws::inset_gd(s" + s + ",sn" + s + "," + a.e + ")", false});
+          const std::string args = "(sl" + s + ",sw" + s + ",s" + s + ",sn" + s + ",";
+          if (a.is_int) st.push_back({"ws::inset_s" + args + a.e + ")", true});
+          else st.push_back({"ws::inset_sd" + args + a.e + ")", false});
           break;
         }
         case WS_OP_SELECT: {
@@ -357,16 +361,25 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   for (int c : tr.encs) o << "  ws::EncView e" << c << ";\n";
   for (int k : tr.iconsts) o << "  long long ik" << k << ";\n";
   for (int k : tr.dconsts) o << "  double k" << k << ";\n";
-  for (int k : tr.sets) o << "  const unsigned WS_GLOBAL* s" << k << ";\n  long long sn" << k << ";\n";
+  for (int k : tr.sets)
+    o << "  const unsigned WS_GLOBAL* s" << k << ";\n  long long sn" << k << ";\n  const unsigned WS_LOCAL* sl" << k << ";\n  int sw" << k
+      << ";\n";
   o << "  __device__ QEval(const ws::ScanDesc& d) : env(d.env) {\n";
   // literals and set bitmaps are the same for every lane: scalar registers
   for (int k : tr.iconsts) o << "    ik" << k << " = ws::uni((long long)ws::dbits(env->consts[" << k << "]));\n";
   for (int k : tr.dconsts) o << "    k" << k << " = ws::bitsd(ws::uni((long long)ws::dbits(env->consts[" << k << "])));\n";
   // an empty set reads its own size word (always readable; every bit is masked off)
   for (int k : tr.sets)
-    o << "    sn" << k << " = ws::uni((long long)env->set_size[" << k << "]); s" << k << " = (const unsigned WS_GLOBAL*)ws::uni(sn" << k
-      << " > 0 ? env->sets[" << k << "] : (const unsigned*)&env->set_size[" << k << "]);\n";
-  o << "  }\n  __device__ void bind(const ws_colview* c) {\n";
+    o << "    sn" << k << " = ws::uni((long long)(env->set_size[" << k << "] > 0 ? env->set_size[" << k << "] : 0)); s" << k << " = (const unsigned WS_GLOBAL*)ws::uni(sn" << k
+      << " > 0 ? env->sets[" << k << "] : (const unsigned*)&env->set_size[" << k << "]); sl" << k << " = ws::lds_spare_word(); sw" << k
+      << " = 0;\n";
+  o << "  }\n";
+  // A kernel with LDS to spare copies the sets' leading words there, in order, within `words`.
+  o << "  __device__ void stage_sets(unsigned* lds, int words) {\n    int off = 0, w = 0;\n";
+  for (int k : tr.sets)
+    o << "    w = ws::stage_set(lds, off, words, s" << k << ", sn" << k << "); if (w > 0) { sl" << k
+      << " = (const unsigned WS_LOCAL*)(lds + off); sw" << k << " = w; } off += w;\n";
+  o << "    (void)off; (void)w;\n  }\n  __device__ void bind(const ws_colview* c) {\n";
   for (int c : tr.cols)
     o << "    v" << c << " = ws::uniform_view(c[" << c << "]); c" << c << " = (" << ctype_of(env.cols[c].t) << ")v" << c << ".p;\n";
   for (int c : tr.encs) o << "    e" << c << " = ws::enc_view(v" << c << ");\n";
@@ -452,8 +465,11 @@ void load_image(ws_jit_program& p, const std::string& image) {
   WS_CK(hipModuleGetFunction(&f, mod, kKernelName[p.fn_kind]));
   p.fn = f;
   if (p.fn_kind == WS_JF_DENSE) WS_CK(hipModuleGetFunction(&p.fn_batch, mod, (std::string(kKernelName[p.fn_kind]) + "_batch").c_str()));
-  if (p.fn_kind == WS_JF_FLAG || p.fn_kind == WS_JF_FLAGM)
+  if (p.fn_kind == WS_JF_FLAG || p.fn_kind == WS_JF_FLAGM) {
     WS_CK(hipModuleGetFunction(&p.fn_chunk, mod, (std::string(kKernelName[p.fn_kind]) + "_chunk").c_str()));
+    WS_CK(hipModuleGetFunction(&p.fn_staged, mod, (std::string(kKernelName[p.fn_kind]) + "_staged").c_str()));
+    WS_CK(hipFuncGetAttribute(&p.staged_static_lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, p.fn_staged));
+  }
   if (p.fn_kind == WS_JF_GROUPED) {
     const std::string base = kKernelName[p.fn_kind];
     WS_CK(hipModuleGetFunction(&p.fn_part[0], mod, (base + "_pcount").c_str()));
@@ -542,10 +558,12 @@ void* jit_function(const ws_jit_program* p, int fn) {
     return p->fn_kind == WS_JF_GROUPED ? (void*)p->fn_part[fn == kJitPartScatter ? 1 : 0] : nullptr;
   if (fn == kJitBatchDense) return p->fn_kind == WS_JF_DENSE ? (void*)p->fn_batch : nullptr;
   if (fn == kJitSelectChunk) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_chunk : nullptr;
+  if (fn == kJitSelectStaged) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_staged : nullptr;
   const bool sel = (fn == WS_JF_FLAG || fn == WS_JF_FLAGM) && (p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM);
   return (p->fn_kind == fn || sel) ? (void*)p->fn : nullptr;
 }
 int jit_rows(const ws_jit_program* p) { return p ? p->rows : 1; }
+int jit_staged_static_lds(const ws_jit_program* p) { return p ? p->staged_static_lds : 0; }
 const char* jit_prof_name(const ws_jit_program* p, const char* base) {
   if (!p || p->kind.empty()) return base;
   static std::mutex mu;

@@ -59,6 +59,7 @@ struct ScanDesc {
   int* part_err;                 // set when a run would exceed its counted length
   int64_t* key_range;            // count pass: [chunk][2 * nkeys] min then max of the integer keys, or null
   const struct PackPlan* pack;   // scatter pass: the packed record layout chosen from them, or null
+  int32_t set_words;             // persistent chunked selection: LDS words for staged set bitmaps
 };
 
 // Packed partial-group records: when every GROUP BY key is an integer (at most kPackKeys of them)
@@ -257,18 +258,38 @@ __device__ __forceinline__ i64 uni(i64 x) {
 template <class T>
 __device__ __forceinline__ T* uni(T* p) { return (T*)uni((i64)p); }
 
-// Set membership against a bitmap in global memory (compiled plans: the bitmap base is scalar).
-// Branch-free: the load is unconditional (index clamped to 0 outside the set) and its bit is
-// masked, so a row's evaluation stays one basic block and the loads of a thread's rows can issue
-// together. `bits` must be readable at word 0 even for an empty set (size <= 0 masks every bit).
-__device__ __forceinline__ i64 inset_g(const unsigned WS_GLOBAL* bits, i64 size, i64 c) {
-  const unsigned in = (unsigned)(c >= 0) & (unsigned)(c < size);
-  const unsigned w = bits[(in ? c : 0) >> 5];
-  return (i64)((w >> (c & 31)) & in);
+// Set membership (compiled plans: the bitmap bases are scalar). Set bitmaps staged in LDS. A kernel that has LDS to spare copies the leading words of each set
+// there (stage_set); a lookup reads words below `staged` from LDS and the rest from global memory.
+// Both loads always issue (branch-free): a lane served by LDS reads global word 0, which all such
+// lanes of a wave share (one cache line), and a lane served globally reads LDS word 0. Unstaged
+// sets point at lds_spare_word() with staged = 0.
+#define WS_LOCAL __attribute__((address_space(3)))
+__device__ __forceinline__ const unsigned WS_LOCAL* lds_spare_word() {
+  __shared__ unsigned spare;  // (never written: its value is always masked off)
+  return (const unsigned WS_LOCAL*)&spare;
 }
-__device__ __forceinline__ double inset_gd(const unsigned WS_GLOBAL* bits, i64 size, double x) {
+// Copies min(set words, words - off) words of a set to lds[off...]; returns the count.
+__device__ __forceinline__ int stage_set(unsigned* lds, int off, int words, const unsigned WS_GLOBAL* g, i64 size) {
+  const i64 need = size > 0 ? (size + 31) >> 5 : 0;
+  const i64 room = words - off > 0 ? words - off : 0;
+  const int w = (int)(need < room ? need : room);
+  for (int i = threadIdx.x; i < w; i += blockDim.x) lds[off + i] = g[i];
+  return w;
+}
+// size is the set's bit count (0 <= size < 2^31), so a member's index fits 32 bits.
+__device__ __forceinline__ i64 inset_s(const unsigned WS_LOCAL* l, int staged, const unsigned WS_GLOBAL* g, i64 size, i64 c) {
+  const bool in = (u64)c < (u64)size;  // also c >= 0
+  const unsigned cc = in ? (unsigned)c : 0u;
+  const unsigned w = cc >> 5;
+  const bool loc = w < (unsigned)staged;
+  const unsigned a = l[loc ? w : 0u];
+  const unsigned b = g[loc ? 0u : w];
+  return (i64)(((loc ? a : b) >> (cc & 31)) & (unsigned)in);
+}
+__device__ __forceinline__ double inset_sd(const unsigned WS_LOCAL* l, int staged, const unsigned WS_GLOBAL* g, i64 size,
+                                           double x) {
   if (x != x) return x;
-  return (double)inset_g(bits, size, (i64)x);
+  return (double)inset_s(l, staged, g, size, (i64)x);
 }
 
 __device__ __forceinline__ int batch_of_tile(const ScanDesc& d, i64 t, int hint) {
@@ -1488,17 +1509,131 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
   __shared__ u64 s_excl;
   __shared__ uint32_t s_w[16];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
-  if (threadIdx.x == 0) s_chunk = atomicAdd(chunk_ctr, 1u);
-  __syncthreads();
-  const i64 c = s_chunk;
-  const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
-  if (t0 >= d.ntiles) return;
-  const int per_tile = kSelChunkRows * nw;             // ballots per tile
-  u64* s_mask = (u64*)ws_smem;                         // [tiles][kSelRows][nw]
-  uint32_t* s_pre = (uint32_t*)(s_mask + (t1 - t0) * per_tile);  // exclusive prefix of their counts
+  const int per_tile = kSelChunkRows * nw;  // ballots per tile
   Ev ev(d);
+  {
+    if (threadIdx.x == 0) s_chunk = atomicAdd(chunk_ctr, 1u);
+    __syncthreads();
+    const i64 c = s_chunk;
+    const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
+    if (t0 >= d.ntiles) return;  // (block-uniform)
+    u64* s_mask = (u64*)ws_smem;                         // [tiles][kSelRows][nw]
+    uint32_t* s_pre = (uint32_t*)(s_mask + (t1 - t0) * per_tile);  // exclusive prefix of their counts
+    int cur = -1;
+    u64 bcount = 0;  // this wave's passing rows in batch cur, added to per_batch when the batch ends
+    for (i64 t = t0; t < t1; ++t) {
+      const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
+      if (b != cur) {
+        if (cur >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+        bcount = 0;
+        stage_batch(d, b, scols);
+        ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
+        cur = b;
+      }
+      const TileRows tr = tile_rows(d, t, b);
+      const uint32_t* del = uni(d.del[b]);
+      bool pass[kSelChunkRows];
+      if (Ev::kPureWhere || !del) {  // (block-uniform)
+        // Every row's WHERE is evaluated (rows past the tile re-read its first row), so the loads of
+        // all the thread's rows issue before any result is needed. Safe when the WHERE has no side
+        // effect, or when no row is deleted: then each evaluated row is a live row anyway.
+        bool w[kSelChunkRows];
+#pragma unroll
+        for (int j = 0; j < kSelChunkRows; ++j) {
+          const i64 i = (i64)j * blockDim.x + threadIdx.x;
+          w[j] = !ev.has_where() || ev.where(tr.first + (i < tr.count ? i : 0));
+        }
+#pragma unroll
+        for (int j = 0; j < kSelChunkRows; ++j) {
+          const i64 i = (i64)j * blockDim.x + threadIdx.x;
+          pass[j] = i < tr.count && w[j] && !is_deleted(del, tr.first + i);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < kSelChunkRows; ++j) {
+          const i64 i = (i64)j * blockDim.x + threadIdx.x;
+          const i64 r = tr.first + i;
+          pass[j] = i < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r));
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSelChunkRows; ++j) {
+        const u64 m = __ballot(pass[j]);
+        if (lane == 0) s_mask[(t - t0) * per_tile + j * nw + wave] = m;
+        bcount += (u64)__popcll(m);
+      }
+    }
+    if (lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+    __syncthreads();
+    // Exclusive prefix of the ballots' counts in (tile, step, wave) order: per thread a run of
+    // consecutive ballots, then a block scan of the run totals.
+    const int nm = (int)(t1 - t0) * per_tile;
+    const int K = (nm + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int m0 = threadIdx.x * K, m1 = m0 + K < nm ? m0 + K : nm;
+    uint32_t run = 0;
+    for (int q = m0; q < m1; ++q) run += (uint32_t)__popcll(s_mask[q]);
+    uint32_t incl = run;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t wbase = 0, count = 0;
+    for (int w = 0; w < nw; ++w) {
+      if (w < wave) wbase += s_w[w];
+      count += s_w[w];
+    }
+    uint32_t pre = wbase + incl - run;
+    for (int q = m0; q < m1; ++q) {
+      s_pre[q] = pre;
+      pre += (uint32_t)__popcll(s_mask[q]);
+    }
+    if (wave == 0) {
+      const u64 excl = lookback_exclusive(state, c, count, err);
+      if (lane == 0) {
+        s_excl = excl;
+        if (t1 == d.ntiles) *total = excl + count;
+      }
+    }
+    __syncthreads();
+    const u64 base = s_excl;
+    cur = -1;
+    for (i64 t = t0; t < t1; ++t) {
+      const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
+      cur = b;
+      const TileRows tr = tile_rows(d, t, b);
+      const i64 g0 = d.row0[b] + tr.first;
+#pragma unroll
+      for (int j = 0; j < kSelChunkRows; ++j) {
+        const int q = (int)(t - t0) * per_tile + j * nw + wave;
+        const u64 m = s_mask[q];
+        if ((m >> lane) & 1)
+          out_idx[base + s_pre[q] + (u64)__popcll(m & ((1ull << lane) - 1))] = (int32_t)(g0 + (i64)j * blockDim.x + threadIdx.x);
+      }
+    }
+  }
+}
+
+// Selection over large set bitmaps, in two kernels. select_mask_body: one block of 1024 threads per
+// CU copies the leading words of the plan's set bitmaps into LDS (d.set_words of them), then scans
+// a contiguous share of the tiles (kSelMaskRows rows per thread per tile) and stores each wave's
+// ballots to masks[tile][tile_rows / 64], in row order; no look-back or barrier in the loop, so the
+// block keeps its loads in flight, and adds each tile's count to tile_cnt[tile]. scan_counts_body
+// and select_expand_body then turn the bits into ascending indices.
+constexpr int kSelMaskRows = 8;
+template <class Ev>
+__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt, u64* per_batch) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  const i64 words = (i64)kSelMaskRows * nw;  // ballots per tile
+  Ev ev(d);
+  ev.stage_sets((unsigned*)ws_smem, d.set_words);
+  __syncthreads();
+  const i64 t0 = (i64)blockIdx.x * d.ntiles / gridDim.x, t1 = (i64)(blockIdx.x + 1) * d.ntiles / gridDim.x;
   int cur = -1;
-  u64 bcount = 0;  // this wave's passing rows in batch cur, added to per_batch when the batch ends
+  u64 bcount = 0;  // this wave's passing rows in batch cur
   for (i64 t = t0; t < t1; ++t) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
     if (b != cur) {
@@ -1510,46 +1645,50 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
     }
     const TileRows tr = tile_rows(d, t, b);
     const uint32_t* del = uni(d.del[b]);
-    bool pass[kSelChunkRows];
-    if (Ev::kPureWhere || !del) {  // (block-uniform)
-      // Every row's WHERE is evaluated (rows past the tile re-read its first row), so the loads of
-      // all the thread's rows issue before any result is needed. Safe when the WHERE has no side
-      // effect, or when no row is deleted: then each evaluated row is a live row anyway.
-      bool w[kSelChunkRows];
+    bool pass[kSelMaskRows];
+    if (Ev::kPureWhere || !del) {  // (block-uniform) every row evaluated, loads issued together
+      bool w[kSelMaskRows];
 #pragma unroll
-      for (int j = 0; j < kSelChunkRows; ++j) {
+      for (int j = 0; j < kSelMaskRows; ++j) {
         const i64 i = (i64)j * blockDim.x + threadIdx.x;
         w[j] = !ev.has_where() || ev.where(tr.first + (i < tr.count ? i : 0));
       }
 #pragma unroll
-      for (int j = 0; j < kSelChunkRows; ++j) {
+      for (int j = 0; j < kSelMaskRows; ++j) {
         const i64 i = (i64)j * blockDim.x + threadIdx.x;
         pass[j] = i < tr.count && w[j] && !is_deleted(del, tr.first + i);
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < kSelChunkRows; ++j) {
+      for (int j = 0; j < kSelMaskRows; ++j) {
         const i64 i = (i64)j * blockDim.x + threadIdx.x;
         const i64 r = tr.first + i;
         pass[j] = i < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r));
       }
     }
+    u64* tm = masks + t * words;
+    uint32_t tc = 0;
 #pragma unroll
-    for (int j = 0; j < kSelChunkRows; ++j) {
+    for (int j = 0; j < kSelMaskRows; ++j) {
       const u64 m = __ballot(pass[j]);
-      if (lane == 0) s_mask[(t - t0) * per_tile + j * nw + wave] = m;
-      bcount += (u64)__popcll(m);
+      if (lane == 0) tm[j * nw + wave] = m;
+      tc += (uint32_t)__popcll(m);
     }
+    if (lane == 0 && tc) atomicAdd(&tile_cnt[t], tc);
+    bcount += tc;
   }
-  if (lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
-  __syncthreads();
-  // Exclusive prefix of the ballots' counts in (tile, step, wave) order: per thread a run of
-  // consecutive ballots, then a block scan of the run totals.
-  const int nm = (int)(t1 - t0) * per_tile;
-  const int K = (nm + (int)blockDim.x - 1) / (int)blockDim.x;
-  const int m0 = threadIdx.x * K, m1 = m0 + K < nm ? m0 + K : nm;
+  if (cur >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+}
+
+// Exclusive prefix of the per-tile counts, in place, by one block; *total gets their sum. Each
+// thread takes a run of consecutive counts, so one block scan covers the whole array.
+__device__ void scan_counts_body(uint32_t* cnt, i64 n, u64* total) {
+  __shared__ uint32_t s_w[16];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  const i64 K = (n + blockDim.x - 1) / blockDim.x;
+  const i64 i0 = threadIdx.x * K, i1 = i0 + K < n ? i0 + K : n;
   uint32_t run = 0;
-  for (int q = m0; q < m1; ++q) run += (uint32_t)__popcll(s_mask[q]);
+  for (i64 i = i0; i < i1; ++i) run += cnt[i];
   uint32_t incl = run;
   for (int o = 1; o < kWave; o <<= 1) {
     const uint32_t y = __shfl_up(incl, o, kWave);
@@ -1557,38 +1696,47 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
   }
   if (lane == kWave - 1) s_w[wave] = incl;
   __syncthreads();
-  uint32_t wbase = 0, count = 0;
+  uint32_t pre = 0, sum = 0;
   for (int w = 0; w < nw; ++w) {
-    if (w < wave) wbase += s_w[w];
-    count += s_w[w];
+    if (w < wave) pre += s_w[w];
+    sum += s_w[w];
   }
-  uint32_t pre = wbase + incl - run;
-  for (int q = m0; q < m1; ++q) {
-    s_pre[q] = pre;
-    pre += (uint32_t)__popcll(s_mask[q]);
+  pre += incl - run;
+  for (i64 i = i0; i < i1; ++i) {
+    const uint32_t c = cnt[i];
+    cnt[i] = pre;
+    pre += c;
   }
-  if (wave == 0) {
-    const u64 excl = lookback_exclusive(state, c, count, err);
-    if (lane == 0) {
-      s_excl = excl;
-      if (t1 == d.ntiles) *total = excl + count;
-    }
+  if (threadIdx.x == 0) *total = sum;
+}
+
+// Expands select_mask_body's ballots of tile t = blockIdx.x (one thread per 64-row word,
+// blockDim.x == tile_rows / 64): the tile's output offset comes from the scanned counts, the
+// thread's from a block scan of the words' bit counts; rows are written as global indices
+// (row0[batch] + row), ascending.
+__device__ void select_expand_body(const ScanDesc& d, const u64* masks, const uint32_t* tile_off, int32_t* out_idx) {
+  __shared__ uint32_t s_w[16];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const i64 t = blockIdx.x;
+  u64 m = masks[t * (i64)blockDim.x + threadIdx.x];
+  const uint32_t c = (uint32_t)__popcll(m);
+  uint32_t incl = c;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += y;
   }
+  if (lane == kWave - 1) s_w[wave] = incl;
   __syncthreads();
-  const u64 base = s_excl;
-  cur = -1;
-  for (i64 t = t0; t < t1; ++t) {
-    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
-    cur = b;
-    const TileRows tr = tile_rows(d, t, b);
-    const i64 g0 = d.row0[b] + tr.first;
-#pragma unroll
-    for (int j = 0; j < kSelChunkRows; ++j) {
-      const int q = (int)(t - t0) * per_tile + j * nw + wave;
-      const u64 m = s_mask[q];
-      if ((m >> lane) & 1)
-        out_idx[base + s_pre[q] + (u64)__popcll(m & ((1ull << lane) - 1))] = (int32_t)(g0 + (i64)j * blockDim.x + threadIdx.x);
-    }
+  uint32_t wbase = 0;
+  for (int w = 0; w < wave; ++w) wbase += s_w[w];
+  const int b = batch_of_tile_bsearch(d, t);
+  const TileRows tr = tile_rows(d, t, b);
+  const i64 g = d.row0[b] + tr.first + (i64)threadIdx.x * 64;
+  int32_t* o = out_idx + (i64)tile_off[t] + wbase + incl - c;
+  while (m) {
+    const int bit = __ffsll((unsigned long long)m) - 1;
+    *o++ = (int32_t)(g + bit);
+    m &= m - 1;
   }
 }
 
@@ -1635,4 +1783,12 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
                                                                  unsigned long long* per_batch,             \
                                                                  unsigned long long* total, int* err) {     \
     ws::select_chunk_body<EV>(*d, state, chunk_ctr, out_idx, per_batch, total, err);                         \
+  }
+#define WS_KERNEL_SELECT_JIT(EV, NAME) WS_KERNEL_SELECT(EV, NAME) WS_KERNEL_SELECT_STAGED(EV, NAME)
+// Compiled plans only (the evaluator stages its sets): the ballot pass of the staged selection.
+#define WS_KERNEL_SELECT_STAGED(EV, NAME)                                                                   \
+  extern "C" __global__ void __launch_bounds__(1024) NAME##_staged(const ws::ScanDesc* d, unsigned long long* masks, \
+                                                                   unsigned* tile_cnt,                      \
+                                                                   unsigned long long* per_batch) {         \
+    ws::select_mask_body<EV>(*d, masks, tile_cnt, per_batch);                                                \
   }

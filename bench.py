@@ -103,9 +103,9 @@ ALGO = {
 }
 
 
-def pmc_traffic(symbol: str):
+def pmc_traffic(symbol: str, name: str = "pmc_c2_web_analytics.json"):
     """HBM bytes per launch of a kernel, from the committed rocprofv3 counter runs (or None)."""
-    path = os.path.join(ROOT, "profiles", "pmc_c2_web_analytics.json")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             d = json.load(f)
@@ -313,6 +313,12 @@ def bench_join(args) -> int:
         key, val = kv.split("=", 1)
         ws.check(L.ws_tune_set(key.encode(), int(val)))
         tuned[key] = int(val)
+    if args.calibrate:
+        src, dst = ws.DevBuf(1 << 30), ws.DevBuf(1 << 30)
+        ws.check(L.ws_copy_kernel(dst.p, src.p, 1 << 30))
+        L.ws_sync()
+        src.free()
+        dst.free()
     g = ws.datagen()
     dkey, dattr = np.empty(JOIN_DIM, np.int32), np.empty(JOIN_DIM, np.int32)
     g.wsg_c3_dim(3, JOIN_DIM, ws.ptr(dkey), ws.ptr(dattr))
@@ -371,7 +377,9 @@ def bench_join(args) -> int:
         "stage_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(warm.items(), key=lambda kv: -kv[1][1])[:8]},
         "roofline": {"kernel": fact_filter + " (compiled fact filter: fk - lo IN dimension-key bitmap)", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": algo,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": pmc_traffic("ws_q_select_staged (fact filter)", "pmc_c3_star_join.json"),
+                     "algorithmic_bytes": algo,
                      "mean_launch_ms": round(kern_ms, 4), "launches": int(launches)},
         "cpu_baseline": {"value": round(JOIN_FACT * JOIN_BYTES / cpu_s / 1e9, 4), "unit": "GB/s", "cores": 1,
                          "kind": "port", "sample": f"the whole query, single-thread CPU oracle ({cpu_s:.2f} s)"},

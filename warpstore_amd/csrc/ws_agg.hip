@@ -349,12 +349,13 @@ __device__ __forceinline__ u64 packed_keys(const PackPlan& pk, int NK, u64 w, u6
 }
 
 // Merge pass, for records of W words: block p combines partition p's records in an LDS table of
-// m.S slots (linear probing from a multiply-shift index of the hash's low word; the partition is
-// its high bits). Full records are [hash, key values, slot values]; packed ones (PK) are [packed
-// keys, slot values] and the hash is recomputed from the unpacked keys, so a group has the same
-// identity either way; the LDS table then keeps the packed word instead of the key values. Each
-// thread loads R records into registers before combining them one by one (a register queue
-// shifted per record), so a block keeps many loads in flight with one copy of the combining code.
+// m.S slots (linear probing from a multiply-shift index; the partition is the hash's high bits).
+// Full records are [hash, key values, slot values] and the table keeps the hash and the key
+// values. Packed ones (PK) are [packed keys, slot values]: the packed word identifies the group
+// exactly, so the table keeps only that word, indexes by a multiplicative hash of it, and the
+// canonical hash (the plain path's, from the unpacked keys) is computed only for a record that
+// spills to the global table. Each thread loads R records into registers before combining them
+// (the step is unrolled), so a block keeps many loads in flight.
 // Lanes of a wave holding the same group as the wave's first live lane fold into that lane first:
 // a hot group costs one LDS update per wave instead of one per lane. A record that finds no LDS
 // slot in 64 probes goes to the spill table (device-scope atomics, as the plain path). At the end
@@ -385,9 +386,9 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
     }
   }
   const u64 kinds = m.kinds;
-  u64* lk = (u64*)ws_smem;
-  u64* lkey = lk + S;                                 // [S][KW]: key bits, or the packed word
-  double* lacc = (double*)(lkey + (int64_t)S * KW);  // [S][NA]
+  u64* lk = (u64*)ws_smem;                                   // [S]: group identity
+  u64* lkey = lk + S;                                        // [S][NK]: key bits (full records)
+  double* lacc = (double*)(lkey + (PK ? 0 : (int64_t)S * KW));  // [S][NA]
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     lk[i] = kEmpty;
     for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = kind_identity(slot_kind(kinds, a));
@@ -412,19 +413,16 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
         q[r][j] = i < end ? v : kEmpty;
       }
     }
-#pragma unroll 1
+#pragma unroll
     for (int r = 0; r < R; ++r) {
       if (i0 + (int64_t)r * blockDim.x >= end) break;  // (block-uniform) the rest of the step is past the partition
       u64 rec[W];
 #pragma unroll
-      for (int j = 0; j < W; ++j) rec[j] = q[0][j];
-#pragma unroll
-      for (int r2 = 0; r2 + 1 < R; ++r2)
-#pragma unroll
-        for (int j = 0; j < W; ++j) q[r2][j] = q[r2 + 1][j];
+      for (int j = 0; j < W; ++j) rec[j] = q[r][j];
       bool live = rec[0] != kEmpty;
-      u64 key[KB] = {};  // key bits (unpacked, or copied for a spill)
-      const u64 h = !live ? kEmpty : PK ? packed_keys(pk, NK, rec[0], key) : rec[0];
+      u64 key[KB] = {};  // key bits (copied for a spill)
+      // Group identity: the packed key word (exact, below 2^63), or the hash of a full record.
+      const u64 h = rec[0];
       u64 val[VA > 0 ? VA : 1];
 #pragma unroll
       for (int a = 0; a < (VA > 0 ? VA : 1); ++a) {
@@ -453,20 +451,19 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
         }
       }
       if (!live) continue;
-      unsigned s = (unsigned)(((h & 0xffffffffull) * (u64)S) >> 32);
+      // Slot: multiply-shift of the hash's low word; a packed word is first folded and multiplied
+      // by a 32-bit odd constant (its high bits then spread the group over the table).
+      const unsigned hx = PK ? ((unsigned)h ^ (unsigned)(h >> 32)) * 0x9e3779b1u : (unsigned)h;
+      unsigned s = (unsigned)(((u64)hx * (u64)S) >> 32);
       int slot = -1;
       for (int probe = 0; probe < 64; ++probe) {
         u64 prev = kEmpty;
         const bool claimed = __hip_atomic_compare_exchange_strong(&lk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (claimed) {
-          if (PK) {
-            lkey[s] = rec[0];
-          } else {
+        if (claimed && !PK) {
 #pragma unroll
-            for (int j = 1; j < W; ++j)
-              if (j <= NK) lkey[(int64_t)s * NK + (j - 1)] = rec[j];
-          }
+          for (int j = 1; j < W; ++j)
+            if (j <= NK) lkey[(int64_t)s * NK + (j - 1)] = rec[j];
         }
         if (claimed || prev == h) {
           slot = (int)s;
@@ -478,11 +475,12 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
 #pragma unroll
         for (int a = 0; a < (VA > 0 ? VA : 1); ++a)
           if (a < NA) atomic_combine_kind<WS_LDS>(slot_kind(kinds, a), &lacc[(int64_t)slot * NA + a], bitsd((i64)val[a]));
+      } else if (PK) {
+        spill_insert(spill, kinds, NK, NA, packed_keys(pk, NK, h, key), key, val, &s_new);
       } else {
-        if (!PK)
 #pragma unroll
-          for (int k = 0; k < KB; ++k)
-            if (k < NK) key[k] = rec[1 + k < W ? 1 + k : 0];
+        for (int k = 0; k < KB; ++k)
+          if (k < NK) key[k] = rec[1 + k < W ? 1 + k : 0];
         spill_insert(spill, kinds, NK, NA, h, key, val, &s_new);
       }
     }
@@ -517,7 +515,7 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
     if (lk[qi] == kEmpty) continue;
     if (PK) {
       u64 kb[kPackKeys];
-      packed_keys(pk, NK, lkey[qi], kb);
+      packed_keys(pk, NK, lk[qi], kb);
       for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = bitsd((i64)kb[k]);
     } else {
       for (int k = 0; k < NK; ++k) m.okv[o * NK + k] = bitsd((i64)lkey[(int64_t)qi * NK + k]);

@@ -1648,10 +1648,16 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
     bool pass[kSelMaskRows];
     if (Ev::kPureWhere || !del) {  // (block-uniform) every row evaluated, loads issued together
       bool w[kSelMaskRows];
+      if (tr.count == d.tile_rows) {  // (block-uniform) full tile: row = first + j * blockDim + thread
+        const i64 r0 = tr.first + threadIdx.x;
 #pragma unroll
-      for (int j = 0; j < kSelMaskRows; ++j) {
-        const i64 i = (i64)j * blockDim.x + threadIdx.x;
-        w[j] = !ev.has_where() || ev.where(tr.first + (i < tr.count ? i : 0));
+        for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);
+      } else {
+#pragma unroll
+        for (int j = 0; j < kSelMaskRows; ++j) {
+          const i64 i = (i64)j * blockDim.x + threadIdx.x;
+          w[j] = !ev.has_where() || ev.where(tr.first + (i < tr.count ? i : 0));
+        }
       }
 #pragma unroll
       for (int j = 0; j < kSelMaskRows; ++j) {

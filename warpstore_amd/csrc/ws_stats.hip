@@ -25,27 +25,30 @@ __host__ __device__ __forceinline__ double dkey_inv(u64 k) {
   return bitsd((i64)b);
 }
 
-struct StatAcc {
-  unsigned regs[kHllRegs];
+struct StatScalars {
   unsigned long long rows, nulls;
   long long imin, imax;
   unsigned long long kmin, kmax;  // dkey of the double min / max
 };
-
-__global__ void stat_init_k(StatAcc* a) {
-  for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) a->regs[i] = 0;
-  if (threadIdx.x == 0) {
-    a->rows = a->nulls = 0;
-    a->imin = kI64Max;
-    a->imax = kI64Min;
-    a->kmin = ~0ull;
-    a->kmax = 0;
-  }
+__device__ __forceinline__ void stat_fold(StatScalars& a, const StatScalars& b) {
+  a.rows += b.rows;
+  a.nulls += b.nulls;
+  a.imin = b.imin < a.imin ? b.imin : a.imin;
+  a.imax = b.imax > a.imax ? b.imax : a.imax;
+  a.kmin = b.kmin < a.kmin ? b.kmin : a.kmin;
+  a.kmax = b.kmax > a.kmax ? b.kmax : a.kmax;
 }
+struct StatAcc {
+  unsigned regs[kHllRegs];
+  StatScalars s;
+};
 
-__global__ void __launch_bounds__(256) col_stats_k(ws_colview c, int64_t n, const uint32_t* del, StatAcc* acc,
+// One partial per block (registers as bytes, scalars as a StatScalars), all plain stores: the
+// folding kernel combines them, so no two blocks touch the same address.
+__global__ void __launch_bounds__(256) col_stats_k(ws_colview c, int64_t n, const uint32_t* del, StatScalars* part_s,
                                                    uint8_t* part_regs) {
   __shared__ unsigned regs[kHllRegs];
+  __shared__ StatScalars s_w[4];
   for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) regs[i] = 0;
   __syncthreads();
   const bool is_int = c.t != WS_T_F64;
@@ -95,32 +98,60 @@ __global__ void __launch_bounds__(256) col_stats_k(ws_colview c, int64_t n, cons
     imin = e < imin ? e : imin;
     imax = f > imax ? f : imax;
   }
-  if ((threadIdx.x & 63) == 0 && rows) {
-    atomicAdd(&acc->rows, (unsigned long long)rows);
-    if (nulls) atomicAdd(&acc->nulls, (unsigned long long)nulls);
-    if (kmin != ~0ull) {
-      atomicMin(&acc->kmin, (unsigned long long)kmin);
-      atomicMax(&acc->kmax, (unsigned long long)kmax);
-    }
-    if (is_int && imin <= imax) {
-      atomicMin(&acc->imin, (long long)imin);
-      atomicMax(&acc->imax, (long long)imax);
-    }
-  }
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = StatScalars{rows, nulls, imin, imax, kmin, kmax};
   __syncthreads();
+  if (threadIdx.x == 0) {
+    StatScalars b = s_w[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) stat_fold(b, s_w[w]);
+    part_s[blockIdx.x] = b;
+  }
   // registers: one partial per block (plain stores), folded by stat_regs_k
   uint8_t* mine = part_regs + (int64_t)blockIdx.x * kHllRegs;
   for (int i = threadIdx.x; i < kHllRegs; i += blockDim.x) mine[i] = (uint8_t)regs[i];
 }
-__global__ void stat_regs_k(const uint8_t* part_regs, int nblocks, StatAcc* acc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= kHllRegs) return;
+// Folds the block partials: block x takes registers [64 (x mod 32), +64) of the partials
+// x / 32, x / 32 + kFoldGroups, ... (its 4 waves split them, 8 independent loads in flight per
+// lane), combines the waves through LDS and max-combines into the zeroed acc->regs (kFoldGroups
+// atomics per register). Block 0 also folds the scalar partials.
+constexpr int kRegsPerFold = 64, kFoldWaves = 4, kFoldGroups = 8;
+__global__ void __launch_bounds__(256) stat_regs_k(const uint8_t* part_regs, const StatScalars* part_s, int nblocks,
+                                                   StatAcc* acc) {
+  __shared__ unsigned s_m[kFoldWaves][kRegsPerFold];
+  __shared__ StatScalars s_f[256];
+  if (blockIdx.x == 0) {  // the scalars: a strided fold per thread, then a tree in LDS
+    StatScalars f{0, 0, kI64Max, kI64Min, ~0ull, 0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) stat_fold(f, part_s[b]);
+    s_f[threadIdx.x] = f;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) stat_fold(s_f[threadIdx.x], s_f[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) acc->s = s_f[0];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = blockIdx.x % (kHllRegs / kRegsPerFold), grp = blockIdx.x / (kHllRegs / kRegsPerFold);
+  const int i = rb * kRegsPerFold + lane;
+  const int step = kFoldWaves * kFoldGroups;
   unsigned m = 0;
-  for (int b = 0; b < nblocks; ++b) {
+  int b = grp * kFoldWaves + wave;
+  for (; b + 7 * step < nblocks; b += 8 * step) {
+    unsigned v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part_regs[(int64_t)(b + u * step) * kHllRegs + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = v[u] > m ? v[u] : m;
+  }
+  for (; b < nblocks; b += step) {
     const unsigned v = part_regs[(int64_t)b * kHllRegs + i];
     m = v > m ? v : m;
   }
-  acc->regs[i] = m;
+  s_m[wave][lane] = m;
+  __syncthreads();
+  if (wave == 0) {
+    for (int w = 1; w < kFoldWaves; ++w) m = s_m[w][lane] > m ? s_m[w][lane] : m;
+    if (m) __hip_atomic_fetch_max(&acc->regs[i], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 }  // namespace ws
@@ -139,22 +170,26 @@ int ws_column_stats(const ws_colview* col, int64_t n, const uint32_t* del, ws_co
     if (n <= 0) return;
     if (col->t == WS_T_VEC) throw Error("column statistics: vector columns have none", WS_ERR_INTERNAL);
     DevBuf<StatAcc> acc(1);
-    hipLaunchKernelGGL(stat_init_k, dim3(1), dim3(256), 0, stream(), acc.p);
-    // about one block per CU; each block leaves its 2048 registers for one folding kernel
-    const unsigned grid = grid_for(n, 256, 1);
+    // about four blocks per CU; each block leaves its partial for one folding kernel
+    const unsigned grid = grid_for(n, 256, 4);
     DevBuf<uint8_t> parts((int64_t)grid * kHllRegs);
-    hipLaunchKernelGGL(col_stats_k, dim3(grid), dim3(256), 0, stream(), *col, n, del, acc.p, parts.p);
-    hipLaunchKernelGGL(stat_regs_k, dim3(kHllRegs / 256), dim3(256), 0, stream(), (const uint8_t*)parts.p, (int)grid, acc.p);
+    DevBuf<StatScalars> parts_s(grid);
+    fill_u32((uint32_t*)acc.p, kHllRegs, 0);  // the registers (the first member)
+    hipLaunchKernelGGL(col_stats_k, dim3(grid), dim3(256), 0, stream(), *col, n, del, parts_s.p, parts.p);
+    hipLaunchKernelGGL(stat_regs_k, dim3(kHllRegs / kRegsPerFold * kFoldGroups), dim3(64 * kFoldWaves), 0, stream(),
+                       (const uint8_t*)parts.p, (const StatScalars*)parts_s.p, (int)grid, acc.p);
     check_launch("column_stats");
     StatAcc h;
     d2h_sync(&h, acc.p, sizeof h);
-    out->rows = (int64_t)h.rows;
-    out->nulls = (int64_t)h.nulls;
-    out->imin = h.imin;
-    out->imax = h.imax;
-    if (h.kmin != ~0ull) {
-      out->dmin = dkey_inv(h.kmin);
-      out->dmax = dkey_inv(h.kmax);
+    out->rows = (int64_t)h.s.rows;
+    out->nulls = (int64_t)h.s.nulls;
+    if (col->t != WS_T_F64 && h.s.imin <= h.s.imax) {
+      out->imin = h.s.imin;
+      out->imax = h.s.imax;
+    }
+    if (h.s.kmin != ~0ull) {
+      out->dmin = dkey_inv(h.s.kmin);
+      out->dmax = dkey_inv(h.s.kmax);
     }
     for (int i = 0; i < kHllRegs; ++i) out->hll[i] = (uint8_t)h.regs[i];
   });

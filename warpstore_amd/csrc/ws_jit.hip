@@ -13,6 +13,7 @@
 #include <filesystem>
 #include <fstream>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -48,6 +49,35 @@ std::atomic<int> g_default_mode{WS_JIT_ON};
 thread_local int t_mode = -1;
 std::mutex g_mu;
 std::map<std::string, std::shared_ptr<ws_jit_program>> g_cache;
+// The same programs by their structural key (what the generated source depends on: kernel kind,
+// plan, bytecode, column types), so a repeated plan skips generating the source to look it up.
+std::unordered_map<std::string, std::shared_ptr<ws_jit_program>> g_by_shape;
+std::string shape_key(const ws_expr_env& env, int ncols, const ws_agg_plan& plan, int fn) {
+  std::string k;
+  k.reserve(2048);
+  auto put = [&](const void* q, size_t n) { k.append((const char*)q, n); };
+  put(&fn, sizeof fn);
+  put(&ncols, sizeof ncols);
+  put(&plan.has_where, 1);
+  put(&plan.where, sizeof plan.where);
+  put(&plan.nkeys, sizeof plan.nkeys);
+  put(plan.keys, sizeof plan.keys);
+  put(&plan.naggs, sizeof plan.naggs);
+  put(plan.aggs, sizeof plan.aggs);
+  put(plan.ops, sizeof plan.ops);
+  put(plan.agg_int, sizeof plan.agg_int);
+  put(plan.agg_bits, sizeof plan.agg_bits);
+  put(plan.agg_part, sizeof plan.agg_part);
+  put(plan.agg_null, sizeof plan.agg_null);
+  put(&plan.dense_priv, 1);
+  put(plan.key_int, sizeof plan.key_int);
+  put(env.code, sizeof env.code);
+  for (int c = 0; c < ncols && c < WS_MAX_COLS; ++c) {
+    const uint8_t t[3] = {env.cols[c].t, env.cols[c].scale, env.cols[c].enc};
+    put(t, 3);
+  }
+  return k;
+}
 std::vector<std::shared_ptr<ws_jit_program>> g_by_id;
 std::string g_disk_dir;
 thread_local std::string t_sql;
@@ -617,27 +647,37 @@ int ws_jit_get(const ws_expr_env* env, int ncols, const ws_agg_plan* plan, int f
     const int mode = mode_now();
     if (mode == WS_JIT_OFF) return;
     if (fn < 0 || fn > WS_JF_FLAGM) throw Error("jit: bad kernel kind", WS_ERR_INTERNAL);
-    int rows = 1;
-    const std::string ev = gen_evaluator(*env, ncols, *plan, &rows);
-    const std::string key = std::string(kKernelMacro[fn]) + "\n" + ev;
+    const std::string shape = shape_key(*env, ncols, *plan, fn);
     std::shared_ptr<ws_jit_program> p;
     bool fresh = false;
     {
       std::lock_guard<std::mutex> g(g_mu);
-      auto it = g_cache.find(key);
-      if (it != g_cache.end()) {
-        p = it->second;
-      } else {
-        p = std::make_shared<ws_jit_program>();
-        p->id = (int)g_by_id.size() + 1;
-        p->fn_kind = fn;
-        p->kind = kind ? kind : "";
-        p->sql = t_sql;
-        p->source = ev;
-        p->rows = rows;
-        g_by_id.push_back(p);
-        g_cache.emplace(key, p);
-        fresh = true;
+      auto it = g_by_shape.find(shape);
+      if (it != g_by_shape.end()) p = it->second;
+    }
+    if (!p) {
+      int rows = 1;
+      const std::string ev = gen_evaluator(*env, ncols, *plan, &rows);
+      const std::string key = std::string(kKernelMacro[fn]) + "\n" + ev;
+      {
+        std::lock_guard<std::mutex> g(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) {
+          p = it->second;
+          g_by_shape.emplace(shape, p);
+        } else {
+          p = std::make_shared<ws_jit_program>();
+          p->id = (int)g_by_id.size() + 1;
+          p->fn_kind = fn;
+          p->kind = kind ? kind : "";
+          p->sql = t_sql;
+          p->source = ev;
+          p->rows = rows;
+          g_by_id.push_back(p);
+          g_cache.emplace(key, p);
+          g_by_shape.emplace(shape, p);
+          fresh = true;
+        }
       }
     }
     ensure_init();
@@ -743,6 +783,7 @@ void ws_jit_clear_cache(void) {
   std::lock_guard<std::mutex> g(g_mu);
   g_by_id.clear();
   g_cache.clear();
+  g_by_shape.clear();
 }
 
 // Host-only check that a plan's generated source compiles for gfx950 (no device needed): returns

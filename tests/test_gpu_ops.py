@@ -228,14 +228,35 @@ def test_hash_join(gpu, outer):
     assert np.array_equal(gp[order], ep[order_e]) and np.array_equal(gb[order], eb[order_e])
 
 
-def test_hash_join_parts_offsets(gpu):
-    H.say("case join parts")
+@pytest.mark.parametrize("outer", [0, 1, 2])
+def test_hash_join_unique_keys(gpu, outer):
+    """Unique build keys take the single-probe form (one probe per row, no look-back)."""
+    H.say(f"case join unique outer={outer}")
+    rng = np.random.default_rng(14)
+    bk = rng.permutation(20_000)[:15_000].astype(np.int64)
+    bk[::97] = abi.NULL_I64
+    pk = rng.integers(-100, 21_000, 300_000).astype(np.int64)
+    pk[::101] = abi.NULL_I64
+    db, dp = ws.DevBuf.from_array(bk), ws.DevBuf.from_array(pk)
+    gb, gp = run_join(gpu, colview(db, abi.T_I64N), len(bk), colview(dp, abi.T_I64N), len(pk), outer)
+    eb, ep = oracle_join(bk, abi.T_I64N, pk, abi.T_I64N, outer)
+    if outer < 2:
+        assert np.all(np.diff(gp) >= 0)
+    order = np.lexsort((gb, gp))
+    order_e = np.lexsort((eb, ep))
+    assert np.array_equal(gp[order], ep[order_e]) and np.array_equal(gb[order], eb[order_e])
+
+
+@pytest.mark.parametrize("unique", [False, True])
+def test_hash_join_parts_offsets(gpu, unique):
+    H.say(f"case join parts unique={unique}")
     rng = np.random.default_rng(17)
     P = 4
     bsz = [3000, 0, 2500, 4000]
     psz = [50_000, 20_000, 0, 33_333]
-    bk = np.concatenate([rng.integers(0, 2000, s) + 10_000 * i for i, s in enumerate(bsz)]).astype(np.int32)
-    pk = np.concatenate([rng.integers(0, 2500, s) + 10_000 * i for i, s in enumerate(psz)]).astype(np.int32)
+    draw = (lambda s: rng.permutation(5000)[:s]) if unique else (lambda s: rng.integers(0, 2000, s))
+    bk = np.concatenate([draw(s) + 10_000 * i for i, s in enumerate(bsz)]).astype(np.int32)
+    pk = np.concatenate([rng.integers(0, 5500 if unique else 2500, s) + 10_000 * i for i, s in enumerate(psz)]).astype(np.int32)
     boff = np.concatenate([[0], np.cumsum(bsz)]).astype(np.int64)
     poff = np.concatenate([[0], np.cumsum(psz)]).astype(np.int64)
     db, dp = ws.DevBuf.from_array(bk), ws.DevBuf.from_array(pk)

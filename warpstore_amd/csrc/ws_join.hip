@@ -107,8 +107,9 @@ __device__ __forceinline__ int probe_row(const ProbeDesc& d, const JEntry* tab, 
 
 // Match counts of a thread's kJRows rows of a tile (rows r0 + jj * kJBlock + tid): every row's key
 // and first table slot are loaded before any chain is walked, so the rows' memory round trips
-// overlap instead of running one after another.
-__device__ __forceinline__ void probe_counts(const ProbeDesc& d, const JEntry* tab, u64 mask, i64 r0, i64 r1, int (&m)[kJRows]) {
+// overlap instead of running one after another. fr (when given) gets each row's first match.
+__device__ __forceinline__ void probe_counts(const ProbeDesc& d, const JEntry* tab, u64 mask, i64 r0, i64 r1, int (&m)[kJRows],
+                                             int32_t* fr = nullptr) {
   i64 k[kJRows];
   u64 s[kJRows];
   bool ok[kJRows];
@@ -127,12 +128,17 @@ __device__ __forceinline__ void probe_counts(const ProbeDesc& d, const JEntry* t
 #pragma unroll
   for (int jj = 0; jj < kJRows; ++jj) {
     int c = 0;
+    int32_t f = -1;
     for (u64 step = 0; step <= mask && e[jj].key != kJEmpty; ++step) {
-      c += e[jj].key == k[jj];
+      if (e[jj].key == k[jj]) {
+        if (c == 0) f = e[jj].row;
+        ++c;
+      }
       s[jj] = (s[jj] + 1) & mask;
       e[jj] = tab[s[jj]];
     }
     m[jj] = c;
+    if (fr) fr[jj] = f;
   }
 }
 
@@ -221,6 +227,88 @@ __global__ void __launch_bounds__(kJBlock) join_write_k(ProbeDesc d, const u64* 
   }
 }
 
+// Single-probe form, for joins where no probe row has a second match (unique build keys, the usual
+// case): one probe per row instead of three, and no look-back. Pass 1 (a block per tile) stores each
+// row's first match (or -1) and the tile's output count (an unmatched row yields one pair in an
+// outer join), and raises *multi when a row has a second match: the caller then runs the two-pass
+// form instead. A one-block scan turns the counts into tile offsets; pass 2 writes the pairs.
+__global__ void __launch_bounds__(kJBlock) join_probe1_k(ProbeDesc d, int32_t* first, uint32_t* tile_cnt, int* multi) {
+  __shared__ uint32_t s_w[kJBlock / 64];
+  const i64 t = blockIdx.x;
+  const int p = part_of_tile(d, t);
+  const JEntry* tab = d.tab[p];
+  const u64 mask = d.mask[p];
+  const i64 r0 = d.p_off[p] + (t - d.tile0[p]) * kJTile;
+  const i64 r1 = min((long long)(r0 + kJTile), (long long)d.p_off[p + 1]);
+  int mc[kJRows];
+  int32_t fm[kJRows];
+  probe_counts(d, tab, mask, r0, r1, mc, fm);
+  uint32_t c = 0;
+  bool many = false;
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) {
+    const i64 j = r0 + (i64)jj * kJBlock + threadIdx.x;
+    if (j < r1) {
+      first[j] = fm[jj];
+      c += d.outer && mc[jj] == 0 ? 1u : (uint32_t)mc[jj];
+      many |= mc[jj] > 1;
+    }
+  }
+  if (many) *multi = 1;
+  c = (uint32_t)wave_sum_u64(c);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_w[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tc = 0;
+    for (int w = 0; w < kJBlock / 64; ++w) tc += s_w[w];
+    tile_cnt[t] = tc;
+  }
+}
+__global__ void __launch_bounds__(1024) join_scan_k(uint32_t* cnt, int64_t n, u64* total) { scan_counts_body(cnt, n, total); }
+__global__ void __launch_bounds__(kJBlock) join_write1_k(ProbeDesc d, const int32_t* first, const uint32_t* tile_off, int32_t* ob,
+                                                         int32_t* op, uint8_t* matched) {
+  __shared__ uint32_t s_wtot[kJBlock / 64];
+  const i64 t = blockIdx.x;
+  const int p = part_of_tile(d, t);
+  const i64 r0 = d.p_off[p] + (t - d.tile0[p]) * kJTile;
+  const i64 r1 = min((long long)(r0 + kJTile), (long long)d.p_off[p + 1]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int32_t f[kJRows];
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) {
+    const i64 j = r0 + (i64)jj * kJBlock + threadIdx.x;
+    f[jj] = j < r1 ? first[j] : -1;
+  }
+  u64 carry = tile_off[t];
+#pragma unroll
+  for (int jj = 0; jj < kJRows; ++jj) {
+    const i64 j = r0 + (i64)jj * kJBlock + threadIdx.x;
+    const bool emit = j < r1 && (f[jj] >= 0 || d.outer);
+    const u64 m = __ballot(emit);
+    if (lane == 0) s_wtot[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    u64 before = 0, all = 0;
+    for (int w = 0; w < kJBlock / 64; ++w) {
+      if (w < wave) before += s_wtot[w];
+      all += s_wtot[w];
+    }
+    if (emit) {
+      const u64 o = carry + before + (u64)__popcll(m & ((1ull << lane) - 1));
+      ob[o] = f[jj];
+      op[o] = (int32_t)j;
+      if (matched && f[jj] >= 0) matched[f[jj]] = 1;
+    }
+    carry += all;
+    __syncthreads();
+  }
+}
+// Output offset of each probe partition from the scanned tile offsets (total past the last).
+__global__ void join_offsets1_k(const uint32_t* tile_off, const int64_t* tile0, int np, int64_t ntiles, const u64* total,
+                                int64_t* out) {
+  for (int p = threadIdx.x; p <= np; p += blockDim.x) out[p] = tile0[p] < ntiles ? (int64_t)tile_off[tile0[p]] : (int64_t)*total;
+}
+
 __global__ void gather_off_k(const u64* tile_off, const int64_t* tile0, int np, int64_t ntiles, u64 total, int64_t* out) {
   for (int p = threadIdx.x; p <= np; p += blockDim.x) out[p] = tile0[p] < ntiles ? (int64_t)tile_off[tile0[p]] : (int64_t)total;
 }
@@ -303,6 +391,39 @@ static JoinResult join_core(ws_colview bkey, const std::vector<int64_t>& b_off, 
   d.mask = (const u64*)(dblob + o_mask);
   d.outer = outer ? 1 : 0;
   if (ntiles == 0) return res;
+  {
+    // single-probe form: [total][multi][out_off (pp + 1)] read back in one round trip
+    DevBuf<int32_t> first(np);
+    DevBuf<uint32_t> tcnt(ntiles);
+    DevBuf<u64> s1(2 + pp + 1);
+    fill_u64(s1.p, s1.n, 0);
+    {
+      ProfScope ps("join_probe");
+      hipLaunchKernelGGL(join_probe1_k, dim3((unsigned)ntiles), dim3(kJBlock), 0, stream(), d, first.p, tcnt.p, (int*)(s1.p + 1));
+      hipLaunchKernelGGL(join_scan_k, dim3(1), dim3(1024), 0, stream(), tcnt.p, ntiles, s1.p);
+      hipLaunchKernelGGL(join_offsets1_k, dim3(1), dim3(64), 0, stream(), (const uint32_t*)tcnt.p, d.tile0, pp, ntiles,
+                         (const u64*)s1.p, (int64_t*)(s1.p + 2));
+      check_launch("join_probe");
+    }
+    std::vector<u64> h(2 + pp + 1);
+    d2h_sync(h.data(), s1.p, sizeof(u64) * h.size());
+    if (!(int)h[1]) {
+      const int64_t m = (int64_t)h[0];
+      if (m > INT32_MAX) throw Error("join result too large (>2^31 rows)", WS_ERR_INTERNAL);
+      for (int p = 0; p <= pp; ++p) res.out_off[p] = (int64_t)h[2 + p];
+      res.m = m;
+      res.b = DevBuf<int32_t>(std::max<int64_t>(m, 1));
+      res.p = DevBuf<int32_t>(std::max<int64_t>(m, 1));
+      if (m) {
+        ProfScope ps("join_write");
+        hipLaunchKernelGGL(join_write1_k, dim3((unsigned)ntiles), dim3(kJBlock), 0, stream(), d, (const int32_t*)first.p,
+                           (const uint32_t*)tcnt.p, res.b.p, res.p.p, matched);
+        check_launch("join_write");
+      }
+      return res;
+    }
+  }
+  // two-pass form (some probe row matches several build rows)
   DevBuf<u64> st(2 * ntiles + 4 + pp + 1);  // [states][tile_off][ctr][total][err][out_off]
   fill_u64(st.p, st.n, 0);
   u64* state = st.p;

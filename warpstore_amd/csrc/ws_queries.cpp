@@ -705,10 +705,15 @@ void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>& di
   std::vector<C4Plan> plans;
   plans.reserve(n);
   std::vector<size_t> dense_idx;
+  // The batched kernel keeps a query's groups in LDS up to batch_lds_slots of them; a query with
+  // more groups would fold them with device-scope atomics there, so it runs alone (its own plan
+  // picks LDS copies or private slots), as does a query that is not dense.
+  int64_t lds_slots = 1024;
+  wsd::ck(ws_tune_get("batch_lds_slots", &lds_slots));
   for (size_t i = 0; i < n; ++i) {
     plans.push_back(c4_plan(*qs[i]));
-    if (plans.back().dense) dense_idx.push_back(i);
-    else digests[i] = c4_alone(*qs[i], &groups[i]);  // not dense: plain path
+    if (plans.back().dense && plans.back().ds.G <= lds_slots) dense_idx.push_back(i);
+    else digests[i] = c4_alone(*qs[i], &groups[i]);
   }
   if (dense_idx.empty()) return;
   const C4Plan& p0 = plans[dense_idx[0]];

@@ -1623,7 +1623,7 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
 // and select_expand_body then turn the bits into ascending indices.
 constexpr int kSelMaskRows = 8;
 template <class Ev>
-__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt, u64* per_batch) {
+__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
   __shared__ ws_colview scols[WS_MAX_COLS];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
@@ -1633,12 +1633,9 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
   __syncthreads();
   const i64 t0 = (i64)blockIdx.x * d.ntiles / gridDim.x, t1 = (i64)(blockIdx.x + 1) * d.ntiles / gridDim.x;
   int cur = -1;
-  u64 bcount = 0;  // this wave's passing rows in batch cur
   for (i64 t = t0; t < t1; ++t) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
     if (b != cur) {
-      if (cur >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
-      bcount = 0;
       stage_batch(d, b, scols);
       ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
@@ -1680,10 +1677,19 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
       if (lane == 0) tm[j * nw + wave] = m;
       tc += (uint32_t)__popcll(m);
     }
-    if (lane == 0 && tc) atomicAdd(&tile_cnt[t], tc);
-    bcount += tc;
+    if (lane == 0 && tc) atomicAdd(&tile_cnt[t], tc);  // (16 per tile address: no queue builds up)
   }
-  if (cur >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+}
+
+// Per-batch counts from the scanned tile offsets (tiles never straddle batches); no atomics, which
+// from every wave at a launch's end would queue on the few per-batch addresses.
+// tile_off holds `stride` offsets per tile (the tile's own offset first).
+__device__ void batch_counts_body(const ScanDesc& d, const uint32_t* tile_off, int stride, const u64* total, u64* per_batch) {
+  for (int b = threadIdx.x; b < d.nb; b += blockDim.x) {
+    const i64 a = d.tile0[b], z = d.tile0[b + 1];
+    const u64 oa = a < d.ntiles ? tile_off[a * stride] : *total, oz = z < d.ntiles ? tile_off[z * stride] : *total;
+    per_batch[b] = oz - oa;
+  }
 }
 
 // Exclusive prefix of the per-tile counts, in place, by one block; *total gets their sum. Each
@@ -1717,10 +1723,10 @@ __device__ void scan_counts_body(uint32_t* cnt, i64 n, u64* total) {
 }
 
 // Expands select_mask_body's ballots of tile t = blockIdx.x (one thread per 64-row word,
-// blockDim.x == tile_rows / 64): the tile's output offset comes from the scanned counts, the
-// thread's from a block scan of the words' bit counts; rows are written as global indices
-// (row0[batch] + row), ascending.
-__device__ void select_expand_body(const ScanDesc& d, const u64* masks, const uint32_t* tile_off, int32_t* out_idx) {
+// blockDim.x == tile_rows / 64): the tile's output offset comes from the scanned counts (`stride`
+// per tile), the thread's from a block scan of the words' bit counts; rows are written as global
+// indices (row0[batch] + row), ascending.
+__device__ void select_expand_body(const ScanDesc& d, const u64* masks, const uint32_t* tile_off, int stride, int32_t* out_idx) {
   __shared__ uint32_t s_w[16];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const i64 t = blockIdx.x;
@@ -1738,7 +1744,7 @@ __device__ void select_expand_body(const ScanDesc& d, const u64* masks, const ui
   const int b = batch_of_tile_bsearch(d, t);
   const TileRows tr = tile_rows(d, t, b);
   const i64 g = d.row0[b] + tr.first + (i64)threadIdx.x * 64;
-  int32_t* o = out_idx + (i64)tile_off[t] + wbase + incl - c;
+  int32_t* o = out_idx + (i64)tile_off[t * stride] + wbase + incl - c;
   while (m) {
     const int bit = __ffsll((unsigned long long)m) - 1;
     *o++ = (int32_t)(g + bit);
@@ -1794,7 +1800,6 @@ __device__ void select_expand_body(const ScanDesc& d, const u64* masks, const ui
 // Compiled plans only (the evaluator stages its sets): the ballot pass of the staged selection.
 #define WS_KERNEL_SELECT_STAGED(EV, NAME)                                                                   \
   extern "C" __global__ void __launch_bounds__(1024) NAME##_staged(const ws::ScanDesc* d, unsigned long long* masks, \
-                                                                   unsigned* tile_cnt,                      \
-                                                                   unsigned long long* per_batch) {         \
-    ws::select_mask_body<EV>(*d, masks, tile_cnt, per_batch);                                                \
+                                                                   unsigned* tile_cnt) {                    \
+    ws::select_mask_body<EV>(*d, masks, tile_cnt);                                                           \
   }

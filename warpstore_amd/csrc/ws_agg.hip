@@ -312,6 +312,27 @@ __device__ __forceinline__ void atomic_combine_kind(int k, double* addr, double 
 __device__ __forceinline__ double wave_combine_kind(int k, double v) {
   return (k & 4) ? bitsd(wave_combine_i(k & 3, dbits(v))) : wave_combine_d(k & 3, v);
 }
+// Additive slots only (every op COUNT or SUM, bit 1 of each kind clear): the combine is an add of
+// int64 bits or a NULL-skipping add of doubles, so the merge's per-record code needs no dispatch
+// on the op.
+__host__ __device__ __forceinline__ bool kinds_additive(u64 kinds, int na) {
+  for (int a = 0; a < na; ++a)
+    if (slot_kind(kinds, a) & 2) return false;
+  return true;
+}
+template <bool AS>
+__device__ __forceinline__ double kind_identity_t(int k) { return AS ? 0.0 : kind_identity(k); }
+template <bool AS, int SCOPE>
+__device__ __forceinline__ void atomic_combine_kind_t(int k, double* addr, double v) {
+  if (!AS) return atomic_combine_kind<SCOPE>(k, addr, v);
+  if (k & 4) __hip_atomic_fetch_add((u64*)addr, (u64)dbits(v), __ATOMIC_RELAXED, SCOPE);
+  else if (v == v) __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, SCOPE);
+}
+template <bool AS>
+__device__ __forceinline__ double wave_combine_kind_t(int k, double v) {
+  if (!AS) return wave_combine_kind(k, v);
+  return (k & 4) ? bitsd(wave_sum_i64(dbits(v))) : wave_combine_d(WS_AGG_SUM, v);
+}
 
 // Inserts one group (hash, key bits, slot values in registers) into the spill table; false when
 // the table overflowed.
@@ -365,7 +386,7 @@ __device__ __forceinline__ u64 packed_keys(const PackPlan& pk, int NK, u64 w, u6
 // the block writes its groups inside the partition's own record range and its group count to
 // gcnt[p]; no block waits on another or on a shared counter. Launched for both layouts; the one
 // that does not match the scatter pass's choice (m.pack) returns at once.
-template <int W, bool PK>
+template <int W, bool PK, bool AS = false>
 __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   constexpr int R = W <= 3 ? 8 : W <= 6 ? 4 : W <= 10 ? 2 : 1;  // records per thread per step
   const bool packed = m.pack && m.pack->ok;
@@ -394,7 +415,7 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
   double* lacc = (double*)(lkey + (PK ? 0 : (int64_t)S * KW));  // [S][NA]
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     lk[i] = kEmpty;
-    for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = kind_identity(slot_kind(kinds, a));
+    for (int a = 0; a < NA; ++a) lacc[(int64_t)i * NA + a] = kind_identity_t<AS>(slot_kind(kinds, a));
   }
   if (threadIdx.x == 0) s_new = 0;
   __syncthreads();
@@ -447,7 +468,7 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
           for (int a = 0; a < (VA > 0 ? VA : 1); ++a) {
             if (a >= NA) break;
             const int k = slot_kind(kinds, a);
-            const double v = wave_combine_kind(k, in ? bitsd((i64)val[a]) : kind_identity(k));
+            const double v = wave_combine_kind_t<AS>(k, in ? bitsd((i64)val[a]) : kind_identity_t<AS>(k));
             if (lane == L) val[a] = (u64)dbits(v);
           }
           if (in && lane != L) live = false;
@@ -477,7 +498,7 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
       if (slot >= 0) {
 #pragma unroll
         for (int a = 0; a < (VA > 0 ? VA : 1); ++a)
-          if (a < NA) atomic_combine_kind<WS_LDS>(slot_kind(kinds, a), &lacc[(int64_t)slot * NA + a], bitsd((i64)val[a]));
+          if (a < NA) atomic_combine_kind_t<AS, WS_LDS>(slot_kind(kinds, a), &lacc[(int64_t)slot * NA + a], bitsd((i64)val[a]));
       } else if (PK) {
         spill_insert(spill, kinds, NK, NA, packed_keys(pk, NK, h, key), key, val, &s_new);
       } else {
@@ -530,15 +551,19 @@ __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
 
 // The merge kernels for records of W words: full layout (3..16 words) and packed (1..16).
 typedef void (*MergeKernel)(PartMerge, GTable);
-template <bool PK, int Base, int... Ws>
+template <bool PK, bool AS, int Base, int... Ws>
 static MergeKernel merge_pick(int W, std::integer_sequence<int, Ws...>) {
   MergeKernel k = nullptr;
-  ((k = W == Base + Ws ? part_merge_k<Base + Ws, PK> : k), ...);
+  ((k = W == Base + Ws ? part_merge_k<Base + Ws, PK, AS> : k), ...);
   return k;
 }
-static MergeKernel merge_kernel(int W, bool packed) {
-  return packed ? merge_pick<true, 1>(W, std::make_integer_sequence<int, 16>{})
-                : merge_pick<false, 3>(W, std::make_integer_sequence<int, 14>{});
+// Packed records with additive slots only (the common COUNT / SUM plans) get the variant without
+// per-record op dispatch.
+static MergeKernel merge_kernel(int W, bool packed, bool additive) {
+  if (packed)
+    return additive ? merge_pick<true, true, 1>(W, std::make_integer_sequence<int, 16>{})
+                    : merge_pick<true, false, 1>(W, std::make_integer_sequence<int, 16>{});
+  return merge_pick<false, false, 3>(W, std::make_integer_sequence<int, 14>{});
 }
 
 // Moves finished groups into a global table: rows of a partitioned result (goff set), or the
@@ -998,10 +1023,10 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     PartMerge m{recs.p, pstart, pp.W, NK, NA, pp.S, kinds, A->okv, A->oacc, gcnt.p, packable ? pack.p : nullptr};
     ProfScope ps("agg_part_merge");
     // both layouts are launched; the one the scatter pass did not choose returns at once
-    hipLaunchKernelGGL(merge_kernel(pp.W, false), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m,
+    hipLaunchKernelGGL(merge_kernel(pp.W, false, false), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m,
                        t);
     if (packable)
-      hipLaunchKernelGGL(merge_kernel(1 + NA, true), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(),
+      hipLaunchKernelGGL(merge_kernel(1 + NA, true, kinds_additive(kinds, NA)), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(),
                          m, t);
     hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)gcnt.p, A->goff, P);
     check_launch("agg_part_merge");

@@ -300,7 +300,8 @@ def test_sort_perm(gpu, m):
 
 @pytest.mark.parametrize("m,k,nk", [(476_610, 10, 1), (2_000_000, 16, 1), (100_000, 1, 3), (50_000, 256, 2),
                                     (300_001, 257, 1), (70_000, 100, 4), (70_000, 12, 5), (9_000, 1_000, 1),
-                                    (5_000, 7, 2), (3, 10, 1), (1_000_003, 64, 1)])
+                                    (5_000, 7, 2), (3, 10, 1), (1_000_003, 64, 1), (100_000, 1, 1),
+                                    (40_000, 13, 1), (10_000_000, 10, 1)])
 def test_topk_perm_is_sort_prefix(gpu, m, k, nk):
     """ORDER BY ... LIMIT k: the first k of the stable multi-key sort, ties (few distinct values),
     NULLs (NaN), -0.0 and descending keys included; selection (up to 4 keys, k <= 256) and the

@@ -341,7 +341,23 @@ static void topk_launch(TopIn in, int64_t m, int64_t kk, int32_t* out) {
   hipLaunchKernelGGL((topk_k<NK, true>), dim3((unsigned)nb), dim3(kTopThreads), 0, stream(), in, m, per, (int)kk, cok.p, ci.p);
   in.cok = cok.p;
   in.ci = ci.p;
-  hipLaunchKernelGGL((topk_k<NK, false>), dim3(1), dim3(kTopThreads), 0, stream(), in, nb * kk, nb * kk, (int)kk, fok.p, out);
+  // While more than one step's worth of candidates remains and k is small enough for a step to
+  // shrink them (k <= 64: 256 candidates -> k), blocks of one step each keep their best k; the
+  // last pass is one block (a single block over all candidates would take a step per 256).
+  int64_t n = nb * kk;
+  DevBuf<u64> cok2[2];
+  DevBuf<int32_t> ci2[2];
+  for (int cur = 0; n > kTopThreads && kk * 4 <= kTopThreads; cur ^= 1) {
+    const int64_t nb2 = (n + kTopThreads - 1) / kTopThreads;
+    cok2[cur] = DevBuf<u64>(nb2 * kk * NK);
+    ci2[cur] = DevBuf<int32_t>(nb2 * kk);
+    hipLaunchKernelGGL((topk_k<NK, false>), dim3((unsigned)nb2), dim3(kTopThreads), 0, stream(), in, n, (int64_t)kTopThreads,
+                       (int)kk, cok2[cur].p, ci2[cur].p);
+    in.cok = cok2[cur].p;
+    in.ci = ci2[cur].p;
+    n = nb2 * kk;
+  }
+  hipLaunchKernelGGL((topk_k<NK, false>), dim3(1), dim3(kTopThreads), 0, stream(), in, n, n, (int)kk, fok.p, out);
   check_launch("topk");
 }
 

@@ -108,6 +108,8 @@ struct EncView {
   const unsigned char WS_GLOBAL* data;
   i64 base, nul;
   int enc;
+  int lg;        // log2 of the code width (0, 1, 2)
+  unsigned top;  // the NULL code, also the code mask
 };
 __device__ __forceinline__ EncView enc_view(const ws_colview& v) {
   EncView e;
@@ -115,21 +117,18 @@ __device__ __forceinline__ EncView enc_view(const ws_colview& v) {
   e.nul = (v.t == WS_T_I32 || v.t == WS_T_I32N) ? (i64)WS_NULL_I32 : (i64)WS_NULL_I64;
   e.base = v.enc ? *(const i64 WS_GLOBAL*)v.p : 0;
   e.data = (const unsigned char WS_GLOBAL*)v.p + WS_ENC_HEADER + (i64)v.off * enc_width(v.enc);
+  e.lg = v.enc == 1 ? 0 : v.enc == 2 ? 1 : 2;
+  e.top = v.enc == 1 ? 0xFFu : v.enc == 2 ? 0xFFFFu : 0xFFFFFFFFu;
   return e;
 }
+// One aligned 32-bit load for every code width, then a shift and mask: no branch on the encoding,
+// so a row's evaluation stays one basic block. The word holding a code lies inside the buffer: the
+// codes start 16 bytes in, and the segment format keeps 16 bytes after the last code.
 __device__ __forceinline__ i64 enc_get(const EncView& e, i64 r) {
-  unsigned code, top;
-  if (e.enc == 1) {
-    code = e.data[r];
-    top = 0xFFu;
-  } else if (e.enc == 2) {
-    code = ((const unsigned short WS_GLOBAL*)e.data)[r];
-    top = 0xFFFFu;
-  } else {
-    code = ((const unsigned WS_GLOBAL*)e.data)[r];
-    top = 0xFFFFFFFFu;
-  }
-  return code == top ? e.nul : e.base + (i64)code;
+  const u64 a = (u64)(e.data + (r << e.lg));
+  const unsigned word = *(const unsigned WS_GLOBAL*)(a & ~(u64)3);
+  const unsigned code = (word >> ((unsigned)(a & 3) * 8)) & e.top;
+  return code == e.top ? e.nul : e.base + (i64)code;
 }
 
 __device__ __forceinline__ u64 mix64(u64 x) {  // splitmix64 finalizer
@@ -1127,7 +1126,14 @@ __device__ void part_count_body(const ScanDesc& d) {
         const i64 r = tr.first + ii;
         h[u] = kEmpty;
         double kv[Ev::kKeys];
-        if (Ev::kPureKeys) {  // key loads issued with the WHERE loads, not after them
+        if (Ev::kPureKeys && Ev::kPureWhere && !del) {
+          // (block-uniform) no side effects, no deleted rows: evaluated without a branch (rows
+          // past the tile re-read its first row), so the step's rows keep their loads together
+          const i64 rr = tr.first + (ii < tr.count ? ii : 0);
+          const u64 hk = row_keys(ev, NK, rr, kv);
+          const bool w = !ev.has_where() || ev.where(rr);
+          h[u] = ii < tr.count && w ? hk : kEmpty;
+        } else if (Ev::kPureKeys) {  // key loads issued with the WHERE loads, not after them
           if (ii < tr.count) {
             const u64 hk = row_keys(ev, NK, r, kv);
             if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
@@ -1252,20 +1258,38 @@ __device__ void part_scatter_body(const ScanDesc& d) {
       double kv[U][Ev::kKeys];
       double av[U][Ev::kAggs];
       u64 h[U];
+      if (Ev::kPureKeys && Ev::kPureVals && Ev::kPureWhere && !del) {
+        // (block-uniform) No side effects and no deleted rows: every row of the step is evaluated
+        // (rows past the tile re-read its first row) with no branch between them, so the loads
+        // of all U rows issue before any result is needed.
+        bool w[U];
+        u64 hk[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const i64 ii = i + (i64)u * blockDim.x;
-        const i64 r = tr.first + ii;
-        h[u] = kEmpty;
-        if (Ev::kPureKeys && Ev::kPureVals) {  // all loads of the row issued together
-          if (ii < tr.count) {
-            const u64 hk = row_keys(ev, NK, r, kv[u]);
-            row_values(ev, plan, NA, r, av[u]);
-            if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
-          }
-        } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
-          h[u] = row_keys(ev, NK, r, kv[u]);
+        for (int u = 0; u < U; ++u) {
+          const i64 ii = i + (i64)u * blockDim.x;
+          const i64 r = tr.first + (ii < tr.count ? ii : 0);
+          hk[u] = row_keys(ev, NK, r, kv[u]);
           row_values(ev, plan, NA, r, av[u]);
+          w[u] = !ev.has_where() || ev.where(r);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) h[u] = i + (i64)u * blockDim.x < tr.count && w[u] ? hk[u] : kEmpty;
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const i64 ii = i + (i64)u * blockDim.x;
+          const i64 r = tr.first + ii;
+          h[u] = kEmpty;
+          if (Ev::kPureKeys && Ev::kPureVals) {  // all loads of the row issued together
+            if (ii < tr.count) {
+              const u64 hk = row_keys(ev, NK, r, kv[u]);
+              row_values(ev, plan, NA, r, av[u]);
+              if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
+            }
+          } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
+            h[u] = row_keys(ev, NK, r, kv[u]);
+            row_values(ev, plan, NA, r, av[u]);
+          }
         }
       }
 #pragma unroll

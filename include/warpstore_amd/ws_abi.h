@@ -107,7 +107,9 @@ typedef struct ws_colview {
   const void* p;
   uint8_t t;     /* WS_T_* */
   uint8_t scale; /* DECIMAL scale */
-  uint8_t enc;   /* 0 raw; 1/2/3 u8/u16/u32 frame of reference; 4 f64 dictionary; 5/6 decimal doubles */
+  uint8_t enc;   /* 0 raw; 1/2/3 u8/u16/u32 frame of reference; 4 f64 dictionary; 5/6 decimal doubles.
+                    An encoded buffer is header + codes + 16 bytes (the segment format's tail, as the
+                    engine's encoders allocate it): kernels read codes as aligned 32-bit words. */
   int32_t off;   /* first row of an encoded column (range scans) */
 } ws_colview;
 

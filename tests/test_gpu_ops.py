@@ -333,6 +333,26 @@ def test_topk_perm_is_sort_prefix(gpu, m, k, nk):
     assert np.array_equal(got, exp[:kk])
 
 
+@pytest.mark.parametrize("ptype,dtype", [(abi.P_I32, np.int32), (abi.P_I64, np.int64)])
+def test_gather_multi_raw(gpu, ptype, dtype):
+    """Gather across raw batches (the LDS-staged kernel): ragged batches, sorted and unsorted
+    indexes, a count that does not fill the last step."""
+    H.say(f"case gather_multi raw {dtype.__name__}")
+    rng = np.random.default_rng(23)
+    sizes = [70_001, 3, 0, 130_000, 99_999]
+    n = sum(sizes)
+    col = rng.integers(-2 ** 30, 2 ** 30, n).astype(dtype)
+    t = H.Table({"c": col}, sizes)
+    bound = H.Bound(t, [("c", abi.T_I32 if dtype == np.int32 else abi.T_I64, 0)], True)
+    boff = np.array(t.offs[:-1], np.int64)
+    for idx in (np.sort(rng.choice(n, 123_457, replace=False)).astype(np.int32), rng.integers(0, n, 77_777).astype(np.int32)):
+        di = ws.DevBuf.from_array(idx)
+        out = ws.DevBuf(len(idx) * col.itemsize)
+        ws.check(gpu.ws_gather_multi(C.cast(bound.views, C.c_void_p), ws.ptr(boff), len(sizes), ptype, col.itemsize, di.p,
+                                     len(idx), out.p))
+        assert np.array_equal(out.to_array(dtype), col[idx])
+
+
 def test_gathers_and_codes(gpu):
     H.say("case gathers")
     rng = np.random.default_rng(19)

@@ -66,8 +66,8 @@ def test_select_multi(gpu, mode):
 
 @pytest.mark.parametrize("mode", ["interp", "jit"])
 def test_select_multi_chunked(gpu, mode):
-    """Inputs above 4 tiles per CU take the chunked selection kernel (one look-back per chunk of
-    tiles): 2.6M rows in ragged batches with deletes, against the oracle."""
+    """The chunked selection kernel (one look-back per chunk of tiles), forced by the tuning knobs:
+    2.6M rows in ragged batches with deletes, against the oracle."""
     H.say(f"case select_multi chunked {mode}")
     n = 2_600_000
     rng = np.random.default_rng(71)
@@ -94,8 +94,21 @@ def test_select_multi_chunked(gpu, mode):
     out = C.c_void_p()
     per = np.zeros(nb, np.int64)
     cnt = C.c_int64()
-    ws.check(gpu.ws_select_multi(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(dev.dels, C.c_void_p),
-                                 C.cast(dev.views, C.c_void_p), 2, C.byref(out), ws.ptr(per), jit, C.byref(cnt)))
+    # one chunk per CU: 2,540 tiles are above 2 x 256 chunks, so the chunked kernel runs (a compiled
+    # plan only when the ballot pass is held off)
+    knobs = {"select_chunks_per_cu": 1, "select_mask_min_rows": 1 << 40}
+    saved = {}
+    for key, val in knobs.items():
+        v = C.c_int64()
+        ws.check(gpu.ws_tune_get(key.encode(), C.byref(v)))
+        saved[key] = v.value
+        ws.check(gpu.ws_tune_set(key.encode(), val))
+    try:
+        ws.check(gpu.ws_select_multi(C.byref(b.env), 1, where, nb, ws.ptr(bn), C.cast(dev.dels, C.c_void_p),
+                                     C.cast(dev.views, C.c_void_p), 2, C.byref(out), ws.ptr(per), jit, C.byref(cnt)))
+    finally:
+        for key, val in saved.items():
+            ws.check(gpu.ws_tune_set(key.encode(), val))
     got = ws.DevBuf(cnt.value * 4, tmp=True, adopt=out.value).to_array(np.int32, cnt.value)
     exp = np.zeros(n, np.int32)
     eper = np.zeros(nb, np.int64)

@@ -157,11 +157,20 @@ struct Translator {
   const ws_expr_env& env;
   int ncols;
   std::set<int> cols, iconsts, dconsts, encs, sets;  // encs: encoded integer columns read; sets: INSET bitmaps
+  // Row mode: column reads come from a Row of values loaded beforehand (x.v<c>: the element, or the
+  // 32-bit word holding an encoded code), so a kernel can issue the loads of a later step early.
+  bool row_mode = false, split_ok = true;
+  std::set<int> row_cols;
 
   std::string raw_int(int c) {
     const ws_colview& v = env.cols[c];
     if (v.enc) encs.insert(c);
-    return v.enc ? "ws::enc_get(e" + std::to_string(c) + ",r)" : "((long long)c" + std::to_string(c) + "[r])";
+    const std::string n = std::to_string(c);
+    if (row_mode) {
+      row_cols.insert(c);
+      return v.enc ? "ws::enc_decode(e" + n + ",x.v" + n + ",r)" : "((long long)x.v" + n + ")";
+    }
+    return v.enc ? "ws::enc_get(e" + n + ",r)" : "((long long)c" + n + "[r])";
   }
   Node run(ws_prog p) {
     std::vector<Node> st;
@@ -180,7 +189,11 @@ struct Translator {
           cols.insert(in.arg);
           const ws_colview& v = env.cols[in.arg];
           const std::string n = std::to_string(in.arg);
-          if (v.t == WS_T_F64) st.push_back({v.enc ? "ws::col_dbl(v" + n + ",r)" : "c" + n + "[r]", false});
+          if (v.t == WS_T_F64) {
+            if (row_mode && !v.enc) row_cols.insert(in.arg);
+            if (row_mode && v.enc) split_ok = false;  // (a dictionary read: no row form)
+            st.push_back({v.enc ? "ws::col_dbl(v" + n + ",r)" : row_mode ? "x.v" + n : "c" + n + "[r]", false});
+          }
           else if (v.t == WS_T_I32N) st.push_back({"ws::null_if_i32(" + raw_int(in.arg) + ")", false});
           else if (v.t == WS_T_I64N) st.push_back({"ws::null_if_i64(" + raw_int(in.arg) + ")", false});
           else if (v.t == WS_T_DECIMAL) {
@@ -341,6 +354,9 @@ struct Translator {
   }
 };
 
+const char* elem_of(int t) {  // element type of a raw column
+  return t == WS_T_F64 ? "double" : (t == WS_T_I32 || t == WS_T_I32N) ? "int" : "long long";
+}
 const char* ctype_of(int t) {  // global-memory pointers (loads issue as global, not flat)
   return (t == WS_T_I32 || t == WS_T_I32N) ? "const int WS_GLOBAL*"
          : (t == WS_T_F64)                 ? "const double WS_GLOBAL*"
@@ -440,6 +456,28 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
   o << " } return 0; }\n";
   o << "  __device__ bool where(long long r) const { return " << (where.is_int ? where.e + " != 0" : "ws::truthy(" + where.e + ")")
     << "; }\n";
+  // The WHERE split in a load half (load_row: the row's column words) and a compute half
+  // (where_row), for kernels that load a later step's rows before computing this one.
+  {
+    Translator trr{env, ncols, {}, {}, {}};
+    trr.row_mode = true;
+    const Node w2 = plan.has_where ? trr.run(plan.where) : Node{"1", true};
+    // (its sets, constants and encoded columns are among the main translation's members)
+    const bool split = plan.has_where && trr.split_ok && !trr.row_cols.empty();
+    o << "  static constexpr bool kSplit = " << (split ? "true" : "false") << ";\n";
+    o << "  struct Row {";
+    if (split)
+      for (int c : trr.row_cols) o << " " << (env.cols[c].enc ? "unsigned" : elem_of(env.cols[c].t)) << " v" << c << ";";
+    o << " int unused_; };\n";
+    o << "  __device__ void load_row(long long r, Row& x) const {";
+    if (split)
+      for (int c : trr.row_cols)
+        o << " x.v" << c << " = " << (env.cols[c].enc ? "ws::enc_word(e" + std::to_string(c) + ",r)" : "c" + std::to_string(c) + "[r]")
+          << ";";
+    o << " }\n";
+    o << "  __device__ bool where_row(const Row& x, long long r) const { return "
+      << (split ? (w2.is_int ? w2.e + " != 0" : "ws::truthy(" + w2.e + ")") : std::string("where(r)")) << "; }\n";
+  }
   o << "  __device__ double key(int k, long long r) const { switch (k) {";
   for (size_t k = 0; k < keys.size(); ++k) o << "\n    case " << k << ": return " << as_d(keys[k]) << ";";
   o << " } return 0; }\n";

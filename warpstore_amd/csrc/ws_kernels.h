@@ -124,6 +124,14 @@ __device__ __forceinline__ EncView enc_view(const ws_colview& v) {
 // One aligned 32-bit load for every code width, then a shift and mask: no branch on the encoding,
 // so a row's evaluation stays one basic block. The word holding a code lies inside the buffer: the
 // codes start 16 bytes in, and the segment format keeps 16 bytes after the last code.
+// The 32-bit word holding row r's code, and the value decoded from it (enc_get in two halves).
+__device__ __forceinline__ unsigned enc_word(const EncView& e, i64 r) {
+  return *(const unsigned WS_GLOBAL*)((u64)(e.data + (r << e.lg)) & ~(u64)3);
+}
+__device__ __forceinline__ i64 enc_decode(const EncView& e, unsigned word, i64 r) {
+  const unsigned code = (word >> ((unsigned)((u64)(e.data + (r << e.lg)) & 3) * 8)) & e.top;
+  return code == e.top ? e.nul : e.base + (i64)code;
+}
 __device__ __forceinline__ i64 enc_get(const EncView& e, i64 r) {
   const u64 a = (u64)(e.data + (r << e.lg));
   const unsigned word = *(const unsigned WS_GLOBAL*)(a & ~(u64)3);
@@ -1657,9 +1665,14 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
   __syncthreads();
   const i64 t0 = (i64)blockIdx.x * d.ntiles / gridDim.x, t1 = (i64)(blockIdx.x + 1) * d.ntiles / gridDim.x;
   int cur = -1;
+  // Compiled WHEREs in two halves (Ev::kSplit): the next full tile's column words load while this
+  // tile is computed (xa holds this tile's rows when `have`).
+  typename Ev::Row xa[kSelMaskRows];
+  bool have = false;
   for (i64 t = t0; t < t1; ++t) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
     if (b != cur) {
+      have = false;
       stage_batch(d, b, scols);
       ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
       cur = b;
@@ -1671,9 +1684,30 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
       bool w[kSelMaskRows];
       if (tr.count == d.tile_rows) {  // (block-uniform) full tile: row = first + j * blockDim + thread
         const i64 r0 = tr.first + threadIdx.x;
+        if constexpr (Ev::kSplit) {
+          if (!have)
 #pragma unroll
-        for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);
+            for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r0 + (i64)j * blockDim.x, xa[j]);
+          // The next tile of this block and batch, when full: its loads issue before this tile's
+          // dependent lookups, so their latency overlaps this tile's work. Issued without a branch
+          // (this tile's rows again when there is no next tile), so the compiler's wait counts
+          // know exactly which loads are outstanding.
+          const bool nxt = t + 1 < t1 && tr.first + 2 * (i64)d.tile_rows <= uni((i64)d.n[b]);
+          const i64 rn = nxt ? r0 + d.tile_rows : r0;
+          typename Ev::Row xb[kSelMaskRows];
+#pragma unroll
+          for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(rn + (i64)j * blockDim.x, xb[j]);
+#pragma unroll
+          for (int j = 0; j < kSelMaskRows; ++j) w[j] = ev.where_row(xa[j], r0 + (i64)j * blockDim.x);
+#pragma unroll
+          for (int j = 0; j < kSelMaskRows; ++j) xa[j] = xb[j];
+          have = nxt;
+        } else {
+#pragma unroll
+          for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);
+        }
       } else {
+        have = false;
 #pragma unroll
         for (int j = 0; j < kSelMaskRows; ++j) {
           const i64 i = (i64)j * blockDim.x + threadIdx.x;
@@ -1686,6 +1720,7 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
         pass[j] = i < tr.count && w[j] && !is_deleted(del, tr.first + i);
       }
     } else {
+      have = false;
 #pragma unroll
       for (int j = 0; j < kSelMaskRows; ++j) {
         const i64 i = (i64)j * blockDim.x + threadIdx.x;

@@ -1665,9 +1665,9 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
   __syncthreads();
   const i64 t0 = (i64)blockIdx.x * d.ntiles / gridDim.x, t1 = (i64)(blockIdx.x + 1) * d.ntiles / gridDim.x;
   int cur = -1;
-  // Compiled WHEREs in two halves (Ev::kSplit): the next full tile's column words load while this
-  // tile is computed (xa holds this tile's rows when `have`).
-  typename Ev::Row xa[kSelMaskRows];
+  // Compiled WHEREs in two halves (Ev::kSplit): later full tiles' column words load while this tile
+  // is computed (xa, xb hold this tile's and the next one's rows when `have`).
+  typename Ev::Row xa[kSelMaskRows], xb[kSelMaskRows];
   bool have = false;
   for (i64 t = t0; t < t1; ++t) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
@@ -1685,23 +1685,33 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
       if (tr.count == d.tile_rows) {  // (block-uniform) full tile: row = first + j * blockDim + thread
         const i64 r0 = tr.first + threadIdx.x;
         if constexpr (Ev::kSplit) {
-          if (!have)
+          // Two tiles ahead: xa holds this tile's rows, xb the next one's; the rows of the tile
+          // after that load before this tile's dependent lookups, so two tiles' loads are in
+          // flight while one is computed. Tiles past the block's range or the batch, or not full,
+          // are replaced by this tile's rows (loaded without a branch, so the compiler's wait
+          // counts know exactly which loads are outstanding) and end the pipeline.
+          const i64 n_b = uni((i64)d.n[b]), tr_ = d.tile_rows;
+          const bool nxt1 = t + 1 < t1 && tr.first + 2 * tr_ <= n_b;
+          const bool nxt2 = t + 2 < t1 && tr.first + 3 * tr_ <= n_b;
+          if (!have) {
+            const i64 r1 = nxt1 ? r0 + tr_ : r0;
 #pragma unroll
             for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r0 + (i64)j * blockDim.x, xa[j]);
-          // The next tile of this block and batch, when full: its loads issue before this tile's
-          // dependent lookups, so their latency overlaps this tile's work. Issued without a branch
-          // (this tile's rows again when there is no next tile), so the compiler's wait counts
-          // know exactly which loads are outstanding.
-          const bool nxt = t + 1 < t1 && tr.first + 2 * (i64)d.tile_rows <= uni((i64)d.n[b]);
-          const i64 rn = nxt ? r0 + d.tile_rows : r0;
-          typename Ev::Row xb[kSelMaskRows];
 #pragma unroll
-          for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(rn + (i64)j * blockDim.x, xb[j]);
+            for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r1 + (i64)j * blockDim.x, xb[j]);
+          }
+          const i64 r2 = nxt2 ? r0 + 2 * tr_ : r0;
+          typename Ev::Row xc[kSelMaskRows];
+#pragma unroll
+          for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r2 + (i64)j * blockDim.x, xc[j]);
 #pragma unroll
           for (int j = 0; j < kSelMaskRows; ++j) w[j] = ev.where_row(xa[j], r0 + (i64)j * blockDim.x);
 #pragma unroll
-          for (int j = 0; j < kSelMaskRows; ++j) xa[j] = xb[j];
-          have = nxt;
+          for (int j = 0; j < kSelMaskRows; ++j) {
+            xa[j] = xb[j];
+            xb[j] = xc[j];
+          }
+          have = nxt1;
         } else {
 #pragma unroll
           for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);

@@ -119,21 +119,22 @@ def test_select_multi_chunked(gpu, mode):
     assert np.array_equal(per, eper)
 
 
-@pytest.mark.parametrize("case", ["partial", "full", "empty"])
+@pytest.mark.parametrize("case", ["partial", "full", "empty", "partial_nodel"])
 def test_select_staged_sets(gpu, case):
     """Compiled selections with set bitmaps: the ballot pass stages the sets' leading words in LDS
     and reads the rest from global memory. A 2^21-bit set only partly fits ("partial"), a 5000-bit
     one fully ("full"), an empty set matches nothing ("empty"); 3M rows in ragged batches with
-    deletes, against the oracle and the chunked single-pass kernel."""
+    deletes (without: full tiles take the pipelined path), against the oracle and the single-pass
+    kernel."""
     H.say(f"case select staged sets {case}")
     n = 3_000_000
     rng = np.random.default_rng(72)
     big = (1 << 21) + 37
     k = rng.integers(-50, big + 50, n).astype(np.int32)
     a = rng.integers(-10, 6000, n).astype(np.int32)
-    deleted = rng.random(n) < 0.05
+    deleted = None if case == "partial_nodel" else rng.random(n) < 0.05  # no deletes: the pipelined path
     t = H.Table({"k": k, "a": a}, [1_048_576, 5, 0, 1_200_001, 751_418], deleted)
-    sizes = {"partial": (big, 5000), "full": (0, 5000), "empty": (0, 0)}[case]
+    sizes = {"partial": (big, 5000), "full": (0, 5000), "empty": (0, 0), "partial_nodel": (big, 5000)}[case]
     bits = [rng.random((s + 31) // 32 * 32) < 0.3 for s in sizes]
     words = [np.packbits(bb, bitorder="little").view(np.uint32).copy() if bb.size else np.zeros(1, np.uint32)
              for bb in bits]

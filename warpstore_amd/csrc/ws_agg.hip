@@ -42,9 +42,9 @@ __global__ void __launch_bounds__(1024) scan_counts_k(uint32_t* cnt, int64_t n, 
 __global__ void batch_counts_k(const ScanDesc* d, const uint32_t* tile_off, int stride, const u64* total, u64* per_batch) {
   batch_counts_body(*d, tile_off, stride, total, per_batch);
 }
-__global__ void __launch_bounds__(256) select_expand_k(const ScanDesc* d, const u64* masks, const uint32_t* tile_off, int stride,
-                                                        int32_t* out_idx) {
-  select_expand_body(*d, masks, tile_off, stride, out_idx);
+__global__ void __launch_bounds__(256) select_expand_k(const ScanDesc* d, const u64* masks, const uint8_t* tile_fmt,
+                                                        const uint32_t* tile_off, int stride, int32_t* out_idx) {
+  select_expand_body(*d, masks, tile_fmt, tile_off, stride, out_idx);
 }
 __global__ void init_table_k(GTable t, int nk, int na, ws_agg_plan plan) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= (int64_t)t.mask; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1617,8 +1617,10 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
     constexpr int nw = 1;  // counts per tile
     DevBuf<u64> masks(ntiles * words);
     DevBuf<uint32_t> cnt(ntiles);  // per-tile counts, then their exclusive prefix
+    DevBuf<uint8_t> fmt(ntiles);   // per-tile word layout (1: raw four-row ballots)
     fill_u32(cnt.p, ntiles, 0);
-    void* args[] = {&up.ddesc, &masks.p, &cnt.p};
+    fill_u8(fmt.p, ntiles, 0);
+    void* args[] = {&up.ddesc, &masks.p, &cnt.p, &fmt.p};
     jit_launch_raw(jf, (unsigned)std::min<int64_t>(ntiles, cu_count()), kStagedBlock, lds, args,
                    jit_prof_name(jit, "select_mask"));
     ProfScope ps("select_expand");
@@ -1626,7 +1628,7 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
     hipLaunchKernelGGL(batch_counts_k, dim3(1), dim3(256), 0, stream(), (const ScanDesc*)up.ddesc, (const uint32_t*)cnt.p, nw,
                        (const u64*)total, pb);
     hipLaunchKernelGGL(select_expand_k, dim3((unsigned)ntiles), dim3(words), 0, stream(), (const ScanDesc*)up.ddesc,
-                       (const u64*)masks.p, (const uint32_t*)cnt.p, nw, out_idx);
+                       (const u64*)masks.p, (const uint8_t*)fmt.p, (const uint32_t*)cnt.p, nw, out_idx);
     check_launch("select_expand");
   } else if (jf) {
     void* args[] = {&up.ddesc, &st.p, &ctr, &out_idx, &pb, &total, &err};

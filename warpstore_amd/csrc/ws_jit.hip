@@ -475,6 +475,32 @@ std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& 
         o << " x.v" << c << " = " << (env.cols[c].enc ? "ws::enc_word(e" + std::to_string(c) + ",r)" : "c" + std::to_string(c) + "[r]")
           << ";";
     o << " }\n";
+    // Four consecutive rows (r a multiple of 4) with 16-byte loads of raw columns: a wave's load
+    // instruction then carries 1 KiB; rows4_ok tells whether every raw column base is 16-byte
+    // aligned (encoded columns read their code words one by one).
+    o << "  __device__ bool rows4_ok() const { return true";
+    if (split)
+      for (int c : trr.row_cols)
+        if (!env.cols[c].enc) o << " && (((unsigned long long)c" << c << " & 15) == 0)";
+    o << "; }\n";
+    o << "  __device__ void load_rows4(long long r, Row* x) const {";
+    if (split)
+      for (int c : trr.row_cols) {
+        const ws_colview& v = env.cols[c];
+        const std::string n = std::to_string(c);
+        if (v.enc) {
+          for (int q = 0; q < 4; ++q) o << " x[" << q << "].v" << n << " = ws::enc_word(e" << n << ",r+" << q << ");";
+        } else if (v.t == WS_T_I32 || v.t == WS_T_I32N) {
+          o << " { const ws::ws_i32x4 q = *(const ws::ws_i32x4 WS_GLOBAL*)(c" << n << " + r); x[0].v" << n << " = q.x; x[1].v" << n
+            << " = q.y; x[2].v" << n << " = q.z; x[3].v" << n << " = q.w; }";
+        } else {
+          const char* vt = v.t == WS_T_F64 ? "ws::ws_f64x2" : "ws::ws_i64x2";
+          o << " { const " << vt << " q0 = *(const " << vt << " WS_GLOBAL*)(c" << n << " + r), q1 = *(const " << vt
+            << " WS_GLOBAL*)(c" << n << " + r + 2); x[0].v" << n << " = q0.x; x[1].v" << n << " = q0.y; x[2].v" << n
+            << " = q1.x; x[3].v" << n << " = q1.y; }";
+        }
+      }
+    o << " }\n";
     o << "  __device__ bool where_row(const Row& x, long long r) const { return "
       << (split ? (w2.is_int ? w2.e + " != 0" : "ws::truthy(" + w2.e + ")") : std::string("where(r)")) << "; }\n";
   }

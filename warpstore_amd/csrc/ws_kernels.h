@@ -104,6 +104,11 @@ __device__ __forceinline__ ws_colview uniform_view(const ws_colview& v) {
 }
 // Frame-of-reference integer column (encodings 1-3): codes from `data` (offset applied), value =
 // base + code, the top code standing for the type's NULL.
+// Builtin vector types for 16-byte loads through address-space-qualified pointers.
+typedef int ws_i32x4 __attribute__((ext_vector_type(4)));
+typedef long long ws_i64x2 __attribute__((ext_vector_type(2)));
+typedef double ws_f64x2 __attribute__((ext_vector_type(2)));
+
 struct EncView {
   const unsigned char WS_GLOBAL* data;
   i64 base, nul;
@@ -1655,7 +1660,7 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
 // and select_expand_body then turn the bits into ascending indices.
 constexpr int kSelMaskRows = 8;
 template <class Ev>
-__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt) {
+__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt, uint8_t* tile_fmt) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
   __shared__ ws_colview scols[WS_MAX_COLS];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
@@ -1667,7 +1672,7 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
   int cur = -1;
   // Compiled WHEREs in two halves (Ev::kSplit): later full tiles' column words load while this tile
   // is computed (xa, xb hold this tile's and the next one's rows when `have`).
-  typename Ev::Row xa[kSelMaskRows], xb[kSelMaskRows];
+  typename Ev::Row xa[kSelMaskRows];
   bool have = false;
   for (i64 t = t0; t < t1; ++t) {
     const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
@@ -1679,43 +1684,54 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
     }
     const TileRows tr = tile_rows(d, t, b);
     const uint32_t* del = uni(d.del[b]);
+    u64* tm = masks + t * words;
+    if constexpr (Ev::kSplit) {
+      if (!del && tr.count == d.tile_rows && ev.rows4_ok()) {  // (block-uniform)
+        // Four consecutive rows per lane and load: lane L of wave w takes rows w * 512 + g * 256 +
+        // 4 L + s of the tile (g = 0, 1; s = 0..3), so a wave's load instruction carries 1 KiB.
+        // Its ballots are stored as they are (bit L of word (w, g, s) is row w * 512 + g * 256 +
+        // 4 L + s) and tile_fmt[t] = 1 tells the expansion to transpose them. The next full tile
+        // of this block and batch loads before this tile's dependent lookups (without a branch:
+        // this tile's rows again when there is none).
+        const i64 tr_ = d.tile_rows;
+        const i64 rb = tr.first + (i64)wave * (64 * kSelMaskRows) + 4 * lane;
+        if (!have) {
+          ev.load_rows4(rb, xa);
+          ev.load_rows4(rb + 256, xa + 4);
+        }
+        const bool nxt = t + 1 < t1 && tr.first + 2 * tr_ <= uni((i64)d.n[b]);
+        const i64 rn = nxt ? rb + tr_ : rb;
+        typename Ev::Row xb[kSelMaskRows];
+        ev.load_rows4(rn, xb);
+        ev.load_rows4(rn + 256, xb + 4);
+        bool p[kSelMaskRows];
+#pragma unroll
+        for (int j = 0; j < kSelMaskRows; ++j) p[j] = ev.where_row(xa[j], rb + (j >> 2) * 256 + (j & 3));
+#pragma unroll
+        for (int j = 0; j < kSelMaskRows; ++j) xa[j] = xb[j];
+        have = nxt;
+        uint32_t tc = 0;
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const u64 B = __ballot(p[g * 4 + q]);
+            tc += (uint32_t)__popcll(B);
+            if (lane == g * 4 + q) tm[wave * 8 + g * 4 + q] = B;
+          }
+        if (lane == 0 && tc) atomicAdd(&tile_cnt[t], tc);
+        if (threadIdx.x == 0) tile_fmt[t] = 1;
+        continue;
+      }
+      have = false;
+    }
     bool pass[kSelMaskRows];
     if (Ev::kPureWhere || !del) {  // (block-uniform) every row evaluated, loads issued together
       bool w[kSelMaskRows];
       if (tr.count == d.tile_rows) {  // (block-uniform) full tile: row = first + j * blockDim + thread
         const i64 r0 = tr.first + threadIdx.x;
-        if constexpr (Ev::kSplit) {
-          // Two tiles ahead: xa holds this tile's rows, xb the next one's; the rows of the tile
-          // after that load before this tile's dependent lookups, so two tiles' loads are in
-          // flight while one is computed. Tiles past the block's range or the batch, or not full,
-          // are replaced by this tile's rows (loaded without a branch, so the compiler's wait
-          // counts know exactly which loads are outstanding) and end the pipeline.
-          const i64 n_b = uni((i64)d.n[b]), tr_ = d.tile_rows;
-          const bool nxt1 = t + 1 < t1 && tr.first + 2 * tr_ <= n_b;
-          const bool nxt2 = t + 2 < t1 && tr.first + 3 * tr_ <= n_b;
-          if (!have) {
-            const i64 r1 = nxt1 ? r0 + tr_ : r0;
 #pragma unroll
-            for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r0 + (i64)j * blockDim.x, xa[j]);
-#pragma unroll
-            for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r1 + (i64)j * blockDim.x, xb[j]);
-          }
-          const i64 r2 = nxt2 ? r0 + 2 * tr_ : r0;
-          typename Ev::Row xc[kSelMaskRows];
-#pragma unroll
-          for (int j = 0; j < kSelMaskRows; ++j) ev.load_row(r2 + (i64)j * blockDim.x, xc[j]);
-#pragma unroll
-          for (int j = 0; j < kSelMaskRows; ++j) w[j] = ev.where_row(xa[j], r0 + (i64)j * blockDim.x);
-#pragma unroll
-          for (int j = 0; j < kSelMaskRows; ++j) {
-            xa[j] = xb[j];
-            xb[j] = xc[j];
-          }
-          have = nxt1;
-        } else {
-#pragma unroll
-          for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);
-        }
+        for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);
       } else {
         have = false;
 #pragma unroll
@@ -1738,7 +1754,6 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
         pass[j] = i < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r));
       }
     }
-    u64* tm = masks + t * words;
     uint32_t tc = 0;
 #pragma unroll
     for (int j = 0; j < kSelMaskRows; ++j) {
@@ -1795,11 +1810,31 @@ __device__ void scan_counts_body(uint32_t* cnt, i64 n, u64* total) {
 // blockDim.x == tile_rows / 64): the tile's output offset comes from the scanned counts (`stride`
 // per tile), the thread's from a block scan of the words' bit counts; rows are written as global
 // indices (row0[batch] + row), ascending.
-__device__ void select_expand_body(const ScanDesc& d, const u64* masks, const uint32_t* tile_off, int stride, int32_t* out_idx) {
+// Bit i of x (x < 2^16) to bit 4 i.
+__device__ __forceinline__ u64 spread4(u64 x) {
+  x = (x | (x << 24)) & 0x000000FF000000FFull;
+  x = (x | (x << 12)) & 0x000F000F000F000Full;
+  x = (x | (x << 6)) & 0x0303030303030303ull;
+  x = (x | (x << 3)) & 0x1111111111111111ull;
+  return x;
+}
+// tile_fmt[t] = 1: the tile's words are raw ballots of four rows per lane (word 4 k + s, bit L =
+// row 256 k + 4 L + s); row-ordered word 4 k + m is then bit 4 i + s = bit 16 m + i of word 4 k + s.
+__device__ void select_expand_body(const ScanDesc& d, const u64* masks, const uint8_t* tile_fmt, const uint32_t* tile_off,
+                                   int stride, int32_t* out_idx) {
   __shared__ uint32_t s_w[16];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const i64 t = blockIdx.x;
-  u64 m = masks[t * (i64)blockDim.x + threadIdx.x];
+  u64 m;
+  if (tile_fmt[t]) {  // (block-uniform)
+    const u64* b4 = masks + t * (i64)blockDim.x + (threadIdx.x & ~3);
+    const int sh = 16 * (threadIdx.x & 3);
+    m = 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) m |= spread4((b4[s] >> sh) & 0xFFFFull) << s;
+  } else {
+    m = masks[t * (i64)blockDim.x + threadIdx.x];
+  }
   const uint32_t c = (uint32_t)__popcll(m);
   uint32_t incl = c;
   for (int o = 1; o < kWave; o <<= 1) {
@@ -1869,6 +1904,6 @@ __device__ void select_expand_body(const ScanDesc& d, const u64* masks, const ui
 // Compiled plans only (the evaluator stages its sets): the ballot pass of the staged selection.
 #define WS_KERNEL_SELECT_STAGED(EV, NAME)                                                                   \
   extern "C" __global__ void __launch_bounds__(1024) NAME##_staged(const ws::ScanDesc* d, unsigned long long* masks, \
-                                                                   unsigned* tile_cnt) {                    \
-    ws::select_mask_body<EV>(*d, masks, tile_cnt);                                                           \
+                                                                   unsigned* tile_cnt, unsigned char* tile_fmt) { \
+    ws::select_mask_body<EV>(*d, masks, tile_cnt, tile_fmt);                                                 \
   }

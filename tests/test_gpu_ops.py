@@ -261,6 +261,28 @@ def test_hash_join_unique_keys(gpu, outer):
     assert np.array_equal(gp[order], ep[order_e]) and np.array_equal(gb[order], eb[order_e])
 
 
+@pytest.mark.parametrize("outer", [0, 1, 2])
+@pytest.mark.parametrize("sizes", [(0, 5000), (5000, 0), (0, 0)])
+def test_hash_join_empty_side(gpu, sizes, outer):
+    """An empty build or probe side: an inner join matches nothing, a left join keeps every probe
+    row (build index -1), a full join also every build row (probe index -1)."""
+    nb, np_ = sizes
+    H.say(f"case join empty side nb={nb} np={np_} outer={outer}")
+    rng = np.random.default_rng(19)
+    bk = rng.integers(0, 100, nb).astype(np.int64)
+    pk = rng.integers(0, 100, np_).astype(np.int64)
+    # a one-element device buffer stands behind an empty side (its pointer is valid, never read)
+    db = ws.DevBuf.from_array(bk if nb else np.zeros(1, np.int64))
+    dp = ws.DevBuf.from_array(pk if np_ else np.zeros(1, np.int64))
+    gb, gp = run_join(gpu, colview(db, abi.T_I64), nb, colview(dp, abi.T_I64), np_, outer)
+    eb, ep = oracle_join(bk, abi.T_I64, pk, abi.T_I64, outer)
+    expect = 0 if outer == 0 else np_ if nb == 0 else nb if outer == 2 else 0
+    assert len(gb) == len(eb) == expect
+    order = np.lexsort((gb, gp))
+    order_e = np.lexsort((eb, ep))
+    assert np.array_equal(gp[order], ep[order_e]) and np.array_equal(gb[order], eb[order_e])
+
+
 @pytest.mark.parametrize("unique", [False, True])
 def test_hash_join_parts_offsets(gpu, unique):
     H.say(f"case join parts unique={unique}")

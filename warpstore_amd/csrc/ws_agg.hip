@@ -388,7 +388,7 @@ __device__ __forceinline__ u64 packed_keys(const PackPlan& pk, int NK, u64 w, u6
 // that does not match the scatter pass's choice (m.pack) returns at once.
 template <int W, bool PK, bool AS = false>
 __global__ void __launch_bounds__(512) part_merge_k(PartMerge m, GTable spill) {
-  constexpr int R = W <= 3 ? 8 : W <= 6 ? 4 : W <= 10 ? 2 : 1;  // records per thread per step
+  constexpr int R = W <= 6 ? 4 : W <= 10 ? 2 : 1;  // records per thread per step (VGPRs: 3 blocks per CU)
   const bool packed = m.pack && m.pack->ok;
   if (packed != PK) return;
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
@@ -1025,8 +1025,8 @@ static void agg_run_part(ws_agg* A, const ws_expr_env& env, const ws_jit_program
     // both layouts are launched; the one the scatter pass did not choose returns at once
     hipLaunchKernelGGL(merge_kernel(pp.W, false, false), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(), m,
                        t);
-    if (packable)
-      hipLaunchKernelGGL(merge_kernel(1 + NA, true, kinds_additive(kinds, NA)), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * pp.W, stream(),
+    if (packable)  // a packed slot is 1 + NA words: less LDS per block, more blocks per CU
+      hipLaunchKernelGGL(merge_kernel(1 + NA, true, kinds_additive(kinds, NA)), dim3((unsigned)P), dim3(kPartMergeBlock), (size_t)pp.S * 8 * (1 + NA), stream(),
                          m, t);
     hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)gcnt.p, A->goff, P);
     check_launch("agg_part_merge");

@@ -903,7 +903,7 @@ struct TuneKnob {
   int64_t lo, hi;
 };
 TuneKnob g_tune[] = {
-    {"part_merge_lds_kib", {60}, 8, 152},     // merge table per block (KiB): 2 blocks per CU at 60
+    {"part_merge_lds_kib", {60}, 8, 152},     // merge table per block (KiB) for full records; sets the slot count, packed tables take less
     {"part_scatter_lds_kib", {120}, 16, 152}, // cursors + pre-aggregation table per scatter block (KiB)
     {"part_chunks_per_cu", {1}, 1, 64},       // count / scatter blocks (chunks) per CU
     {"part_auto_rows", {1 << 20}, 0, (int64_t)1 << 40},   // automatic choice: at least this many rows

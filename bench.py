@@ -1,26 +1,32 @@
 """Benchmark of the analytic hot path on one MI355X.
 
-Workload (one step = one query over GPU-resident segments, 2^20 rows each):
-  web-analytics table, 10M rows, seed 2: user_id int64 Zipf(1.1) over 1e6 ids, region int16 in
-  [0,256) stored frame-of-reference encoded (uint16 codes), event_date int32 over 365 days, url
-  int32 dictionary code Zipf(1.1) over 1e5, dur int32 lognormal(ln 30000, 1);
-  SELECT user_id, COUNT(*) c, SUM(dur), COUNT(DISTINCT url) FROM h
-  WHERE event_date < 19723+300 AND region < 224 GROUP BY user_id ORDER BY c DESC LIMIT 10
-run through the engine-side driver exactly as the planner issues it (compiled phase-1 plan,
-two-phase COUNT DISTINCT, GPU top-N), with results checked against the CPU oracle.
+Default workload (one step = one query over GPU-resident segments of 2^20 rows): the star join,
+fact 60M rows (fk int32 into the dimension keys, qty int64 [1,50], price DECIMAL(15,2) as scaled
+int64, flag [0,25)), dimension 1.5M rows (shuffled unique int32 key, attr int32 [0,1000)), seed 3:
+  SELECT f.flag, COUNT(*), SUM(f.price), SUM(f.qty) FROM fact f JOIN dim d ON f.fk = d.key
+  WHERE d.attr < 50 GROUP BY f.flag
+run through the engine-side driver as the planner issues it, with results checked against the CPU
+oracle. It is the largest single-GPU configuration of the metric (1.27 GB of referenced columns).
 
-value = GB/s scanned: 22 bytes of referenced columns per row (8 + 2 + 4 + 4 + 4) x 10M rows per
-query, over the wall time of the timed steps. Inputs are resident in HBM before timing starts;
-steps rotate over copies of the table that together exceed the 256 MB Infinity Cache.
-roofline: the dominant kernel (most GPU time over the warm-up queries, among the scan kernels of
-the compiled phase-1 aggregation) timed live with HIP events on its launch stream during the timed
-steps; achieved = its algorithmic bytes per launch (the referenced columns of the 10M rows it scans:
-22 B/row, or 18 B/row for the partition count pass, which does not read `dur`) / mean duration.
-cpu_baseline: the CPU oracle's restatement of the same query on this host, one thread.
+value = GB/s read: the bytes the query has to read per step (fk of every fact row, key and attr of
+every dimension row, and qty, price and flag of the fact rows that survive the join filter) over
+the wall time of the timed steps; inputs are resident in HBM before timing starts.
+effective_gbs = 24 B of referenced fact columns x 60M rows over the same time (the figure a
+column engine would quote for a full scan; most of qty / price / flag is never read).
+roofline: the dominant kernel (most GPU time in a profiled untimed query), timed live with HIP
+events on its launch stream during the timed steps; achieved = its algorithmic bytes per launch /
+mean launch duration.
+cpu_baseline: the multithreaded CPU restatement of the same query (oracle/cpu_baseline.cpp) on
+this host's cores, plus its single-thread figure.
+
+Other workloads (--workload): web (10M-row web-analytics GROUP BY user_id + COUNT DISTINCT + top
+10), filter (1M-row filter + GROUP BY, rotating over 320 MB of table copies), batch (256 batched
+GROUP BY queries), serve (the open-loop mixed stream).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -37,6 +43,54 @@ ROWS = 10_000_000
 BYTES_PER_ROW = 22
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_lib():
+    """The CPU baseline library (test infrastructure: timed beside the GPU path, never the product)."""
+    B = C.CDLL(os.path.join(ROOT, "oracle", "libcpu_baseline.so"))
+    P, I64, I = C.c_void_p, C.c_int64, C.c_int
+    B.cb_hardware_threads.restype = I
+    B.cb_c1.restype = I64
+    B.cb_c1.argtypes = [P, P, P, I64, C.c_int32, I64, I, P, P, I64]
+    B.cb_c2.restype = I64
+    B.cb_c2.argtypes = [P, P, P, P, P, I64, C.c_int32, C.c_int32, I, P, I64]
+    B.cb_c3.restype = I64
+    B.cb_c3.argtypes = [P, P, P, P, I64, P, P, I64, C.c_int32, I, P, I64]
+    return B
+
+
+def time_cpu(run, target_s: float):
+    """Runs run() repeatedly for about target_s seconds (at least once); returns seconds per run."""
+    t0 = time.perf_counter()
+    run()
+    dt = time.perf_counter() - t0
+    reps = int(max(1, min(20, target_s / max(dt, 1e-6) - 1)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    return (time.perf_counter() - t0) / reps
+
+
+def cpu_baseline_entry(bytes_per_query: float, run_threads, target_s: float, sample: str) -> dict:
+    """Times the CPU restatement with all hardware threads and with one thread."""
+    B = cpu_lib()
+    threads = int(B.cb_hardware_threads())
+    mt = time_cpu(lambda: run_threads(threads), target_s * 0.6)
+    st = time_cpu(lambda: run_threads(1), target_s * 0.4)
+    return {"value": round(bytes_per_query / mt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": sample, "ms_per_query": round(mt * 1e3, 3), "single_thread_value": round(bytes_per_query / st / 1e9, 4),
+            "single_thread_ms_per_query": round(st * 1e3, 3), "cpu_model": cpu_model()}
+
+
 def gen_c2(n, seed=2):
     import warpstore_amd as ws
     g = ws.datagen()
@@ -50,12 +104,13 @@ def gen_c2(n, seed=2):
 
 
 def oracle_lib():
-    import ctypes as C
     path = os.path.join(ROOT, "oracle", "liboracle.so")
     O = C.CDLL(path)
     P, I64 = C.c_void_p, C.c_int64
     O.or_c2_query.restype = I64
     O.or_c2_query.argtypes = [P, P, P, P, P, I64, C.c_int32, C.c_int32, P, I64]
+    O.or_c3_query.restype = I64
+    O.or_c3_query.argtypes = [P, P, P, P, I64, P, P, I64, C.c_int32, P, I64]
     return O
 
 
@@ -75,35 +130,7 @@ def check_top(top, full):
     return all(int(r[0]) in by_key and np.array_equal(r, by_key[int(r[0])]) for r in top)
 
 
-def cpu_baseline(cols, target_s: float):
-    """Times the oracle on a bounded prefix sample of the same table (about target_s seconds)."""
-    O = oracle_lib()
-    n = 1_000_000
-    t0 = time.perf_counter()
-    oracle_c2(O, cols, n)
-    dt = time.perf_counter() - t0
-    n2 = int(min(ROWS, max(n, n * target_s / max(dt, 1e-3))))
-    if n2 > n:
-        t0 = time.perf_counter()
-        oracle_c2(O, cols, n2)
-        dt = time.perf_counter() - t0
-        n = n2
-    return {"value": round(n * BYTES_PER_ROW / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} rows of the same table, same query, single-thread CPU oracle ({dt:.2f} s)"}
-
-
-# Scan kernels of the phase-1 aggregation that can dominate, with their algorithmic bytes per launch
-# and the kernel symbol they launch.
-PHASE1 = "[aggregate (distinct pairs)]"  # the plan-cache kind of the phase-1 plan: its launches' profile tag
-ALGO = {
-    "agg_hash_jit" + PHASE1: (22 * ROWS, "ws_q_grouped", "compiled filter + shared-table hash GROUP BY (user_id, url)"),
-    "agg_part_scatter_jit" + PHASE1: (22 * ROWS, "ws_q_grouped_pscatter",
-                                      "compiled filter + partition scatter of (user_id, url) partial groups"),
-    "agg_part_count_jit" + PHASE1: (18 * ROWS, "ws_q_grouped_pcount", "compiled filter + partition count"),
-}
-
-
-def pmc_traffic(symbol: str, name: str = "pmc_c2_web_analytics.json"):
+def pmc_traffic(symbol: str, name: str):
     """HBM bytes per launch of a kernel, from the committed rocprofv3 counter runs (or None)."""
     path = os.path.join(ROOT, "profiles", name)
     try:
@@ -114,36 +141,9 @@ def pmc_traffic(symbol: str, name: str = "pmc_c2_web_analytics.json"):
         return None
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--copies", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
-                    help="set a tuning knob of the physical plans (ws_tune_set) before running")
-    ap.add_argument("--calibrate", action="store_true",
-                    help="first run a plain 1 GiB device copy kernel (known bytes, for counter calibration)")
-    ap.add_argument("--workload", choices=["web", "batch", "serve", "join"], default="web",
-                    help="web: the headline 10M-row web-analytics query (default); batch: 256 small GROUP BY "
-                         "queries, one batched launch vs per-query; serve: the open-loop mixed stream, "
-                         "continuous batching vs per-query serving")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the path's multi-GPU sharding is a design note, no collective runs here)", file=sys.stderr)
-        return 2
-    if args.workload == "batch":
-        return bench_batch(args)
-    if args.workload == "join":
-        return bench_join(args)
-    if args.workload == "serve":
-        return bench_serve(args)
-
+def setup(args):
     import warpstore_amd as ws
-    from warpstore_amd import abi, queries as Q
+    from warpstore_amd import abi
     L = ws.lib()
     ws.check(L.ws_init())
     L.ws_jit_set_mode(abi.JIT_SYNC)
@@ -158,84 +158,191 @@ def main() -> int:
         L.ws_sync()
         src.free()
         dst.free()
+    return ws, L, tuned
 
-    cols = gen_c2(ROWS)
-    types = [abi.T_I64, abi.T_I32, abi.T_I32, abi.T_I32, abi.T_I32]
-    tables = [Q.DeviceTable(list(cols), types, [0, 1, 0, 0, 0]) for _ in range(args.copies)]
 
-    top = None
-    for w in range(max(args.warmup, 1)):
-        top, stats = Q.c2(tables[w % args.copies], 10)
+def timed_steps(ws, L, step, steps: int, prof_kernel: str):
+    """Times exactly `steps` calls of step() between two device synchronizations, with the given
+    kernel (profile name) bracketed by HIP events on its launch stream; returns
+    (elapsed s, per-step latencies s, launches, kernel total ms, last result)."""
+    L.ws_prof_reset()
+    L.ws_prof_only(prof_kernel.encode())
+    L.ws_prof_enable(1)
     L.ws_sync()
-    # the dominant kernel, from one more (untimed) query after the warm-ups
+    lat = []
+    out = None
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ts = time.perf_counter()
+        out = step(s)
+        lat.append(time.perf_counter() - ts)
+    L.ws_sync()
+    elapsed = time.perf_counter() - t0
+    prof = ws.prof_rows()
+    L.ws_prof_enable(0)
+    launches, total_ms = prof.get(prof_kernel, (0, 0.0))
+    return elapsed, lat, launches, total_ms, out
+
+
+def profiled_query(ws, L, step):
+    """One untimed query with every kernel bracketed by events: {profile name: (launches, ms)}."""
     L.ws_prof_reset()
     L.ws_prof_only(b"")
     L.ws_prof_enable(1)
-    top, stats = Q.c2(tables[0], 10)
+    out = step(0)
     L.ws_sync()
     warm = ws.prof_rows()
     L.ws_prof_enable(0)
-    dom = max((n for n in ALGO if n in warm), key=lambda n: warm[n][1], default="agg_hash_jit" + PHASE1)
-    algo_bytes, symbol, what = ALGO[dom]
+    return warm, out
 
-    L.ws_prof_reset()
-    L.ws_prof_only(dom.encode())
-    L.ws_prof_enable(1)
-    lat = []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        ts = time.perf_counter()
-        top, stats = Q.c2(tables[s % args.copies], 10)
-        lat.append(time.perf_counter() - ts)
-    L.ws_sync()
-    t1 = time.perf_counter()
-    prof = ws.prof_rows()
-    L.ws_prof_enable(0)
 
-    elapsed = t1 - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    gbs = ROWS * BYTES_PER_ROW * args.steps / elapsed / 1e9
-    launches, total_ms = prof.get(dom, (0, 0.0))
-    kern_ms = total_ms / max(launches, 1)
-    achieved = algo_bytes / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
-
-    full = oracle_c2(oracle_lib(), cols, ROWS)
-    parity = bool(check_top(top, full))
-
-    result = {
-        "metric": METRIC,
-        "value": round(gbs, 3),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int64",
-        "data": "synthetic (seeded generator, seed 2)",
-        "config": {
-            "workload": "web-analytics GROUP BY user_id + COUNT DISTINCT url + top 10, 10M rows",
-            "rows": ROWS, "bytes_per_row": BYTES_PER_ROW, "segments_rows": 1 << 20, "table_copies": args.copies,
-            "distinct_pairs": int(stats[0]), "groups": int(stats[1]), "phase1_capacity": int(stats[2]),
-            "overflow_retries": int(stats[3]),
-            **({"tuning": tuned} if tuned else {}),
-        },
-        "pct_of_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 3),
+def base_line(args, value, elapsed, lat, data, config, dtype="int64"):
+    return {
+        "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data, "config": config,
+        "pct_of_hbm_peak": round(100 * value / HBM_PEAK_GBS, 3),
         "queries_per_s": round(args.steps / elapsed, 3),
         "p50_ms": round(float(np.percentile(np.array(lat) * 1e3, 50)), 4),
         "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 4),
-        "parity_vs_oracle": parity,
-        "roofline": {
-            "kernel": f"{symbol} ({what})",
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": pmc_traffic(symbol),
-            "algorithmic_bytes": algo_bytes, "mean_launch_ms": round(kern_ms, 4), "launches": int(launches),
-        },
     }
+
+
+def roofline(kernel_desc, algo_bytes, launches, total_ms, traffic):
+    kern_ms = total_ms / max(launches, 1)
+    achieved = algo_bytes / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    return {"kernel": kernel_desc, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "algorithmic_bytes": int(algo_bytes),
+            "mean_launch_ms": round(kern_ms, 4), "launches": int(launches)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--copies", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a tuning knob of the physical plans (ws_tune_set) before running")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 1 GiB device copy kernel (known bytes, for counter calibration)")
+    ap.add_argument("--workload", choices=["join", "web", "filter", "batch", "serve"], default="join",
+                    help="join: the star join, fact 60M x dimension 1.5M (default); web: the 10M-row "
+                         "web-analytics query; filter: the 1M-row filter + GROUP BY; batch: 256 small GROUP BY "
+                         "queries, one batched launch vs per-query; serve: the open-loop mixed stream")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path's multi-GPU sharding is a design note, no collective runs here)", file=sys.stderr)
+        return 2
+    return {"join": bench_join, "web": bench_web, "filter": bench_filter, "batch": bench_batch,
+            "serve": bench_serve}[args.workload](args)
+
+
+# Scan kernels of the web query's phase-1 aggregation that can dominate, with their algorithmic
+# bytes per launch and the kernel symbol they launch.
+PHASE1 = "[aggregate (distinct pairs)]"  # the plan-cache kind of the phase-1 plan: its launches' profile tag
+WEB_ALGO = {
+    "agg_hash_jit" + PHASE1: (22 * ROWS, "ws_q_grouped", "compiled filter + shared-table hash GROUP BY (user_id, url)"),
+    "agg_part_scatter_jit" + PHASE1: (22 * ROWS, "ws_q_grouped_pscatter",
+                                      "compiled filter + partition scatter of (user_id, url) partial groups"),
+    "agg_part_count_jit" + PHASE1: (18 * ROWS, "ws_q_grouped_pcount", "compiled filter + partition count"),
+}
+
+
+def bench_web(args) -> int:
+    """The web-analytics query (seed 2): 10M rows, user_id int64 Zipf(1.1) over 1e6 ids, region in
+    [0,256) stored frame-of-reference encoded (uint16 codes), event_date over 365 days, url code
+    Zipf(1.1) over 1e5, dur lognormal; SELECT user_id, COUNT(*) c, SUM(dur), COUNT(DISTINCT url) FROM
+    h WHERE event_date < 19723+300 AND region < 224 GROUP BY user_id ORDER BY c DESC LIMIT 10.
+    value = 22 B of referenced columns per row x 10M rows per query / wall time; steps rotate over
+    copies of the table that together exceed the 256 MB Infinity Cache."""
+    from warpstore_amd import abi, queries as Q
+    ws, L, tuned = setup(args)
+    cols = gen_c2(ROWS)
+    types = [abi.T_I64, abi.T_I32, abi.T_I32, abi.T_I32, abi.T_I32]
+    tables = [Q.DeviceTable(list(cols), types, [0, 1, 0, 0, 0]) for _ in range(args.copies)]
+    for w in range(max(args.warmup, 1)):
+        Q.c2(tables[w % args.copies], 10)
+    L.ws_sync()
+    warm, _ = profiled_query(ws, L, lambda s: Q.c2(tables[0], 10))
+    dom = max((n for n in WEB_ALGO if n in warm), key=lambda n: warm[n][1], default="agg_hash_jit" + PHASE1)
+    algo_bytes, symbol, what = WEB_ALGO[dom]
+    elapsed, lat, launches, total_ms, (top, stats) = timed_steps(
+        ws, L, lambda s: Q.c2(tables[s % args.copies], 10), args.steps, dom)
+    gbs = ROWS * BYTES_PER_ROW * args.steps / elapsed / 1e9
+    full = oracle_c2(oracle_lib(), cols, ROWS)
+    parity = bool(check_top(top, full))
+    result = base_line(args, gbs, elapsed, lat, "synthetic (seeded generator, seed 2)", {
+        "workload": "web-analytics GROUP BY user_id + COUNT DISTINCT url + top 10, 10M rows",
+        "rows": ROWS, "bytes_per_row": BYTES_PER_ROW, "segments_rows": 1 << 20, "table_copies": args.copies,
+        "distinct_pairs": int(stats[0]), "groups": int(stats[1]), "phase1_capacity": int(stats[2]),
+        "overflow_retries": int(stats[3]), **({"tuning": tuned} if tuned else {})})
+    result["rows_per_s"] = round(ROWS * args.steps / elapsed, 1)
+    result["parity_vs_oracle"] = parity
+    result["roofline"] = roofline(f"{symbol} ({what})", algo_bytes, launches, total_ms,
+                                  pmc_traffic(symbol, "pmc_c2_web_analytics.json"))
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(cols, args.cpu_seconds)
+        B = cpu_lib()
+        cap = ROWS + 1
+        out = np.zeros((cap, 4), np.int64)
+        ptrs = [c.ctypes.data for c in cols]
+        result["cpu_baseline"] = cpu_baseline_entry(
+            ROWS * BYTES_PER_ROW, lambda t: B.cb_c2(*ptrs, ROWS, 19723 + 300, 224, t, out.ctypes.data, cap), args.cpu_seconds,
+            "the whole query over the same 10M rows (filter, partition by user, sort, group, order)")
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0 if parity else 1
+
+
+FILTER_ROWS = 1_000_000
+FILTER_BYTES = 20  # k 4 + v 8 + p 8
+
+
+def bench_filter(args) -> int:
+    """The plain filter + GROUP BY (seed 1): 1M rows, k int32 [0,1000), v int64 [0,1e6), p double
+    [0,1000); SELECT k, COUNT(*), SUM(v), MIN(v), MAX(v), AVG(p) FROM t WHERE k < 500 AND v > 1000
+    GROUP BY k. One 20 MB table fits the 256 MB Infinity Cache, so steps rotate over 16 copies
+    (320 MB) and every step reads a copy the previous 15 steps did not touch."""
+    from warpstore_amd import abi, queries as Q
+    ws, L, tuned = setup(args)
+    g = ws.datagen()
+    k, v, p = np.empty(FILTER_ROWS, np.int32), np.empty(FILTER_ROWS, np.int64), np.empty(FILTER_ROWS, np.float64)
+    g.wsg_c1(1, FILTER_ROWS, ws.ptr(k), ws.ptr(v), ws.ptr(p))
+    ncopies = 16
+    tables = [Q.DeviceTable([k, v, p], [abi.T_I32, abi.T_I64, abi.T_F64]) for _ in range(ncopies)]
+    for w in range(max(args.warmup, 1)):
+        Q.c1(tables[w % ncopies], True)
+    L.ws_sync()
+    warm, _ = profiled_query(ws, L, lambda s: Q.c1(tables[0], True))
+    scan = [n for n in warm if n.startswith("agg_dense") or n.startswith("agg_hash") or n.startswith("agg_nokeys")]
+    dom = max(scan, key=lambda n: warm[n][1]) if scan else max(warm, key=lambda n: warm[n][1])
+    elapsed, lat, launches, total_ms, (rows, avg) = timed_steps(
+        ws, L, lambda s: Q.c1(tables[(s + args.warmup) % ncopies], True), args.steps, dom)
+    gbs = FILTER_ROWS * FILTER_BYTES * args.steps / elapsed / 1e9
+    # parity against the CPU restatement
+    B = cpu_lib()
+    out = np.zeros((1000, 5), np.int64)
+    psum = np.zeros(1000)
+    ng = B.cb_c1(k.ctypes.data, v.ctypes.data, p.ctypes.data, FILTER_ROWS, 500, 1000, 1, out.ctypes.data, psum.ctypes.data, 1000)
+    order = np.argsort(rows[:, 0])
+    parity = bool(len(rows) == ng and np.array_equal(rows[order], out[:ng]) and
+                  np.allclose(avg[order], psum[:ng] / out[:ng, 1], rtol=1e-12, atol=0))
+    result = base_line(args, gbs, elapsed, lat, "synthetic (seeded generator, seed 1)", {
+        "workload": "filter + GROUP BY k, 1M rows (k < 500 AND v > 1000; COUNT, SUM, MIN, MAX v, AVG p)",
+        "rows": FILTER_ROWS, "bytes_per_row": FILTER_BYTES, "table_copies": ncopies, "groups": int(len(rows)),
+        **({"tuning": tuned} if tuned else {})})
+    result["rows_per_s"] = round(FILTER_ROWS * args.steps / elapsed, 1)
+    result["parity_vs_oracle"] = parity
+    result["roofline"] = roofline(dom, FILTER_ROWS * FILTER_BYTES, launches, total_ms, None)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline_entry(
+            FILTER_ROWS * FILTER_BYTES,
+            lambda t: B.cb_c1(k.ctypes.data, v.ctypes.data, p.ctypes.data, FILTER_ROWS, 500, 1000, t, out.ctypes.data,
+                              psum.ctypes.data, 1000), args.cpu_seconds, "the whole query over the same 1M rows")
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
@@ -299,98 +406,68 @@ def oracle_c3(O, fact, dim):
     return out[:g]
 
 
-def bench_join(args) -> int:
-    """Star join (seed 3): fact 60M rows (fk int32 into the dimension keys, qty int64, price
-    DECIMAL(15,2) as int64, flag < 25), dimension 1.5M rows (shuffled unique int32 key, attr
-    < 1000). SELECT flag, COUNT(*), SUM(price), SUM(qty) FROM fact JOIN dim ON fk = key WHERE
-    attr < 50 GROUP BY flag: dimension filter, exact key bitmap, fused fact filter, hash join
-    (build on the dimension), gathers, dense GROUP BY. GB/s = 24 B of referenced fact columns per
-    fact row / query time."""
-    import warpstore_amd as ws
-    from warpstore_amd import abi, queries as Q
-    L = ws.lib()
-    ws.check(L.ws_init())
-    L.ws_jit_set_mode(abi.JIT_SYNC)
-    tuned = {}
-    for kv in args.tune:
-        key, val = kv.split("=", 1)
-        ws.check(L.ws_tune_set(key.encode(), int(val)))
-        tuned[key] = int(val)
-    if args.calibrate:
-        src, dst = ws.DevBuf(1 << 30), ws.DevBuf(1 << 30)
-        ws.check(L.ws_copy_kernel(dst.p, src.p, 1 << 30))
-        L.ws_sync()
-        src.free()
-        dst.free()
+def c3_data(ws):
     g = ws.datagen()
     dkey, dattr = np.empty(JOIN_DIM, np.int32), np.empty(JOIN_DIM, np.int32)
     g.wsg_c3_dim(3, JOIN_DIM, ws.ptr(dkey), ws.ptr(dattr))
     fk, flag = np.empty(JOIN_FACT, np.int32), np.empty(JOIN_FACT, np.int32)
     qty, price = np.empty(JOIN_FACT, np.int64), np.empty(JOIN_FACT, np.int64)
     g.wsg_c3_fact(3, JOIN_FACT, JOIN_DIM, ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag))
+    return (fk, qty, price, flag), (dkey, dattr)
+
+
+def c3_bytes_read(fact_selected: int) -> int:
+    """Bytes the star join has to read: fk of every fact row, key and attr of every dimension row,
+    and qty (8 B), price (8 B) and flag (4 B) of each fact row that survives the join filter."""
+    return 4 * JOIN_FACT + 8 * JOIN_DIM + 20 * int(fact_selected)
+
+
+def bench_join(args) -> int:
+    """Star join (seed 3): fact 60M rows (fk int32 into the dimension keys, qty int64, price
+    DECIMAL(15,2) as int64, flag < 25), dimension 1.5M rows (shuffled unique int32 key, attr
+    < 1000). SELECT flag, COUNT(*), SUM(price), SUM(qty) FROM fact JOIN dim ON fk = key WHERE
+    attr < 50 GROUP BY flag: dimension filter, exact key bitmap, fused fact filter, hash join
+    (build on the dimension), gathers, dense GROUP BY."""
+    from warpstore_amd import abi, queries as Q
+    ws, L, tuned = setup(args)
+    (fk, qty, price, flag), (dkey, dattr) = c3_data(ws)
     fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
     dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
     for _ in range(max(args.warmup, 1)):
-        rows, st = Q.c3(fact, dim)
+        Q.c3(fact, dim)
     L.ws_sync()
-    # stage times from one more (untimed) query: a warm-up's first launches also load code objects,
-    # which would land between a stage's events
-    L.ws_prof_reset()
-    L.ws_prof_only(b"")
-    L.ws_prof_enable(1)
-    rows, st = Q.c3(fact, dim)
-    L.ws_sync()
-    warm = ws.prof_rows()
+    # stage times from one more (untimed) query: a warm-up's first launches also load code objects
+    warm, _ = profiled_query(ws, L, lambda s: Q.c3(fact, dim))
     # the fact filter's launch: the persistent (sets staged in LDS) or the chunked selection kernel
-    fact_filter = next((k for k in warm if k.endswith("[filter (fact by join keys)]")), "select_jit[filter (fact by join keys)]")
-    L.ws_prof_enable(0)
-    L.ws_prof_reset()
-    L.ws_prof_only(fact_filter.encode())
-    L.ws_prof_enable(1)
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        rows, st = Q.c3(fact, dim)
-        lat.append(time.perf_counter() - ts)
-    L.ws_sync()
-    elapsed = time.perf_counter() - t0
-    prof = ws.prof_rows()
-    L.ws_prof_enable(0)
-    launches, total_ms = prof.get(fact_filter, (0, 0.0))
-    kern_ms = total_ms / max(launches, 1)
-    algo = 4 * JOIN_FACT  # the fused fact filter reads fk (4 B) per fact row
-    achieved = algo / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    dom = next((k for k in warm if k.endswith("[filter (fact by join keys)]")), "select_jit[filter (fact by join keys)]")
+    elapsed, lat, launches, total_ms, (rows, st) = timed_steps(ws, L, lambda s: Q.c3(fact, dim), args.steps, dom)
     O = oracle_lib()
-    t1 = time.perf_counter()
     exp = oracle_c3(O, (fk, qty, price, flag), (dkey, dattr))
-    cpu_s = time.perf_counter() - t1
-    parity = bool(np.array_equal(np.asarray(rows)[np.argsort(np.asarray(rows)[:, 0])], exp[np.argsort(exp[:, 0])]))
-    gbs = JOIN_FACT * JOIN_BYTES * args.steps / elapsed / 1e9
-    out = {
-        "metric": "GB/s scanned (% of HBM peak) for filter+agg/join; queries/s at p99, 1 GPU", "value": round(gbs, 3),
-        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded generator, seed 3)",
-        "config": {"workload": "star join: fact 60M x dim 1.5M, 5 % dimension filter, GROUP BY flag",
-                   "fact_rows": JOIN_FACT, "dim_rows": JOIN_DIM, "bytes_per_fact_row": JOIN_BYTES,
-                   "dim_selected": int(st[0]), "fact_selected": int(st[1]), "pairs": int(st[2]),
-                   **({"tuning": tuned} if tuned else {})},
-        "pct_of_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 3),
-        "queries_per_s": round(args.steps / elapsed, 3),
-        "p50_ms": round(float(np.percentile(np.array(lat) * 1e3, 50)), 4),
-        "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 4),
-        "parity_vs_oracle": parity,
-        "stage_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(warm.items(), key=lambda kv: -kv[1][1])[:8]},
-        "roofline": {"kernel": fact_filter + " (compiled fact filter: fk - lo IN dimension-key bitmap)", "bound": "hbm",
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": pmc_traffic("ws_q_select_staged (fact filter)", "pmc_c3_star_join.json"),
-                     "algorithmic_bytes": algo,
-                     "mean_launch_ms": round(kern_ms, 4), "launches": int(launches)},
-        "cpu_baseline": {"value": round(JOIN_FACT * JOIN_BYTES / cpu_s / 1e9, 4), "unit": "GB/s", "cores": 1,
-                         "kind": "port", "sample": f"the whole query, single-thread CPU oracle ({cpu_s:.2f} s)"},
-    }
+    rows = np.asarray(rows)
+    parity = bool(np.array_equal(rows[np.argsort(rows[:, 0])], exp[np.argsort(exp[:, 0])]))
+    bytes_read = c3_bytes_read(int(st[1]))
+    gbs = bytes_read * args.steps / elapsed / 1e9
+    out = base_line(args, gbs, elapsed, lat, "synthetic (seeded generator, seed 3)", {
+        "workload": "star join: fact 60M x dim 1.5M, 5 % dimension filter, GROUP BY flag",
+        "fact_rows": JOIN_FACT, "dim_rows": JOIN_DIM, "bytes_read_per_query": bytes_read,
+        "dim_selected": int(st[0]), "fact_selected": int(st[1]), "pairs": int(st[2]),
+        **({"tuning": tuned} if tuned else {})})
+    out["effective_gbs"] = round(JOIN_FACT * JOIN_BYTES * args.steps / elapsed / 1e9, 3)
+    out["fact_rows_per_s"] = round(JOIN_FACT * args.steps / elapsed, 1)
+    out["parity_vs_oracle"] = parity
+    out["stage_ms"] = {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(warm.items(), key=lambda kv: -kv[1][1])[:8]}
+    out["roofline"] = roofline(dom + " (compiled fact filter: fk - lo IN dimension-key bitmap)", 4 * JOIN_FACT, launches,
+                               total_ms, pmc_traffic("ws_q_select_staged (fact filter)", "pmc_c3_star_join.json"))
+    if not args.no_cpu:
+        B = cpu_lib()
+        res = np.zeros((256, 4), np.int64)
+        ptrs = [a.ctypes.data for a in (fk, qty, price, flag)]
+        out["cpu_baseline"] = cpu_baseline_entry(
+            bytes_read, lambda t: B.cb_c3(*ptrs, JOIN_FACT, dkey.ctypes.data, dattr.ctypes.data, JOIN_DIM, 50, t,
+                                          res.ctypes.data, 256), args.cpu_seconds,
+            "the whole query over the same 60M x 1.5M tables (filtered build with multiplicities, probe, GROUP BY flag)")
+    else:
+        out["cpu_baseline"] = None
     fact.free()
     dim.free()
     print(json.dumps(out), flush=True)

@@ -2,6 +2,8 @@
 reference harness in oracle/_ref/ (its src/devtypes.h compiled as it lies; nothing is copied).
 
   oracle/liboracle.so               the CPU restatement (ws_oracle.cpp)
+  oracle/libcpu_baseline.so         multithreaded CPU restatements of the measured queries (the
+                                    benchmark's CPU baseline; cpu_baseline.cpp)
   oracle/_ref/libref_devtypes.so    reference expression semantics (ref_devtypes_harness.cpp)
 """
 from __future__ import annotations
@@ -29,6 +31,16 @@ def build_oracle() -> str:
     return out
 
 
+def build_cpu_baseline() -> str:
+    # x86-64-v3 (AVX2) rather than -march=native: the library is built here and runs on the GPU
+    # box's host, whose CPU model differs
+    out = os.path.join(HERE, "libcpu_baseline.so")
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-pthread",
+          os.path.join(HERE, "cpu_baseline.cpp"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_ref() -> str | None:
     if not os.path.exists(os.path.join(REF_SRC, "devtypes.h")):
         return None
@@ -42,6 +54,7 @@ def build_ref() -> str | None:
 
 def build() -> None:
     build_oracle()
+    build_cpu_baseline()
     build_ref()
 
 

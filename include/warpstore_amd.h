@@ -203,6 +203,17 @@ int ws_set_bits(uint32_t* bits, const int32_t* idx, int64_t m);                 
 int ws_gather_null(const void* src, int ptype, const int32_t* idx, int64_t m, void* dst);        /* gpuGatherNull gpu.h:193 */
 int ws_gather_bytes(const void* src, int esize, const int32_t* idx, int64_t m, void* dst, int nulls); /* gpuGatherBytes gpu.h:195 */
 
+/* ------------------------------------------------------------------ segment encodings (host side)
+ * Table::encodeColumn storage.cu:1097-1194: a sealed segment column in the compressed format the
+ * kernels decode inline (ws_colview.enc): 1 / 2 / 3 integer frame of reference (u8 / u16 / u32
+ * codes, top code = NULL), 4 double dictionary (<= 255 distinct values), 5 / 6 decimal doubles
+ * (u16 / u32 codes of round(x * 10^s), s <= 4). *enc = 0 when no encoding fits (keep the column
+ * raw); otherwise *bytes bytes were written to out (cap >= ws_encoded_bytes_max(n)). The
+ * reference encodes only from 1024 rows on and under its compression setting; that policy stays
+ * with the caller. */
+size_t ws_encoded_bytes_max(int64_t n);
+int ws_encode_column(const void* col, int ptype, int64_t n, uint8_t* out, size_t cap, int* enc, size_t* bytes);
+
 /* ------------------------------------------------------------------ joins (outputs are tmp buffers) */
 int ws_hash_join(ws_colview build_key, int64_t nb, ws_colview probe_key, int64_t np, int32_t** out_build,
                  int32_t** out_probe, int64_t* m); /* gpuHashJoin gpu.h:178-179 */

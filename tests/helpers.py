@@ -91,6 +91,17 @@ def ref_lib():
     return _ref
 
 
+def encode_column(x: np.ndarray):
+    """(encoded bytes, encoding) of one segment column through the library's encoder."""
+    x = np.ascontiguousarray(x)
+    ptype = {np.dtype(np.int32): abi.P_I32, np.dtype(np.int64): abi.P_I64, np.dtype(np.float64): abi.P_F64}[x.dtype]
+    L = ws.lib()
+    out = np.zeros(L.ws_encoded_bytes_max(len(x)), np.uint8)
+    e, nbytes = C.c_int(), C.c_size_t()
+    ws.check(L.ws_encode_column(ws.ptr(x), ptype, len(x), ws.ptr(out), len(out), C.byref(e), C.byref(nbytes)))
+    return out[:nbytes.value].copy(), e.value
+
+
 def say(*a) -> None:
     """Names the case before a GPU step (flushed, so a lost box still shows where it was)."""
     print(*a, flush=True)
@@ -119,16 +130,16 @@ class Table:
         self.enc = {}
 
     def encode(self, name: str) -> int:
-        """Stores column `name` per batch in the frame-of-reference segment encoding."""
-        g = ws.datagen()
+        """Stores column `name` per batch in its segment encoding (ws_encode_column: frame of
+        reference for integers, dictionary or decimal codes for doubles); returns the last
+        batch's encoding."""
         x = self.cols[name]
         pieces = []
         e = None
         for b in range(len(self.sizes)):
             part = np.ascontiguousarray(x[self.offs[b]:self.offs[b + 1]])
-            out = np.zeros(16 + part.nbytes + 16, np.uint8)
-            e = g.wsg_encode_int(ws.ptr(part), 1 if part.dtype == np.int64 else 0, len(part), ws.ptr(out))
-            assert e, "range does not fit an encoding"
+            out, e = encode_column(part)
+            assert e, "column does not fit an encoding"
             pieces.append((out, e))
         self.enc[name] = pieces
         return e

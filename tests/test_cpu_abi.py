@@ -93,7 +93,7 @@ def test_encoded_and_nullable_plan_compiles():
                                        C.byref(n))
     assert rc == 0, buf.value.decode()[:2000]
     src = buf.value.decode()
-    assert "ws::enc_get(e0,r)" in src and "ws::dec_of(" in src and "ws::inset_s(" in src
+    assert "ws::enc_get(enc0,r)" in src and "ws::dec_of(" in src and "ws::inset_s(" in src
 
 
 def test_prog_is_int_typing():

@@ -307,14 +307,12 @@ def test_c3_query_oracle_vs_numpy():
 def test_encoded_decode_roundtrip_via_generator():
     """Frame-of-reference encoding decodes back to the original values (NULL sentinel kept)."""
     rng = np.random.default_rng(9)
-    g = ws.datagen()
     O = H.oracle()
     for dtype, lo, hi, tcode in ((np.int32, 0, 250, abi.T_I32N), (np.int32, -1000, 60000, abi.T_I32N),
                                  (np.int64, 10 ** 12, 10 ** 12 + 3 * 10 ** 9, abi.T_I64N)):
         x = rng.integers(lo, hi, 3000).astype(dtype)
         x[::29] = abi.NULL_I32 if dtype == np.int32 else abi.NULL_I64
-        out = np.zeros(16 + x.nbytes + 16, np.uint8)
-        e = g.wsg_encode_int(x.ctypes.data, 1 if dtype == np.int64 else 0, len(x), out.ctypes.data)
+        out, e = H.encode_column(x)
         assert e in (1, 2, 3)
         v = abi.ColView()
         v.p, v.t, v.enc = out.ctypes.data, tcode, e

@@ -96,6 +96,7 @@ _SIGS = {
                                  C.POINTER(SZ)]),
     "ws_prof_enable": (None, [I]), "ws_prof_only": (None, [C.c_char_p]), "ws_prof_reset": (None, []), "ws_prof_collect": (I, [C.POINTER(abi.ProfRow), I]),
     "ws_copy_kernel": (I, [P, P, SZ]),
+    "ws_encoded_bytes_max": (SZ, [I64]), "ws_encode_column": (I, [P, I, I64, P, SZ, C.POINTER(I), C.POINTER(SZ)]),
 }
 
 
@@ -127,8 +128,6 @@ def datagen():
         G.wsg_c3_fact.argtypes = [U64, I64, I64, P, P, P, P]
         G.wsg_c4_shapes.argtypes = [U64, I64, P, P, P]
         G.wsg_c4_data.argtypes = [U64, I64, I64, I64, P, P]
-        G.wsg_encode_int.argtypes = [P, I, I64, P]
-        G.wsg_encode_int.restype = I
         for f in ("wsg_c1", "wsg_c2", "wsg_c3_dim", "wsg_c3_fact", "wsg_c4_shapes", "wsg_c4_data"):
             getattr(G, f).restype = None
         _gen = G

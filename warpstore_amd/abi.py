@@ -29,10 +29,12 @@ JF_NOKEYS, JF_GROUPED, JF_DENSE, JF_FLAG, JF_FLAGM = range(5)
 JIT_OFF, JIT_ON, JIT_SYNC = 0, 1, 2
 # scalar function ids used in tests
 FN_ABS, FN_CEIL, FN_FLOOR, FN_ROUND, FN_SQRT = 0, 1, 2, 3, 4
+FN_EXP, FN_LN, FN_LOG10, FN_LOG2, FN_SIN, FN_COS, FN_TAN, FN_SIGN = 5, 6, 7, 8, 9, 10, 11, 12
+FN_ASIN, FN_ACOS, FN_ATAN = 30, 31, 32
 FN_MONTH, FN_DAY, FN_DAYOFWEEK, FN_DAYOFYEAR, FN_QUARTER, FN_WEEKDAY, FN_LAST_DAY = 13, 14, 15, 16, 17, 18, 19
 FN_WEEK, FN_YEARWEEK = 35, 39
 FN_ROUND2, FN_POW, FN_ATAN2, FN_LEAST, FN_GREATEST, FN_TRUNCATE2, FN_ADD_MONTHS = 100, 101, 102, 103, 104, 105, 106
-FN_IDIV, FN_DATE_TRUNC_MONTH = 109, 110
+FN_LOG_B, FN_IDIV, FN_DATE_TRUNC_MONTH = 108, 109, 110
 
 
 class Instr(C.Structure):

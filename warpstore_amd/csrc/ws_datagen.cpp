@@ -136,35 +136,4 @@ void wsg_c4_data(uint64_t seed, int64_t q, int64_t rows, int64_t distinct, int32
   }
 }
 
-// Frame-of-reference encoding of an int32 / int64 column (i64 != 0) as stored in a GPU segment:
-// 16-byte header (int64 base = min), then u8 / u16 / u32 codes (value - base), the top code meaning
-// NULL. Returns the encoding (1, 2, 3; 3 only for int64) or 0 when the range does not fit; out
-// must hold 16 + n * width + 16 bytes.
-int wsg_encode_int(const void* x, int i64, int64_t n, uint8_t* out) {
-  const long long nul = i64 ? (long long)WS_NULL_I64 : (long long)WS_NULL_I32;
-  long long mn = LLONG_MAX, mx = LLONG_MIN;
-  for (int64_t i = 0; i < n; ++i) {
-    const long long v = i64 ? (long long)((const int64_t*)x)[i] : (long long)((const int32_t*)x)[i];
-    if (v == nul) continue;
-    mn = std::min(mn, v);
-    mx = std::max(mx, v);
-  }
-  if (mn > mx) mn = mx = 0;
-  const unsigned long long range = (unsigned long long)mx - (unsigned long long)mn;
-  const int e = range < 0xFFull ? 1 : range < 0xFFFFull ? 2 : (i64 && range < 0xFFFFFFFFull) ? 3 : 0;
-  if (!e) return 0;
-  const int w = e == 1 ? 1 : e == 2 ? 2 : 4;
-  std::memset(out, 0, WS_ENC_HEADER + (size_t)n * w + 16);
-  std::memcpy(out, &mn, 8);
-  uint8_t* d = out + WS_ENC_HEADER;
-  for (int64_t i = 0; i < n; ++i) {
-    const long long v = i64 ? (long long)((const int64_t*)x)[i] : (long long)((const int32_t*)x)[i];
-    const unsigned long long code = v == nul ? (w == 1 ? 0xFFull : w == 2 ? 0xFFFFull : 0xFFFFFFFFull) : (unsigned long long)(v - mn);
-    if (w == 1) d[i] = (uint8_t)code;
-    else if (w == 2) ((uint16_t*)d)[i] = (uint16_t)code;
-    else ((uint32_t*)d)[i] = (uint32_t)code;
-  }
-  return e;
-}
-
 }  // extern "C"

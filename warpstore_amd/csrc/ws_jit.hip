@@ -20,6 +20,12 @@
 #include <sstream>
 #include <thread>
 
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <cstddef>
+
+#include "ws_codegen.h"
 #include "ws_internal.h"
 #include "ws_jit_prelude.inc"  // kWsJitPrelude: ws_abi.h + ws_vm.h + ws_kernels.h (generated at build)
 
@@ -89,445 +95,109 @@ const char* kKernelMacro[] = {"WS_KERNEL_NOKEYS", "WS_KERNEL_GROUPED", "WS_KERNE
 const char* kKernelName[] = {"ws_q_nokeys", "ws_q_grouped", "ws_q_dense", "ws_q_select", "ws_q_select"};
 
 // ---------------------------------------------------------------- persistent plan cache
+//
+// One file per compiled plan, named by a 128-bit digest of its key (hipRTC version, options,
+// target and full source): a 64-byte header, the key (checked byte for byte on load, so a digest
+// collision can only miss), then the gfx950 code object. The header carries digests of itself and
+// of the image; a file that fails either check is discarded and the plan recompiled. Files are
+// written to a private name and renamed into place, so readers never see a partial file.
 
-uint64_t fnv64(const std::string& s, uint64_t h) {
-  for (unsigned char c : s) h = (h ^ c) * 0x100000001b3ull;
-  return h;
+constexpr char kPlanMagic[8] = {'W', 'S', 'G', 'F', 'X', 'P', 'L', 'N'};
+constexpr uint32_t kPlanVersion = 2;
+constexpr uint64_t kMaxImageBytes = 256ull << 20;
+
+struct PlanHeader {
+  char magic[8];
+  uint32_t version, header_bytes;
+  uint64_t key_digest[2];
+  uint64_t key_bytes, image_bytes, image_digest;
+  uint64_t header_digest;  // digest of the 56 bytes before it
+};
+static_assert(sizeof(PlanHeader) == 64, "plan file header is 64 bytes");
+
+// 64-bit digest of a byte string: 8-byte little-endian words folded by multiply-rotate, the tail
+// zero-padded, the length mixed in, then the splitmix64 finalizer.
+uint64_t digest64(const void* data, size_t n, uint64_t seed) {
+  const uint8_t* p = (const uint8_t*)data;
+  uint64_t h = seed ^ (n * 0x9e3779b97f4a7c15ull);
+  for (size_t i = 0; i < n; i += 8) {
+    uint64_t w = 0;
+    std::memcpy(&w, p + i, std::min<size_t>(8, n - i));
+    h ^= w * 0xc2b2ae3d27d4eb4full;
+    h = ((h << 29) | (h >> 35)) * 0x165667b19e3779f9ull;
+  }
+  h ^= h >> 30;
+  h *= 0xbf58476d1ce4e5b9ull;
+  h ^= h >> 27;
+  h *= 0x94d049bb133111ebull;
+  return h ^ (h >> 31);
 }
-std::string disk_path(const std::string& key) {
-  char name[64];
-  snprintf(name, sizeof name, "%016llx%016llx.wsplan", (unsigned long long)fnv64(key, 0xcbf29ce484222325ull),
-           (unsigned long long)fnv64(key, 0x84222325cbf29ce4ull));
+std::string plan_file(const std::string& key) {
+  char name[48];
+  snprintf(name, sizeof name, "%016llx%016llx.wsplan", (unsigned long long)digest64(key.data(), key.size(), 1),
+           (unsigned long long)digest64(key.data(), key.size(), 2));
   return g_disk_dir + "/" + name;
 }
-constexpr char kMagic[9] = "WSPLAN01";
-bool disk_load(const std::string& path, const std::string& key, std::string& image) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) return false;
-  std::string magic(8, '\0');
-  uint64_t kl = 0, il = 0;
-  f.read(magic.data(), 8);
-  f.read((char*)&kl, 8);
-  if (!f || magic != kMagic || kl != key.size()) return false;
-  std::string k(kl, '\0');
-  f.read(k.data(), (std::streamsize)kl);
-  f.read((char*)&il, 8);
-  if (!f || k != key || il > (1ull << 30)) return false;
-  image.assign(il, '\0');
-  f.read(image.data(), (std::streamsize)il);
-  uint64_t sum = 0;
-  f.read((char*)&sum, 8);
-  if (!f || sum != fnv64(image, 0xcbf29ce484222325ull)) {
-    fprintf(stderr, "[warpstore_amd] plan cache file %s is corrupt; recompiling\n", path.c_str());
-    return false;
+bool read_all(int fd, void* dst, size_t n) {
+  uint8_t* p = (uint8_t*)dst;
+  while (n) {
+    const ssize_t r = ::read(fd, p, n);
+    if (r <= 0) return false;
+    p += r, n -= (size_t)r;
   }
   return true;
 }
-void disk_store(const std::string& path, const std::string& key, const std::string& image) {
-  const std::string tmp = path + ".tmp" + std::to_string((unsigned long long)std::hash<std::thread::id>{}(std::this_thread::get_id()));
-  {
-    std::ofstream f(tmp, std::ios::binary);
-    const uint64_t kl = key.size(), il = image.size();
-    f.write(kMagic, 8);
-    f.write((const char*)&kl, 8);
-    f.write(key.data(), (std::streamsize)kl);
-    f.write((const char*)&il, 8);
-    f.write(image.data(), (std::streamsize)il);
-    const uint64_t sum = fnv64(image, 0xcbf29ce484222325ull);
-    f.write((const char*)&sum, 8);
-    if (!f) return;
+bool write_all(int fd, const void* src, size_t n) {
+  const uint8_t* p = (const uint8_t*)src;
+  while (n) {
+    const ssize_t r = ::write(fd, p, n);
+    if (r <= 0) return false;
+    p += r, n -= (size_t)r;
   }
-  std::error_code ec;
-  std::filesystem::rename(tmp, path, ec);  // atomic publish
+  return true;
 }
-
-// ---------------------------------------------------------------- bytecode -> typed C++
-
-struct Node {
-  std::string e;
-  bool is_int;
-};
-std::string as_d(const Node& n) { return n.is_int ? "((double)" + n.e + ")" : n.e; }
-std::string as_i(const Node& n) { return n.is_int ? n.e : "((long long)" + n.e + ")"; }
-
-// Rebuilds typed expression trees from the stack bytecode. Integer columns (INT, BIGINT, DATE,
-// dictionary codes) and integer literals stay in int64 arithmetic (+ - * checked for overflow into
-// env->ovf); anything NULL-able or fractional is double with three-valued helpers.
-struct Translator {
-  const ws_expr_env& env;
-  int ncols;
-  std::set<int> cols, iconsts, dconsts, encs, sets;  // encs: encoded integer columns read; sets: INSET bitmaps
-  // Row mode: column reads come from a Row of values loaded beforehand (x.v<c>: the element, or the
-  // 32-bit word holding an encoded code), so a kernel can issue the loads of a later step early.
-  bool row_mode = false, split_ok = true;
-  std::set<int> row_cols;
-
-  std::string raw_int(int c) {
-    const ws_colview& v = env.cols[c];
-    if (v.enc) encs.insert(c);
-    const std::string n = std::to_string(c);
-    if (row_mode) {
-      row_cols.insert(c);
-      return v.enc ? "ws::enc_decode(e" + n + ",x.v" + n + ",r)" : "((long long)x.v" + n + ")";
-    }
-    return v.enc ? "ws::enc_get(e" + n + ",r)" : "((long long)c" + n + "[r])";
+// Loads the image stored under `key`; false when absent, for another key, or damaged.
+bool plan_file_load(const std::string& path, const std::string& key, std::string& image) {
+  const int fd = ::open(path.c_str(), O_RDONLY | O_CLOEXEC);
+  if (fd < 0) return false;
+  struct Closer {
+    int fd;
+    ~Closer() { ::close(fd); }
+  } closer{fd};
+  PlanHeader h;
+  if (!read_all(fd, &h, sizeof h) || std::memcmp(h.magic, kPlanMagic, 8) != 0 || h.version != kPlanVersion) return false;
+  const bool header_ok = h.header_bytes == sizeof h && h.header_digest == digest64(&h, offsetof(PlanHeader, header_digest), 3) &&
+                         h.image_bytes <= kMaxImageBytes;
+  std::string k;
+  if (header_ok && h.key_bytes == key.size()) {
+    k.resize(key.size());
+    if (!read_all(fd, k.data(), k.size()) || k != key) return false;  // another key under this name
+    image.resize(h.image_bytes);
+    if (read_all(fd, image.data(), image.size()) && digest64(image.data(), image.size(), 4) == h.image_digest) return true;
+  } else if (header_ok) {
+    return false;
   }
-  Node run(ws_prog p) {
-    std::vector<Node> st;
-    auto pop = [&]() {
-      if (st.empty()) throw Error("codegen: malformed expression", WS_ERR_INTERNAL);
-      Node n = st.back();
-      st.pop_back();
-      return n;
-    };
-    for (int pc = p.start; pc < p.start + p.len; ++pc) {
-      if (pc < 0 || pc >= WS_MAX_CODE) throw Error("codegen: program out of range", WS_ERR_INTERNAL);
-      const ws_instr in = env.code[pc];
-      switch (in.op) {
-        case WS_OP_COL: {
-          if (in.arg >= ncols) throw Error("codegen: column slot out of range", WS_ERR_INTERNAL);
-          cols.insert(in.arg);
-          const ws_colview& v = env.cols[in.arg];
-          const std::string n = std::to_string(in.arg);
-          if (v.t == WS_T_F64) {
-            if (row_mode && !v.enc) row_cols.insert(in.arg);
-            if (row_mode && v.enc) split_ok = false;  // (a dictionary read: no row form)
-            st.push_back({v.enc ? "ws::col_dbl(v" + n + ",r)" : row_mode ? "x.v" + n : "c" + n + "[r]", false});
-          }
-          else if (v.t == WS_T_I32N) st.push_back({"ws::null_if_i32(" + raw_int(in.arg) + ")", false});
-          else if (v.t == WS_T_I64N) st.push_back({"ws::null_if_i64(" + raw_int(in.arg) + ")", false});
-          else if (v.t == WS_T_DECIMAL) {
-            char p10[64];
-            snprintf(p10, sizeof p10, "%.1f", pow10i(v.scale));
-            st.push_back({"ws::dec_of(" + raw_int(in.arg) + "," + p10 + ")", false});
-          } else if (v.t == WS_T_VEC) {
-            throw Error("codegen: vector column used as a scalar", WS_ERR_INTERNAL);
-          } else st.push_back({raw_int(in.arg), true});
-          break;
-        }
-        case WS_OP_COLRAW: {
-          if (in.arg >= ncols) throw Error("codegen: column slot out of range", WS_ERR_INTERNAL);
-          cols.insert(in.arg);
-          if (env.cols[in.arg].t == WS_T_DECIMAL && !in.is_int) st.push_back({"ws::null_if_i64(" + raw_int(in.arg) + ")", false});
-          else st.push_back({raw_int(in.arg), true});
-          break;
-        }
-        case WS_OP_CONST:
-          if (in.arg >= WS_MAX_CONSTS) throw Error("codegen: constant out of range", WS_ERR_INTERNAL);
-          if (in.is_int) {
-            iconsts.insert(in.arg);
-            st.push_back({"ik" + std::to_string(in.arg), true});
-          } else {
-            dconsts.insert(in.arg);
-            st.push_back({"k" + std::to_string(in.arg), false});
-          }
-          break;
-        case WS_OP_ADD: case WS_OP_SUB: case WS_OP_MUL: {
-          Node b = pop(), a = pop();
-          if (a.is_int && b.is_int) {
-            const char* f = in.op == WS_OP_ADD ? "ws::add_chk(" : in.op == WS_OP_SUB ? "ws::sub_chk(" : "ws::mul_chk(";
-            st.push_back({std::string(f) + a.e + "," + b.e + ",&ovf_)", true});
-          } else {
-            const char* o = in.op == WS_OP_ADD ? "+" : in.op == WS_OP_SUB ? "-" : "*";
-            st.push_back({"(" + as_d(a) + o + as_d(b) + ")", false});
-          }
-          break;
-        }
-        case WS_OP_DIV: {
-          Node b = pop(), a = pop();
-          st.push_back({"ws::divz(" + as_d(a) + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_MOD: {
-          Node b = pop(), a = pop();
-          st.push_back({"fmod(" + as_d(a) + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_NEG: {
-          Node a = pop();
-          if (a.is_int) st.push_back({"((long long)(0ull-(unsigned long long)" + a.e + "))", true});
-          else st.push_back({"(-" + a.e + ")", false});
-          break;
-        }
-        case WS_OP_EQ: case WS_OP_NE: case WS_OP_LT: case WS_OP_LE: case WS_OP_GT: case WS_OP_GE: {
-          Node b = pop(), a = pop();
-          static const char* ops[] = {"==", "!=", "<", "<=", ">", ">="};
-          const int k = in.op - WS_OP_EQ;
-          if (a.is_int && b.is_int) st.push_back({"((long long)(" + a.e + ops[k] + b.e + "))", true});  // never NULL
-          else st.push_back({"ws::cmp3(" + std::to_string(k) + "," + as_d(a) + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_AND: case WS_OP_OR: {
-          Node b = pop(), a = pop();
-          // Both operands are evaluated (bitwise & / |) unless one can raise the overflow flag:
-          // their column loads then issue together instead of one memory round trip after the
-          // other. Operands that can set env->ovf keep the short circuit.
-          const bool fx = a.e.find("&ovf_") != std::string::npos || b.e.find("&ovf_") != std::string::npos;
-          const char* join = in.op == WS_OP_AND ? (fx ? "&&" : "&") : (fx ? "||" : "|");
-          if (a.is_int && b.is_int)
-            st.push_back({"((long long)(((" + a.e + ")!=0)" + join + "((" + b.e + ")!=0)))", true});
-          else st.push_back({std::string(in.op == WS_OP_AND ? "ws::and3(" : "ws::or3(") + as_d(a) + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_NOT: {
-          Node a = pop();
-          if (a.is_int) st.push_back({"((long long)(" + a.e + "==0))", true});
-          else st.push_back({"ws::not3(" + a.e + ")", false});
-          break;
-        }
-        case WS_OP_INSET: {
-          Node a = pop();
-          const std::string s = std::to_string(in.arg);
-          if (in.arg >= WS_MAX_SETS) throw Error("codegen: set out of range", WS_ERR_INTERNAL);
-          sets.insert(in.arg);
-          const std::string args = "(sl" + s + ",sw" + s + ",s" + s + ",sn" + s + ",";
-          if (a.is_int) st.push_back({"ws::inset_s" + args + a.e + ")", true});
-          else st.push_back({"ws::inset_sd" + args + a.e + ")", false});
-          break;
-        }
-        case WS_OP_SELECT: {
-          Node e = pop(), t = pop(), c = pop();
-          const std::string cond = c.is_int ? "(" + c.e + "!=0)" : "ws::truthy(" + c.e + ")";
-          if (t.is_int && e.is_int) st.push_back({"(" + cond + "?" + t.e + ":" + e.e + ")", true});
-          else st.push_back({"(" + cond + "?" + as_d(t) + ":" + as_d(e) + ")", false});
-          break;
-        }
-        case WS_OP_YEAR: {
-          Node a = pop();
-          if (a.is_int) st.push_back({"ws::year_of_days(" + a.e + ")", true});
-          else st.push_back({"ws::call1(WS_FN_DT_YEAR,ws::call1(WS_FN_DATE_TO_DT," + a.e + "))", false});
-          break;
-        }
-        case WS_OP_ISNULL: {
-          Node a = pop();
-          st.push_back({a.is_int ? std::string("0ll") : "((long long)ws::isnan_(" + a.e + "))", true});
-          break;
-        }
-        case WS_OP_COALESCE: {
-          Node b = pop(), a = pop();
-          if (a.is_int) st.push_back(a);
-          else st.push_back({"ws::coalesce2(" + a.e + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_NULLIF: {
-          Node b = pop(), a = pop();
-          st.push_back({"ws::nullif2(" + as_d(a) + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_CALL1: {
-          Node a = pop();
-          st.push_back({"ws::call1(" + std::to_string(in.arg) + "," + as_d(a) + ")", false});
-          break;
-        }
-        case WS_OP_CALL2: {
-          Node b = pop(), a = pop();
-          st.push_back({"ws::call2(" + std::to_string(in.arg) + "," + as_d(a) + "," + as_d(b) + ")", false});
-          break;
-        }
-        case WS_OP_VDOT: case WS_OP_VEUC: {
-          const int slot = in.arg & 255, m = in.arg >> 8;
-          if (slot >= ncols || m >= WS_MAX_SETS) throw Error("codegen: vector operand out of range", WS_ERR_INTERNAL);
-          cols.insert(slot);
-          const std::string ms = std::to_string(m);
-          st.push_back({std::string(in.op == WS_OP_VDOT ? "ws::vdot_d(" : "ws::veuc_d(") + "(const float*)v" +
-                            std::to_string(slot) + ".p,r,env->maps[" + ms + "],env->map_size[" + ms + "])",
-                        false});
-          break;
-        }
-        case WS_OP_INSORTED: {
-          Node a = pop();
-          const std::string s = std::to_string(in.arg);
-          st.push_back({"ws::insorted_d(env->maps[" + s + "],env->map_size[" + s + "]," + as_d(a) + ")", false});
-          break;
-        }
-        case WS_OP_DICTMAP: {
-          Node a = pop();
-          const std::string s = std::to_string(in.arg);
-          st.push_back({"ws::dictmap_d(env->maps[" + s + "],env->map_size[" + s + "]," + as_d(a) + ")", false});
-          break;
-        }
-        default: throw Error("codegen: unknown opcode", WS_ERR_INTERNAL);
-      }
-    }
-    if (st.size() != 1) throw Error("codegen: malformed expression", WS_ERR_INTERNAL);
-    return st[0];
-  }
-};
-
-const char* elem_of(int t) {  // element type of a raw column
-  return t == WS_T_F64 ? "double" : (t == WS_T_I32 || t == WS_T_I32N) ? "int" : "long long";
+  fprintf(stderr, "[warpstore_amd] discarding damaged plan file %s; the plan is compiled again\n", path.c_str());
+  return false;
 }
-const char* ctype_of(int t) {  // global-memory pointers (loads issue as global, not flat)
-  return (t == WS_T_I32 || t == WS_T_I32N) ? "const int WS_GLOBAL*"
-         : (t == WS_T_F64)                 ? "const double WS_GLOBAL*"
-         : t == WS_T_VEC                   ? "const float WS_GLOBAL*"
-                                           : "const long long WS_GLOBAL*";
-}
-
-int dense_kind_of(int op, bool is_int) {
-  return op == WS_AGG_COUNT ? WS_DK_COUNT
-         : op == WS_AGG_SUM ? (is_int ? WS_DK_SUM_I : WS_DK_SUM_D)
-         : op == WS_AGG_MIN ? (is_int ? WS_DK_MIN_I : WS_DK_MIN_D)
-                            : (is_int ? WS_DK_MAX_I : WS_DK_MAX_D);
-}
-
-// The evaluator struct for one plan: shape queries are compile-time constants, so the unrolled
-// loops of the kernel bodies collapse to the work this query needs.
-std::string gen_evaluator(const ws_expr_env& env, int ncols, const ws_agg_plan& plan, int* rows_out) {
-  Translator tr{env, ncols, {}, {}, {}};
-  const Node where = plan.has_where ? tr.run(plan.where) : Node{"1", true};
-  std::vector<Node> keys, aggs;
-  for (int k = 0; k < plan.nkeys; ++k) keys.push_back(tr.run(plan.keys[k]));
-  for (int a = 0; a < plan.naggs; ++a) aggs.push_back(plan.ops[a] == WS_AGG_COUNT ? Node{"1ll", true} : tr.run(plan.aggs[a]));
-  const int nk = std::max(plan.nkeys, 1), na = std::max(plan.naggs, 1);
-  const int rows = std::max(1, std::min(4, 12 / (nk + na)));  // rows per thread per step (registers)
-  *rows_out = rows;
-  std::ostringstream o;
-  o << "struct QEval {\n";
-  // A program that is one column or one literal has no side effect (no checked-arithmetic flag),
-  // so the partition passes may evaluate it for rows the WHERE then drops.
-  auto pure = [&](ws_prog p) {
-    return p.len == 1 && (env.code[p.start].op == WS_OP_COL || env.code[p.start].op == WS_OP_CONST);
-  };
-  bool pure_keys = true, pure_vals = true;
-  for (int k = 0; k < plan.nkeys; ++k) pure_keys = pure_keys && pure(plan.keys[k]);
-  for (int a = 0; a < plan.naggs; ++a) pure_vals = pure_vals && (plan.ops[a] == WS_AGG_COUNT || pure(plan.aggs[a]));
-  o << "  static constexpr int kKeys = " << nk << ", kAggs = " << na << ", kRows = " << rows << ";\n";
-  o << "  static constexpr bool kPureKeys = " << (pure_keys ? "true" : "false") << ", kPureVals = "
-    << (pure_vals ? "true" : "false") << ";\n";
-  o << "  static constexpr bool kGlobalCols = true;\n";
-  o << "  static constexpr bool kPureWhere = " << (where.e.find("&ovf_") == std::string::npos ? "true" : "false") << ";\n";
-  o << "  const ws_expr_env* env;\n";
-  // Overflow of checked integer arithmetic: a per-thread flag, stored to env->ovf once when the
-  // evaluator goes out of scope (a possible store in the row loop would pin every later load
-  // behind it).
-  o << "  mutable int ovf_ = 0;\n";
-  o << "  __device__ ~QEval() { if (ovf_ && env->ovf) *env->ovf = 1; }\n";
-  for (int c : tr.cols) o << "  " << ctype_of(env.cols[c].t) << " c" << c << ";\n  ws_colview v" << c << ";\n";
-  for (int c : tr.encs) o << "  ws::EncView e" << c << ";\n";
-  for (int k : tr.iconsts) o << "  long long ik" << k << ";\n";
-  for (int k : tr.dconsts) o << "  double k" << k << ";\n";
-  for (int k : tr.sets)
-    o << "  const unsigned WS_GLOBAL* s" << k << ";\n  long long sn" << k << ";\n  const unsigned WS_LOCAL* sl" << k << ";\n  int sw" << k
-      << ";\n";
-  o << "  __device__ QEval(const ws::ScanDesc& d) : env(d.env) {\n";
-  // literals and set bitmaps are the same for every lane: scalar registers
-  for (int k : tr.iconsts) o << "    ik" << k << " = ws::uni((long long)ws::dbits(env->consts[" << k << "]));\n";
-  for (int k : tr.dconsts) o << "    k" << k << " = ws::bitsd(ws::uni((long long)ws::dbits(env->consts[" << k << "])));\n";
-  // an empty set reads its own size word (always readable; every bit is masked off)
-  for (int k : tr.sets)
-    o << "    sn" << k << " = ws::uni((long long)(env->set_size[" << k << "] > 0 ? env->set_size[" << k << "] : 0)); s" << k << " = (const unsigned WS_GLOBAL*)ws::uni(sn" << k
-      << " > 0 ? env->sets[" << k << "] : (const unsigned*)&env->set_size[" << k << "]); sl" << k << " = ws::lds_spare_word(); sw" << k
-      << " = 0;\n";
-  o << "  }\n";
-  // A kernel with LDS to spare copies the sets' leading words there, in order, within `words`.
-  o << "  __device__ void stage_sets(unsigned* lds, int words) {\n    int off = 0, w = 0;\n";
-  for (int k : tr.sets)
-    o << "    w = ws::stage_set(lds, off, words, s" << k << ", sn" << k << "); if (w > 0) { sl" << k
-      << " = (const unsigned WS_LOCAL*)(lds + off); sw" << k << " = w; } off += w;\n";
-  o << "    (void)off; (void)w;\n  }\n  __device__ void bind(const ws_colview* c) {\n";
-  for (int c : tr.cols)
-    o << "    v" << c << " = ws::uniform_view(c[" << c << "]); c" << c << " = (" << ctype_of(env.cols[c].t) << ")v" << c << ".p;\n";
-  for (int c : tr.encs) o << "    e" << c << " = ws::enc_view(v" << c << ");\n";
-  o << "  }\n";
-  o << "  __device__ int nkeys() const { return " << plan.nkeys << "; }\n";
-  o << "  __device__ int naggs() const { return " << plan.naggs << "; }\n";
-  o << "  __device__ bool has_where() const { return " << (plan.has_where ? "true" : "false") << "; }\n";
-  o << "  __device__ int op(int a) const { switch (a) {";
-  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (int)plan.ops[a] << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ bool agg_int(int a) const { switch (a) {";
-  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (plan.agg_int[a] ? "true" : "false") << ";";
-  o << " } return false; }\n";
-  o << "  __device__ bool agg_bits(int a) const { switch (a) {";
-  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (plan.agg_bits[a] ? "true" : "false") << ";";
-  o << " } return false; }\n";
-  o << "  __device__ int agg_part(int a) const { switch (a) {";
-  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (int)plan.agg_part[a] << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ int agg_null(int a) const { switch (a) {";
-  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << (int)plan.agg_null[a] << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ bool key_int(int k) const { switch (k) {";
-  for (int k = 0; k < plan.nkeys; ++k) o << " case " << k << ": return " << (plan.key_int[k] ? "true" : "false") << ";";
-  o << " } return false; }\n";
-  o << "  __device__ uint8_t dkind(int a, const ws_dense_spec&) const { switch (a) {";
-  for (int a = 0; a < plan.naggs; ++a) o << " case " << a << ": return " << dense_kind_of(plan.ops[a], plan.agg_int[a] != 0) << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ bool where(long long r) const { return " << (where.is_int ? where.e + " != 0" : "ws::truthy(" + where.e + ")")
-    << "; }\n";
-  // The WHERE split in a load half (load_row: the row's column words) and a compute half
-  // (where_row), for kernels that load a later step's rows before computing this one.
-  {
-    Translator trr{env, ncols, {}, {}, {}};
-    trr.row_mode = true;
-    const Node w2 = plan.has_where ? trr.run(plan.where) : Node{"1", true};
-    // (its sets, constants and encoded columns are among the main translation's members)
-    const bool split = plan.has_where && trr.split_ok && !trr.row_cols.empty();
-    o << "  static constexpr bool kSplit = " << (split ? "true" : "false") << ";\n";
-    o << "  struct Row {";
-    if (split)
-      for (int c : trr.row_cols) o << " " << (env.cols[c].enc ? "unsigned" : elem_of(env.cols[c].t)) << " v" << c << ";";
-    o << " int unused_; };\n";
-    o << "  __device__ void load_row(long long r, Row& x) const {";
-    if (split)
-      for (int c : trr.row_cols)
-        o << " x.v" << c << " = " << (env.cols[c].enc ? "ws::enc_word(e" + std::to_string(c) + ",r)" : "c" + std::to_string(c) + "[r]")
-          << ";";
-    o << " }\n";
-    // Four consecutive rows (r a multiple of 4) with 16-byte loads of raw columns: a wave's load
-    // instruction then carries 1 KiB; rows4_ok tells whether every raw column base is 16-byte
-    // aligned (encoded columns read their code words one by one).
-    o << "  __device__ bool rows4_ok() const { return true";
-    if (split)
-      for (int c : trr.row_cols)
-        if (!env.cols[c].enc) o << " && (((unsigned long long)c" << c << " & 15) == 0)";
-    o << "; }\n";
-    o << "  __device__ void load_rows4(long long r, Row* x) const {";
-    if (split)
-      for (int c : trr.row_cols) {
-        const ws_colview& v = env.cols[c];
-        const std::string n = std::to_string(c);
-        if (v.enc) {
-          for (int q = 0; q < 4; ++q) o << " x[" << q << "].v" << n << " = ws::enc_word(e" << n << ",r+" << q << ");";
-        } else if (v.t == WS_T_I32 || v.t == WS_T_I32N) {
-          o << " { const ws::ws_i32x4 q = *(const ws::ws_i32x4 WS_GLOBAL*)(c" << n << " + r); x[0].v" << n << " = q.x; x[1].v" << n
-            << " = q.y; x[2].v" << n << " = q.z; x[3].v" << n << " = q.w; }";
-        } else {
-          const char* vt = v.t == WS_T_F64 ? "ws::ws_f64x2" : "ws::ws_i64x2";
-          o << " { const " << vt << " q0 = *(const " << vt << " WS_GLOBAL*)(c" << n << " + r), q1 = *(const " << vt
-            << " WS_GLOBAL*)(c" << n << " + r + 2); x[0].v" << n << " = q0.x; x[1].v" << n << " = q0.y; x[2].v" << n
-            << " = q1.x; x[3].v" << n << " = q1.y; }";
-        }
-      }
-    o << " }\n";
-    o << "  __device__ bool where_row(const Row& x, long long r) const { return "
-      << (split ? (w2.is_int ? w2.e + " != 0" : "ws::truthy(" + w2.e + ")") : std::string("where(r)")) << "; }\n";
-  }
-  o << "  __device__ double key(int k, long long r) const { switch (k) {";
-  for (size_t k = 0; k < keys.size(); ++k) o << "\n    case " << k << ": return " << as_d(keys[k]) << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ long long ikey(int k, long long r) const { switch (k) {";
-  for (size_t k = 0; k < keys.size(); ++k) o << "\n    case " << k << ": return " << as_i(keys[k]) << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ double agg(int a, long long r) const { switch (a) {";
-  for (size_t a = 0; a < aggs.size(); ++a) o << "\n    case " << a << ": return " << as_d(aggs[a]) << ";";
-  o << " } return 0; }\n";
-  o << "  __device__ long long iagg(int a, long long r) const { switch (a) {";
-  for (size_t a = 0; a < aggs.size(); ++a) {
-    // A nullable int column: its raw value, NULL sentinel included (the kernel skips it).
-    std::string e = as_i(aggs[a]);
-    if (plan.agg_null[a]) e = tr.raw_int(env.code[plan.aggs[a].start].arg);
-    o << "\n    case " << a << ": return " << e << ";";
-  }
-  o << " } return 0; }\n};\n";
-  if (plan.agg_null[0] || plan.nkeys > WS_MAX_KEYS || plan.naggs > WS_MAX_AGGS)
-    throw Error("codegen: plan out of range", WS_ERR_INTERNAL);
-  for (int a = 0; a < plan.naggs; ++a)
-    if (plan.agg_null[a] && !(plan.aggs[a].len == 1 && env.code[plan.aggs[a].start].op == WS_OP_COL))
-      throw Error("codegen: nullable aggregate argument is not a column", WS_ERR_INTERNAL);
-  // (The columns named by agg_null arguments were added by the translator above.)
-  return o.str();
+void plan_file_store(const std::string& path, const std::string& key, const std::string& image) {
+  PlanHeader h{};
+  std::memcpy(h.magic, kPlanMagic, 8);
+  h.version = kPlanVersion;
+  h.header_bytes = sizeof h;
+  h.key_digest[0] = digest64(key.data(), key.size(), 1);
+  h.key_digest[1] = digest64(key.data(), key.size(), 2);
+  h.key_bytes = key.size();
+  h.image_bytes = image.size();
+  h.image_digest = digest64(image.data(), image.size(), 4);
+  h.header_digest = digest64(&h, offsetof(PlanHeader, header_digest), 3);
+  static std::atomic<uint64_t> seq{0};
+  const std::string tmp = path + ".part" + std::to_string((unsigned long long)::getpid()) + "." + std::to_string(seq++);
+  const int fd = ::open(tmp.c_str(), O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+  if (fd < 0) return;
+  const bool ok = write_all(fd, &h, sizeof h) && write_all(fd, key.data(), key.size()) && write_all(fd, image.data(), image.size());
+  ::close(fd);
+  if (!ok || ::rename(tmp.c_str(), path.c_str()) != 0) ::unlink(tmp.c_str());
 }
 
 struct PlanBuild {
@@ -545,7 +215,7 @@ PlanBuild plan_build(const ws_jit_program& p) {
     b.key = "hiprtc " + std::to_string(maj) + "." + std::to_string(min) + "\n";
     for (auto& o : b.opts) b.key += o + "\n";
     b.key += b.src;
-    b.path = disk_path(b.key);
+    b.path = plan_file(b.key);
   }
   return b;
 }
@@ -576,7 +246,7 @@ bool try_disk(ws_jit_program& p) {
   const auto t0 = std::chrono::steady_clock::now();
   PlanBuild b = plan_build(p);
   std::string image;
-  if (!disk_load(b.path, b.key, image)) return false;
+  if (!plan_file_load(b.path, b.key, image)) return false;
   try {
     load_image(p, image);
   } catch (const std::exception&) {
@@ -623,7 +293,7 @@ bool compile(ws_jit_program& p) {
     return false;
   }
   const double tc = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  if (!b.path.empty()) disk_store(b.path, b.key, image);
+  if (!b.path.empty()) plan_file_store(b.path, b.key, image);
   load_image(p, image);
   if (getenv("WARPSTORE_TRACE"))
     fprintf(stderr, "[codegen] plan #%d %s: hiprtc %.1f ms, %zu source bytes\n", p.id, kKernelName[p.fn_kind], tc, b.src.size());
@@ -673,7 +343,7 @@ void jit_launch_raw(void* fn, unsigned grid, unsigned block, size_t shmem, void*
 // Generates the evaluator and the full source for a plan without compiling (tests, tooling).
 std::string jit_source_for(const ws_expr_env& env, int ncols, const ws_agg_plan& plan, int fn) {
   int rows = 1;
-  return std::string(kKernelMacro[fn]) + "\n" + gen_evaluator(env, ncols, plan, &rows);
+  return std::string(kKernelMacro[fn]) + "\n" + codegen::evaluator_source(env, ncols, plan, &rows);
 }
 
 }  // namespace ws
@@ -699,8 +369,7 @@ void ws_jit_set_cache_dir(const char* dir) {
 
 int ws_prog_is_int(const ws_expr_env* env, ws_prog p, int* out) {
   return guard([&] {
-    Translator tr{*env, WS_MAX_COLS, {}, {}, {}};
-    *out = tr.run(p).is_int ? 1 : 0;
+    *out = codegen::program_is_int(*env, p) ? 1 : 0;
   });
 }
 
@@ -721,7 +390,7 @@ int ws_jit_get(const ws_expr_env* env, int ncols, const ws_agg_plan* plan, int f
     }
     if (!p) {
       int rows = 1;
-      const std::string ev = gen_evaluator(*env, ncols, *plan, &rows);
+      const std::string ev = codegen::evaluator_source(*env, ncols, *plan, &rows);
       const std::string key = std::string(kKernelMacro[fn]) + "\n" + ev;
       {
         std::lock_guard<std::mutex> g(g_mu);
@@ -858,7 +527,7 @@ int ws_jit_compile_check(const ws_expr_env* env, int ncols, const ws_agg_plan* p
     ws_jit_program p;
     p.fn_kind = fn;
     int rows = 1;
-    p.source = gen_evaluator(*env, ncols, *plan, &rows);
+    p.source = codegen::evaluator_source(*env, ncols, *plan, &rows);
     PlanBuild b;
     b.src = std::string(kWsJitPrelude) + "\n" + p.source + kKernelMacro[fn] + "(QEval, " + kKernelName[fn] + ")\n";
     b.opts = {std::string("--offload-arch=") + (arch ? arch : "gfx950"), "-O3", "-std=c++17", "-munsafe-fp-atomics",

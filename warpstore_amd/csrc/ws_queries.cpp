@@ -29,7 +29,6 @@
 #include "warpstore_amd/gpu.h"
 #include "warpstore_amd/jit.h"
 
-extern "C" int wsg_encode_int(const void* x, int i64, int64_t n, uint8_t* out);
 extern "C" void wsg_c4_shapes(uint64_t seed, int64_t nq, int64_t* rows, int64_t* distinct, int64_t* theta);
 extern "C" void wsg_c4_data(uint64_t seed, int64_t q, int64_t rows, int64_t distinct, int32_t* key, int64_t* val);
 extern "C" int wsq_table_load(int ncols, const int* types, const int* encode, int64_t n, const void* const* host, void** out);
@@ -965,10 +964,13 @@ int wsq_table_load(int ncols, const int* types, const int* encode, int64_t n, co
         }
         SegCol sc;
         if (encode && encode[c]) {
-          std::vector<uint8_t> enc((size_t)(WS_ENC_HEADER + rows * w + 16));
-          sc.enc = (uint8_t)wsg_encode_int(src, w == 8, rows, enc.data());
-          if (!sc.enc) throw SqlError("column does not fit an encoding", 1105);
-          const size_t bytes = (size_t)(WS_ENC_HEADER + rows * (sc.enc == 1 ? 1 : sc.enc == 2 ? 2 : 4) + 16);
+          std::vector<uint8_t> enc(ws_encoded_bytes_max(rows));
+          int e = 0;
+          size_t bytes = 0;
+          wsd::ck(ws_encode_column(src, col.t == WS_T_F64 ? WS_P_F64 : w == 8 ? WS_P_I64 : WS_P_I32, rows, enc.data(), enc.size(),
+                                   &e, &bytes));
+          if (!e) throw SqlError("column does not fit an encoding", 1105);
+          sc.enc = (uint8_t)e;
           sc.dev = devAlloc(bytes);
           h2d(sc.dev, enc.data(), bytes);
         } else {

@@ -144,7 +144,8 @@ WS_HD i64 mul_chk(i64 a, i64 b, int* f) {
 }
 
 // ---- three-valued helpers (NULL = NaN; devtypes.h:247-304)
-WS_HD double cmp3(int op, double a, double b) {  // op: 0 =, 1 <>, 2 <, 3 <=, 4 >, 5 >=
+// Comparison k of WS_OP_EQ + k (equal, unequal, less, less-or-equal, greater, greater-or-equal).
+WS_HD double tri_cmp(int op, double a, double b) {
   if (a != a || b != b) return nullv();
   bool r;
   switch (op) {
@@ -157,20 +158,20 @@ WS_HD double cmp3(int op, double a, double b) {  // op: 0 =, 1 <>, 2 <, 3 <=, 4 
   }
   return r ? 1.0 : 0.0;
 }
-WS_HD double and3(double a, double b) {
+WS_HD double tri_and(double a, double b) {
   const bool af = a == a && a == 0, bf = b == b && b == 0;
   if (af || bf) return 0.0;
   return (a != a || b != b) ? nullv() : 1.0;
 }
-WS_HD double or3(double a, double b) {
+WS_HD double tri_or(double a, double b) {
   const bool at = a == a && a != 0, bt = b == b && b != 0;
   if (at || bt) return 1.0;
   return (a != a || b != b) ? nullv() : 0.0;
 }
-WS_HD double not3(double a) { return a != a ? a : (a == 0 ? 1.0 : 0.0); }
-WS_HD double coalesce2(double a, double b) { return a != a ? b : a; }
-WS_HD double nullif2(double a, double b) { return a == b ? nullv() : a; }
-WS_HD double divz(double a, double b) { return b == 0 ? nullv() : a / b; }  // x / 0 = NULL
+WS_HD double tri_not(double a) { return a != a ? a : (a == 0 ? 1.0 : 0.0); }
+WS_HD double first_non_null(double a, double b) { return a != a ? b : a; }
+WS_HD double null_if_equal(double a, double b) { return a == b ? nullv() : a; }
+WS_HD double div_or_null(double a, double b) { return b == 0 ? nullv() : a / b; }  // x / 0 = NULL
 WS_HD i64 inset_i(const uint32_t* bits, int size, i64 c) {
   return (c >= 0 && c < size) ? (i64)((bits[c >> 5] >> (c & 31)) & 1u) : 0;
 }
@@ -350,23 +351,23 @@ WS_HD double eval_prog_t(const ws_expr_env& env, const ws_colview* cols, ws_prog
       case WS_OP_ADD: --sp; st[sp - 1] += st[sp]; break;
       case WS_OP_SUB: --sp; st[sp - 1] -= st[sp]; break;
       case WS_OP_MUL: --sp; st[sp - 1] *= st[sp]; break;
-      case WS_OP_DIV: --sp; st[sp - 1] = divz(st[sp - 1], st[sp]); break;
+      case WS_OP_DIV: --sp; st[sp - 1] = div_or_null(st[sp - 1], st[sp]); break;
       case WS_OP_MOD: --sp; st[sp - 1] = kFull ? fmod(st[sp - 1], st[sp]) : nullv(); break;
       case WS_OP_NEG: st[sp - 1] = -st[sp - 1]; break;
       case WS_OP_EQ: case WS_OP_NE: case WS_OP_LT: case WS_OP_LE: case WS_OP_GT: case WS_OP_GE:
         --sp;
-        st[sp - 1] = cmp3(in.op - WS_OP_EQ, st[sp - 1], st[sp]);
+        st[sp - 1] = tri_cmp(in.op - WS_OP_EQ, st[sp - 1], st[sp]);
         break;
-      case WS_OP_AND: --sp; st[sp - 1] = and3(st[sp - 1], st[sp]); break;
-      case WS_OP_OR: --sp; st[sp - 1] = or3(st[sp - 1], st[sp]); break;
-      case WS_OP_NOT: st[sp - 1] = not3(st[sp - 1]); break;
+      case WS_OP_AND: --sp; st[sp - 1] = tri_and(st[sp - 1], st[sp]); break;
+      case WS_OP_OR: --sp; st[sp - 1] = tri_or(st[sp - 1], st[sp]); break;
+      case WS_OP_NOT: st[sp - 1] = tri_not(st[sp - 1]); break;
       case WS_OP_INSET: st[sp - 1] = inset_d(env.sets[in.arg], env.set_size[in.arg], st[sp - 1]); break;
       case WS_OP_SELECT:
         sp -= 2;
         st[sp - 1] = truthy(st[sp - 1]) ? st[sp] : st[sp + 1];
         break;
       case WS_OP_ISNULL: st[sp - 1] = st[sp - 1] != st[sp - 1] ? 1.0 : 0.0; break;
-      case WS_OP_COALESCE: --sp; st[sp - 1] = coalesce2(st[sp - 1], st[sp]); break;
+      case WS_OP_COALESCE: --sp; st[sp - 1] = first_non_null(st[sp - 1], st[sp]); break;
       case WS_OP_NULLIF: --sp; if (st[sp - 1] == st[sp]) st[sp - 1] = nullv(); break;
       case WS_OP_CALL1: st[sp - 1] = kFull ? call1(in.arg, st[sp - 1]) : nullv(); break;
       case WS_OP_CALL2: --sp; st[sp - 1] = kFull ? call2(in.arg, st[sp - 1], st[sp]) : nullv(); break;

@@ -55,6 +55,18 @@ double from_bits(i64 v) {
   return d;
 }
 
+// Double -> int64 truncation as x86-64 performs it (cvttsd2si: out of range or NaN -> INT64_MIN).
+// C++ leaves the out-of-range case undefined and other processors answer differently, so every
+// such conversion is recorded in g_out_of_range: a result that depended on one is not portable.
+thread_local int g_out_of_range = 0;
+i64 to_i64(double x) {
+  if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) {
+    g_out_of_range = 1;
+    return I64MIN;
+  }
+  return (i64)x;
+}
+
 // ------------------------------------------------------------ dates (Hinnant, public domain)
 i64 civil_to_days(i64 y, unsigned m, unsigned d) {
   if (m <= 2) y -= 1;
@@ -179,7 +191,7 @@ double fn1(int f, double x) {
     case WS_FN_SIGN: return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0);
     default: break;
   }
-  xi = (i64)x;
+  xi = to_i64(x);
   switch (f) {
     case WS_FN_MONTH: days_to_civil(xi, y, m, d); return m;
     case WS_FN_DAY: days_to_civil(xi, y, m, d); return d;
@@ -217,15 +229,15 @@ double fn2(int f, double a, double b) {
     case WS_FN_ATAN2: return std::atan2(a, b);
     case WS_FN_LOG_B: return (a <= 0 || b <= 0) ? NaN() : std::log(b) / std::log(a);
     case WS_FN_IDIV: return b == 0 ? NaN() : std::trunc(a / b);
-    case WS_FN_ADD_MONTHS: return (double)add_months((i64)a, (i64)b);
+    case WS_FN_ADD_MONTHS: return (double)add_months(to_i64(a), to_i64(b));
     case WS_FN_DT_ADD_MONTHS: {
-      i64 us = (i64)a, days = fdiv(us, 86400000000ll);
-      return (double)(add_months(days, (i64)b) * 86400000000ll + (us - days * 86400000000ll));
+      i64 us = to_i64(a), days = fdiv(us, 86400000000ll);
+      return (double)(add_months(days, to_i64(b)) * 86400000000ll + (us - days * 86400000000ll));
     }
     case WS_FN_DATE_TRUNC_MONTH: {
       i64 y;
       unsigned m, d;
-      days_to_civil((i64)a, y, m, d);
+      days_to_civil(to_i64(a), y, m, d);
       return (double)civil_to_days(y, b > 0 ? 1 : m, 1);
     }
   }
@@ -282,7 +294,7 @@ double run(const ws_expr_env& env, const ws_colview* cols, ws_prog pg, i64 row) 
       case WS_OP_INSET: {
         double x = pop();
         if (std::isnan(x)) { st.push_back(x); break; }
-        i64 c = (i64)x;
+        i64 c = to_i64(x);
         const uint32_t* bits = env.sets[in.arg];
         st.push_back(c >= 0 && c < env.set_size[in.arg] ? (double)((bits[c >> 5] >> (c & 31)) & 1u) : 0.0);
         break;
@@ -297,7 +309,7 @@ double run(const ws_expr_env& env, const ws_colview* cols, ws_prog pg, i64 row) 
         if (std::isnan(x)) { st.push_back(x); break; }
         i64 y;
         unsigned m, d;
-        days_to_civil((i64)x, y, m, d);
+        days_to_civil(to_i64(x), y, m, d);
         st.push_back((double)y);
         break;
       }
@@ -309,7 +321,7 @@ double run(const ws_expr_env& env, const ws_colview* cols, ws_prog pg, i64 row) 
       case WS_OP_DICTMAP: {
         double x = pop();
         if (std::isnan(x)) { st.push_back(x); break; }
-        i64 c = (i64)x;
+        i64 c = to_i64(x);
         st.push_back(c >= 0 && c < env.map_size[in.arg] ? env.maps[in.arg][c] : NaN());
         break;
       }
@@ -341,6 +353,214 @@ double run(const ws_expr_env& env, const ws_colview* cols, ws_prog pg, i64 row) 
   return st.empty() ? NaN() : st[0];
 }
 
+// ------------------------------------------------------------ compiled-plan (typed) evaluation
+// The per-query compiler types every subexpression as exact int64 or as double (NULL = NaN) and
+// evaluates int64 nodes in integer arithmetic (reference src/jit.cu:107-272): integer columns and
+// literals are int64; + - * of two int64 wrap (and raise the overflow flag); comparisons, AND /
+// OR / NOT, IN-set, CASE, YEAR and COALESCE of int64 operands stay int64; a DECIMAL's raw stored
+// integer is int64 unless read as a nullable value; YEAR of a double goes through the datetime
+// functions (days -> microseconds -> year). Everything else is the VM's double operator. On most
+// rows this equals the VM's value; it differs where double arithmetic cannot be exact (int64
+// operands or results beyond 2^53, wrapped overflow), where a raw NULL sentinel is read as an
+// integer, or where YEAR's datetime path truncates differently. Each node checks its typed value
+// against the VM operator on the same operands and classifies a difference:
+//   kTypedExplained:   the difference has one of the causes above (checked from the operands);
+//   kTypedUnexplained: a difference without such a cause;
+//   kTypedNotPortable: the value went through an out-of-range double -> int64 conversion.
+enum { kTypedExplained = 1, kTypedUnexplained = 2, kTypedNotPortable = 4 };
+struct TV {
+  bool is_int;
+  i64 i;
+  double d;
+  double as_d() const { return is_int ? (double)i : d; }
+};
+constexpr double kExactLimit = 9007199254740992.0;  // 2^53
+bool beyond_exact(i64 v) { return v > (i64)kExactLimit || v < -(i64)kExactLimit; }
+bool same_double(double a, double b) { return (std::isnan(a) && std::isnan(b)) || bits_of(a) == bits_of(b); }
+
+TV typed_run(const ws_expr_env& env, const ws_colview* cols, ws_prog pg, i64 row, int* flags) {
+  std::vector<TV> st;
+  auto pop = [&] {
+    TV v = st.back();
+    st.pop_back();
+    return v;
+  };
+  auto I = [](i64 v) { return TV{true, v, 0.0}; };
+  auto D = [](double v) { return TV{false, 0, v}; };
+  auto differ = [&](double typed, double vm, bool cause) {
+    if (!same_double(typed, vm)) *flags |= cause ? kTypedExplained : kTypedUnexplained;
+  };
+  for (int pc = pg.start; pc < pg.start + pg.len; ++pc) {
+    const ws_instr in = env.code[pc];
+    switch (in.op) {
+      case WS_OP_COL: {
+        const ws_colview& c = cols[in.arg];
+        if (c.t == WS_T_I32 || c.t == WS_T_I64) st.push_back(I(stored_int(c, row)));
+        else st.push_back(D(value(c, row)));
+        break;
+      }
+      case WS_OP_COLRAW: {
+        const ws_colview& c = cols[in.arg];
+        const i64 v = stored_int(c, row);
+        if (c.t == WS_T_DECIMAL && !in.is_int) {
+          st.push_back(D(v == (i64)WS_NULL_I64 ? NaN() : (double)v));
+        } else {
+          st.push_back(I(v));
+          differ((double)v, v == (i64)WS_NULL_I64 ? NaN() : (double)v, v == (i64)WS_NULL_I64);
+        }
+        break;
+      }
+      case WS_OP_CONST: st.push_back(in.is_int ? I(bits_of(env.consts[in.arg])) : D(env.consts[in.arg])); break;
+      case WS_OP_ADD: case WS_OP_SUB: case WS_OP_MUL: {
+        TV b = pop(), a = pop();
+        const double x = a.as_d(), y = b.as_d();
+        const double vm = in.op == WS_OP_ADD ? x + y : in.op == WS_OP_SUB ? x - y : x * y;
+        if (a.is_int && b.is_int) {
+          const __int128 exact = in.op == WS_OP_ADD ? (__int128)a.i + b.i : in.op == WS_OP_SUB ? (__int128)a.i - b.i
+                                                                                              : (__int128)a.i * b.i;
+          const i64 r = (i64)(u64)(unsigned __int128)exact;
+          const bool inexact = beyond_exact(a.i) || beyond_exact(b.i) || exact > (__int128)kExactLimit ||
+                               exact < -(__int128)kExactLimit;
+          // an integer zero has no sign: 0 * -5 is +0 in int64 and -0.0 in doubles
+          differ((double)r, vm, inexact || (r == 0 && vm == 0));
+          st.push_back(I(r));
+        } else {
+          st.push_back(D(vm));
+        }
+        break;
+      }
+      case WS_OP_DIV: { TV b = pop(), a = pop(); st.push_back(D(b.as_d() == 0 ? NaN() : a.as_d() / b.as_d())); break; }
+      case WS_OP_MOD: { TV b = pop(), a = pop(); st.push_back(D(std::fmod(a.as_d(), b.as_d()))); break; }
+      case WS_OP_NEG: {
+        TV a = pop();
+        if (a.is_int) {
+          const i64 r = (i64)(0ull - (u64)a.i);
+          differ((double)r, -(double)a.i, a.i == I64MIN || a.i == 0);  // wraps; -0 is +0 in int64
+          st.push_back(I(r));
+        } else {
+          st.push_back(D(-a.d));
+        }
+        break;
+      }
+      case WS_OP_EQ: case WS_OP_NE: case WS_OP_LT: case WS_OP_LE: case WS_OP_GT: case WS_OP_GE: {
+        TV b = pop(), a = pop();
+        auto cmp = [&](auto x, auto y) {
+          return in.op == WS_OP_EQ ? x == y : in.op == WS_OP_NE ? x != y : in.op == WS_OP_LT ? x < y
+               : in.op == WS_OP_LE ? x <= y : in.op == WS_OP_GT ? x > y : x >= y;
+        };
+        const double x = a.as_d(), y = b.as_d();
+        const double vm = std::isnan(x) || std::isnan(y) ? NaN() : tv(cmp(x, y));
+        if (a.is_int && b.is_int) {
+          const i64 r = cmp(a.i, b.i) ? 1 : 0;
+          differ((double)r, vm, beyond_exact(a.i) || beyond_exact(b.i));
+          st.push_back(I(r));
+        } else {
+          st.push_back(D(vm));
+        }
+        break;
+      }
+      case WS_OP_AND: case WS_OP_OR: {
+        TV b = pop(), a = pop();
+        if (a.is_int && b.is_int) {
+          st.push_back(I(in.op == WS_OP_AND ? (a.i != 0 && b.i != 0) : (a.i != 0 || b.i != 0)));
+          break;
+        }
+        const double x = a.as_d(), y = b.as_d();
+        if (in.op == WS_OP_AND) {
+          if ((!std::isnan(x) && x == 0) || (!std::isnan(y) && y == 0)) st.push_back(D(0.0));
+          else st.push_back(D(std::isnan(x) || std::isnan(y) ? NaN() : 1.0));
+        } else {
+          if ((!std::isnan(x) && x != 0) || (!std::isnan(y) && y != 0)) st.push_back(D(1.0));
+          else st.push_back(D(std::isnan(x) || std::isnan(y) ? NaN() : 0.0));
+        }
+        break;
+      }
+      case WS_OP_NOT: {
+        TV a = pop();
+        st.push_back(a.is_int ? I(a.i == 0) : D(std::isnan(a.d) ? a.d : tv(a.d == 0)));
+        break;
+      }
+      case WS_OP_INSET: {
+        TV a = pop();
+        const uint32_t* bits = env.sets[in.arg];
+        const i64 n = env.set_size[in.arg];
+        auto bit = [&](i64 c) { return c >= 0 && c < n ? (i64)((bits[c >> 5] >> (c & 31)) & 1u) : 0; };
+        if (a.is_int) {
+          const i64 r = bit(a.i);
+          const int saved = g_out_of_range;
+          differ((double)r, (double)bit(to_i64((double)a.i)), beyond_exact(a.i));
+          g_out_of_range = saved;
+          st.push_back(I(r));
+        } else {
+          st.push_back(D(std::isnan(a.d) ? a.d : (double)bit(to_i64(a.d))));
+        }
+        break;
+      }
+      case WS_OP_SELECT: {
+        TV e = pop(), t = pop(), c = pop();
+        const bool cond = c.is_int ? c.i != 0 : (!std::isnan(c.d) && c.d != 0);
+        if (t.is_int && e.is_int) st.push_back(cond ? t : e);
+        else st.push_back(D(cond ? t.as_d() : e.as_d()));
+        break;
+      }
+      case WS_OP_YEAR: {
+        TV a = pop();
+        i64 y;
+        unsigned m, d;
+        if (a.is_int) {
+          days_to_civil(a.i, y, m, d);
+          const i64 r = y;
+          const int saved = g_out_of_range;
+          days_to_civil(to_i64((double)a.i), y, m, d);
+          g_out_of_range = saved;
+          differ((double)r, (double)y, beyond_exact(a.i));
+          st.push_back(I(r));
+        } else if (std::isnan(a.d)) {
+          st.push_back(a);
+        } else {
+          const int saved = g_out_of_range;
+          g_out_of_range = 0;
+          const double r = fn1(WS_FN_DT_YEAR, fn1(WS_FN_DATE_TO_DT, a.d));
+          const bool out_of_range = g_out_of_range;
+          i64 vm_year = 0;
+          if (a.d >= -9223372036854775808.0 && a.d < 9223372036854775808.0) {
+            days_to_civil((i64)a.d, y, m, d);
+            vm_year = y;
+          }
+          g_out_of_range = saved || out_of_range;
+          differ(r, (double)vm_year, out_of_range || a.d != std::trunc(a.d));
+          st.push_back(D(r));
+        }
+        break;
+      }
+      case WS_OP_ISNULL: { TV a = pop(); st.push_back(I(a.is_int ? 0 : std::isnan(a.d))); break; }
+      case WS_OP_COALESCE: {
+        TV b = pop(), a = pop();
+        if (a.is_int) st.push_back(a);
+        else st.push_back(D(std::isnan(a.d) ? b.as_d() : a.d));
+        break;
+      }
+      case WS_OP_NULLIF: { TV b = pop(), a = pop(); st.push_back(D(a.as_d() == b.as_d() ? NaN() : a.as_d())); break; }
+      case WS_OP_CALL1: { TV a = pop(); st.push_back(D(fn1(in.arg, a.as_d()))); break; }
+      case WS_OP_CALL2: { TV b = pop(), a = pop(); st.push_back(D(fn2(in.arg, a.as_d(), b.as_d()))); break; }
+      default: {
+        // DICTMAP, INSORTED, VDOT, VEUC: double operators on double operands, as in the VM
+        ws_expr_env one = env;
+        const int nin = (in.op == WS_OP_VDOT || in.op == WS_OP_VEUC) ? 0 : 1;
+        double arg = 0;
+        if (nin) arg = pop().as_d();
+        one.consts[0] = arg;
+        one.code[0] = ws_instr{WS_OP_CONST, 0, 0};
+        one.code[1] = in;
+        st.push_back(D(run(one, cols, ws_prog{(int16_t)(nin ? 0 : 1), (int16_t)(nin + 1)}, row)));
+        break;
+      }
+    }
+  }
+  if (g_out_of_range) *flags |= kTypedNotPortable;
+  return st.empty() ? TV{false, 0, NaN()} : st[0];
+}
+
 bool is_col(const ws_expr_env& env, ws_prog p, int* slot) {
   if (p.len == 1 && env.code[p.start].op == WS_OP_COL) {
     *slot = env.code[p.start].arg;
@@ -353,7 +573,7 @@ bool is_col(const ws_expr_env& env, ws_prog p, int* slot) {
 i64 int_value(const ws_expr_env& env, const ws_colview* cols, ws_prog p, i64 r) {
   int s;
   if (is_col(env, p, &s) && cols[s].t != WS_T_F64) return stored_int(cols[s], r);
-  return (i64)run(env, cols, p, r);
+  return to_i64(run(env, cols, p, r));
 }
 
 // ------------------------------------------------------------ aggregation
@@ -388,6 +608,15 @@ extern "C" {
 
 double or_eval_prog(const ws_expr_env* env, const ws_colview* cols, ws_prog p, int64_t row) {
   return run(*env, cols ? cols : env->cols, p, row);
+}
+// A program's value as a compiled plan computes it (int64 nodes in integer arithmetic, see
+// typed_run); *flags gets kTypedExplained / kTypedUnexplained / kTypedNotPortable.
+double or_eval_typed(const ws_expr_env* env, const ws_colview* cols, ws_prog p, int64_t row, int* flags) {
+  g_out_of_range = 0;
+  *flags = 0;
+  const double v = typed_run(*env, cols ? cols : env->cols, p, row, flags).as_d();
+  g_out_of_range = 0;
+  return v;
 }
 int64_t or_stored_int(const ws_colview* c, int64_t r) { return stored_int(*c, r); }
 double or_stored_dbl(const ws_colview* c, int64_t r) { return stored_dbl(*c, r); }

@@ -45,6 +45,61 @@ def test_vm_matches_reference_golden(golden):
     assert not bad, f"{len(bad)} mismatches, first: {bad[:3]}"
 
 
+def test_typed_evaluation_matches_golden_where_exact(golden):
+    """The compiled-plan (typed) restatement equals the reference VM's golden value on every row
+    where no int64 node changed the value; every row where one did has a cause the double VM
+    cannot represent (operands or results beyond 2^53, wrapped overflow, a raw NULL sentinel read
+    as an integer, YEAR of a fractional or out-of-range double through the datetime path). A
+    mistyped or mis-dispatched operator shows up as an unexplained difference or as a mismatch."""
+    O = H.oracle()
+    O.or_eval_typed.restype = C.c_double
+    O.or_eval_typed.argtypes = [C.POINTER(abi.ExprEnv), C.c_void_p, abi.Prog, C.c_int64, C.POINTER(C.c_int)]
+    bad, explained, exact = [], 0, 0
+    for i, case in enumerate(golden["vm"]):
+        code = [tuple(c) for c in case["code"]]
+        c = Case(case["cols"], code, case["sets"], case["maps"])
+        for r in range(NROWS):
+            fl = C.c_int()
+            got = O.or_eval_typed(C.byref(c.env), None, c.prog, r, C.byref(fl))
+            want = struct.unpack("<d", bytes.fromhex(case["expect"][r]))[0]
+            if fl.value & 2:
+                bad.append(("unexplained", i, r, fbits(got), case["expect"][r], code))
+            elif fl.value & 1:
+                explained += 1
+            elif not ((np.isnan(got) and np.isnan(want)) or fbits(got) == case["expect"][r]):
+                bad.append(("mismatch", i, r, fbits(got), case["expect"][r], code))
+            else:
+                exact += 1
+    assert not bad, f"{len(bad)} bad rows, first: {bad[:3]}"
+    assert exact > 0.9 * len(golden["vm"]) * NROWS, (exact, explained)
+
+
+def test_typed_evaluation_closed_forms():
+    """The typed rules on their own: int64 + - * wrap (2^62 * 4 -> 0), NEG of INT64_MIN wraps,
+    comparisons beyond 2^53 are exact, a raw DECIMAL sentinel read as an integer is the sentinel,
+    YEAR of an integer day is the civil year."""
+    O = H.oracle()
+    O.or_eval_typed.restype = C.c_double
+    O.or_eval_typed.argtypes = [C.POINTER(abi.ExprEnv), C.c_void_p, abi.Prog, C.c_int64, C.POINTER(C.c_int)]
+    big = np.array([2 ** 62, -2 ** 63, 2 ** 53 + 1, 19723], np.int64)
+    dec = np.array([abi.NULL_I64, 5, 7, 9], np.int64)
+    b = abi.EnvBuilder()
+    sb, sd = b.slot(abi.T_I64), b.slot(abi.T_DECIMAL, 2)
+    b.env.cols[sb].p, b.env.cols[sd].p = big.ctypes.data, dec.ctypes.data
+
+    def ev(prog, row):
+        fl = C.c_int()
+        return O.or_eval_typed(C.byref(b.env), None, prog, row, C.byref(fl)), fl.value
+
+    assert ev(b.prog([(abi.OP_COL, sb), b.ilit(4), (abi.OP_MUL,)]), 0) == (0.0, 1)  # wraps; the VM gives 2^64
+    assert ev(b.prog([(abi.OP_COL, sb), (abi.OP_NEG,)]), 1) == (float(-2 ** 63), 1)
+    assert ev(b.prog([(abi.OP_COL, sb), b.ilit(2 ** 53), (abi.OP_GT,)]), 2) == (1.0, 1)  # doubles say equal
+    assert ev(b.prog([(abi.OP_COLRAW, sd, 1)]), 0) == (float(abi.NULL_I64), 1)
+    assert ev(b.prog([(abi.OP_COLRAW, sd, 0)]), 0)[0] != ev(b.prog([(abi.OP_COLRAW, sd, 0)]), 0)[0]  # NULL
+    assert ev(b.prog([(abi.OP_COL, sb), (abi.OP_YEAR,)]), 3) == (2024.0, 0)
+    assert ev(b.prog([(abi.OP_COL, sb), b.ilit(3), (abi.OP_ADD,)]), 3) == (19726.0, 0)
+
+
 def test_vm_matches_reference_harness_live():
     R = H.ref_lib()
     if R is None:
@@ -277,6 +332,148 @@ def test_select_join_sort_bruteforce():
     n1 = np.where(k1 == 0, 0.0, k1)
     exp = np.lexsort((np.arange(m), n1, -n0))
     assert np.array_equal(perm, exp)
+
+
+def _and3(a, b):
+    """SQL AND over {1, 0, NaN = NULL}: false wins, then NULL."""
+    return np.where((a == 0) | (b == 0), 0.0, np.where(np.isnan(a) | np.isnan(b), np.nan, 1.0))
+
+
+def _or3(a, b):
+    return np.where((a == 1) | (b == 1), 1.0, np.where(np.isnan(a) | np.isnan(b), np.nan, 0.0))
+
+
+def _cmp3(mask, *nulls):
+    out = mask.astype(np.float64)
+    for n in nulls:
+        out[n] = np.nan
+    return out
+
+
+def test_select_matches_numpy_mask():
+    """or_select against a numpy three-valued mask and np.nonzero: ragged batches with an empty
+    one, deleted rows, NULL sentinels (I64N, I32N) and NaN doubles, AND / OR / NOT over NULLs."""
+    O = H.oracle()
+    rng = np.random.default_rng(31)
+    n = 30_000
+    k = rng.integers(0, 10, n).astype(np.int32)
+    k[rng.random(n) < 0.07] = abi.NULL_I32
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    v[rng.random(n) < 0.1] = abi.NULL_I64
+    x = rng.standard_normal(n)
+    x[rng.random(n) < 0.1] = np.nan
+    deleted = rng.random(n) < 0.2
+    sizes = [9000, 0, 12345, 8655]
+    t = H.Table({"k": k, "v": v, "x": x}, sizes, deleted)
+    host = H.Bound(t, [("k", abi.T_I32N, 0), ("v", abi.T_I64N, 0), ("x", abi.T_F64, 0)], False)
+    b = abi.EnvBuilder()
+    sk, sv, sx = b.slot(abi.T_I32N), b.slot(abi.T_I64N), b.slot(abi.T_F64)
+    # WHERE (x < 0.5 AND v > -200) OR NOT (k <> 3)
+    where = b.prog([(abi.OP_COL, sx), b.dlit(0.5), (abi.OP_LT,), (abi.OP_COL, sv), b.ilit(-200), (abi.OP_GT,), (abi.OP_AND,),
+                    (abi.OP_COL, sk), b.ilit(3), (abi.OP_NE,), (abi.OP_NOT,), (abi.OP_OR,)])
+    xl = _cmp3(x < 0.5, np.isnan(x))
+    vg = _cmp3(v > -200, v == abi.NULL_I64)
+    kne = _cmp3(k != 3, k == abi.NULL_I32)
+    knot = np.where(np.isnan(kne), np.nan, 1.0 - kne)
+    res = _or3(_and3(xl, vg), knot)
+    for has_where, sel in ((1, (res == 1) & ~deleted), (0, ~deleted)):
+        out = np.zeros(n, np.int32)
+        per = np.zeros(len(sizes), np.int64)
+        m = O.or_select(C.byref(b.env), has_where, where, len(sizes), ws.ptr(np.array(sizes, np.int64)),
+                        C.cast(host.dels, C.c_void_p), C.cast(host.views, C.c_void_p), host.nslots, ws.ptr(out), ws.ptr(per))
+        exp = np.nonzero(sel)[0]
+        assert np.array_equal(out[:m], exp)
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        assert per.tolist() == [int(sel[offs[i]:offs[i + 1]].sum()) for i in range(len(sizes))]
+    # a NULL WHERE (NaN) never selects; a non-zero non-NaN value always does
+    b = abi.EnvBuilder()
+    sx = b.slot(abi.T_F64)
+    t2 = H.Table({"x": np.array([np.nan, 0.0, -0.0, 2.5, -1e-300])}, 5)
+    h2 = H.Bound(t2, [("x", abi.T_F64, 0)], False)
+    out = np.zeros(5, np.int32)
+    m = O.or_select(C.byref(b.env), 1, b.col(sx), 1, ws.ptr(np.array([5], np.int64)), C.cast(h2.dels, C.c_void_p),
+                    C.cast(h2.views, C.c_void_p), 1, ws.ptr(out), None)
+    assert out[:m].tolist() == [3, 4]
+
+
+def test_remap_codes_matches_numpy():
+    """or_remap_codes against np.where: NULL stays NULL, codes outside [0, len(map)) -> -2."""
+    O = H.oracle()
+    rng = np.random.default_rng(32)
+    cmap = rng.integers(-2, 500, 300).astype(np.int32)
+    codes = rng.integers(-5, 320, 10_000).astype(np.int32)
+    codes[::11] = abi.NULL_I32
+    out = np.zeros_like(codes)
+    O.or_remap_codes(ws.ptr(codes), len(codes), ws.ptr(cmap), len(cmap), ws.ptr(out))
+    inside = (codes >= 0) & (codes < len(cmap))
+    exp = np.where(codes == abi.NULL_I32, abi.NULL_I32, np.where(inside, cmap[np.clip(codes, 0, len(cmap) - 1)], -2))
+    assert np.array_equal(out, exp)
+    O.or_remap_codes(ws.ptr(codes), len(codes), ws.ptr(cmap), 0, ws.ptr(out))  # empty map: everything absent
+    assert np.array_equal(out, np.where(codes == abi.NULL_I32, abi.NULL_I32, -2))
+
+
+def test_aggregate_dense_matches_bruteforce():
+    """or_aggregate(dense=1) against Python-integer brute force: COUNT; integer SUM without
+    agg_bits as the double of the exact int64 sum (values near 2^56, so sums pass 2^53 and the
+    rounding matters); split BIGINT SUM halves; integer MIN / MAX as doubles (no agg_bits) and as
+    exact bits (agg_bits); double SUM / MIN / MAX skipping NaN; deleted rows; two keys."""
+    rng = np.random.default_rng(33)
+    n = 12_000
+    k0 = rng.integers(3, 9, n).astype(np.int32)
+    k1 = rng.integers(-2, 2, n).astype(np.int64)
+    v = rng.integers(2 ** 55, 2 ** 56, n).astype(np.int64)
+    v[rng.random(n) < 0.5] *= -1
+    w = rng.integers(-2 ** 40, 2 ** 40, n).astype(np.int64)
+    w[rng.random(n) < 0.1] = abi.NULL_I64
+    x = rng.standard_normal(n)
+    x[rng.random(n) < 0.1] = np.nan
+    deleted = rng.random(n) < 0.1
+    t = H.Table({"k0": k0, "k1": k1, "v": v, "w": w, "x": x}, [5000, 7000], deleted)
+    host = H.Bound(t, [("k0", abi.T_I32, 0), ("k1", abi.T_I64, 0), ("v", abi.T_I64, 0), ("w", abi.T_I64N, 0),
+                       ("x", abi.T_F64, 0)], False)
+    b = abi.EnvBuilder()
+    s0, s1, sv, sw, sx = (b.slot(tc) for tc in (abi.T_I32, abi.T_I64, abi.T_I64, abi.T_I64N, abi.T_F64))
+    plan = abi.AggPlan()
+    plan.nkeys = 2
+    plan.keys[0], plan.keys[1] = b.col(s0), b.col(s1)
+    plan.key_int[0] = plan.key_int[1] = 1
+    # 0 COUNT(*), 1 SUM(v) no bits, 2 MIN(v) no bits, 3 MAX(v) no bits, 4/5 SUM(w) lo/hi, 6 MIN(w) bits,
+    # 7 MAX(w) bits, 8 SUM(x), 9 MIN(x), 10 MAX(x)
+    ops = [abi.AGG_COUNT, abi.AGG_SUM, abi.AGG_MIN, abi.AGG_MAX, abi.AGG_SUM, abi.AGG_SUM, abi.AGG_MIN, abi.AGG_MAX,
+           abi.AGG_SUM, abi.AGG_MIN, abi.AGG_MAX]
+    plan.naggs = len(ops)
+    for a, o in enumerate(ops):
+        plan.ops[a] = o
+    plan.agg_int[0] = 1
+    for a in (1, 2, 3):
+        plan.aggs[a], plan.agg_int[a] = b.col(sv), 1
+    for a in (4, 5, 6, 7):
+        plan.aggs[a], plan.agg_int[a], plan.agg_bits[a], plan.agg_null[a] = b.col(sw), 1, 1, abi.T_I64N
+    plan.agg_part[4], plan.agg_part[5] = 1, 2
+    for a in (8, 9, 10):
+        plan.aggs[a] = b.col(sx)
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, host, dense=True)
+    live = ~deleted
+    keys = sorted({(int(a), int(c)) for a, c in zip(k0[live], k1[live])}, key=lambda kk: (kk[0], kk[1]))
+    assert [(int(a), int(c)) for a, c in ok.view(np.int64)] == keys
+    beyond = 0
+    for i, (a0, a1) in enumerate(keys):
+        m = live & (k0 == a0) & (k1 == a1)
+        vv = [int(q) for q in v[m]]
+        ww = [int(q) for q in w[m] if q != abi.NULL_I64]
+        xx = x[m][~np.isnan(x[m])]
+        row = oa[i]
+        assert row[0] == m.sum()
+        exact = sum(vv)
+        beyond += abs(exact) > 2 ** 53
+        assert row[1] == float(exact)  # the double nearest the exact sum (correct rounding)
+        assert row[2] == float(min(vv)) and row[3] == float(max(vv))
+        bits = row.view(np.int64)
+        assert bits[4] == sum(q & 0xffffffff for q in ww) and bits[5] == sum(q >> 32 for q in ww)
+        assert bits[6] == (min(ww) if ww else 2 ** 63 - 1) and bits[7] == (max(ww) if ww else -2 ** 63)
+        np.testing.assert_allclose(row[8], xx.sum(), rtol=1e-12)
+        assert row[9] == (xx.min() if len(xx) else np.inf) and row[10] == (xx.max() if len(xx) else -np.inf)
+    assert beyond > 0  # the no-bits sums really pass 2^53
 
 
 def test_c2_query_oracle_vs_numpy():

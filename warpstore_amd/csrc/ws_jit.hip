@@ -209,6 +209,11 @@ PlanBuild plan_build(const ws_jit_program& p) {
   b.src = std::string(kWsJitPrelude) + "\n" + p.source + kKernelMacro[p.fn_kind] + "(QEval, " + kKernelName[p.fn_kind] + ")\n";
   b.opts = {"--offload-arch=" + arch_name(), "-O3", "-std=c++17", "-munsafe-fp-atomics", "-Xclang", "-target-feature",
             "-Xclang", "-packed-fp32-ops", "-DNDEBUG"};
+  // extra hipRTC options, whitespace-separated (part of the disk cache key like every option)
+  if (const char* extra = getenv("WARPSTORE_HIPRTC_OPTS")) {
+    std::istringstream in(extra);
+    for (std::string o; in >> o;) b.opts.push_back(o);
+  }
   if (!g_disk_dir.empty()) {
     int maj = 0, min = 0;
     hiprtcVersion(&maj, &min);

@@ -177,6 +177,15 @@ int ws_select(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, c
 int ws_select_multi(const ws_expr_env* env, int has_where, ws_prog where, int nb, const int64_t* bn,
                     const uint32_t* const* bdel, const ws_colview* bcols, int ncols, int32_t** out_idx,
                     int64_t* per_batch, const ws_jit_program* jit, int64_t* count);
+/* No counterpart in the reference: ws_select for callers that cannot allocate device memory
+   (code under graph capture, serving loops that forbid allocation). Interpreted plans only, one
+   batch of at most 2^28 rows (268,435,456; larger inputs fail with 1105). Its descriptor and
+   look-back state live in a 2 MiB device array per GPU held until the process ends; no device
+   allocation or free happens in the call. Calls that overlap on one GPU share that array and race:
+   serialize them. Same results as ws_select (ascending row indices of rows passing the WHERE and
+   not deleted). */
+int ws_select_noalloc(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, const uint32_t* del,
+                      int32_t* out_idx, int64_t* count);
 int ws_gather_multi(const ws_colview* src, const int64_t* boff, int nb, int ptype, int esize, const int32_t* idx,
                     int64_t m, void* dst);                                                   /* gpuGatherMulti gpu.h:142-143 */
 int ws_remap_codes(const int32_t* in, int64_t n, const int32_t* map, int64_t map_n, int32_t* out); /* gpuRemapCodes gpu.h:145 */

@@ -68,6 +68,7 @@ _SIGS = {
     "ws_agg_fetch_many": (I, [I, P, P, C.POINTER(I64), I64, P, P]), "ws_sizing_log_size": (I64, []), "ws_sizing_log_copy": (I64, [P, P, I64]),
     "ws_sizing_log_clear": (None, []),
     "ws_select": (I, [C.POINTER(ExprEnv), I, Prog, I64, P, P, P, C.POINTER(I64)]),
+    "ws_select_noalloc": (I, [C.POINTER(ExprEnv), I, Prog, I64, P, P, C.POINTER(I64)]),
     "ws_select_multi": (I, [C.POINTER(ExprEnv), I, Prog, I, P, P, P, I, PP, P, P, C.POINTER(I64)]),
     "ws_gather_multi": (I, [P, P, I, I, I, P, I64, P]), "ws_remap_codes": (I, [P, I64, P, I64, P]),
     "ws_bits_to_double": (I, [P, I64, P]), "ws_eval_at_int": (I, [C.POINTER(ExprEnv), Prog, P, I64, P]),

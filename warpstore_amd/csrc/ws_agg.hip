@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <utility>
 
 #include "ws_internal.h"
@@ -1691,6 +1692,147 @@ int ws_select_multi(const ws_expr_env* env, int has_where, ws_prog where, int nb
     *count = select_launch(*env, has_where, where, in, false, jit, WS_JF_FLAGM, out.p, &pb);
     for (int b = 0; b < nb; ++b) per_batch[b] = pb[b];
     *out_idx = out.release();
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ selection without allocation
+
+// ws_select for callers that cannot allocate device memory (code under graph capture, serving
+// loops that forbid it): the scan descriptor and the look-back state live in `scratch`, a
+// file-scope device array that every GPU holds for the life of the process, instead of in pool
+// blocks; the read-back goes through the caller's stack. The interpreted selection kernels are
+// shared with ws_select. All calls on one GPU share that GPU's array: calls that overlap on one
+// GPU (two host threads, or two streams) race and must be serialized by the caller.
+namespace ws {
+namespace {
+constexpr size_t kScratchBytes = (size_t)2 << 20;     // 2 MiB per GPU
+constexpr size_t kScratchDescBytes = 16 << 10;        // descriptor region at the front
+constexpr int64_t kNoAllocMaxRows = (int64_t)1 << 28; // 268,435,456 rows
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The current GPU's copy of `scratch` as a device address (looked up once per GPU, with that GPU
+// current). Host code never uses the array's name as an address: only this pointer.
+uint8_t* scratch_address() {
+  int dev = 0;
+  WS_CK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error("ws_select_noalloc: device ordinal out of range", WS_ERR_INTERNAL);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      throw Error(std::string("ws_select_noalloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e), WS_ERR_INTERNAL);
+    }
+    g_scratch_ptr[dev] = (uint8_t*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Host image of the descriptor region: ScanDesc, plan, env, then the one-batch arrays, each at a
+// 16-byte boundary, with device addresses resolved against the region's base.
+struct NoAllocDesc {
+  alignas(16) uint8_t host[kScratchDescBytes];
+  size_t used = 0;
+  size_t put(const void* src, size_t bytes) {
+    const size_t off = (used + 15) & ~(size_t)15;
+    if (off + bytes > sizeof host) throw Error("ws_select_noalloc: descriptor region overflow", WS_ERR_INTERNAL);
+    if (src) std::memcpy(host + off, src, bytes);
+    else std::memset(host + off, 0, bytes);
+    used = off + bytes;
+    return off;
+  }
+};
+
+int64_t select_noalloc(const ws_expr_env& env, int has_where, ws_prog where, int64_t n, const uint32_t* del,
+                       int32_t* out_idx) {
+  if (n > kNoAllocMaxRows)
+    throw Error("ws_select_noalloc: " + std::to_string(n) + " rows exceed the maximum of " + std::to_string(kNoAllocMaxRows) +
+                    " (the scratch array holds the look-back state of at most that many)",
+                WS_ERR_INTERNAL);
+  uint8_t* base = scratch_address();
+  // tile geometry as ws_select chooses it for interpreted plans
+  const int64_t want = (int64_t)cu_count() * tune(kTuneSelectChunks);
+  int64_t ntiles = (n + kSelTileRows - 1) / kSelTileRows;
+  const bool chunked = ntiles > 2 * want;
+  ScanDesc d{};
+  d.tile_rows = kSelTileRows;
+  int64_t nunits = ntiles;
+  size_t lds = 0;
+  if (chunked) {
+    d.tile_rows = kSelChunkRows * kBlock;
+    ntiles = (n + d.tile_rows - 1) / d.tile_rows;
+    d.chunk_tiles = (int32_t)std::min<int64_t>(85, (ntiles + want - 1) / want);
+    nunits = (ntiles + d.chunk_tiles - 1) / d.chunk_tiles;
+    lds = (size_t)d.chunk_tiles * kSelChunkRows * (kBlock / 64) * 12;
+  }
+  // [tile (or chunk) states][counter][total][err][per-batch count]
+  const int64_t nstate = ntiles + 4;
+  if (kScratchDescBytes + (size_t)nstate * 8 > kScratchBytes)
+    throw Error("ws_select_noalloc: the look-back state of " + std::to_string(ntiles) + " tiles exceeds the scratch array",
+                WS_ERR_INTERNAL);
+  ws_agg_plan fp{};
+  fp.has_where = (uint8_t)(has_where != 0);
+  fp.where = where;
+  NoAllocDesc h;
+  const size_t o_desc = h.put(nullptr, sizeof(ScanDesc));
+  const size_t o_plan = h.put(&fp, sizeof fp);
+  const size_t o_env = h.put(&env, sizeof env);
+  const size_t o_n = h.put(&n, 8);
+  const size_t o_del = h.put(&del, 8);
+  const int64_t tile0[2] = {0, ntiles}, row0 = 0;
+  const size_t o_t0 = h.put(tile0, sizeof tile0);
+  const size_t o_r0 = h.put(&row0, 8);
+  d.nb = 1;
+  d.ncols = WS_MAX_COLS;
+  d.ntiles = ntiles;
+  d.n = (const int64_t*)(base + o_n);
+  d.del = (const uint32_t* const*)(base + o_del);
+  d.tile0 = (const int64_t*)(base + o_t0);
+  d.row0 = (const int64_t*)(base + o_r0);
+  d.env = (const ws_expr_env*)(base + o_env);
+  d.plan = (const ws_agg_plan*)(base + o_plan);
+  d.cols = (const ws_colview*)(base + o_env + offsetof(ws_expr_env, cols));
+  std::memcpy(h.host + o_desc, &d, sizeof d);
+  WS_CK(hipMemcpyAsync(base, h.host, h.used, hipMemcpyHostToDevice, stream()));
+  u64* st = (u64*)(base + kScratchDescBytes);
+  fill_u64(st, nstate, 0);  // a fill kernel: the previous call's state is still in the array
+  unsigned* ctr = (unsigned*)(st + ntiles);
+  u64* total = st + ntiles + 1;
+  int* err = (int*)(st + ntiles + 2);
+  u64* pb = st + ntiles + 3;
+  const ScanDesc* ddesc = (const ScanDesc*)(base + o_desc);
+  const int iv = !has_where || prog_is_leaf(env, where) ? kIvCols : prog_is_simple(env, where) ? kIvLean : kIvFull;
+  if (chunked)
+    hipLaunchKernelGGL(IV_PICK(iv, select_chunk), dim3((unsigned)nunits), dim3(kBlock), lds, stream(), ddesc, st, ctr, out_idx,
+                       pb, total, err);
+  else
+    hipLaunchKernelGGL(IV_PICK(iv, select), dim3((unsigned)ntiles), dim3(kBlock), 0, stream(), ddesc, st, ctr, out_idx, pb,
+                       total, err);
+  check_launch("select_noalloc");
+  u64 back[3];  // total, err, per-batch count
+  WS_CK(hipMemcpyAsync(back, total, sizeof back, hipMemcpyDeviceToHost, stream()));
+  WS_CK(hipStreamSynchronize(stream()));
+  if ((int)back[1]) throw Error("selection look-back did not complete", WS_ERR_INTERNAL);
+  return (int64_t)back[0];
+}
+}  // namespace
+}  // namespace ws
+
+extern "C" {
+
+int ws_select_noalloc(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, const uint32_t* del, int32_t* out_idx,
+                      int64_t* count) {
+  return ws::guard([&] {
+    *count = 0;
+    if (n <= 0) return;
+    *count = ws::select_noalloc(*env, has_where, where, n, del, out_idx);
   });
 }
 

@@ -43,6 +43,10 @@ int ws_describe(char* buf, size_t cap);              /* gpuDescribe    gpu.h:15 
 int ws_dev_alloc(size_t bytes, void** out);          /* devAlloc       gpu.h:16 */
 void ws_dev_free(void* p);                           /* devFree        gpu.h:17 */
 int ws_tmp_alloc(size_t bytes, void** out);          /* tmpAlloc       gpu.h:50 */
+/* Page-locked host memory (the reference keeps a thread-local pinned stage, gpu.cu:226-240):
+   result buffers the caller reads many groups into copy at link speed. No reference entry point. */
+int ws_pinned_alloc(size_t bytes, void** out);
+void ws_pinned_free(void* p);
 void ws_trim_pool(void);                             /* gpuTrimPool    gpu.h:51 */
 void ws_tmp_free(void* p);                           /* tmpFree        gpu.h:52 */
 int ws_h2d(void* dst, const void* src, size_t bytes);/* h2d            gpu.h:53 */

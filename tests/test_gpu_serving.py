@@ -49,3 +49,36 @@ def test_serving_mix_results_equal_alone(gpu):
         for t in (web, fact, dim):
             t.free()
         pool.free()
+
+
+def test_autotuner_choices_keep_results(gpu):
+    """The autotuner times its candidates on the workload itself and records a choice per shape;
+    the batched results under the chosen knobs equal the per-query results, and the web query's
+    result is the same before and after tuning its partition knobs."""
+    H.say("case autotuner (batched GROUP BY and partitioned web query)")
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    s = Q.C4Set(seed=14, nq=24)
+    try:
+        dp, gp = s.run(batched=False)
+        r = s.autotune(reps=1)
+        assert r["trials"] > 5 and r["tuned_ms"] <= r["start_ms"] * 1.0001
+        assert r["chosen"]["batch_tile_rows"] in (1024, 2048, 4096, 8192, 16384, 32768)
+        db, gb = s.run(batched=True)
+        assert np.array_equal(db, dp) and np.array_equal(gb, gp)
+    finally:
+        s.free()
+    user, region, date, url, dur = H.gen_c2(1_000_000)
+    web = Q.DeviceTable([user, region, date, url, dur], [abi.T_I64, abi.T_I32, abi.T_I32, abi.T_I32, abi.T_I32],
+                        [0, 1, 0, 0, 0])
+    try:
+        before, st0 = Q.c2(web, 10)
+        r = Q.autotune_web(web, reps=1)
+        assert r["trials"] > 3
+        after, st1 = Q.c2(web, 10)
+        assert np.array_equal(st0[:2], st1[:2])
+        H.check_topn(after, H.oracle_c2(user, region, date, url, dur))
+    finally:
+        web.free()
+        for k, v in (("batch_tile_rows", 4096), ("batch_lds_slots", 1024), ("batch_blocks_per_cu", 4), ("part_chunks_per_cu", 1),
+                     ("part_scatter_lds_kib", 120), ("part_merge_lds_kib", 60)):
+            gpu.ws_tune_set(k.encode(), v)

@@ -41,6 +41,7 @@ _SIGS = {
     "ws_init": (I, []), "ws_sm_count": (I, []), "ws_arch": (I, []), "ws_describe": (I, [C.c_char_p, SZ]),
     "ws_dev_alloc": (I, [SZ, PP]), "ws_dev_free": (None, [P]), "ws_tmp_alloc": (I, [SZ, PP]),
     "ws_trim_pool": (None, []), "ws_tmp_free": (None, [P]),
+    "ws_pinned_alloc": (I, [SZ, PP]), "ws_pinned_free": (None, [P]),
     "ws_h2d": (I, [P, P, SZ]), "ws_d2h": (I, [P, P, SZ]), "ws_d2d": (I, [P, P, SZ]), "ws_sync": (I, []),
     "ws_set_vram_limit": (None, [I64]), "ws_vram_used": (I64, []), "ws_vram_fill": (D, []),
     "ws_place_column": (I, [P, SZ, I, C.POINTER(I), PP]), "ws_release_column": (None, [P, P, SZ, I]),

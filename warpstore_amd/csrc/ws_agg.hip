@@ -53,6 +53,7 @@ __global__ void init_table_k(GTable t, int nk, int na, ws_agg_plan plan) {
     for (int a = 0; a < na; ++a) t.acc[i * na + a] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
   }
 }
+__global__ void __launch_bounds__(256) batch_init_k(const BatchQuery* qs) { batch_init_body(qs); }
 __global__ void init_dense_k(double* acc, int64_t G, int na, ws_agg_plan plan) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < G * na; i += (int64_t)gridDim.x * blockDim.x) {
     const int a = (int)(i % na);
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(256) dense_count_many_k(const FetchQuery* qs, 
 }
 // Block q: its groups with rows, ascending group index, row-major keys | accs from row offs[q].
 __global__ void __launch_bounds__(256) dense_pack_many_k(const FetchQuery* qs, ws_agg_plan plan, int nk, int na,
-                                                         const uint32_t* offs, double* out) {
+                                                         const uint32_t* offs, double* out_k, double* out_a) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_base;
   const FetchQuery& Q = qs[blockIdx.x];
@@ -132,12 +133,12 @@ __global__ void __launch_bounds__(256) dense_pack_many_k(const FetchQuery* qs, w
       total += s_w[j];
     }
     if (has) {
-      double* row = out + (int64_t)(s_base + before + rank) * (nk + na);
+      const int64_t i = (int64_t)(s_base + before + rank);
       for (int k = 0; k < nk; ++k) {
         const i64 v = Q.ds.kmin[k] + (g / Q.ds.stride[k]) % Q.ds.range[k];
-        row[k] = plan.key_int[k] ? bitsd(v) : (double)v;
+        out_k[i * nk + k] = plan.key_int[k] ? bitsd(v) : (double)v;
       }
-      for (int a = 0; a < na; ++a) row[nk + a] = Q.acc[g * na + a];
+      for (int a = 0; a < na; ++a) out_a[i * na + a] = Q.acc[g * na + a];
     }
     __syncthreads();
     if (threadIdx.x == 0) s_base += total;
@@ -799,6 +800,9 @@ struct ws_agg {
   // fetch state
   int64_t fetched = -1;
   DevBuf<double> fetch_buf;
+  // the flags, counter and dense accumulators still need their initial values (deferred from
+  // creation so a batch of aggregations is initialized by one launch)
+  bool pending_init = false;
   void free_part() {
     for (void* p : {(void*)okv, (void*)oacc, (void*)pstart, (void*)goff}) tmp_free(p);
     okv = oacc = nullptr;
@@ -834,8 +838,21 @@ static void ensure_table(ws_agg* A, int64_t slots) {
   hipLaunchKernelGGL(init_table_k, dim3(grid_for(slots)), dim3(256), 0, stream(), t, nk, A->plan.naggs, A->plan);
   check_launch("init_table");
 }
+// Gives the flags, counter and dense accumulators their initial values if creation deferred it.
+static void prepare(ws_agg* A) {
+  if (!A->pending_init) return;
+  A->pending_init = false;
+  fill_u32((uint32_t*)A->flags, 4, 0);
+  fill_u64(A->count, 1, 0);
+  if (A->is_dense) {
+    hipLaunchKernelGGL(init_dense_k, dim3(grid_for(A->cap * std::max(A->plan.naggs, 1))), dim3(256), 0, stream(), A->acc, A->cap,
+                       A->plan.naggs, A->plan);
+    check_launch("init_dense");
+  }
+}
 // Reads the aggregation's small results in one round trip (cached until the next launch).
 static const ws_agg::Status& status(ws_agg* A) {
+  prepare(A);
   if (A->st_valid) return A->st;
   A->st = ws_agg::Status{};
   D2HPiece pc[3];
@@ -1079,6 +1096,7 @@ static void part_to_table(ws_agg* A) {
 }
 
 static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit, bool allow_part) {
+  prepare(A);
   const ws_agg_plan& plan = A->plan;
   if (&env != &A->last_env) A->last_env = env;
   A->last_jit = jit;
@@ -1199,6 +1217,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
 // column-major; with `abuf` the accs go there instead (two separately owned buffers).
 static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor, DevBuf<double>* abuf = nullptr) {
   const int nk = A->plan.nkeys, na = A->plan.naggs;
+  prepare(A);
   settle(A);
   if (!A->is_dense && !A->keys) {  // no run yet: no groups
     buf = DevBuf<double>(1);
@@ -1265,14 +1284,9 @@ int ws_agg_create(const ws_agg_plan* plan, int64_t capacity, const ws_dense_spec
       const int na = std::max(plan->naggs, 1);
       A->flags = (int*)tmp_alloc(sizeof(int) * 4);
       A->count = (u64*)tmp_alloc(sizeof(u64));
-      fill_u32((uint32_t*)A->flags, 4, 0);
-      fill_u64(A->count, 1, 0);
       // The hash table is allocated by the first run, which knows whether it is partitioned.
-      if (A->is_dense) {
-        A->acc = (double*)tmp_alloc(sizeof(double) * (size_t)A->cap * na);
-        hipLaunchKernelGGL(init_dense_k, dim3(grid_for(A->cap * na)), dim3(256), 0, stream(), A->acc, A->cap, plan->naggs, A->plan);
-        check_launch("init_dense");
-      }
+      if (A->is_dense) A->acc = (double*)tmp_alloc(sizeof(double) * (size_t)A->cap * na);
+      A->pending_init = true;  // initial values at the first use (prepare, or a batched run)
     }
     *out = A.release();
   });
@@ -1433,6 +1447,9 @@ int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs,
       o_t0[q] = blob.put(t0.data(), 8 * t0.size());
     }
     uint8_t* dev = (uint8_t*)tmp_alloc(blob.host.size());
+    int pending = 0;
+    int64_t gmax = 1;
+    for (int q = 0; q < n; ++q) gmax = std::max<int64_t>(gmax, aggs[q]->dense.G * std::max(na, 1));
     for (int q = 0; q < n; ++q) {
       ws_agg* A = aggs[q];
       BatchQuery bq{};
@@ -1450,17 +1467,28 @@ int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs,
       d.ds = (const ws_dense_spec*)(dev + o_ds[q]);
       bq.gacc = A->acc;
       bq.flags = A->flags;
+      bq.count = A->count;
+      bq.init = A->pending_init ? 1 : 0;
+      if (A->pending_init) ++pending;
       *blob.at<BatchQuery>(o_q + sizeof(BatchQuery) * q) = bq;
       A->fetched = -1;
       A->st_valid = false;
       ++A->runs;
     }
     h2d_async_staged(dev, blob.host.data(), blob.host.size());
-    const int S = (int)tune(kTuneBatchSlots);
-    const size_t lds = (((size_t)S * 4 + 15) & ~(size_t)15) + (size_t)S * na * 8;
+    const BatchQuery* dq = (const BatchQuery*)(dev + o_q);
+    if (pending) {  // the deferred initial values of the whole batch, one launch
+      const unsigned gx = (unsigned)std::min<int64_t>(64, (gmax + 2047) / 2048);
+      hipLaunchKernelGGL(batch_init_k, dim3(gx, (unsigned)n), dim3(256), 0, stream(), dq);
+      check_launch("batch_init");
+      for (int q = 0; q < n; ++q) aggs[q]->pending_init = false;
+    }
+    int S = 64;  // the kernel's per-tile tables are powers of two up to S, within the LDS
+    auto lds_of = [&](int slots) { return (((size_t)slots * 4 + 15) & ~(size_t)15) + (size_t)slots * std::max(na, 1) * 8; };
+    while (S * 2 <= (int)tune(kTuneBatchSlots) && lds_of(S * 2) + kStaticLds <= kLdsPerCu) S *= 2;
+    const size_t lds = lds_of(S);
     const unsigned grid =
         (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (int64_t)cu_count() * tune(kTuneBatchBlocksPerCu)));
-    const BatchQuery* dq = (const BatchQuery*)(dev + o_q);
     const BatchTile* dt = (const BatchTile*)(dev + o_t);
     long long nt = (long long)tiles.size();
     int sl = S;
@@ -1486,7 +1514,8 @@ int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* tota
     const int nk = aggs[0]->plan.nkeys, na = aggs[0]->plan.naggs;
     std::vector<FetchQuery> fq(n);
     for (int q = 0; q < n; ++q) {
-      const ws_agg* A = aggs[q];
+      ws_agg* A = aggs[q];
+      prepare(A);  // never run: its identities (no groups)
       if (!A->is_dense || A->plan.nkeys != nk || A->plan.naggs != na)
         throw Error("batched fetch: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
       fq[q] = FetchQuery{A->acc, A->dense, A->dense.G};
@@ -1503,16 +1532,14 @@ int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* tota
     *total = h[n];
     if (*total > cap) throw Error("batched fetch: more groups than the buffers hold", WS_ERR_INTERNAL);
     if (!*total) return;
-    DevBuf<double> packed(*total * (nk + na));
+    DevBuf<double> pk(*total * nk), pa(*total * na);
     hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)dq.p, aggs[0]->plan, nk,
-                       na, (const uint32_t*)offs.p, packed.p);
+                       na, (const uint32_t*)offs.p, pk.p, pa.p);
     check_launch("fetch_many_pack");
-    std::vector<double> hp(*total * (nk + na));
-    d2h_sync(hp.data(), packed.p, sizeof(double) * hp.size());
-    for (int64_t i = 0; i < *total; ++i) {
-      std::memcpy(keys + i * nk, &hp[i * (nk + na)], sizeof(double) * nk);
-      std::memcpy(accs + i * na, &hp[i * (nk + na) + nk], sizeof(double) * na);
-    }
+    // straight into the caller's arrays (pinned ones, e.g. from ws_pinned_alloc, copy at link speed)
+    if (nk) WS_CK(hipMemcpyAsync(keys, pk.p, sizeof(double) * (size_t)(*total * nk), hipMemcpyDeviceToHost, stream()));
+    if (na) WS_CK(hipMemcpyAsync(accs, pa.p, sizeof(double) * (size_t)(*total * na), hipMemcpyDeviceToHost, stream()));
+    WS_CK(hipStreamSynchronize(stream()));
   });
 }
 

@@ -207,6 +207,13 @@ __device__ __forceinline__ void atomic_combine_d(int op, double* addr, double v)
 }
 template <int SCOPE>
 __device__ __forceinline__ void atomic_combine_i(int op, i64* addr, i64 v) {
+  if (SCOPE == __HIP_MEMORY_SCOPE_AGENT && (op == WS_AGG_MIN || op == WS_AGG_MAX)) {
+    // A global MIN only decreases (MAX only grows) while a kernel runs, and caches start clean at
+    // every launch, so even a stale cached value that already beats v proves the update
+    // unnecessary: most rows of a group then skip the read-modify-write.
+    const i64 cur = __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (op == WS_AGG_MIN ? cur <= v : cur >= v) return;
+  }
   if (op == WS_AGG_MIN) __hip_atomic_fetch_min(addr, v, __ATOMIC_RELAXED, SCOPE);
   else if (op == WS_AGG_MAX) __hip_atomic_fetch_max(addr, v, __ATOMIC_RELAXED, SCOPE);
   else __hip_atomic_fetch_add((u64*)addr, (u64)v, __ATOMIC_RELAXED, SCOPE);
@@ -984,14 +991,38 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
 // Many dense aggregations (one plan shape, each with its own batches, constants and table) in one
 // launch: the host lists every tile of every query; blocks take tiles in a grid-stride loop (no
 // queue counter) and pre-aggregate each tile in an LDS table keyed by the dense group index, then
-// fold it into the query's accumulators with device-scope atomics. Values are combined in the
-// final slot encoding of the shared-table dense layout (int64 bits for agg_bits slots, doubles
-// otherwise), so the results equal a plain run of each query (up to double summation order).
+// fold it into the query's accumulators with device-scope atomics. A query with at most S groups
+// keeps each group at its own slot (no hashing, no probes); otherwise the table holds twice the
+// tile's possible groups (min(rows, G)), hashed, when that fits S; a tile whose possible groups do
+// not fit, or of a query with many more groups than rows (little to combine), folds its rows
+// straight into the query's accumulators. Values are combined in the final slot encoding of the shared-table
+// dense layout (int64 bits for agg_bits slots, doubles otherwise), so the results equal a plain
+// run of each query (up to double summation order).
 struct BatchQuery {
   ScanDesc d;    // batches, column views, env, plan and dense spec of the query
   double* gacc;  // [G][NA] accumulators
-  int* flags;    // [1] dense key out of range
+  int* flags;    // [4] flags; [1] dense key out of range
+  u64* count;    // group counter of the aggregation
+  int32_t init;  // the accumulators, flags and counter still need their initial values
+  int32_t pad;
 };
+// Initial values of the batch's accumulators, flags and counters (queries with init set), one
+// launch for the whole batch: block (x, q) takes a share of query q's [G][NA] slots.
+__device__ inline void batch_init_body(const BatchQuery* qs) {
+  const BatchQuery& Q = qs[blockIdx.y];
+  if (!Q.init) return;
+  const ws_agg_plan& plan = *Q.d.plan;
+  const int NA = plan.naggs > 0 ? plan.naggs : 1;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 4) Q.flags[threadIdx.x] = 0;
+    if (threadIdx.x == 4) *Q.count = 0;
+  }
+  const i64 n = Q.d.ds->G * NA;
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+    const int a = (int)(i % NA);
+    Q.gacc[i] = a < plan.naggs ? slot_identity(plan.agg_bits[a] != 0, plan.ops[a]) : 0.0;
+  }
+}
 struct BatchTile {
   int32_t q, b;   // query, batch of the query
   int32_t count;  // rows
@@ -1006,6 +1037,8 @@ __device__ __forceinline__ void dense_fold(const ws_agg_plan& plan, uint8_t k, i
   else atomic_combine_d<SCOPE>(dk_op(k), dst, dv);
 }
 
+constexpr int kBatchRows = 4;
+constexpr i64 kSkipRow = -2;  // a row the step does not aggregate (dense_index gives -1 for a key out of range)
 template <class Ev>
 __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i64 ntiles, int S) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
@@ -1020,9 +1053,19 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
     const ws_dense_spec& ds = *d.ds;
     Ev ev(d);
     const int NK = ev.nkeys(), NA = ev.naggs();
+    // this tile's table: every group of the query at its own slot when G <= S (direct); else 2 x
+    // the tile's possible groups, hashed (a power of two, >= 64, <= S); else none
+    const i64 G = ds.G, reach = bt.count < G ? (i64)bt.count : G;
+    const bool direct = G <= (i64)S;
+    int St = direct ? (int)G : 0;
+    if (!direct && G <= 4 * (i64)bt.count && 2 * reach <= S) {
+      St = 64;
+      while (St < S && St < 2 * reach) St <<= 1;
+      St = St < S ? St : S;
+    }
     __syncthreads();  // the previous tile's fold has read the table
     for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)bt.b * d.ncols + i];
-    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    for (int i = threadIdx.x; i < St; i += blockDim.x) {
       lg[i] = -1;
       for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, dk_op(ds.kind[a]));
     }
@@ -1030,26 +1073,53 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
     ev.bind(Ev::kGlobalCols ? d.cols + (i64)bt.b * d.ncols : scols);
     const uint32_t* del = uni(d.del[bt.b]);
     bool bad = false;
-    for (int i = threadIdx.x; i < bt.count; i += blockDim.x) {
-      const i64 r = bt.first + i;
-      if (is_deleted(del, r)) continue;
-      if (ev.has_where() && !ev.where(r)) continue;
-      const i64 g = dense_index(ev, ds, NK, r);
+    // kBatchRows rows per thread per step: their filter and key loads are issued before any fold
+    // (evaluated for every row, rows past the tile re-reading its first, when the plan has no side
+    // effects and the batch no deleted rows; row by row otherwise)
+    constexpr int U = kBatchRows;
+    const bool spec = Ev::kPureKeys && Ev::kPureWhere && !del;
+    for (int i0 = threadIdx.x; i0 < bt.count; i0 += blockDim.x * U) {
+      i64 gs[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * (int)blockDim.x;
+        gs[u] = kSkipRow;
+        if (spec) {
+          const i64 r = bt.first + (i < bt.count ? i : 0);
+          const bool w = !ev.has_where() || ev.where(r);
+          const i64 g = dense_index(ev, ds, NK, r);
+          gs[u] = i < bt.count && w ? g : kSkipRow;
+        } else if (i < bt.count) {
+          const i64 r = bt.first + i;
+          if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) gs[u] = dense_index(ev, ds, NK, r);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+      const i64 g = gs[u];
+      if (g == kSkipRow) continue;
+      const i64 r = bt.first + i0 + u * (int)blockDim.x;
       if (g < 0) {
         bad = true;
         continue;
       }
-      int s = (int)(g % S), slot = -1;
-      for (int probe = 0; probe < 8; ++probe) {
-        int prev = __hip_atomic_load(&lg[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (prev == -1 && __hip_atomic_compare_exchange_strong(&lg[s], &prev, (int)g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_WORKGROUP))
-          prev = (int)g;
-        if (prev == (int)g) {
-          slot = s;
-          break;
+      int slot = -1;
+      if (direct) {
+        slot = (int)g;
+        if (lg[slot] < 0) lg[slot] = slot;  // occupied (a plain store: every writer stores the same)
+      } else if (St > 0) {
+        int s = (int)(g & (St - 1));
+        for (int probe = 0; probe < 8; ++probe) {
+          int prev = __hip_atomic_load(&lg[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (prev == -1 && __hip_atomic_compare_exchange_strong(&lg[s], &prev, (int)g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
+            prev = (int)g;
+          if (prev == (int)g) {
+            slot = s;
+            break;
+          }
+          s = (s + 1) & (St - 1);
         }
-        s = s + 1 == S ? 0 : s + 1;
       }
       double* dst = slot >= 0 ? lacc + (i64)slot * NA : Q.gacc + g * NA;
 #pragma unroll
@@ -1062,10 +1132,11 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
         if (slot >= 0) dense_fold<WS_LDS>(plan, k, a, &dst[a], iv, dv);
         else dense_fold<WS_DEV>(plan, k, a, &dst[a], iv, dv);
       }
+      }
     }
     if (bad) Q.flags[1] = 1;
     __syncthreads();
-    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    for (int i = threadIdx.x; i < St; i += blockDim.x) {
       const int g = lg[i];
       if (g < 0) continue;
       for (int a = 0; a < NA; ++a) {

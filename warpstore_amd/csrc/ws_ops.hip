@@ -7,8 +7,8 @@
 //     the four random reads, then the stores, so each lane keeps four reads in flight;
 //   - wide elements (VECTOR rows) are copied by a lane group per row, so a row's words are read as
 //     one coalesced run instead of one lane per word;
-//   - gathers over many batches find the batch of a global row through a host-built bucket table
-//     (top bits of the row -> first batch that reaches the bucket) instead of a binary search;
+//   - gathers over many batches stage the batch starts and views in LDS and find each row's batch
+//     from the previous row's (ascending selections), with a binary search in LDS otherwise;
 //   - bitmaps are built in LDS when the block's share of rows is large against the bitmap, then
 //     OR-flushed word by word (only words that gained bits); otherwise with global atomics that
 //     skip bits already set;
@@ -98,36 +98,60 @@ __global__ void __launch_bounds__(kBlock) gather_rows_k(const uint32_t* __restri
   }
 }
 
-// Gather from the concatenation of nb batches (batch b = rows [boff[b], boff[b+1])): the batch of
-// a global row g starts from bucket_first[g >> shift] and walks forward over the batches that
-// begin inside the bucket. Encoded batches are decoded into the plain type t (NULL sentinels of
-// the nullable types are kept as stored).
+// Gather from the concatenation of nb batches (batch b = rows [boff[b], boff[b+1])). Each block
+// stages the batch starts and column views in LDS (kStaged; up to kStageBatches of them, else they
+// are read from global memory), and every lane takes four consecutive indices: the batch of the
+// first comes from a binary search over the starts, each next one is tried at the previous one's
+// batch first (selections are ascending, so that almost always hits). Encoded batches are decoded
+// into the plain type t (NULL sentinels of the nullable types are kept as stored).
+constexpr int kStageBatches = 512;
 struct MultiSrc {
   const ws_colview* views;  // [nb], types already set to the nullable form of t
   const int64_t* boff;      // [nb]
-  const int32_t* bucket_first;
-  int nb, shift, t, esize;
-  int64_t last_bucket;
+  int nb, t, esize;
 };
-__device__ __forceinline__ int batch_of_row(const MultiSrc& s, int64_t g) {
-  const int64_t k = g >> s.shift;
-  int b = s.bucket_first[k < s.last_bucket ? k : s.last_bucket];
-  while (b + 1 < s.nb && s.boff[b + 1] <= g) ++b;
-  return b;
-}
+template <bool kStaged>
 __global__ void __launch_bounds__(kBlock) gather_batches_k(MultiSrc s, const int32_t* __restrict__ idx, int64_t m,
                                                            uint8_t* __restrict__ dst) {
+  __shared__ int64_t s_off[kStaged ? kStageBatches : 1];
+  __shared__ ws_colview s_view[kStaged ? kStageBatches : 1];
+  if (kStaged) {
+    for (int i = threadIdx.x; i < s.nb; i += blockDim.x) {
+      s_off[i] = s.boff[i];
+      s_view[i] = s.views[i];
+    }
+    __syncthreads();
+  }
+  const int nb = s.nb;
+  auto off = [&](int b) { return kStaged ? s_off[b] : s.boff[b]; };
+  auto batch = [&](int64_t g, int hint) {  // the last batch starting at or before g
+    if (hint >= 0 && off(hint) <= g && (hint + 1 >= nb || g < off(hint + 1))) return hint;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off(mid) <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  };
   const int64_t step = (int64_t)gridDim.x * blockDim.x * 4;
+  const bool vec = ((uintptr_t)idx & 15) == 0;
   for (int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; j < m; j += step) {
     int64_t g[4];
+    if (vec && j + 4 <= m) {
+      const int4 q = *(const int4*)(idx + j);
+      g[0] = q.x, g[1] = q.y, g[2] = q.z, g[3] = q.w;
+    } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) g[q] = j + q < m ? idx[j + q] : 0;
+      for (int q = 0; q < 4; ++q) g[q] = j + q < m ? idx[j + q] : 0;
+    }
+    int b = -1;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (j + q >= m) break;
-      const int b = batch_of_row(s, g[q]);
-      const ws_colview v = s.views[b];
-      const int64_t r = g[q] - s.boff[b], o = j + q;
+      b = batch(g[q], b);
+      const ws_colview v = kStaged ? s_view[b] : s.views[b];
+      const int64_t r = g[q] - off(b), o = j + q;
       if (v.enc) {
         if (s.t == WS_P_F64) ((double*)dst)[o] = enc_dbl(v, r);
         else if (s.t == WS_P_I32) ((int32_t*)dst)[o] = (int32_t)enc_int(v, r);
@@ -490,30 +514,20 @@ int ws_gather_multi(const ws_colview* src, const int64_t* boff, int nb, int ptyp
   return guard([&] {
     if (m <= 0) return;
     if (nb <= 0) throw Error("gather_multi: no batches", WS_ERR_INTERNAL);
-    // bucket table up to the last batch's start (rows past it belong to the last batch): at most
-    // 4096 buckets of at least 1024 rows
-    int shift = 10;
-    while ((boff[nb - 1] >> shift) >= 4096) ++shift;
-    const int64_t nbuckets = (boff[nb - 1] >> shift) + 1;
-    std::vector<int32_t> first((size_t)nbuckets);
-    int b = 0;
-    for (int64_t k = 0; k < nbuckets; ++k) {
-      const int64_t lo = k << shift;
-      while (b + 1 < nb && boff[b + 1] <= lo) ++b;
-      first[(size_t)k] = b;
-    }
     std::vector<ws_colview> views(src, src + nb);
     for (auto& v : views) v.t = nullable_type_of(ptype);  // encoded batches keep their NULL sentinels
     Blob blob;
     const size_t o_v = blob.put(views.data(), sizeof(ws_colview) * (size_t)nb);
     const size_t o_o = blob.put(boff, 8 * (size_t)nb);
-    const size_t o_f = blob.put(first.data(), 4 * first.size());
     uint8_t* d = upload_blob(blob);
-    MultiSrc s{(const ws_colview*)(d + o_v), (const int64_t*)(d + o_o), (const int32_t*)(d + o_f), nb, shift, ptype, esize,
-                nbuckets - 1};
+    MultiSrc s{(const ws_colview*)(d + o_v), (const int64_t*)(d + o_o), nb, ptype, esize};
     {
       ProfScope ps("gather_multi");
-      hipLaunchKernelGGL(gather_batches_k, dim3(grid_for((m + 3) / 4)), dim3(kBlock), 0, stream(), s, idx, m, (uint8_t*)dst);
+      const unsigned grid = grid_for((m + 3) / 4);
+      if (nb <= kStageBatches)
+        hipLaunchKernelGGL(gather_batches_k<true>, dim3(grid), dim3(kBlock), 0, stream(), s, idx, m, (uint8_t*)dst);
+      else
+        hipLaunchKernelGGL(gather_batches_k<false>, dim3(grid), dim3(kBlock), 0, stream(), s, idx, m, (uint8_t*)dst);
       check_launch("gather_multi");
     }
     tmp_free(d);

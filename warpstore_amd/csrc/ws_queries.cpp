@@ -17,6 +17,8 @@
 #include <condition_variable>
 #include <deque>
 #include <exception>
+#include <functional>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <climits>
@@ -273,6 +275,81 @@ int64_t size_groupby(const std::vector<const DevColumn*>& keys, int64_t rows, do
   return g;
 }
 
+// ------------------------------------------------------------------ autotuner
+
+// Physical-plan knobs (ws_tune_set) chosen per operator shape by measuring the shape's own
+// workload: coordinate descent from the current values, each knob in turn trying each candidate
+// with the others fixed, a value kept when the median of `reps` timed runs beats the best by more
+// than 2 %; at most two sweeps. The choice is recorded under the shape's name and applied before
+// every later run of that shape (results never depend on the knobs).
+struct KnobSpace {
+  const char* name;
+  std::vector<int64_t> values;
+};
+struct TuneResult {
+  std::vector<std::pair<std::string, int64_t>> chosen;
+  double default_ms = 0, best_ms = 0;
+  int trials = 0;
+};
+std::mutex g_tuned_mu;
+std::map<std::string, std::vector<std::pair<std::string, int64_t>>> g_tuned;
+
+void apply_tuned(const std::string& shape) {
+  std::vector<std::pair<std::string, int64_t>> kv;
+  {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    auto it = g_tuned.find(shape);
+    if (it == g_tuned.end()) return;
+    kv = it->second;
+  }
+  for (auto& e : kv) wsd::ck(ws_tune_set(e.first.c_str(), e.second));
+}
+
+TuneResult autotune(const std::string& shape, const std::vector<KnobSpace>& space, int reps, const std::function<void()>& work) {
+  auto timed = [&] {
+    std::vector<double> t;
+    for (int r = 0; r < std::max(reps, 1); ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      work();
+      t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+  };
+  TuneResult res;
+  std::vector<int64_t> cur(space.size());
+  for (size_t i = 0; i < space.size(); ++i) wsd::ck(ws_tune_get(space[i].name, &cur[i]));
+  work();  // compiles and warms what the shape needs
+  double best = timed();
+  res.default_ms = best;
+  res.trials = 1;
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    bool changed = false;
+    for (size_t i = 0; i < space.size(); ++i)
+      for (int64_t v : space[i].values) {
+        if (v == cur[i]) continue;
+        wsd::ck(ws_tune_set(space[i].name, v));
+        const double t = timed();
+        ++res.trials;
+        if (t < best * 0.98) {
+          best = t;
+          cur[i] = v;
+          changed = true;
+        } else {
+          wsd::ck(ws_tune_set(space[i].name, cur[i]));
+        }
+      }
+    if (!changed) break;
+  }
+  for (size_t i = 0; i < space.size(); ++i) res.chosen.push_back({space[i].name, cur[i]});
+  res.best_ms = best;
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  g_tuned[shape] = res.chosen;
+  return res;
+}
+const char* const kC4Shape = "dense GROUP BY batch (1 key, 5 slots)";
+const char* const kC2Shape = "partitioned GROUP BY (web query, both phases)";
+
 // ------------------------------------------------------------------ C2: COUNT DISTINCT + top 10
 
 // SELECT user_id, COUNT(*) c, SUM(dur), COUNT(DISTINCT url) FROM h WHERE event_date < 19723+300 AND
@@ -283,6 +360,7 @@ struct C2Stats {
   int64_t pairs = 0, groups = 0, cap1 = 0, retries = 0;
 };
 int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
+  apply_tuned(kC2Shape);
   Comp c;
   // slots in the engine's compile order: WHERE, keys, aggregates, then the DISTINCT argument
   const int s_date = c.slot(WS_T_I32), s_region = c.slot(WS_T_I32, t.cols[1].segs[0].enc);
@@ -649,7 +727,7 @@ C4Plan c4_plan(const C4Query& q) {
 
 // Order-independent digest of one query's result rows (key, count, sum, min, max): equal digests
 // mean equal row sets.
-uint64_t c4_digest(const std::vector<double>& keys, const std::vector<double>& accs, int64_t g) {
+uint64_t c4_digest(const double* keys, const double* accs, int64_t g) {
   uint64_t d = 0x12345678ull;
   for (int64_t i = 0; i < g; ++i) {
     int64_t key, lo, hi, mn, mx;
@@ -672,7 +750,7 @@ uint64_t c4_digest(const std::vector<double>& keys, const std::vector<double>& a
 }
 
 // One query alone through the plain per-query path (the engine's aggregation driver).
-uint64_t c4_alone(const C4Query& q, int64_t* groups) {
+uint64_t c4_alone(const C4Query& q, int64_t* groups, bool digest = true) {
   C4Plan p = c4_plan(q);
   const JitProgram* jit = jitGet(p.c.env, p.c.ncols, p.plan, p.dense ? JF_DENSE : JF_GROUPED, "aggregate");
   std::vector<double> keys, accs;
@@ -692,29 +770,63 @@ uint64_t c4_alone(const C4Query& q, int64_t* groups) {
     break;
   }
   if (groups) *groups = g;
-  return c4_digest(keys, accs, g);
+  return digest ? c4_digest(keys.data(), accs.data(), g) : 0;
 }
 
-// Dense queries of one shape together: one batched launch and one batched fetch.
-void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>& digests, std::vector<int64_t>& groups) {
+// Page-locked result buffers of the batched fetch (per thread, grown as needed): the groups of a
+// whole batch (millions for the 256-query workload) cross the link at its full speed.
+struct PinnedResults {
+  double* keys = nullptr;
+  double* accs = nullptr;
+  int64_t cap = 0;
+  void reserve(int64_t groups, int nk, int na) {
+    if (groups <= cap) return;
+    release();
+    int64_t c = 1 << 16;
+    while (c < groups) c <<= 1;
+    void* k = nullptr;
+    void* a = nullptr;
+    wsd::ck(ws_pinned_alloc(sizeof(double) * (size_t)(c * std::max(nk, 1)), &k));
+    const int rc = ws_pinned_alloc(sizeof(double) * (size_t)(c * std::max(na, 1)), &a);
+    if (rc) {
+      ws_pinned_free(k);
+      wsd::ck(rc);
+    }
+    keys = (double*)k, accs = (double*)a, cap = c;
+  }
+  void release() {
+    ws_pinned_free(keys);
+    ws_pinned_free(accs);
+    keys = accs = nullptr;
+    cap = 0;
+  }
+  ~PinnedResults() { release(); }
+};
+thread_local PinnedResults t_results;
+
+// Queries of one plan shape together: every dense query (any group count: a tile whose query has
+// more groups than its LDS table folds into the query's accumulators directly) in one batched
+// launch and one batched fetch; a query that is not dense runs alone. digests may be null (the
+// results are fetched but not digested).
+void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>* digests, std::vector<int64_t>& groups) {
   const size_t n = qs.size();
-  digests.assign(n, 0);
+  if (digests) digests->assign(n, 0);
   groups.assign(n, 0);
   if (!n) return;
   std::vector<C4Plan> plans;
   plans.reserve(n);
   std::vector<size_t> dense_idx;
-  // The batched kernel keeps a query's groups in LDS up to batch_lds_slots of them; a query with
-  // more groups would fold them with device-scope atomics there, so it runs alone (its own plan
-  // picks LDS copies or private slots), as does a query that is not dense.
-  int64_t lds_slots = 1024;
-  wsd::ck(ws_tune_get("batch_lds_slots", &lds_slots));
   for (size_t i = 0; i < n; ++i) {
     plans.push_back(c4_plan(*qs[i]));
-    if (plans.back().dense && plans.back().ds.G <= lds_slots) dense_idx.push_back(i);
-    else digests[i] = c4_alone(*qs[i], &groups[i]);
+    if (plans.back().dense) {
+      dense_idx.push_back(i);
+    } else {
+      const uint64_t d = c4_alone(*qs[i], &groups[i]);
+      if (digests) (*digests)[i] = d;
+    }
   }
   if (dense_idx.empty()) return;
+  apply_tuned(kC4Shape);
   const C4Plan& p0 = plans[dense_idx[0]];
   const JitProgram* jit = jitGet(p0.c.env, p0.c.ncols, p0.plan, JF_DENSE, "aggregate (batched)");
   std::vector<std::unique_ptr<GpuAggregator>> aggs;
@@ -736,14 +848,13 @@ void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>& di
   wsd::ck(ws_agg_run_batch(m, handles.data(), envs.data(), jit ? jit->handle : nullptr));
   std::vector<int64_t> counts((size_t)m);
   int64_t total = 0;
-  std::vector<double> keys((size_t)std::max<int64_t>(cap, 1)), accs((size_t)std::max<int64_t>(cap * 5, 1));
-  wsd::ck(ws_agg_fetch_many(m, handles.data(), counts.data(), &total, cap, keys.data(), accs.data()));
+  t_results.reserve(std::max<int64_t>(cap, 1), 1, 5);
+  wsd::ck(ws_agg_fetch_many(m, handles.data(), counts.data(), &total, t_results.cap, t_results.keys, t_results.accs));
   int64_t off = 0;
   for (int j = 0; j < m; ++j) {
     const size_t i = dense_idx[(size_t)j];
     const int64_t g = counts[(size_t)j];
-    std::vector<double> k(keys.begin() + off, keys.begin() + off + g), a(accs.begin() + off * 5, accs.begin() + (off + g) * 5);
-    digests[i] = c4_digest(k, a, g);
+    if (digests) (*digests)[i] = c4_digest(t_results.keys + off, t_results.accs + off * 5, g);
     groups[i] = g;
     off += g;
   }
@@ -887,7 +998,7 @@ void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, 
         }
         std::vector<uint64_t> dg;
         std::vector<int64_t> gr;
-        c4_batched(qp, dg, gr);
+        c4_batched(qp, &dg, gr);
         const double d = now();
         for (size_t j = 0; j < batch.size(); ++j) rs[batch[j]].digest = dg[j], rs[batch[j]].done = d;
       }
@@ -1056,7 +1167,8 @@ void wsq_c4_free(void* set) {
   delete v;
 }
 // Runs every query of the set: batched (one launch + one fetch) or one by one (plain path).
-// digests[q] / groups[q] per query.
+// digests[q] / groups[q] per query; digests may be null (results fetched, not digested: what a
+// timing run measures).
 int wsq_c4_run(void* set, int batched, uint64_t* digests, int64_t* groups) {
   return guard([&] {
     auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)set;
@@ -1067,11 +1179,60 @@ int wsq_c4_run(void* set, int batched, uint64_t* digests, int64_t* groups) {
       for (auto& q : qs) qp.push_back(&q);
       std::vector<uint64_t> d;
       std::vector<int64_t> g;
-      c4_batched(qp, d, g);
-      for (size_t i = 0; i < qs.size(); ++i) digests[i] = d[i], groups[i] = g[i];
+      c4_batched(qp, digests ? &d : nullptr, g);
+      for (size_t i = 0; i < qs.size(); ++i) {
+        if (digests) digests[i] = d[i];
+        groups[i] = g[i];
+      }
     } else {
-      for (size_t i = 0; i < qs.size(); ++i) digests[i] = c4_alone(qs[i], &groups[i]);
+      for (size_t i = 0; i < qs.size(); ++i) {
+        const uint64_t d = c4_alone(qs[i], &groups[i], digests != nullptr);
+        if (digests) digests[i] = d;
+      }
     }
+  });
+}
+
+// The autotuner (see autotune()) on the batched workload of a C4 set: knobs batch_tile_rows,
+// batch_lds_slots, batch_blocks_per_cu. chosen[3] gets their values, ms[2] the median time of one
+// batched run with the starting and the chosen values; *trials the configurations timed.
+int wsq_c4_autotune(void* set, int reps, int64_t* chosen, double* ms, int* trials) {
+  return guard([&] {
+    auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)set;
+    std::vector<C4Query> qs;
+    for (auto& e : *v) qs.push_back(C4Query{e.first, e.second});
+    std::vector<const C4Query*> qp;
+    for (auto& q : qs) qp.push_back(&q);
+    {
+      std::lock_guard<std::mutex> g(g_tuned_mu);
+      g_tuned.erase(kC4Shape);  // tune from the current knob values
+    }
+    const std::vector<KnobSpace> space = {{"batch_tile_rows", {1024, 2048, 4096, 8192, 16384, 32768}},
+                                          {"batch_lds_slots", {256, 512, 1024, 2048, 4096}},
+                                          {"batch_blocks_per_cu", {1, 2, 4, 8, 16}}};
+    std::vector<int64_t> g;
+    const TuneResult r = autotune(kC4Shape, space, reps, [&] { c4_batched(qp, nullptr, g); });
+    for (size_t i = 0; i < r.chosen.size(); ++i) chosen[i] = r.chosen[i].second;
+    ms[0] = r.default_ms, ms[1] = r.best_ms;
+    *trials = r.trials;
+  });
+}
+// The autotuner on the web query over table t: knobs part_chunks_per_cu, part_scatter_lds_kib,
+// part_merge_lds_kib (chosen[3]), as wsq_c4_autotune.
+int wsq_c2_autotune(void* t, int reps, int64_t* chosen, double* ms, int* trials) {
+  return guard([&] {
+    {
+      std::lock_guard<std::mutex> g(g_tuned_mu);
+      g_tuned.erase(kC2Shape);
+    }
+    const std::vector<KnobSpace> space = {{"part_chunks_per_cu", {1, 2, 4}},
+                                          {"part_scatter_lds_kib", {32, 48, 64, 96, 120}},
+                                          {"part_merge_lds_kib", {32, 48, 60, 96, 120}}};
+    int64_t out[40];
+    const TuneResult r = autotune(kC2Shape, space, reps, [&] { query_c2(*(DevTable*)t, out, 10, nullptr); });
+    for (size_t i = 0; i < r.chosen.size(); ++i) chosen[i] = r.chosen[i].second;
+    ms[0] = r.default_ms, ms[1] = r.best_ms;
+    *trials = r.trials;
   });
 }
 

@@ -444,6 +444,16 @@ void ws_dev_free(void* p) {
   if (p) (void)hipFree(p);
 }
 int ws_tmp_alloc(size_t bytes, void** out) { return guard([&] { *out = tmp_alloc(bytes); }); }
+int ws_pinned_alloc(size_t bytes, void** out) {
+  return guard([&] {
+    ensure_init();
+    *out = nullptr;
+    if (bytes) WS_CK(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  });
+}
+void ws_pinned_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
 void ws_trim_pool(void) { trim_pool(); }
 void ws_tmp_free(void* p) { tmp_free(p); }
 int ws_h2d(void* dst, const void* src, size_t b) { return guard([&] { h2d_sync(dst, src, b); }); }

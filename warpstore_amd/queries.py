@@ -43,6 +43,9 @@ def qlib():
         Q.wsq_serve_alone.restype = I
         Q.wsq_set_sizer.argtypes = [P]
         Q.wsq_set_sizer.restype = None
+        for f in ("wsq_c4_autotune", "wsq_c2_autotune"):
+            getattr(Q, f).argtypes = [P, I, P, P, C.POINTER(I)]
+            getattr(Q, f).restype = I
         _q = Q
     return _q
 
@@ -116,17 +119,40 @@ class C4Set:
         _ck(qlib().wsq_c4_load(seed, nq, rows_override, C.byref(h), C.byref(tot)))
         self.h, self.nq, self.rows = h, nq, tot.value
 
-    def run(self, batched: bool):
+    def run(self, batched: bool, digests: bool = True):
+        """(digests, group counts) per query; digests=False fetches the results without digesting
+        them (what a timing run measures) and returns None for them."""
         d = np.zeros(self.nq, np.uint64)
         g = np.zeros(self.nq, np.int64)
-        _ck(qlib().wsq_c4_run(self.h, 1 if batched else 0, d.ctypes.data, g.ctypes.data))
-        return d, g
+        _ck(qlib().wsq_c4_run(self.h, 1 if batched else 0, d.ctypes.data if digests else None, g.ctypes.data))
+        return (d if digests else None), g
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Runs the autotuner on this set's batched workload; returns its choice and timings."""
+        chosen = np.zeros(3, np.int64)
+        ms = np.zeros(2)
+        trials = C.c_int()
+        _ck(qlib().wsq_c4_autotune(self.h, reps, chosen.ctypes.data, ms.ctypes.data, C.byref(trials)))
+        return {"shape": "dense GROUP BY batch (1 key, 5 slots)",
+                "chosen": dict(zip(["batch_tile_rows", "batch_lds_slots", "batch_blocks_per_cu"], chosen.tolist())),
+                "start_ms": round(float(ms[0]), 3), "tuned_ms": round(float(ms[1]), 3), "trials": trials.value}
+
 
     def free(self) -> None:
         if self.h:
             qlib().wsq_c4_free(self.h)
             self.h = None
 
+
+def autotune_web(t: "DeviceTable", reps: int = 3) -> dict:
+    """Runs the autotuner on the web query over t; returns its choice and timings."""
+    chosen = np.zeros(3, np.int64)
+    ms = np.zeros(2)
+    trials = C.c_int()
+    _ck(qlib().wsq_c2_autotune(t.h, reps, chosen.ctypes.data, ms.ctypes.data, C.byref(trials)))
+    return {"shape": "partitioned GROUP BY (web query, both phases)",
+            "chosen": dict(zip(["part_chunks_per_cu", "part_scatter_lds_kib", "part_merge_lds_kib"], chosen.tolist())),
+            "start_ms": round(float(ms[0]), 3), "tuned_ms": round(float(ms[1]), 3), "trials": trials.value}
 
 def serve(pool: C4Set, web=None, fact=None, dim=None, rate: float = 100.0, count: int = 200, mode: int = 1,
           workers: int = 2, max_batch: int = 256, seed: int = 5):

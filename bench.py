@@ -227,6 +227,9 @@ def main() -> int:
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="set a tuning knob of the physical plans (ws_tune_set) before running")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: skip the autotuner, use the knobs as set")
+    ap.add_argument("--no-learned", action="store_true", help="join / web workloads: skip the learned-sizing comparison")
+    ap.add_argument("--budget-mb", type=int, default=1536,
+                    help="serve workload: HBM budget for the requests' estimated temporaries (0: none)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 1 GiB device copy kernel (known bytes, for counter calibration)")
     ap.add_argument("--workload", choices=["join", "web", "filter", "batch", "serve"], default="join",
@@ -285,6 +288,8 @@ def bench_web(args) -> int:
     result["parity_vs_oracle"] = parity
     result["roofline"] = roofline(f"{symbol} ({what})", algo_bytes, launches, total_ms,
                                   pmc_traffic(symbol, "pmc_c2_web_analytics.json"))
+    if not args.no_learned:
+        result["learned_sizing"] = web_learned(ws, L, Q, tables, full, args, elapsed / args.steps)
     if not args.no_cpu:
         B = cpu_lib()
         cap = ROWS + 1
@@ -473,6 +478,8 @@ def bench_join(args) -> int:
     out["stage_ms"] = {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(warm.items(), key=lambda kv: -kv[1][1])[:8]}
     out["roofline"] = roofline(dom + " (compiled fact filter: fk - lo IN dimension-key bitmap)", 4 * JOIN_FACT, launches,
                                total_ms, pmc_traffic("ws_q_select_staged (fact filter)", "pmc_c3_star_join.json"))
+    if not args.no_learned:
+        out["learned_sizing"] = join_learned(ws, L, Q, fact, dim, exp, args, elapsed / args.steps)
     if not args.no_cpu:
         B = cpu_lib()
         res = np.zeros((256, 4), np.int64)
@@ -489,40 +496,123 @@ def bench_join(args) -> int:
     return 0 if parity else 1
 
 
+JOIN_TRAIN_FILTERS = (3, 8, 15, 25, 35, 65, 90, 130, 200, 300, 450, 700)  # dimension filters attr < x (not 50)
+
+
+def join_learned(ws, L, Q, fact, dim, exp, args, plain_s):
+    """The star join with the learned join-size model: trained on the GPU on the exact output
+    sizes of the same join under 12 other dimension filters (the measured filter attr < 50 is held
+    out), installed in the drivers, then timed like the plain path; results checked again."""
+    from warpstore_amd import sizing
+    sizing.log_clear("joins")
+    for a in JOIN_TRAIN_FILTERS:
+        Q.c3(fact, dim, attr_lt=a)
+    model = sizing.train_from_log("joins", epochs=600)
+    Q.set_sizers(joins=model.h)
+    try:
+        for _ in range(max(args.warmup, 1)):
+            Q.c3(fact, dim)
+        L.ws_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rows, st = Q.c3(fact, dim)
+        L.ws_sync()
+        learned_s = (time.perf_counter() - t0) / args.steps
+    finally:
+        Q.set_sizers(joins=None)
+    rows = np.asarray(rows)
+    same = bool(np.array_equal(rows[np.argsort(rows[:, 0])], exp[np.argsort(exp[:, 0])]))
+    return {"model": "MLP 16-32-32-1 on join key statistics, trained on the GPU", "training_joins": model.samples,
+            "train_loss": round(model.loss, 6), "pairs": int(st[2]), "predicted_pairs": int(st[3]),
+            "plain_ms_per_query": round(plain_s * 1e3, 4), "learned_ms_per_query": round(learned_s * 1e3, 4),
+            "speedup": round(plain_s / learned_s, 3), "parity_vs_oracle": same}
+
+
+WEB_TRAIN_FILTERS = [(19723 + d, r) for d in (30, 90, 150, 220, 365) for r in (16, 64, 128, 192, 256)]
+
+
+def web_learned(ws, L, Q, tables, full, args, plain_s):
+    """The web query with the learned group-count model: trained on the GPU on the exact group
+    counts both phases log under 25 other filters (the measured filter is held out), installed,
+    then timed over the same table copies; results checked again."""
+    from warpstore_amd import sizing
+    sizing.log_clear("groups")
+    for d, r in WEB_TRAIN_FILTERS:
+        Q.c2(tables[0], 10, date_lt=d, region_lt=r)
+    model = sizing.train_from_log("groups", epochs=600)
+    Q.set_sizers(groups=model.h)
+    try:
+        for w in range(max(args.warmup, 1)):
+            Q.c2(tables[w % len(tables)], 10)
+        L.ws_sync()
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            top, st = Q.c2(tables[s % len(tables)], 10)
+        L.ws_sync()
+        learned_s = (time.perf_counter() - t0) / args.steps
+    finally:
+        Q.set_sizers(groups=None)
+    return {"model": "MLP 16-32-32-1 on segment statistics, trained on the GPU", "training_aggregations": model.samples,
+            "train_loss": round(model.loss, 6), "phase1_capacity": int(st[2]), "overflow_retries": int(st[3]),
+            "plain_ms_per_query": round(plain_s * 1e3, 4), "learned_ms_per_query": round(learned_s * 1e3, 4),
+            "speedup": round(plain_s / learned_s, 3), "parity_vs_oracle": bool(check_top(top, full))}
+
+
 def bench_serve(args) -> int:
     """The open-loop mixed stream (seed 5): Poisson arrivals, 70 % small aggregates on 64k-row
     segments, 25 % web-analytics queries on the 10M-row table, 5 % star joins (fact 60M, dim 1.5M).
-    Per offered rate, continuous batching (one thread batches the small aggregates, 2 threads run
-    the others concurrently) against per-query serving (4 concurrent queries, each alone); reports
-    queries/s and p50 / p99 latency."""
+    Both forms admit requests under the same HBM budget for their estimated temporaries
+    (--budget-mb). Per offered rate: the serving form (continuous batching: one thread batches the
+    small aggregates, 2 threads run the others concurrently; learned group-count and join-size
+    models installed, so estimates are predictions) against the plain per-query form (4 concurrent
+    queries, each alone; the range rules' upper-bound estimates). The models are trained first on
+    the GPU from the exact counts of the same queries under other filters. Reports queries/s with
+    p50 / p99 latency, one line per rate and form, then a summary."""
     import warpstore_amd as ws
-    from warpstore_amd import abi, queries as Q
+    from warpstore_amd import abi, queries as Q, sizing
     L = ws.lib()
     ws.check(L.ws_init())
     L.ws_jit_set_mode(abi.JIT_SYNC)
     pool = Q.C4Set(seed=5, nq=256, rows_override=65536)
     cols = gen_c2(ROWS)
     web = Q.DeviceTable(list(cols), [abi.T_I64, abi.T_I32, abi.T_I32, abi.T_I32, abi.T_I32], [0, 1, 0, 0, 0])
-    g = ws.datagen()
-    nf, nd = 60_000_000, 1_500_000
-    dkey, dattr = np.empty(nd, np.int32), np.empty(nd, np.int32)
-    g.wsg_c3_dim(3, nd, ws.ptr(dkey), ws.ptr(dattr))
-    fk, qty, price, flag = np.empty(nf, np.int32), np.empty(nf, np.int64), np.empty(nf, np.int64), np.empty(nf, np.int32)
-    g.wsg_c3_fact(3, nf, nd, ws.ptr(fk), ws.ptr(qty), ws.ptr(price), ws.ptr(flag))
+    (fk, qty, price, flag), (dkey, dattr) = c3_data(ws)
     fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
     dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
     del fk, qty, price, flag
-    Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=1, seed=1)  # warm-up: compile plans
-    Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=0, seed=1)
+    sizing.log_clear("groups")
+    sizing.log_clear("joins")
+    for d, r in WEB_TRAIN_FILTERS:
+        Q.c2(web, 10, date_lt=d, region_lt=r)
+    for a in JOIN_TRAIN_FILTERS:
+        Q.c3(fact, dim, attr_lt=a)
+    gm, jm = sizing.train_from_log("groups", epochs=600), sizing.train_from_log("joins", epochs=600)
+    budget = args.budget_mb << 20
+    forms = (("learned_batched", 1, 2, (gm.h, jm.h)), ("plain_per_query", 0, 4, (None, None)))
+    for name, mode, workers, models in forms:  # warm-up: compile plans
+        Q.set_sizers(*models)
+        Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=mode, workers=workers, seed=1, budget_bytes=budget)
     rows = []
     for rate in (400.0, 1600.0, 3200.0, 6400.0):
         count = int(min(4000, max(400, rate * 1.5)))
-        for mode, name in ((1, "continuous_batching"), (0, "per_query")):
-            st, _, _ = Q.serve(pool, web, fact, dim, rate=rate, count=count, mode=mode, workers=2 if mode else 4,
-                               max_batch=256, seed=5)
-            st.update({"offered_rate": rate, "mode": name})
+        for name, mode, workers, models in forms:
+            Q.set_sizers(*models)
+            st, _, _ = Q.serve(pool, web, fact, dim, rate=rate, count=count, mode=mode, workers=workers, max_batch=256, seed=5,
+                               budget_bytes=budget)
+            st.update({"offered_rate": rate, "form": name, "budget_mb": args.budget_mb})
             rows.append(st)
             print(json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}), flush=True)
+    Q.set_sizers(None, None)
+    # capacity at equal p99: the highest sustained q/s of each form among the rates where its p99
+    # stays within the plain form's p99 at its own highest sustained rate
+    summary = {}
+    for name in ("learned_batched", "plain_per_query"):
+        rs = [r for r in rows if r["form"] == name]
+        summary[name] = {"max_qps": round(max(r["qps"] for r in rs), 1),
+                         "p99_ms_at_max": round(max(rs, key=lambda r: r["qps"])["p99_ms"], 3)}
+    summary["qps_ratio"] = round(summary["learned_batched"]["max_qps"] / summary["plain_per_query"]["max_qps"], 3)
+    print(json.dumps({"serving_summary": summary, "models": {"groups_trained_on": gm.samples, "joins_trained_on": jm.samples}}),
+          flush=True)
     for t in (web, fact, dim):
         t.free()
     pool.free()

@@ -173,6 +173,15 @@ int ws_agg_set_features(ws_agg* a, const float* features); /* log (features, exa
 int64_t ws_sizing_log_size(void);
 int64_t ws_sizing_log_copy(float* x, float* t, int64_t cap); /* x: cap x 16; t: log2(groups + 1) / 32 */
 void ws_sizing_log_clear(void);
+/* Join output size model (no reference counterpart; the reference counts, scans and reads the size
+   back, gpu.cu:1129-1193): features of an equi-join from its key columns' statistics, and a log of
+   exact output sizes for training a sizer model, whose ws_sizer_groups then predicts pairs. */
+int ws_join_features(const ws_col_stats* build_key, int64_t build_rows, int64_t build_table_rows, const ws_col_stats* probe_key,
+                     int64_t probe_rows, float* out);
+void ws_join_log_add(const float* features, int64_t pairs);
+int64_t ws_join_log_size(void);
+int64_t ws_join_log_copy(float* x, float* t, int64_t cap); /* x: cap x 16; t: log2(pairs + 1) / 32 */
+void ws_join_log_clear(void);
 
 /* ------------------------------------------------------------------ selection / gathers */
 int ws_select(const ws_expr_env* env, int has_where, ws_prog where, int64_t n, const uint32_t* del, int32_t* out_idx,
@@ -236,6 +245,11 @@ int ws_hash_join_parts(ws_colview build_key, const int64_t* b_off, int pb, ws_co
                        int pp, int32_t** out_build, int32_t** out_probe, int64_t* out_off, int64_t* m);
 int ws_hash_join_outer(ws_colview build_key, int64_t nb, ws_colview probe_key, int64_t np, int32_t** out_build,
                        int32_t** out_probe, int full, int64_t* m); /* gpuHashJoinOuter gpu.h:189-190 */
+/* gpuHashJoin with a predicted output size (expected_pairs > 0): the single-probe form writes into
+   buffers of that size without reading the count back first; a smaller real size is exact, a larger
+   one costs a second write pass. Same pairs and order as ws_hash_join. */
+int ws_hash_join_sized(ws_colview build_key, int64_t nb, ws_colview probe_key, int64_t np, int64_t expected_pairs,
+                       int32_t** out_build, int32_t** out_probe, int64_t* m);
 int ws_cross_join(int64_t na, int64_t nb, int32_t** out_a, int32_t** out_b, int64_t* m); /* gpuCrossJoin gpu.h:191 */
 
 /* ------------------------------------------------------------------ sorting */

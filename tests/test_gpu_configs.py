@@ -4,8 +4,9 @@
     keys), batched and one by one, against numpy per query (and the oracle's aggregation on a
     sample of them);
   - the serving mix on the 10M-row web table, the 60M x 1.5M star join and 256 segments of 64k
-    rows: every request's result in the continuously batched stream equals its result alone, in
-    the per-query stream, and the CPU's (numpy / oracle) result.
+    rows: every request's result in the continuously batched stream (learned sizing models on,
+    HBM-budget admission) equals its result alone, in the per-query stream, and the CPU's (numpy /
+    oracle) result.
 Results are compared through the drivers' order-independent digests, recomputed here on the CPU
 from the independent results."""
 import numpy as np
@@ -145,9 +146,22 @@ def test_c5_serving_full_tables(gpu):
     fk, qty, price, flag, dkey, dattr = H.gen_c3(60_000_000, 1_500_000)
     fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
     dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
+    from warpstore_amd import sizing
     try:
+        # the serving form with learned sizing on: group-count and join-size models trained on the
+        # GPU from the same queries under other filters, and a tight HBM budget for admission
+        sizing.log_clear("groups")
+        sizing.log_clear("joins")
+        for d, r in ((19723 + 90, 64), (19723 + 200, 128), (19723 + 365, 256), (19723 + 30, 200)):
+            Q.c2(web, 10, date_lt=d, region_lt=r)
+        for a in (10, 30, 100, 300):
+            Q.c3(fact, dim, attr_lt=a)
+        gm, jm = sizing.train_from_log("groups", epochs=300), sizing.train_from_log("joins", epochs=300)
         n, rate = 120, 300.0
-        st1, d1, ty = Q.serve(pool, web, fact, dim, rate=rate, count=n, mode=1, workers=2, max_batch=256, seed=11)
+        Q.set_sizers(gm.h, jm.h)
+        st1, d1, ty = Q.serve(pool, web, fact, dim, rate=rate, count=n, mode=1, workers=2, max_batch=256, seed=11,
+                              budget_bytes=1 << 30)
+        Q.set_sizers(None, None)
         st0, d0, ty0 = Q.serve(pool, web, fact, dim, rate=rate, count=n, mode=0, workers=4, seed=11)
         assert st1["completed"] == n and st0["completed"] == n
         assert np.array_equal(ty, ty0) and set(ty.tolist()) == {0, 1, 2}
@@ -162,6 +176,7 @@ def test_c5_serving_full_tables(gpu):
         H.say(f"  batched {st1}")
         H.say(f"  per-query {st0}")
     finally:
+        Q.set_sizers(None, None)
         for t in (web, fact, dim):
             t.free()
         pool.free()

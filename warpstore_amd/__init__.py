@@ -84,6 +84,9 @@ _SIGS = {
     "ws_hash_join_parts": (I, [ColView, P, I, ColView, P, I, PP, PP, P, C.POINTER(I64)]),
     "ws_hash_join_outer": (I, [ColView, I64, ColView, I64, PP, PP, I, C.POINTER(I64)]),
     "ws_cross_join": (I, [I64, I64, PP, PP, C.POINTER(I64)]),
+    "ws_hash_join_sized": (I, [ColView, I64, ColView, I64, I64, PP, PP, C.POINTER(I64)]),
+    "ws_join_features": (I, [P, I64, I64, P, I64, P]), "ws_join_log_add": (None, [P, I64]),
+    "ws_join_log_size": (I64, []), "ws_join_log_copy": (I64, [P, P, I64]), "ws_join_log_clear": (None, []),
     "ws_sort_perm": (I, [P, P, I, I64, PP]), "ws_topk_perm": (I, [P, P, I, I64, I64, PP]), "ws_map_rank": (I, [P, P, I64]),
     "ws_sort_perm_host": (I, [P, I, I64, P]),
     "ws_jit_enabled": (I, []), "ws_jit_mode": (I, []), "ws_jit_set_mode": (None, [I]),

@@ -267,8 +267,9 @@ __global__ void __launch_bounds__(kJBlock) join_probe1_k(ProbeDesc d, int32_t* f
 }
 __global__ void __launch_bounds__(1024) join_scan_k(uint32_t* cnt, int64_t n, u64* total) { scan_counts_body(cnt, n, total); }
 __global__ void __launch_bounds__(kJBlock) join_write1_k(ProbeDesc d, const int32_t* first, const uint32_t* tile_off, int32_t* ob,
-                                                         int32_t* op, uint8_t* matched) {
+                                                         int32_t* op, uint8_t* matched, const u64* total, u64 cap) {
   __shared__ uint32_t s_wtot[kJBlock / 64];
+  if (*total > cap) return;  // buffers sized ahead of the count (learned size) and too small: written later
   const i64 t = blockIdx.x;
   const int p = part_of_tile(d, t);
   const i64 r0 = d.p_off[p] + (t - d.tile0[p]) * kJTile;
@@ -332,8 +333,11 @@ struct JoinResult {
 
 // Builds one table per build partition and probes every probe partition against its table
 // (pb == 1: the one table is shared). outer: every probe row appears; matched: flags of build rows.
+// expect > 0 (a predicted output size, e.g. from the learned join model): the single-probe form
+// allocates its pair buffers at that size and writes them without first reading the count back;
+// a prediction below the real size costs one more write pass, never a wrong result.
 static JoinResult join_core(ws_colview bkey, const std::vector<int64_t>& b_off, ws_colview pkey, const std::vector<int64_t>& p_off,
-                            bool outer, uint8_t* matched) {
+                            bool outer, uint8_t* matched, int64_t expect = 0) {
   JoinResult res;
   const int pb = (int)b_off.size() - 1, pp = (int)p_off.size() - 1;
   if (pb != 1 && pb != pp) throw Error("internal: partition count mismatch in join", WS_ERR_INTERNAL);
@@ -405,6 +409,16 @@ static JoinResult join_core(ws_colview bkey, const std::vector<int64_t>& b_off, 
                          (const u64*)s1.p, (int64_t*)(s1.p + 2));
       check_launch("join_probe");
     }
+    const int64_t cap = expect > 0 ? std::min<int64_t>(expect, (int64_t)INT32_MAX) : 0;
+    DevBuf<int32_t> eb, ep;
+    if (cap > 0) {  // sized ahead: the write is queued behind the count, no round trip between
+      eb = DevBuf<int32_t>(cap);
+      ep = DevBuf<int32_t>(cap);
+      ProfScope ps("join_write");
+      hipLaunchKernelGGL(join_write1_k, dim3((unsigned)ntiles), dim3(kJBlock), 0, stream(), d, (const int32_t*)first.p,
+                         (const uint32_t*)tcnt.p, eb.p, ep.p, matched, (const u64*)s1.p, (u64)cap);
+      check_launch("join_write");
+    }
     std::vector<u64> h(2 + pp + 1);
     d2h_sync(h.data(), s1.p, sizeof(u64) * h.size());
     if (!(int)h[1]) {
@@ -412,12 +426,17 @@ static JoinResult join_core(ws_colview bkey, const std::vector<int64_t>& b_off, 
       if (m > INT32_MAX) throw Error("join result too large (>2^31 rows)", WS_ERR_INTERNAL);
       for (int p = 0; p <= pp; ++p) res.out_off[p] = (int64_t)h[2 + p];
       res.m = m;
+      if (cap > 0 && m <= cap) {
+        res.b = std::move(eb);
+        res.p = std::move(ep);
+        return res;
+      }
       res.b = DevBuf<int32_t>(std::max<int64_t>(m, 1));
       res.p = DevBuf<int32_t>(std::max<int64_t>(m, 1));
       if (m) {
         ProfScope ps("join_write");
         hipLaunchKernelGGL(join_write1_k, dim3((unsigned)ntiles), dim3(kJBlock), 0, stream(), d, (const int32_t*)first.p,
-                           (const uint32_t*)tcnt.p, res.b.p, res.p.p, matched);
+                           (const uint32_t*)tcnt.p, res.b.p, res.p.p, matched, (const u64*)s1.p, (u64)m);
         check_launch("join_write");
       }
       return res;
@@ -477,6 +496,20 @@ int ws_hash_join_parts(ws_colview build_key, const int64_t* b_off, int pb, ws_co
     JoinResult r = join_core(build_key, bo, probe_key, po, false, nullptr);
     if (out_off)
       for (int p = 0; p <= pp; ++p) out_off[p] = r.out_off[p];
+    *m = r.m;
+    if (r.m) {
+      *out_build = r.b.release();
+      *out_probe = r.p.release();
+    }
+  });
+}
+int ws_hash_join_sized(ws_colview build_key, int64_t nb, ws_colview probe_key, int64_t np, int64_t expected_pairs,
+                       int32_t** out_build, int32_t** out_probe, int64_t* m) {
+  return guard([&] {
+    *out_build = *out_probe = nullptr;
+    *m = 0;
+    if (nb <= 0 || np <= 0) return;  // as ws_hash_join: no pairs, no buffers
+    JoinResult r = join_core(build_key, {0, nb}, probe_key, {0, np}, false, nullptr, expected_pairs);
     *m = r.m;
     if (r.m) {
       *out_build = r.b.release();

@@ -359,7 +359,8 @@ const char* const kC2Shape = "partitioned GROUP BY (web query, both phases)";
 struct C2Stats {
   int64_t pairs = 0, groups = 0, cap1 = 0, retries = 0;
 };
-int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
+int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st, int64_t date_lt = 19723 + 300,
+                 int64_t region_lt = 224) {
   apply_tuned(kC2Shape);
   Comp c;
   // slots in the engine's compile order: WHERE, keys, aggregates, then the DISTINCT argument
@@ -369,7 +370,7 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
   AggPlan plan;
   std::memset(&plan, 0, sizeof plan);
   plan.has_where = 1;
-  plan.where = c.prog({c.col(s_date), c.ilit(19723 + 300), Comp::op(WS_OP_LT), c.col(s_region), c.ilit(224),
+  plan.where = c.prog({c.col(s_date), c.ilit(date_lt), Comp::op(WS_OP_LT), c.col(s_region), c.ilit(region_lt),
                        Comp::op(WS_OP_LT), Comp::op(WS_OP_AND)});
   plan.nkeys = 1;
   plan.keys[0] = c.prog({c.col(s_user)});
@@ -391,7 +392,7 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st) {
   const JitProgram* jit1 = jitGet(c.env, c.ncols, plan1, JF_GROUPED, "aggregate (distinct pairs)");
   int64_t cap1 = pow2_at_least(std::min<int64_t>(std::max<int64_t>(t.rows, 1), 1 << 22), 1024);
   float f1[WS_SIZER_INPUTS];
-  const double sel = frac_below(t.cols[2], 19723 + 300) * frac_below(t.cols[1], 224);
+  const double sel = frac_below(t.cols[2], date_lt) * frac_below(t.cols[1], region_lt);
   const int64_t pred1 = size_groupby({&t.cols[0], &t.cols[3]}, t.rows, sel, f1);
   if (pred1 > 0) cap1 = pow2_at_least(pred1 * 5 / 2, 1024);  // 2x headroom + 25 % for the error
   DevVec<double> dk, da;
@@ -571,13 +572,16 @@ int64_t materialize(const DevTable& t, Comp& c, bool has_where, Prog where, cons
 // dim columns: 0 key (INT), 1 attr (INT). Result rows: flag, count, raw price sum (scaled by
 // 100), qty sum.
 struct C3Stats {
-  int64_t dim_rows = 0, fact_rows = 0, pairs = 0;
+  int64_t dim_rows = 0, fact_rows = 0, pairs = 0, predicted_pairs = 0;
 };
-int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stats* st) {
-  // dim: materialize(dim, [attr < 50], {key})
+// The join output model the drivers consult (null: the reference's count, scan, read-back sizing).
+std::atomic<ws_sizer*> g_join_sizer{nullptr};
+
+int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stats* st, int64_t attr_lt = 50) {
+  // dim: materialize(dim, [attr < attr_lt], {key})
   Comp cd;
   const int d_attr = cd.slot(WS_T_I32);
-  const Prog dwhere = cd.prog({cd.col(d_attr), cd.ilit(50), Comp::op(WS_OP_LT)});
+  const Prog dwhere = cd.prog({cd.col(d_attr), cd.ilit(attr_lt), Comp::op(WS_OP_LT)});
   std::vector<RelCol> drel;
   const int64_t nd = materialize(dim, cd, true, dwhere, {1}, {0}, drel, "filter (dimension)");
   // join filter: the dim keys were reduced below half their table, fact is large: exact bitmap
@@ -601,7 +605,22 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
   ColView bk{}, pk{};
   bk.p = drel[0].data.p, bk.t = WS_T_I32;
   pk.p = frel[0].data.p, pk.t = WS_T_I32;
-  const int64_t m = gpuHashJoin(bk, nd, pk, nf, ib, ip);
+  // output size: predicted by the join model when one is installed (pair buffers sized ahead, no
+  // read-back between count and write), else counted first; the exact size is logged either way
+  float jf[WS_SIZER_INPUTS];
+  wsd::ck(ws_join_features(&dim.cols[0].stats, nd, dim.rows, &fact.cols[0].stats, nf, jf));
+  int64_t pred = 0;
+  if (ws_sizer* model = g_join_sizer.load()) wsd::ck(ws_sizer_groups(model, jf, &pred));
+  int64_t m = 0;
+  if (pred > 0) {
+    int32_t *b = nullptr, *p = nullptr;
+    wsd::ck(ws_hash_join_sized(wsd::view(bk), nd, wsd::view(pk), nf, pred + pred / 8 + 1024, &b, &p, &m));
+    ib = DevVec<int32_t>::adopt(b, b ? m : 0);
+    ip = DevVec<int32_t>::adopt(p, p ? m : 0);
+  } else {
+    m = gpuHashJoin(bk, nd, pk, nf, ib, ip);
+  }
+  ws_join_log_add(jf, m);
   // joined relation: gather every needed column of both sides by pair index
   std::vector<DevVec<uint8_t>> jcols;
   auto gather_side = [&](const RelCol& rc, const int32_t* idx) {
@@ -661,7 +680,7 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
     out[i * 4 + 2] = (int64_t)accs[i * 4 + 1];  // exact: per-group raw sums < 2^53
     out[i * 4 + 3] = (int64_t)((__int128)hi2 * ((__int128)1 << 32) + lo2);
   }
-  if (st) st->dim_rows = nd, st->fact_rows = nf, st->pairs = m;
+  if (st) st->dim_rows = nd, st->fact_rows = nf, st->pairs = m, st->predicted_pairs = pred;
   return g;
 }
 
@@ -924,6 +943,62 @@ uint64_t run_heavy(const ServeSet& ss, const Request& r) {
   return d;
 }
 
+// Device memory a request's temporaries take, estimated before it runs: from the installed sizing
+// models when there are (predicted groups and join pairs), else from the plain rules' upper bounds
+// (every filtered row a group or a pair). Used by the serving loop's HBM-budget admission.
+int64_t need_of(const ServeSet& ss, const Request& r) {
+  if (r.type == kReqSmall) {
+    const DevTable& t = *ss.pool[(size_t)r.pick];
+    return (t.cols[0].mx - t.cols[0].mn + 1) * 48 + (64 << 10);  // dense table + fetch buffers
+  }
+  if (r.type == kReqWeb) {
+    const DevTable& t = *ss.web;
+    const double sel = frac_below(t.cols[2], 19723 + 300) * frac_below(t.cols[1], 224);
+    float f[WS_SIZER_INPUTS];
+    const int64_t pred = size_groupby({&t.cols[0], &t.cols[3]}, t.rows, sel, f);
+    const int64_t pass = (int64_t)((double)t.rows * sel) + 1;
+    const int64_t groups = pred > 0 ? std::min<int64_t>(pred + pred / 4, pass) : pass;
+    return pass * 24 + groups * 96;  // phase-1 records + both phases' groups and tables
+  }
+  const DevTable &f = *ss.fact, &d = *ss.dim;
+  const double sel = frac_below(d.cols[1], 50);
+  const int64_t nd = (int64_t)((double)d.rows * sel) + 1, nf = (int64_t)((double)f.rows * sel) + 1;
+  int64_t pairs = f.rows;  // plain bound: every fact row matches
+  if (ws_sizer* model = g_join_sizer.load()) {
+    float jf[WS_SIZER_INPUTS];
+    wsd::ck(ws_join_features(&d.cols[0].stats, nd, d.rows, &f.cols[0].stats, nf, jf));
+    int64_t pred = 0;
+    wsd::ck(ws_sizer_groups(model, jf, &pred));
+    pairs = std::min<int64_t>(f.rows, pred + pred / 4);
+  }
+  return d.rows * 8 + pairs * 72;  // dimension relation + fact relation, join pairs and gathers
+}
+
+// HBM-budget admission: a request starts when its estimated temporaries fit the budget next to
+// those of the requests running (one that alone exceeds the budget runs by itself).
+struct Admission {
+  int64_t budget = 0;  // 0: no budget
+  int64_t reserved = 0, peak = 0, waits = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  int64_t acquire(int64_t need) {
+    if (budget <= 0) return 0;
+    need = std::min(need, budget);
+    std::unique_lock<std::mutex> g(mu);
+    if (reserved + need > budget) ++waits;
+    cv.wait(g, [&] { return reserved + need <= budget; });
+    reserved += need;
+    peak = std::max(peak, reserved);
+    return need;
+  }
+  void release(int64_t held) {
+    if (held <= 0) return;
+    std::lock_guard<std::mutex> g(mu);
+    reserved -= held;
+    cv.notify_all();
+  }
+};
+
 uint64_t run_alone(const ServeSet& ss, const Request& r) {
   if (r.type != kReqSmall) return run_heavy(ss, r);
   C4Query q{ss.pool[(size_t)r.pick], r.theta};
@@ -940,7 +1015,7 @@ namespace {
 // small request that has arrived (up to `max_batch`) into one batched launch and fetch; `workers`
 // threads serve the web / join queries at the same time, so their work overlaps on the device.
 // Requests that arrive while others run join the next batch. Latency = completion - arrival.
-void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, int max_batch) {
+void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, int max_batch, Admission& adm) {
   std::mutex mu;
   std::condition_variable cv;
   std::deque<size_t> small_q, heavy_q, all_q;
@@ -973,7 +1048,9 @@ void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, 
             i = all_q.front();
             all_q.pop_front();
           }
+          const int64_t held = adm.acquire(need_of(ss, rs[i]));
           rs[i].digest = run_alone(ss, rs[i]);
+          adm.release(held);
           rs[i].done = now();
         }
       }));
@@ -998,7 +1075,11 @@ void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, 
         }
         std::vector<uint64_t> dg;
         std::vector<int64_t> gr;
+        int64_t need = 0;
+        for (size_t j : batch) need += need_of(ss, rs[j]);
+        const int64_t held = adm.acquire(need);
         c4_batched(qp, &dg, gr);
+        adm.release(held);
         const double d = now();
         for (size_t j = 0; j < batch.size(); ++j) rs[batch[j]].digest = dg[j], rs[batch[j]].done = d;
       }
@@ -1014,7 +1095,9 @@ void serve(const ServeSet& ss, std::vector<Request>& rs, int mode, int workers, 
             i = heavy_q.front();
             heavy_q.pop_front();
           }
+          const int64_t held = adm.acquire(need_of(ss, rs[i]));
           rs[i].digest = run_heavy(ss, rs[i]);
+          adm.release(held);
           rs[i].done = now();
         }
       }));
@@ -1118,14 +1201,25 @@ int wsq_c2(void* t, int64_t limit, int64_t* out, int64_t* nrows, int64_t* stats)
     if (stats) stats[0] = st.pairs, stats[1] = st.groups, stats[2] = st.cap1, stats[3] = st.retries;
   });
 }
-// stats: dim rows, fact rows after the join filter, join pairs
-int wsq_c3(void* fact, void* dim, int64_t* out, int64_t* groups, int64_t* stats) {
+// The same query with WHERE event_date < date_lt AND region < region_lt.
+int wsq_c2_where(void* t, int64_t date_lt, int64_t region_lt, int64_t limit, int64_t* out, int64_t* nrows, int64_t* stats) {
   return guard([&] {
-    C3Stats st;
-    *groups = query_c3(*(DevTable*)fact, *(DevTable*)dim, out, &st);
-    if (stats) stats[0] = st.dim_rows, stats[1] = st.fact_rows, stats[2] = st.pairs;
+    C2Stats st;
+    *nrows = query_c2(*(DevTable*)t, out, limit, &st, date_lt, region_lt);
+    if (stats) stats[0] = st.pairs, stats[1] = st.groups, stats[2] = st.cap1, stats[3] = st.retries;
   });
 }
+// stats: dim rows, fact rows after the join filter, join pairs, predicted pairs (0: no model).
+// The dimension filter is attr < attr_lt (the measured query: 50).
+int wsq_c3(void* fact, void* dim, int64_t attr_lt, int64_t* out, int64_t* groups, int64_t* stats) {
+  return guard([&] {
+    C3Stats st;
+    *groups = query_c3(*(DevTable*)fact, *(DevTable*)dim, out, &st, attr_lt);
+    if (stats) stats[0] = st.dim_rows, stats[1] = st.fact_rows, stats[2] = st.pairs, stats[3] = st.predicted_pairs;
+  });
+}
+// The join output model (null: count first, as the reference). Not owned.
+void wsq_set_join_sizer(ws_sizer* m) { g_join_sizer.store(m); }
 
 // The sizing model the drivers consult (null to go back to the range rule alone). Not owned.
 void wsq_set_sizer(ws_sizer* m) { g_sizer.store(m); }
@@ -1240,8 +1334,10 @@ int wsq_c2_autotune(void* t, int reps, int64_t* chosen, double* ms, int* trials)
 // the small aggregates; web / fact / dim may be null (the mix then shifts to what exists).
 // stats: [0] completed, [1] wall seconds, [2] queries/s, [3] p50 ms, [4] p99 ms, [5] small p50,
 // [6] small p99, [7] heavy p50, [8] heavy p99 (ms). digests / types per request (may be null).
+// budget_bytes > 0: HBM-budget admission of the requests' estimated temporaries. stats gets 11
+// values: the 9 above, then [9] requests that waited for memory, [10] peak estimated bytes held.
 int wsq_serve(void* pool, void* web, void* fact, void* dim, double rate, int64_t count, int mode, int workers, int max_batch,
-              uint64_t seed, double* stats, uint64_t* digests, int32_t* types) {
+              uint64_t seed, int64_t budget_bytes, double* stats, uint64_t* digests, int32_t* types) {
   return guard([&] {
     auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)pool;
     ServeSet ss;
@@ -1250,7 +1346,9 @@ int wsq_serve(void* pool, void* web, void* fact, void* dim, double rate, int64_t
     ss.fact = (const DevTable*)fact;
     ss.dim = (const DevTable*)dim;
     std::vector<Request> rs = make_stream(ss, rate, count, seed);
-    serve(ss, rs, mode, workers, max_batch);
+    Admission adm;
+    adm.budget = budget_bytes;
+    serve(ss, rs, mode, workers, max_batch, adm);
     std::vector<double> all, sm, hv;
     double end = 0;
     for (size_t i = 0; i < rs.size(); ++i) {
@@ -1262,8 +1360,8 @@ int wsq_serve(void* pool, void* web, void* fact, void* dim, double rate, int64_t
       if (types) types[i] = rs[i].type;
     }
     const double wall = end - (rs.empty() ? 0 : rs[0].arrive);
-    const double out[9] = {(double)rs.size(), wall, wall > 0 ? (double)rs.size() / wall : 0, pct(all, 0.5), pct(all, 0.99),
-                           pct(sm, 0.5), pct(sm, 0.99), pct(hv, 0.5), pct(hv, 0.99)};
+    const double out[11] = {(double)rs.size(), wall, wall > 0 ? (double)rs.size() / wall : 0, pct(all, 0.5), pct(all, 0.99),
+                            pct(sm, 0.5), pct(sm, 0.99), pct(hv, 0.5), pct(hv, 0.99), (double)adm.waits, (double)adm.peak};
     std::memcpy(stats, out, sizeof out);
   });
 }

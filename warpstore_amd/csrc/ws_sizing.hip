@@ -165,6 +165,9 @@ void sizing_log(const float* features, int64_t groups) {
   g_log_x.insert(g_log_x.end(), features, features + kIn);
   g_log_t.push_back((float)(std::log2((double)groups + 1.0) / 32.0));
 }
+// Exact join output sizes logged by the callers (ws_join_log_add), the join model's training set.
+std::mutex g_jlog_mu;
+std::vector<float> g_jlog_x, g_jlog_t;
 
 }  // namespace ws
 
@@ -340,6 +343,62 @@ int ws_sizer_groups(ws_sizer* s, const float* features, int64_t* groups) {
     const double g = std::exp2(std::min(62.0, std::max(0.0, 32.0 * (double)y))) - 1.0;
     *groups = std::max<int64_t>(1, (int64_t)std::llround(g));
   });
+}
+
+// Features of an equi-join's output size from the key columns' statistics (table-level: the
+// build side may have been filtered to build_rows of build_table_rows rows, so its distinct keys
+// are capped there). Slots: log2 probe rows, log2 build rows, build selectivity, log2 distinct
+// build / probe keys, log2 key spans, key-range overlap over the probe span, log2 build rows per
+// distinct key, log2 of the containment estimate |B| |P| / max(d_B, d_P), NULL fractions.
+int ws_join_features(const ws_col_stats* build_key, int64_t build_rows, int64_t build_table_rows, const ws_col_stats* probe_key,
+                     int64_t probe_rows, float* out) {
+  return guard([&] {
+    std::memset(out, 0, sizeof(float) * kIn);
+    const double nb = (double)std::max<int64_t>(build_rows, 0), np = (double)std::max<int64_t>(probe_rows, 0);
+    const double sel = build_table_rows > 0 ? std::min(1.0, nb / (double)build_table_rows) : 1.0;
+    auto span = [](const ws_col_stats& c) { return c.imin <= c.imax ? (double)c.imax - (double)c.imin + 1 : 0.0; };
+    const double db = std::min(nb, ws_hll_estimate(build_key->hll) * sel + 1), dp = ws_hll_estimate(probe_key->hll);
+    const double sb = span(*build_key), sp = span(*probe_key);
+    double overlap = 0;
+    if (sb > 0 && sp > 0) {
+      const double lo = std::max((double)build_key->imin, (double)probe_key->imin);
+      const double hi = std::min((double)build_key->imax, (double)probe_key->imax);
+      overlap = hi >= lo ? (hi - lo + 1) / sp : 0.0;
+    }
+    out[0] = (float)(std::log2(np + 1) / 32);
+    out[1] = (float)(std::log2(nb + 1) / 32);
+    out[2] = (float)sel;
+    out[3] = (float)(std::log2(db + 1) / 32);
+    out[4] = (float)(std::log2(dp + 1) / 32);
+    out[5] = (float)(std::log2(sb + 1) / 32);
+    out[6] = (float)(std::log2(sp + 1) / 32);
+    out[7] = (float)overlap;
+    out[8] = (float)(std::log2(nb / std::max(db, 1.0) + 1) / 32);
+    out[9] = (float)(std::log2(nb * np * overlap / std::max(std::max(db, dp), 1.0) + 1) / 32);
+    out[10] = (float)(probe_key->rows ? (double)probe_key->nulls / (double)probe_key->rows : 0.0);
+    out[11] = (float)(build_key->rows ? (double)build_key->nulls / (double)build_key->rows : 0.0);
+  });
+}
+void ws_join_log_add(const float* features, int64_t pairs) {
+  std::lock_guard<std::mutex> g(g_jlog_mu);
+  g_jlog_x.insert(g_jlog_x.end(), features, features + kIn);
+  g_jlog_t.push_back((float)(std::log2((double)std::max<int64_t>(pairs, 0) + 1.0) / 32.0));
+}
+int64_t ws_join_log_size(void) {
+  std::lock_guard<std::mutex> g(g_jlog_mu);
+  return (int64_t)g_jlog_t.size();
+}
+int64_t ws_join_log_copy(float* x, float* t, int64_t cap) {
+  std::lock_guard<std::mutex> g(g_jlog_mu);
+  const int64_t n = std::min<int64_t>(cap, (int64_t)g_jlog_t.size());
+  if (x) std::memcpy(x, g_jlog_x.data(), sizeof(float) * kIn * n);
+  if (t) std::memcpy(t, g_jlog_t.data(), sizeof(float) * n);
+  return n;
+}
+void ws_join_log_clear(void) {
+  std::lock_guard<std::mutex> g(g_jlog_mu);
+  g_jlog_x.clear();
+  g_jlog_t.clear();
 }
 
 int64_t ws_sizing_log_size(void) {

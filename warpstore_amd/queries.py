@@ -30,14 +30,18 @@ def qlib():
         Q.wsq_c1.restype = I
         Q.wsq_c2.argtypes = [P, I64, P, C.POINTER(I64), P]
         Q.wsq_c2.restype = I
-        Q.wsq_c3.argtypes = [P, P, P, C.POINTER(I64), P]
+        Q.wsq_c2_where.argtypes = [P, I64, I64, I64, P, C.POINTER(I64), P]
+        Q.wsq_c2_where.restype = I
+        Q.wsq_c3.argtypes = [P, P, I64, P, C.POINTER(I64), P]
         Q.wsq_c3.restype = I
+        Q.wsq_set_join_sizer.argtypes = [P]
+        Q.wsq_set_join_sizer.restype = None
         Q.wsq_c4_load.argtypes = [C.c_uint64, I64, I64, C.POINTER(C.c_void_p), C.POINTER(I64)]
         Q.wsq_c4_load.restype = I
         Q.wsq_c4_free.argtypes = [P]
         Q.wsq_c4_run.argtypes = [P, I, P, P]
         Q.wsq_c4_run.restype = I
-        Q.wsq_serve.argtypes = [P, P, P, P, C.c_double, I64, I, I, I, C.c_uint64, P, P, P]
+        Q.wsq_serve.argtypes = [P, P, P, P, C.c_double, I64, I, I, I, C.c_uint64, I64, P, P, P]
         Q.wsq_serve.restype = I
         Q.wsq_serve_alone.argtypes = [P, P, P, P, C.c_double, I64, C.c_uint64, I64, C.POINTER(C.c_uint64)]
         Q.wsq_serve_alone.restype = I
@@ -91,22 +95,29 @@ def c1(t: DeviceTable, dense: bool = True):
     return oi[:g.value], od[:g.value]
 
 
-def c2(t: DeviceTable, limit: int = 10):
+def c2(t: DeviceTable, limit: int = 10, date_lt: int = 19723 + 300, region_lt: int = 224):
     """Top rows (user_id, count, sum(dur), count distinct url) and stats (pairs, groups, cap1, retries)."""
     out = np.zeros((limit, 4), np.int64)
     st = np.zeros(4, np.int64)
     n = C.c_int64()
-    _ck(qlib().wsq_c2(t.h, limit, out.ctypes.data, C.byref(n), st.ctypes.data))
+    _ck(qlib().wsq_c2_where(t.h, date_lt, region_lt, limit, out.ctypes.data, C.byref(n), st.ctypes.data))
     return out[:n.value], st
 
 
-def c3(fact: DeviceTable, dim: DeviceTable):
-    """Rows (flag, count, raw price sum, qty sum) and stats (dim rows, filtered fact rows, pairs)."""
+def c3(fact: DeviceTable, dim: DeviceTable, attr_lt: int = 50):
+    """Rows (flag, count, raw price sum, qty sum) and stats (dim rows, filtered fact rows, pairs,
+    predicted pairs: 0 without a join model)."""
     out = np.zeros((64, 4), np.int64)
-    st = np.zeros(3, np.int64)
+    st = np.zeros(4, np.int64)
     g = C.c_int64()
-    _ck(qlib().wsq_c3(fact.h, dim.h, out.ctypes.data, C.byref(g), st.ctypes.data))
+    _ck(qlib().wsq_c3(fact.h, dim.h, attr_lt, out.ctypes.data, C.byref(g), st.ctypes.data))
     return out[:g.value], st
+
+
+def set_sizers(groups=None, joins=None) -> None:
+    """Installs sizing models (ws_sizer handles, or None for the plain rules) in the drivers."""
+    qlib().wsq_set_sizer(groups)
+    qlib().wsq_set_join_sizer(joins)
 
 
 class C4Set:
@@ -155,14 +166,16 @@ def autotune_web(t: "DeviceTable", reps: int = 3) -> dict:
             "start_ms": round(float(ms[0]), 3), "tuned_ms": round(float(ms[1]), 3), "trials": trials.value}
 
 def serve(pool: C4Set, web=None, fact=None, dim=None, rate: float = 100.0, count: int = 200, mode: int = 1,
-          workers: int = 2, max_batch: int = 256, seed: int = 5):
-    """Runs the open-loop mixed stream; returns (stats dict, digests, types)."""
-    st = np.zeros(9)
+          workers: int = 2, max_batch: int = 256, seed: int = 5, budget_bytes: int = 0):
+    """Runs the open-loop mixed stream; returns (stats dict, digests, types). budget_bytes > 0
+    admits requests only while their estimated temporaries fit that much device memory."""
+    st = np.zeros(11)
     d = np.zeros(count, np.uint64)
     ty = np.zeros(count, np.int32)
     _ck(qlib().wsq_serve(pool.h, web.h if web else None, fact.h if fact else None, dim.h if dim else None, rate, count, mode,
-                         workers, max_batch, seed, st.ctypes.data, d.ctypes.data, ty.ctypes.data))
-    keys = ["completed", "wall_s", "qps", "p50_ms", "p99_ms", "small_p50_ms", "small_p99_ms", "heavy_p50_ms", "heavy_p99_ms"]
+                         workers, max_batch, seed, budget_bytes, st.ctypes.data, d.ctypes.data, ty.ctypes.data))
+    keys = ["completed", "wall_s", "qps", "p50_ms", "p99_ms", "small_p50_ms", "small_p99_ms", "heavy_p50_ms", "heavy_p99_ms",
+            "memory_waits", "peak_reserved_bytes"]
     return dict(zip(keys, st.tolist())), d, ty
 
 

@@ -67,35 +67,47 @@ i64 to_i64(double x) {
   return (i64)x;
 }
 
+// Two's-complement wrapping int64 arithmetic: date and datetime arithmetic on out-of-range inputs
+// wraps on the processors the engine runs on; spelled out so the restatement has no signed overflow.
+i64 wadd(i64 a, i64 b) { return (i64)((u64)a + (u64)b); }
+i64 wsub(i64 a, i64 b) { return (i64)((u64)a - (u64)b); }
+i64 wmul(i64 a, i64 b) { return (i64)((u64)a * (u64)b); }
+
 // ------------------------------------------------------------ dates (Hinnant, public domain)
 i64 civil_to_days(i64 y, unsigned m, unsigned d) {
-  if (m <= 2) y -= 1;
-  i64 era = (y >= 0 ? y : y - 399) / 400;
-  unsigned yoe = (unsigned)(y - era * 400);
+  if (m <= 2) y = wsub(y, 1);
+  i64 era = (y >= 0 ? y : wsub(y, 399)) / 400;
+  unsigned yoe = (unsigned)wsub(y, wmul(era, 400));
   unsigned doy = (153 * (m > 2 ? m - 3 : m + 9) + 2) / 5 + d - 1;
   unsigned doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
-  return era * 146097 + (i64)doe - 719468;
+  return wsub(wadd(wmul(era, 146097), (i64)doe), 719468);
 }
 void days_to_civil(i64 z, i64& y, unsigned& m, unsigned& d) {
-  z += 719468;
-  i64 era = (z >= 0 ? z : z - 146096) / 146097;
-  unsigned doe = (unsigned)(z - era * 146097);
+  z = wadd(z, 719468);
+  i64 era = (z >= 0 ? z : wsub(z, 146096)) / 146097;
+  unsigned doe = (unsigned)wsub(z, wmul(era, 146097));
   unsigned yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
-  y = (i64)yoe + era * 400;
+  y = wadd((i64)yoe, wmul(era, 400));
   unsigned doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
   unsigned mp = (5 * doy + 2) / 153;
   d = doy - (153 * mp + 2) / 5 + 1;
   m = mp < 10 ? mp + 3 : mp - 9;
-  if (m <= 2) y += 1;
+  if (m <= 2) y = wadd(y, 1);
 }
-i64 fdiv(i64 a, i64 b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+// Floor division for b > 0, as -((-a + b - 1) / b) for negative a, with the negations and the sum
+// wrapping in two's complement (the value x86-64 gives for a = INT64_MIN, without signed overflow).
+i64 fdiv(i64 a, i64 b) {
+  if (a >= 0) return a / b;
+  const i64 t = (i64)((0ull - (u64)a) + (u64)(b - 1));
+  return (i64)(0ull - (u64)(t / b));
+}
 i64 add_months(i64 days, i64 n) {
   i64 y;
   unsigned m, d;
   days_to_civil(days, y, m, d);
-  i64 months = y * 12 + (m - 1) + n;
-  y = months >= 0 ? months / 12 : (months - 11) / 12;
-  m = (unsigned)(months - y * 12) + 1;
+  i64 months = wadd(wadd(wmul(y, 12), (i64)(m - 1)), n);
+  y = months >= 0 ? months / 12 : wsub(months, 11) / 12;
+  m = (unsigned)wsub(months, wmul(y, 12)) + 1;
   static const unsigned len[] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
   bool leap = (y % 4 == 0 && y % 100 != 0) || y % 400 == 0;
   unsigned lim = len[m - 1] + (m == 2 && leap ? 1 : 0);
@@ -106,9 +118,9 @@ double mysql_week(i64 x) {
   unsigned m, d;
   days_to_civil(x, y, m, d);
   i64 jan1 = civil_to_days(y, 1, 1);
-  i64 dow = (jan1 + 4) - fdiv(jan1 + 4, 7) * 7;
-  i64 sun = jan1 + (7 - dow) % 7;
-  return x < sun ? 0.0 : (double)((x - sun) / 7 + 1);
+  i64 dow = wsub(wadd(jan1, 4), wmul(fdiv(wadd(jan1, 4), 7), 7));
+  i64 sun = wadd(jan1, (7 - dow) % 7);
+  return x < sun ? 0.0 : (double)(wsub(x, sun) / 7 + 1);
 }
 
 // ------------------------------------------------------------ column decode
@@ -195,20 +207,20 @@ double fn1(int f, double x) {
   switch (f) {
     case WS_FN_MONTH: days_to_civil(xi, y, m, d); return m;
     case WS_FN_DAY: days_to_civil(xi, y, m, d); return d;
-    case WS_FN_DAYOFWEEK: return (double)((xi + 4) - fdiv(xi + 4, 7) * 7 + 1);
-    case WS_FN_WEEKDAY: return (double)((xi + 3) - fdiv(xi + 3, 7) * 7);
-    case WS_FN_DAYOFYEAR: days_to_civil(xi, y, m, d); return (double)(xi - civil_to_days(y, 1, 1) + 1);
+    case WS_FN_DAYOFWEEK: return (double)(wsub(wadd(xi, 4), wmul(fdiv(wadd(xi, 4), 7), 7)) + 1);
+    case WS_FN_WEEKDAY: return (double)wsub(wadd(xi, 3), wmul(fdiv(wadd(xi, 3), 7), 7));
+    case WS_FN_DAYOFYEAR: days_to_civil(xi, y, m, d); return (double)wadd(wsub(xi, civil_to_days(y, 1, 1)), 1);
     case WS_FN_QUARTER: days_to_civil(xi, y, m, d); return (double)((m - 1) / 3 + 1);
     case WS_FN_WEEK: return mysql_week(xi);
-    case WS_FN_YEARWEEK: days_to_civil(xi, y, m, d); return (double)(y * 100) + mysql_week(xi);
-    case WS_FN_LAST_DAY: days_to_civil(xi, y, m, d); return (double)(add_months(civil_to_days(y, m, 1), 1) - 1);
+    case WS_FN_YEARWEEK: days_to_civil(xi, y, m, d); return (double)wmul(y, 100) + mysql_week(xi);
+    case WS_FN_LAST_DAY: days_to_civil(xi, y, m, d); return (double)wsub(add_months(civil_to_days(y, m, 1), 1), 1);
     case WS_FN_TO_DAYS: return x + 719528;
     case WS_FN_FROM_DAYS: return x - 719528;
     case WS_FN_DT_DATE: return (double)fdiv(xi, 86400000000ll);
-    case WS_FN_DT_HOUR: return (double)((xi - fdiv(xi, 86400000000ll) * 86400000000ll) / 3600000000ll);
-    case WS_FN_DT_MINUTE: return (double)(((xi - fdiv(xi, 86400000000ll) * 86400000000ll) / 60000000ll) % 60);
-    case WS_FN_DT_SECOND: return (double)(((xi - fdiv(xi, 86400000000ll) * 86400000000ll) / 1000000ll) % 60);
-    case WS_FN_DT_MICRO: return (double)(xi - fdiv(xi, 1000000ll) * 1000000ll);
+    case WS_FN_DT_HOUR: return (double)(wsub(xi, wmul(fdiv(xi, 86400000000ll), 86400000000ll)) / 3600000000ll);
+    case WS_FN_DT_MINUTE: return (double)((wsub(xi, wmul(fdiv(xi, 86400000000ll), 86400000000ll)) / 60000000ll) % 60);
+    case WS_FN_DT_SECOND: return (double)((wsub(xi, wmul(fdiv(xi, 86400000000ll), 86400000000ll)) / 1000000ll) % 60);
+    case WS_FN_DT_MICRO: return (double)wsub(xi, wmul(fdiv(xi, 1000000ll), 1000000ll));
     case WS_FN_DATE_TO_DT: return x * 86400000000.0;
     case WS_FN_DT_UNIX: return std::floor(x / 1e6);
     case WS_FN_UNIX_DT: return x * 1e6;
@@ -232,7 +244,7 @@ double fn2(int f, double a, double b) {
     case WS_FN_ADD_MONTHS: return (double)add_months(to_i64(a), to_i64(b));
     case WS_FN_DT_ADD_MONTHS: {
       i64 us = to_i64(a), days = fdiv(us, 86400000000ll);
-      return (double)(add_months(days, to_i64(b)) * 86400000000ll + (us - days * 86400000000ll));
+      return (double)wadd(wmul(add_months(days, to_i64(b)), 86400000000ll), wsub(us, wmul(days, 86400000000ll)));
     }
     case WS_FN_DATE_TRUNC_MONTH: {
       i64 y;

@@ -504,3 +504,70 @@ def test_hash_join_sized_equals_plain(gpu, keys):
     ob, op, m = C.c_void_p(), C.c_void_p(), C.c_int64()
     ws.check(gpu.ws_hash_join_sized(bv, 0, pv, len(pk), 100, C.byref(ob), C.byref(op), C.byref(m)))
     assert m.value == 0 and not ob.value and not op.value
+
+
+@pytest.mark.parametrize("width", ["i64n_f64", "encoded"])
+def test_select_ballot_pass_wide_and_encoded_columns(gpu, width):
+    """The ballot pass's four-rows-per-lane loads on tiles without deleted rows, for 8-byte columns
+    (nullable BIGINT with NULLs, DOUBLE with NaN: 16-byte pair loads) and for frame-of-reference
+    encoded columns (u8 / u16 codes, word loads); ragged batches put batch bases off 16-byte
+    alignment so some batches take the row-by-row fallback. Against the oracle and the chunked
+    single-pass kernel."""
+    H.say(f"case ballot pass {width}")
+    n = 2_500_000
+    rng = np.random.default_rng(73)
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    v[rng.random(n) < 0.05] = abi.NULL_I64
+    x = rng.standard_normal(n)
+    x[rng.random(n) < 0.05] = np.nan
+    r = rng.integers(0, 200, n).astype(np.int32)  # u8 frame-of-reference codes
+    u = rng.integers(0, 40_000, n).astype(np.int32)  # u16 codes
+    sizes = [1_048_576, 3, 700_001, 751_420]  # 3 and 700_001 leave later batch bases unaligned
+    t = H.Table({"v": v, "x": x, "r": r, "u": u}, sizes)
+    b = abi.EnvBuilder()
+    if width == "i64n_f64":
+        slots = [("v", abi.T_I64N, 0), ("x", abi.T_F64, 0)]
+        s0, s1 = b.slot(abi.T_I64N), b.slot(abi.T_F64)
+        # WHERE v > 100 OR x < -0.5 (NULL and NaN rows take three-valued logic)
+        where = b.prog([(abi.OP_COL, s0), b.ilit(100), (abi.OP_GT,), (abi.OP_COL, s1), b.dlit(-0.5), (abi.OP_LT,), (abi.OP_OR,)])
+    else:
+        e0 = t.encode("r")
+        e1 = t.encode("u")
+        assert (e0, e1) == (1, 2)
+        slots = [("r", abi.T_I32, 0), ("u", abi.T_I32, 0)]
+        s0, s1 = b.slot(abi.T_I32, enc=e0), b.slot(abi.T_I32, enc=e1)
+        where = b.prog([(abi.OP_COL, s0), b.ilit(37), (abi.OP_LT,), (abi.OP_COL, s1), b.ilit(30_000), (abi.OP_GT,), (abi.OP_AND,)])
+    dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
+    nb = len(t.sizes)
+    bn = np.array(t.sizes, np.int64)
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    fp = abi.AggPlan()
+    fp.has_where, fp.where = 1, where
+    jit = C.c_void_p()
+    ws.check(gpu.ws_jit_get(C.byref(b.env), 2, C.byref(fp), abi.JF_FLAGM, b"filter", C.byref(jit)))
+    assert jit.value
+
+    def run():
+        out = C.c_void_p()
+        per = np.zeros(nb, np.int64)
+        cnt = C.c_int64()
+        ws.check(gpu.ws_select_multi(C.byref(b.env), 1, where, nb, ws.ptr(bn), None, C.cast(dev.views, C.c_void_p), 2,
+                                     C.byref(out), ws.ptr(per), jit, C.byref(cnt)))
+        return ws.DevBuf(cnt.value * 4, tmp=True, adopt=out.value).to_array(np.int32, cnt.value), per
+
+    v_old = C.c_int64()
+    ws.check(gpu.ws_tune_get(b"select_mask_min_rows", C.byref(v_old)))
+    try:
+        ws.check(gpu.ws_tune_set(b"select_mask_min_rows", 1 << 40))
+        plain, plain_per = run()
+        ws.check(gpu.ws_tune_set(b"select_mask_min_rows", 0))
+        got, per = run()
+    finally:
+        ws.check(gpu.ws_tune_set(b"select_mask_min_rows", v_old.value))
+    exp = np.zeros(n, np.int32)
+    eper = np.zeros(nb, np.int64)
+    m = H.oracle().or_select(C.byref(b.env), 1, where, nb, ws.ptr(bn), None, C.cast(host.views, C.c_void_p), 2, ws.ptr(exp),
+                             ws.ptr(eper))
+    assert m > 10_000
+    assert len(got) == m and np.array_equal(got, exp[:m]) and np.array_equal(per, eper)
+    assert np.array_equal(plain, exp[:m]) and np.array_equal(plain_per, eper)

@@ -359,8 +359,9 @@ def bench_batch(args) -> int:
     """256 filter + GROUP BY queries (seed 4): rows lognormal in [4096, 4M], keys with 10 to 1e5
     distinct values, WHERE val > theta_q; 12 B/row scanned. One step = all 256 queries, run as one
     batched launch + one batched fetch into page-locked memory, and (separately timed) one by one
-    through the per-query path. Both timed forms fetch every group to the host; the digests that
-    compare the two forms are computed after the timing. Unless --no-autotune, the autotuner first
+    through the per-query path, and as a batch prepared once whose launches replay from a HIP graph
+    (the repeated-dashboard form). Every timed form fetches every group to the host; the digests
+    that compare the forms are computed after the timing. Unless --no-autotune, the autotuner first
     picks the batched form's tile, LDS table and grid knobs on this workload."""
     import warpstore_amd as ws
     from warpstore_amd import abi, queries as Q
@@ -388,9 +389,21 @@ def bench_batch(args) -> int:
             L.ws_sync()
             elapsed = time.perf_counter() - t0
         res[mode] = elapsed / args.steps
+    # the same batch prepared once (its launches recorded in a HIP graph) and replayed
+    pr = s.prepare()
+    for _ in range(max(args.warmup, 1)):
+        pr.run(digests=False)
+    L.ws_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pr.run(digests=False)
+    L.ws_sync()
+    res["replay"] = (time.perf_counter() - t0) / args.steps
+    dr, gr = pr.run()
+    pr.free()
     db, gb = s.run(batched=True)
     dp, gp = s.run(batched=False)
-    same = bool(np.array_equal(db, dp) and np.array_equal(gb, gp))
+    same = bool(np.array_equal(db, dp) and np.array_equal(gb, gp) and np.array_equal(dr, dp) and np.array_equal(gr, gp))
     bt, pt = res["batched"], res["per_query"]
     scan = s.rows * 12
     out = {
@@ -401,6 +414,7 @@ def bench_batch(args) -> int:
         "queries_per_s": round(256 / bt, 1), "per_query_ms_per_step": round(pt * 1e3, 4),
         "per_query_queries_per_s": round(256 / pt, 1), "speedup_vs_per_query": round(pt / bt, 3),
         "results_equal_per_query": same, "groups": int(gb.sum()), "autotuner": tuned,
+        "graph_replay_ms_per_step": round(res["replay"] * 1e3, 4), "graph_replay_value": round(scan / res["replay"] / 1e9, 3),
         "roofline": roofline("agg_batch_dense_jit (all 256 queries' filter + dense GROUP BY in one launch)", scan, launches,
                              total_ms, None),
     }

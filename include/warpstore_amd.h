@@ -108,6 +108,15 @@ int ws_agg_group_count(ws_agg* a, int64_t* out);           /* groupCount gpu.h:1
  * ascending group index; fails with *total set when *total > cap). jit: a compiled dense plan of
  * the shape, or NULL for the interpreter. Results equal separate runs (double sums up to order). */
 int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit);
+/* A batch prepared once and replayed (no reference counterpart: repeated dashboards, a serving
+   loop's standing query sets). The aggregators (dense, one plan shape, batches added) must outlive
+   the plan; every replay starts their tables from the initial values, runs the batched kernel and
+   fetches like ws_agg_fetch_many. The launches are recorded in a HIP graph at prepare time. */
+typedef struct ws_agg_batch_plan ws_agg_batch_plan;
+int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit,
+                         ws_agg_batch_plan** out);
+int ws_agg_batch_replay(ws_agg_batch_plan* p, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs);
+void ws_agg_batch_release(ws_agg_batch_plan* p);
 int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs);
 /* Aggregation strategy of a hash GROUP BY (no reference counterpart: the hook through which a
  * sizing model, or a test, chooses the physical plan; results do not depend on it).

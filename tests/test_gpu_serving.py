@@ -82,3 +82,26 @@ def test_autotuner_choices_keep_results(gpu):
         for k, v in (("batch_tile_rows", 4096), ("batch_lds_slots", 1024), ("batch_blocks_per_cu", 4), ("part_chunks_per_cu", 1),
                      ("part_scatter_lds_kib", 120), ("part_merge_lds_kib", 60)):
             gpu.ws_tune_set(k.encode(), v)
+
+
+def test_prepared_batch_graph_replays(gpu):
+    """A batch prepared once (its launches recorded in a HIP graph) and replayed three times, with
+    the program's own eager work and synchronizations between the replays: every replay's results
+    equal the per-query path's (the tables start from their initial values each time)."""
+    H.say("case prepared batch: graph replays")
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    s = Q.C4Set(seed=21, nq=40)
+    pr = None
+    try:
+        dp, gp = s.run(batched=False)
+        pr = s.prepare()
+        for rep in range(3):
+            dr, gr = pr.run()
+            assert np.array_equal(dr, dp) and np.array_equal(gr, gp), rep
+            db, gb = s.run(batched=True)  # eager work between replays
+            assert np.array_equal(db, dp)
+            gpu.ws_sync()
+    finally:
+        if pr is not None:
+            pr.free()
+        s.free()

@@ -1264,6 +1264,142 @@ static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor, D
   return g;
 }
 
+// ------------------------------------------------------------------ batched runs
+
+// A batched run's launch description: the blob [BatchQuery n][BatchTile T] then per query plan,
+// dense spec, env, n[], del[], cols[], tile0[], with device addresses resolved against `dev`.
+struct BatchBuild {
+  Blob blob;
+  uint8_t* dev = nullptr;
+  const BatchQuery* dq = nullptr;
+  const BatchTile* dt = nullptr;
+  int64_t ntiles = 0, gmax = 1;
+  int pending = 0, iv = kIvCols, na = 0;
+  void* jf = nullptr;
+};
+template <class Alloc>
+static BatchBuild build_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit,
+                              bool init_all, Alloc alloc) {
+  BatchBuild r;
+  const ws_agg* A0 = aggs[0];
+  const int nk = A0->plan.nkeys, na = A0->plan.naggs;
+  r.na = na;
+  for (int q = 0; q < n; ++q) {
+    const ws_agg* A = aggs[q];
+    if (!A->is_dense || A->plan.nkeys != nk || A->plan.naggs != na)
+      throw Error("batched run: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
+    r.iv = std::min(r.iv, interp_of(*envs[q], A->plan));
+  }
+  if (!jit && (na > WS_INTERP_AGGS || nk > WS_INTERP_KEYS))
+    throw Error("this aggregation needs a compiled plan (more than 16 aggregates or 4 GROUP BY keys); codegen failed or is off",
+                WS_ERR_NEEDS_PLAN);
+  r.jf = jit ? jit_function(jit, kJitBatchDense) : nullptr;
+  const int tile_rows = (int)tune(kTuneBatchTile);
+  std::vector<BatchTile> tiles;
+  for (int q = 0; q < n; ++q)
+    for (size_t b = 0; b < aggs[q]->bn.size(); ++b)
+      for (int64_t f = 0; f < aggs[q]->bn[b]; f += tile_rows)
+        tiles.push_back(BatchTile{q, (int32_t)b, (int32_t)std::min<int64_t>(tile_rows, aggs[q]->bn[b] - f), 0, f});
+  r.ntiles = (int64_t)tiles.size();
+  Blob& blob = r.blob;
+  const size_t o_q = blob.reserve(sizeof(BatchQuery) * n);
+  const size_t o_t = blob.put(tiles.data(), sizeof(BatchTile) * tiles.size());
+  std::vector<size_t> o_plan(n), o_ds(n), o_env(n), o_n(n), o_del(n), o_cols(n), o_t0(n);
+  for (int q = 0; q < n; ++q) {
+    const ws_agg* A = aggs[q];
+    o_plan[q] = blob.put(&A->plan, sizeof A->plan);
+    o_ds[q] = blob.put(&A->dense, sizeof A->dense);
+    o_env[q] = blob.put(envs[q], sizeof(ws_expr_env));
+    o_n[q] = blob.put(A->bn.data(), 8 * A->bn.size());
+    o_del[q] = blob.put(A->bdel.data(), 8 * A->bdel.size());
+    o_cols[q] = blob.put(A->bcols.data(), sizeof(ws_colview) * A->bcols.size());
+    std::vector<int64_t> t0(A->bn.size() + 1, 0);
+    o_t0[q] = blob.put(t0.data(), 8 * t0.size());
+  }
+  uint8_t* dev = alloc(blob.host.size());
+  r.dev = dev;
+  for (int q = 0; q < n; ++q) r.gmax = std::max<int64_t>(r.gmax, aggs[q]->dense.G * std::max(na, 1));
+  for (int q = 0; q < n; ++q) {
+    ws_agg* A = aggs[q];
+    BatchQuery bq{};
+    ScanDesc& d = bq.d;
+    d.nb = (int32_t)A->bn.size();
+    d.ncols = A->ncols;
+    d.tile_rows = tile_rows;
+    d.n = (const int64_t*)(dev + o_n[q]);
+    d.del = (const uint32_t* const*)(dev + o_del[q]);
+    d.cols = (const ws_colview*)(dev + o_cols[q]);
+    d.tile0 = (const int64_t*)(dev + o_t0[q]);
+    d.row0 = d.tile0;
+    d.env = (const ws_expr_env*)(dev + o_env[q]);
+    d.plan = (const ws_agg_plan*)(dev + o_plan[q]);
+    d.ds = (const ws_dense_spec*)(dev + o_ds[q]);
+    bq.gacc = A->acc;
+    bq.flags = A->flags;
+    bq.count = A->count;
+    bq.init = (init_all || A->pending_init) ? 1 : 0;
+    r.pending += bq.init;
+    *blob.at<BatchQuery>(o_q + sizeof(BatchQuery) * q) = bq;
+    A->fetched = -1;
+    A->st_valid = false;
+    ++A->runs;
+  }
+  r.dq = (const BatchQuery*)(dev + o_q);
+  r.dt = (const BatchTile*)(dev + o_t);
+  return r;
+}
+// The batch's launches: the initial values (queries whose BatchQuery asks for them) and the
+// batched kernel. prof = false leaves out the profiling events (inside a graph capture).
+static void launch_batch(const BatchBuild& b, int n, bool prof) {
+  if (b.pending) {  // the deferred initial values of the whole batch, one launch
+    const unsigned gx = (unsigned)std::min<int64_t>(64, (b.gmax + 2047) / 2048);
+    hipLaunchKernelGGL(batch_init_k, dim3(gx, (unsigned)n), dim3(256), 0, stream(), b.dq);
+    check_launch("batch_init");
+  }
+  const int na = b.na;
+  int S = 64;  // the kernel's per-tile tables are powers of two up to S, within the LDS
+  auto lds_of = [&](int slots) { return (((size_t)slots * 4 + 15) & ~(size_t)15) + (size_t)slots * std::max(na, 1) * 8; };
+  while (S * 2 <= (int)tune(kTuneBatchSlots) && lds_of(S * 2) + kStaticLds <= kLdsPerCu) S *= 2;
+  const size_t lds = lds_of(S);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(b.ntiles, (int64_t)cu_count() * tune(kTuneBatchBlocksPerCu)));
+  const BatchQuery* dq = b.dq;
+  const BatchTile* dt = b.dt;
+  long long nt = (long long)b.ntiles;
+  int sl = S;
+  if (b.jf) {
+    void* args[] = {&dq, &dt, &nt, &sl};
+    if (prof) jit_launch_raw(b.jf, grid, kBlock, lds, args, "agg_batch_dense_jit");
+    else WS_CK(hipModuleLaunchKernel((hipFunction_t)b.jf, grid, 1, 1, kBlock, 1, 1, (unsigned)lds, stream(), args, nullptr));
+  } else {
+    std::unique_ptr<ProfScope> ps(prof ? new ProfScope("agg_batch_dense") : nullptr);
+    const int iv = b.iv;
+    hipLaunchKernelGGL(IV_PICK(iv, dense_batch), dim3(grid), dim3(kBlock), lds, stream(), dq, dt, nt, sl);
+    check_launch("agg_batch_dense");
+  }
+}
+
+// A batch of dense aggregations prepared once and replayed: its descriptor blob stays resident,
+// and its launches (initial values, the batched kernel, then the fetch's count, scan and pack
+// kernels) are recorded in a HIP graph; a replay is one graph launch and two read-backs.
+struct ws_agg_batch_plan {
+  int n = 0, nk = 0, na = 0;
+  ws_agg_plan plan{};
+  BatchBuild b;
+  ws::DevBuf<ws::FetchQuery> fq;
+  ws::DevBuf<uint32_t> cnt, offs;
+  ws::DevBuf<double> pk, pa;
+  int64_t gcap = 0;
+  hipStream_t cs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~ws_agg_batch_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (b.dev) ws::tmp_free(b.dev);
+  }
+};
+
 extern "C" {
 
 int ws_agg_create(const ws_agg_plan* plan, int64_t capacity, const ws_dense_spec* dense, ws_agg** out) {
@@ -1410,97 +1546,15 @@ int ws_agg_compact(ws_agg* a, double** keys, double** accs, int64_t* groups) {
 int ws_agg_run_batch(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit) {
   return guard([&] {
     if (n <= 0) return;
-    const ws_agg* A0 = aggs[0];
-    const int nk = A0->plan.nkeys, na = A0->plan.naggs;
-    int iv = kIvCols;
-    for (int q = 0; q < n; ++q) {
-      const ws_agg* A = aggs[q];
-      if (!A->is_dense || A->plan.nkeys != nk || A->plan.naggs != na)
-        throw Error("batched run: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
-      iv = std::min(iv, interp_of(*envs[q], A->plan));
+    BatchBuild b = build_batch(n, aggs, envs, jit, false, [](size_t bytes) { return (uint8_t*)tmp_alloc(bytes); });
+    if (b.ntiles == 0) {
+      tmp_free(b.dev);
+      return;
     }
-    if (!jit && (na > WS_INTERP_AGGS || nk > WS_INTERP_KEYS))
-      throw Error("this aggregation needs a compiled plan (more than 16 aggregates or 4 GROUP BY keys); codegen failed or is off",
-                  WS_ERR_NEEDS_PLAN);
-    void* jf = jit ? jit_function(jit, kJitBatchDense) : nullptr;
-    const int tile_rows = (int)tune(kTuneBatchTile);
-    // blob: [BatchQuery n][BatchTile T] then per query plan, dense spec, env, n[], del[], cols[], tile0[], row0[]
-    std::vector<BatchTile> tiles;
-    for (int q = 0; q < n; ++q)
-      for (size_t b = 0; b < aggs[q]->bn.size(); ++b)
-        for (int64_t f = 0; f < aggs[q]->bn[b]; f += tile_rows)
-          tiles.push_back(BatchTile{q, (int32_t)b, (int32_t)std::min<int64_t>(tile_rows, aggs[q]->bn[b] - f), 0, f});
-    if (tiles.empty()) return;
-    Blob blob;
-    const size_t o_q = blob.reserve(sizeof(BatchQuery) * n);
-    const size_t o_t = blob.put(tiles.data(), sizeof(BatchTile) * tiles.size());
-    std::vector<size_t> o_plan(n), o_ds(n), o_env(n), o_n(n), o_del(n), o_cols(n), o_t0(n);
-    for (int q = 0; q < n; ++q) {
-      const ws_agg* A = aggs[q];
-      o_plan[q] = blob.put(&A->plan, sizeof A->plan);
-      o_ds[q] = blob.put(&A->dense, sizeof A->dense);
-      o_env[q] = blob.put(envs[q], sizeof(ws_expr_env));
-      o_n[q] = blob.put(A->bn.data(), 8 * A->bn.size());
-      o_del[q] = blob.put(A->bdel.data(), 8 * A->bdel.size());
-      o_cols[q] = blob.put(A->bcols.data(), sizeof(ws_colview) * A->bcols.size());
-      std::vector<int64_t> t0(A->bn.size() + 1, 0);
-      o_t0[q] = blob.put(t0.data(), 8 * t0.size());
-    }
-    uint8_t* dev = (uint8_t*)tmp_alloc(blob.host.size());
-    int pending = 0;
-    int64_t gmax = 1;
-    for (int q = 0; q < n; ++q) gmax = std::max<int64_t>(gmax, aggs[q]->dense.G * std::max(na, 1));
-    for (int q = 0; q < n; ++q) {
-      ws_agg* A = aggs[q];
-      BatchQuery bq{};
-      ScanDesc& d = bq.d;
-      d.nb = (int32_t)A->bn.size();
-      d.ncols = A->ncols;
-      d.tile_rows = tile_rows;
-      d.n = (const int64_t*)(dev + o_n[q]);
-      d.del = (const uint32_t* const*)(dev + o_del[q]);
-      d.cols = (const ws_colview*)(dev + o_cols[q]);
-      d.tile0 = (const int64_t*)(dev + o_t0[q]);
-      d.row0 = d.tile0;
-      d.env = (const ws_expr_env*)(dev + o_env[q]);
-      d.plan = (const ws_agg_plan*)(dev + o_plan[q]);
-      d.ds = (const ws_dense_spec*)(dev + o_ds[q]);
-      bq.gacc = A->acc;
-      bq.flags = A->flags;
-      bq.count = A->count;
-      bq.init = A->pending_init ? 1 : 0;
-      if (A->pending_init) ++pending;
-      *blob.at<BatchQuery>(o_q + sizeof(BatchQuery) * q) = bq;
-      A->fetched = -1;
-      A->st_valid = false;
-      ++A->runs;
-    }
-    h2d_async_staged(dev, blob.host.data(), blob.host.size());
-    const BatchQuery* dq = (const BatchQuery*)(dev + o_q);
-    if (pending) {  // the deferred initial values of the whole batch, one launch
-      const unsigned gx = (unsigned)std::min<int64_t>(64, (gmax + 2047) / 2048);
-      hipLaunchKernelGGL(batch_init_k, dim3(gx, (unsigned)n), dim3(256), 0, stream(), dq);
-      check_launch("batch_init");
-      for (int q = 0; q < n; ++q) aggs[q]->pending_init = false;
-    }
-    int S = 64;  // the kernel's per-tile tables are powers of two up to S, within the LDS
-    auto lds_of = [&](int slots) { return (((size_t)slots * 4 + 15) & ~(size_t)15) + (size_t)slots * std::max(na, 1) * 8; };
-    while (S * 2 <= (int)tune(kTuneBatchSlots) && lds_of(S * 2) + kStaticLds <= kLdsPerCu) S *= 2;
-    const size_t lds = lds_of(S);
-    const unsigned grid =
-        (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)tiles.size(), (int64_t)cu_count() * tune(kTuneBatchBlocksPerCu)));
-    const BatchTile* dt = (const BatchTile*)(dev + o_t);
-    long long nt = (long long)tiles.size();
-    int sl = S;
-    if (jf) {
-      void* args[] = {&dq, &dt, &nt, &sl};
-      jit_launch_raw(jf, grid, kBlock, lds, args, "agg_batch_dense_jit");
-    } else {
-      ProfScope ps("agg_batch_dense");
-      hipLaunchKernelGGL(IV_PICK(iv, dense_batch), dim3(grid), dim3(kBlock), lds, stream(), dq, dt, nt, sl);
-      check_launch("agg_batch_dense");
-    }
-    tmp_free(dev);
+    h2d_async_staged(b.dev, b.blob.host.data(), b.blob.host.size());
+    launch_batch(b, n, true);
+    for (int q = 0; q < n; ++q) aggs[q]->pending_init = false;
+    tmp_free(b.dev);
   });
 }
 
@@ -1542,6 +1596,71 @@ int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* tota
     WS_CK(hipStreamSynchronize(stream()));
   });
 }
+
+int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit,
+                         ws_agg_batch_plan** out) {
+  return guard([&] {
+    *out = nullptr;
+    if (n <= 0 || n > 4096) throw Error("prepared batch: 1 to 4096 aggregations", WS_ERR_INTERNAL);
+    auto P = std::make_unique<ws_agg_batch_plan>();
+    P->n = n;
+    P->nk = aggs[0]->plan.nkeys;
+    P->na = aggs[0]->plan.naggs;
+    P->plan = aggs[0]->plan;
+    P->b = build_batch(n, aggs, envs, jit, true, [](size_t bytes) { return (uint8_t*)tmp_alloc(bytes); });
+    if (P->b.ntiles == 0) throw Error("prepared batch: no rows", WS_ERR_INTERNAL);
+    h2d_sync(P->b.dev, P->b.blob.host.data(), P->b.blob.host.size());
+    std::vector<FetchQuery> fq(n);
+    for (int q = 0; q < n; ++q) {
+      fq[q] = FetchQuery{aggs[q]->acc, aggs[q]->dense, aggs[q]->dense.G};
+      P->gcap += aggs[q]->dense.G;
+      aggs[q]->pending_init = false;  // every replay gives the tables their initial values
+    }
+    P->fq = DevBuf<FetchQuery>(n);
+    h2d_sync(P->fq.p, fq.data(), sizeof(FetchQuery) * n);
+    P->cnt = DevBuf<uint32_t>(n);
+    P->offs = DevBuf<uint32_t>(n + 1);
+    P->pk = DevBuf<double>(std::max<int64_t>(P->gcap * P->nk, 1));
+    P->pa = DevBuf<double>(std::max<int64_t>(P->gcap * P->na, 1));
+    WS_CK(hipStreamSynchronize(stream()));
+    WS_CK(hipStreamCreateWithFlags(&P->cs, hipStreamNonBlocking));
+    WS_CK(hipStreamBeginCapture(P->cs, hipStreamCaptureModeThreadLocal));
+    try {
+      StreamOverride so(P->cs);
+      launch_batch(P->b, n, false);
+      hipLaunchKernelGGL(dense_count_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)P->fq.p, P->na,
+                         P->cnt.p);
+      hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)P->cnt.p, P->offs.p, n);
+      hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)P->fq.p, P->plan,
+                         P->nk, P->na, (const uint32_t*)P->offs.p, P->pk.p, P->pa.p);
+      check_launch("batch_plan_capture");
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(P->cs, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    WS_CK(hipStreamEndCapture(P->cs, &P->graph));
+    WS_CK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    *out = P.release();
+  });
+}
+int ws_agg_batch_replay(ws_agg_batch_plan* P, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs) {
+  return guard([&] {
+    *total = 0;
+    WS_CK(hipGraphLaunch(P->exec, stream()));
+    std::vector<uint32_t> h((size_t)P->n + 1);
+    d2h_sync(h.data(), P->offs.p, sizeof(uint32_t) * h.size());
+    for (int q = 0; q < P->n; ++q) counts[q] = (int64_t)(h[q + 1] - h[q]);
+    *total = h[P->n];
+    if (*total > cap) throw Error("prepared batch: more groups than the buffers hold", WS_ERR_INTERNAL);
+    if (!*total) return;
+    if (P->nk) WS_CK(hipMemcpyAsync(keys, P->pk.p, sizeof(double) * (size_t)(*total * P->nk), hipMemcpyDeviceToHost, stream()));
+    if (P->na) WS_CK(hipMemcpyAsync(accs, P->pa.p, sizeof(double) * (size_t)(*total * P->na), hipMemcpyDeviceToHost, stream()));
+    WS_CK(hipStreamSynchronize(stream()));
+  });
+}
+void ws_agg_batch_release(ws_agg_batch_plan* P) { delete P; }
 
 int64_t ws_agg_capacity(const ws_agg* a) { return a->cap; }
 int ws_agg_group_count(ws_agg* a, int64_t* out) {

@@ -50,7 +50,15 @@ int guard(F&& f) {
 void ensure_init();
 int cu_count();
 const std::string& arch_name();  // e.g. "gfx950:sramecc+:xnack-"
-inline hipStream_t stream() { return hipStreamPerThread; }
+// The stream a thread's work goes to: its per-thread stream, or a capture stream while the thread
+// records a graph (set through StreamOverride).
+extern thread_local hipStream_t t_stream_override;
+inline hipStream_t stream() { return t_stream_override ? t_stream_override : hipStreamPerThread; }
+struct StreamOverride {
+  hipStream_t saved;
+  explicit StreamOverride(hipStream_t s) : saved(t_stream_override) { t_stream_override = s; }
+  ~StreamOverride() { t_stream_override = saved; }
+};
 
 // Grid for a grid-stride kernel over n items.
 inline unsigned grid_for(int64_t n, int block = 256, int per_cu = 8) {

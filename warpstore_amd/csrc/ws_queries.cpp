@@ -718,7 +718,10 @@ struct C4Plan {
 };
 const std::vector<int> kC4Slots = {0, 1};
 
-C4Plan c4_plan(const C4Query& q) {
+// batched: the plan of the batched forms, which keep every query dense (a dense table for a query
+// with many possible keys and few rows costs memory, not correctness: dense and hash give the same
+// groups); the per-query path keeps the engine's rule.
+C4Plan c4_plan(const C4Query& q, bool batched = false) {
   C4Plan p;
   const int sk = p.c.slot(WS_T_I32), sv = p.c.slot(WS_T_I64);
   std::memset(&p.plan, 0, sizeof p.plan);
@@ -737,7 +740,7 @@ C4Plan c4_plan(const C4Query& q) {
   }
   p.plan.agg_part[1] = 1, p.plan.agg_part[2] = 2;
   const DevColumn& k = q.t->cols[0];
-  p.dense = plan_dense(p.plan, {{k.mn, k.mx}}, q.t->rows, p.ds);
+  p.dense = plan_dense(p.plan, {{k.mn, k.mx}}, batched ? ((int64_t)1 << 40) : q.t->rows, p.ds);
   p.plan.dense_priv = p.dense ? (p.ds.priv ? 1 : 2) : 0;
   const double est = std::min<double>((double)(k.mx - k.mn + 1), (double)std::max<int64_t>(q.t->rows, 1));
   while (p.cap < est * 2) p.cap <<= 1;
@@ -836,7 +839,7 @@ void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>* di
   plans.reserve(n);
   std::vector<size_t> dense_idx;
   for (size_t i = 0; i < n; ++i) {
-    plans.push_back(c4_plan(*qs[i]));
+    plans.push_back(c4_plan(*qs[i], true));
     if (plans.back().dense) {
       dense_idx.push_back(i);
     } else {
@@ -876,6 +879,56 @@ void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>* di
     if (digests) (*digests)[i] = c4_digest(t_results.keys + off, t_results.accs + off * 5, g);
     groups[i] = g;
     off += g;
+  }
+}
+
+// A C4 set prepared once (plans, aggregators, the batch's launches recorded in a graph) and replayed:
+// the repeated-dashboard form of the batched workload.
+struct C4Prepared {
+  std::vector<C4Plan> plans;
+  std::vector<std::unique_ptr<GpuAggregator>> aggs;
+  ws_agg_batch_plan* plan = nullptr;
+  int64_t cap = 0;
+  ~C4Prepared() { ws_agg_batch_release(plan); }
+};
+C4Prepared* c4_prepare(const std::vector<const C4Query*>& qs) {
+  auto P = std::make_unique<C4Prepared>();
+  apply_tuned(kC4Shape);
+  for (const C4Query* q : qs) {
+    P->plans.push_back(c4_plan(*q, true));
+    if (!P->plans.back().dense) throw SqlError("prepared batch: every query must be dense", 1105);
+  }
+  const C4Plan& p0 = P->plans[0];
+  const JitProgram* jit = jitGet(p0.c.env, p0.c.ncols, p0.plan, JF_DENSE, "aggregate (batched)");
+  std::vector<ws_agg*> handles;
+  std::vector<const ws_expr_env*> envs;
+  for (size_t i = 0; i < qs.size(); ++i) {
+    const C4Plan& p = P->plans[i];
+    P->aggs.emplace_back(new GpuAggregator(p.plan, p.cap, &p.ds));
+    for (size_t s = 0; s < qs[i]->t->seg_rows.size(); ++s) {
+      ExprEnv e = bind_segment(p.c, *qs[i]->t, kC4Slots, s);
+      P->aggs.back()->addBatch(e, p.c.ncols, qs[i]->t->seg_rows[s], nullptr);
+    }
+    handles.push_back(P->aggs.back()->handle());
+    envs.push_back(reinterpret_cast<const ws_expr_env*>(&p.c.env));
+    P->cap += p.ds.G;
+  }
+  wsd::ck(ws_agg_batch_prepare((int)handles.size(), handles.data(), envs.data(), jit ? jit->handle : nullptr, &P->plan));
+  return P.release();
+}
+void c4_replay(C4Prepared& P, std::vector<uint64_t>* digests, std::vector<int64_t>& groups) {
+  const size_t n = P.plans.size();
+  std::vector<int64_t> counts(n);
+  int64_t total = 0;
+  t_results.reserve(std::max<int64_t>(P.cap, 1), 1, 5);
+  wsd::ck(ws_agg_batch_replay(P.plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs));
+  if (digests) digests->assign(n, 0);
+  groups.assign(n, 0);
+  int64_t off = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (digests) (*digests)[i] = c4_digest(t_results.keys + off, t_results.accs + off * 5, counts[i]);
+    groups[i] = counts[i];
+    off += counts[i];
   }
 }
 
@@ -1286,6 +1339,32 @@ int wsq_c4_run(void* set, int batched, uint64_t* digests, int64_t* groups) {
     }
   });
 }
+
+// The set prepared once for replays (see c4_prepare); wsq_c4_replay runs every query again with the
+// recorded launches; wsq_c4_release frees it (the set must outlive it).
+int wsq_c4_prepare(void* set, void** out) {
+  return guard([&] {
+    auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)set;
+    std::vector<C4Query> qs;
+    for (auto& e : *v) qs.push_back(C4Query{e.first, e.second});
+    std::vector<const C4Query*> qp;
+    for (auto& q : qs) qp.push_back(&q);
+    *out = c4_prepare(qp);
+  });
+}
+int wsq_c4_replay(void* prepared, uint64_t* digests, int64_t* groups) {
+  return guard([&] {
+    auto* P = (C4Prepared*)prepared;
+    std::vector<uint64_t> d;
+    std::vector<int64_t> g;
+    c4_replay(*P, digests ? &d : nullptr, g);
+    for (size_t i = 0; i < g.size(); ++i) {
+      if (digests) digests[i] = d[i];
+      groups[i] = g[i];
+    }
+  });
+}
+void wsq_c4_release(void* prepared) { delete (C4Prepared*)prepared; }
 
 // The autotuner (see autotune()) on the batched workload of a C4 set: knobs batch_tile_rows,
 // batch_lds_slots, batch_blocks_per_cu. chosen[3] gets their values, ms[2] the median time of one

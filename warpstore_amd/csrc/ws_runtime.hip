@@ -15,6 +15,7 @@ namespace ws {
 // ------------------------------------------------------------------ errors
 
 static thread_local std::string t_last_error;
+thread_local hipStream_t t_stream_override = nullptr;
 void set_last_error(const std::string& msg) { t_last_error = msg; }
 
 void hip_fail(hipError_t e, const char* what, const char* file, int line) {

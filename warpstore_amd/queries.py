@@ -47,6 +47,12 @@ def qlib():
         Q.wsq_serve_alone.restype = I
         Q.wsq_set_sizer.argtypes = [P]
         Q.wsq_set_sizer.restype = None
+        Q.wsq_c4_prepare.argtypes = [P, C.POINTER(C.c_void_p)]
+        Q.wsq_c4_prepare.restype = I
+        Q.wsq_c4_replay.argtypes = [P, P, P]
+        Q.wsq_c4_replay.restype = I
+        Q.wsq_c4_release.argtypes = [P]
+        Q.wsq_c4_release.restype = None
         for f in ("wsq_c4_autotune", "wsq_c2_autotune"):
             getattr(Q, f).argtypes = [P, I, P, P, C.POINTER(I)]
             getattr(Q, f).restype = I
@@ -138,6 +144,12 @@ class C4Set:
         _ck(qlib().wsq_c4_run(self.h, 1 if batched else 0, d.ctypes.data if digests else None, g.ctypes.data))
         return (d if digests else None), g
 
+    def prepare(self) -> "PreparedC4":
+        """The set's batch prepared once (launches recorded in a HIP graph) for replays."""
+        h = C.c_void_p()
+        _ck(qlib().wsq_c4_prepare(self.h, C.byref(h)))
+        return PreparedC4(h, self.nq)
+
     def autotune(self, reps: int = 3) -> dict:
         """Runs the autotuner on this set's batched workload; returns its choice and timings."""
         chosen = np.zeros(3, np.int64)
@@ -152,6 +164,24 @@ class C4Set:
     def free(self) -> None:
         if self.h:
             qlib().wsq_c4_free(self.h)
+            self.h = None
+
+
+class PreparedC4:
+    """A prepared batch of the C4 workload: run() replays it (digests as C4Set.run)."""
+
+    def __init__(self, h, nq: int) -> None:
+        self.h, self.nq = h, nq
+
+    def run(self, digests: bool = True):
+        d = np.zeros(self.nq, np.uint64)
+        g = np.zeros(self.nq, np.int64)
+        _ck(qlib().wsq_c4_replay(self.h, d.ctypes.data if digests else None, g.ctypes.data))
+        return (d if digests else None), g
+
+    def free(self) -> None:
+        if self.h:
+            qlib().wsq_c4_release(self.h)
             self.h = None
 
 

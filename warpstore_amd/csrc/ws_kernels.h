@@ -192,6 +192,8 @@ __device__ __forceinline__ i64 int_part(const Ev& ev, int a, i64 v) {
 template <int SCOPE>
 __device__ __forceinline__ void atomic_combine_d(int op, double* addr, double v) {
   if (v != v) return;
+  // adding zero to a global sum changes nothing (a sum slot starts at +0.0, so it never holds -0.0)
+  if (SCOPE == __HIP_MEMORY_SCOPE_AGENT && op == WS_AGG_SUM && v == 0.0) return;
   if (op == WS_AGG_SUM || op == WS_AGG_COUNT) {
     __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, SCOPE);
     return;
@@ -207,6 +209,7 @@ __device__ __forceinline__ void atomic_combine_d(int op, double* addr, double v)
 }
 template <int SCOPE>
 __device__ __forceinline__ void atomic_combine_i(int op, i64* addr, i64 v) {
+  if (SCOPE == __HIP_MEMORY_SCOPE_AGENT && op != WS_AGG_MIN && op != WS_AGG_MAX && v == 0) return;  // + 0: no change
   if (SCOPE == __HIP_MEMORY_SCOPE_AGENT && (op == WS_AGG_MIN || op == WS_AGG_MAX)) {
     // A global MIN only decreases (MAX only grows) while a kernel runs, and caches start clean at
     // every launch, so even a stale cached value that already beats v proves the update
@@ -1039,13 +1042,39 @@ __device__ __forceinline__ void dense_fold(const ws_agg_plan& plan, uint8_t k, i
 
 constexpr int kBatchRows = 4;
 constexpr i64 kSkipRow = -2;  // a row the step does not aggregate (dense_index gives -1 for a key out of range)
+constexpr i64 kCopyGroups = 64;  // direct tables of at most this many groups are replicated per lane group
+// Folds the LDS table (slots [0, St) of query Q; lg[i] >= 0: slot i holds group lg[i], a group may
+// have several slots) into Q's accumulators.
+__device__ inline void batch_flush(const BatchQuery& Q, const int* lg, const double* lacc, int St) {
+  const ws_agg_plan& plan = *Q.d.plan;
+  const ws_dense_spec& ds = *Q.d.ds;
+  const int NA = plan.naggs;
+  for (int i = threadIdx.x; i < St; i += blockDim.x) {
+    const int g = lg[i];
+    if (g < 0) continue;
+    for (int a = 0; a < NA; ++a) {
+      const double v = lacc[(i64)i * NA + a];
+      const uint8_t k = ds.kind[a];
+      if (plan.agg_bits[a]) atomic_combine_i<WS_DEV>(dk_op(k), (i64*)&Q.gacc[(i64)g * NA + a], dbits(v));
+      else atomic_combine_d<WS_DEV>(a == 0 ? WS_AGG_SUM : dk_op(k), &Q.gacc[(i64)g * NA + a], v);
+    }
+  }
+}
+// Each block takes a contiguous range of the tile list (a query's tiles are consecutive there), so
+// a query whose groups fit the LDS table directly keeps them there across its tiles and folds them
+// into its accumulators once per block, when the block moves on to another query.
 template <class Ev>
 __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i64 ntiles, int S) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
   __shared__ ws_colview scols[WS_MAX_COLS];
   int* lg = (int*)ws_smem;                                    // [S] group index, -1 = free
   double* lacc = (double*)(ws_smem + (((i64)S * 4 + 15) & ~(i64)15));  // [S][NA]
-  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const i64 per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const i64 t0 = (i64)blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  int tab_q = -1, tab_St = 0;  // the query whose groups the table holds, and its slots in use
+  bool tab_keep = false;       // the table holds a direct layout that the query's next tile continues
+  for (i64 t = t0; t < t1; ++t) {
     const BatchTile bt = tiles[t];
     const BatchQuery& Q = qs[bt.q];
     const ScanDesc& d = Q.d;
@@ -1053,32 +1082,47 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
     const ws_dense_spec& ds = *d.ds;
     Ev ev(d);
     const int NK = ev.nkeys(), NA = ev.naggs();
-    // this tile's table: every group of the query at its own slot when G <= S (direct); else 2 x
-    // the tile's possible groups, hashed (a power of two, >= 64, <= S); else none
+    // this tile's table: every group of the query at its own slot when G <= S (direct, kept across
+    // the query's tiles); else 2 x the tile's possible groups, hashed (a power of two, >= 64,
+    // <= S); else none
     const i64 G = ds.G, reach = bt.count < G ? (i64)bt.count : G;
     const bool direct = G <= (i64)S;
-    int St = direct ? (int)G : 0;
+    // a few groups: C copies of the direct table, lane l using copy l % C, so the lanes of one LDS
+    // atomic spread over C x G slots instead of contending for G
+    const int C = direct && G <= kCopyGroups ? (int)(S / G < 8 ? S / G : 8) : 1;
+    int St = direct ? (int)G * C : 0;
     if (!direct && G <= 4 * (i64)bt.count && 2 * reach <= S) {
       St = 64;
       while (St < S && St < 2 * reach) St <<= 1;
       St = St < S ? St : S;
     }
-    __syncthreads();  // the previous tile's fold has read the table
-    for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)bt.b * d.ncols + i];
-    for (int i = threadIdx.x; i < St; i += blockDim.x) {
-      lg[i] = -1;
-      for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, dk_op(ds.kind[a]));
+    const bool cont = tab_keep && tab_q == bt.q;
+    __syncthreads();  // the previous tile's rows are folded; its column views are no longer read
+    if (!cont) {
+      if (tab_q >= 0) {
+        batch_flush(qs[tab_q], lg, lacc, tab_St);
+        __syncthreads();
+      }
+      for (int i = threadIdx.x; i < St; i += blockDim.x) {
+        lg[i] = -1;
+        for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(plan.agg_bits[a] != 0, dk_op(ds.kind[a]));
+      }
+      tab_q = St > 0 ? bt.q : -1;
+      tab_St = St;
+      tab_keep = direct;
     }
+    for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)bt.b * d.ncols + i];
     __syncthreads();
     ev.bind(Ev::kGlobalCols ? d.cols + (i64)bt.b * d.ncols : scols);
     const uint32_t* del = uni(d.del[bt.b]);
+    const int copy_base = (lane % C) * (int)G;
     bool bad = false;
     // kBatchRows rows per thread per step: their filter and key loads are issued before any fold
     // (evaluated for every row, rows past the tile re-reading its first, when the plan has no side
     // effects and the batch no deleted rows; row by row otherwise)
     constexpr int U = kBatchRows;
     const bool spec = Ev::kPureKeys && Ev::kPureWhere && !del;
-    for (int i0 = threadIdx.x; i0 < bt.count; i0 += blockDim.x * U) {
+    for (int i0 = threadIdx.x; i0 < bt.count; i0 += (int)blockDim.x * U) {
       i64 gs[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -1096,57 +1140,46 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-      const i64 g = gs[u];
-      if (g == kSkipRow) continue;
-      const i64 r = bt.first + i0 + u * (int)blockDim.x;
-      if (g < 0) {
-        bad = true;
-        continue;
-      }
-      int slot = -1;
-      if (direct) {
-        slot = (int)g;
-        if (lg[slot] < 0) lg[slot] = slot;  // occupied (a plain store: every writer stores the same)
-      } else if (St > 0) {
-        int s = (int)(g & (St - 1));
-        for (int probe = 0; probe < 8; ++probe) {
-          int prev = __hip_atomic_load(&lg[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (prev == -1 && __hip_atomic_compare_exchange_strong(&lg[s], &prev, (int)g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
-            prev = (int)g;
-          if (prev == (int)g) {
-            slot = s;
-            break;
+        const i64 g = gs[u];
+        const i64 r = bt.first + i0 + u * (int)blockDim.x;
+        if (g == -1) bad = true;
+        const bool valid = g >= 0;
+        if (!valid) continue;
+        int slot = -1;
+        if (direct) {
+          slot = copy_base + (int)g;
+          if (lg[slot] < 0) lg[slot] = (int)g;  // occupied (a plain store: every writer stores the same)
+        } else if (St > 0) {
+          int s = (int)(g & (St - 1));
+          for (int probe = 0; probe < 8; ++probe) {
+            int prev = __hip_atomic_load(&lg[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prev == -1 && __hip_atomic_compare_exchange_strong(&lg[s], &prev, (int)g, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_WORKGROUP))
+              prev = (int)g;
+            if (prev == (int)g) {
+              slot = s;
+              break;
+            }
+            s = (s + 1) & (St - 1);
           }
-          s = (s + 1) & (St - 1);
         }
-      }
-      double* dst = slot >= 0 ? lacc + (i64)slot * NA : Q.gacc + g * NA;
+        double* dst = slot >= 0 ? lacc + (i64)slot * NA : Q.gacc + g * NA;
 #pragma unroll
-      for (int a = 0; a < Ev::kAggs; ++a) {
-        if (a >= NA) break;
-        const uint8_t k = ev.dkind(a, ds);
-        i64 iv = 0;
-        double dv = 0;
-        dense_value(ev, plan, k, a, r, iv, dv);
-        if (slot >= 0) dense_fold<WS_LDS>(plan, k, a, &dst[a], iv, dv);
-        else dense_fold<WS_DEV>(plan, k, a, &dst[a], iv, dv);
-      }
+        for (int a = 0; a < Ev::kAggs; ++a) {
+          if (a >= NA) break;
+          const uint8_t k = ev.dkind(a, ds);
+          i64 iv = 0;
+          double dv = 0;
+          dense_value(ev, plan, k, a, r, iv, dv);
+          if (slot >= 0) dense_fold<WS_LDS>(plan, k, a, &dst[a], iv, dv);
+          else dense_fold<WS_DEV>(plan, k, a, &dst[a], iv, dv);
+        }
       }
     }
     if (bad) Q.flags[1] = 1;
-    __syncthreads();
-    for (int i = threadIdx.x; i < St; i += blockDim.x) {
-      const int g = lg[i];
-      if (g < 0) continue;
-      for (int a = 0; a < NA; ++a) {
-        const double v = lacc[(i64)i * NA + a];
-        const uint8_t k = ds.kind[a];
-        if (plan.agg_bits[a]) atomic_combine_i<WS_DEV>(dk_op(k), (i64*)&Q.gacc[(i64)g * NA + a], dbits(v));
-        else atomic_combine_d<WS_DEV>(a == 0 ? WS_AGG_SUM : dk_op(k), &Q.gacc[(i64)g * NA + a], v);
-      }
-    }
   }
+  __syncthreads();
+  if (tab_q >= 0) batch_flush(qs[tab_q], lg, lacc, tab_St);
 }
 
 // ------------------------------------------------------------------ partitioned hash GROUP BY

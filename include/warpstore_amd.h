@@ -231,6 +231,9 @@ int ws_gather(const void* src, int ptype, const int32_t* idx, int64_t m, void* d
 int ws_gather_enc(const ws_colview* v, int ptype, const int32_t* idx, int64_t m, void* dst);     /* gpuGatherEnc gpu.h:172 */
 int ws_key_bitmap(ws_colview key, int64_t n, int64_t lo, int64_t nbits, uint32_t* bits);         /* gpuKeyBitmap gpu.h:174 */
 int ws_set_bits(uint32_t* bits, const int32_t* idx, int64_t m);                                  /* gpuSetBits gpu.h:176 */
+/* No reference entry point: the number of set bits among `words` words (synchronous). The star-join
+   driver compares it with the filtered build rows: equal means the build keys are distinct. */
+int ws_bitmap_count(const uint32_t* bits, int64_t words, int64_t* count);
 int ws_gather_null(const void* src, int ptype, const int32_t* idx, int64_t m, void* dst);        /* gpuGatherNull gpu.h:193 */
 int ws_gather_bytes(const void* src, int esize, const int32_t* idx, int64_t m, void* dst, int nulls); /* gpuGatherBytes gpu.h:195 */
 

@@ -299,6 +299,11 @@ inline void gpuKeyBitmap(ColView key, int64_t n, int64_t lo, int64_t nbits, uint
   wsd::ck(ws_key_bitmap(wsd::view(key), n, lo, nbits, bits));
 }
 inline void gpuSetBits(uint32_t* bits, const int32_t* idx, int64_t m) { wsd::ck(ws_set_bits(bits, idx, m)); }
+inline int64_t gpuBitmapCount(const uint32_t* bits, int64_t words) {  // (no reference counterpart)
+  int64_t c = 0;
+  wsd::ck(ws_bitmap_count(bits, words, &c));
+  return c;
+}
 inline void gpuGatherNull(const void* src, PType t, const int32_t* idx, int64_t m, void* dst) {
   wsd::ck(ws_gather_null(src, (int)t, idx, m, dst));
 }

@@ -266,3 +266,80 @@ def test_needs_compiled_plan_error(gpu):
         assert gpu.ws_agg_run(h, C.byref(e), None) == ws.WS_ERR_NEEDS_PLAN
     finally:
         gpu.ws_agg_destroy(h)
+
+
+@pytest.mark.parametrize("case", ["sparse", "dense_filter", "deletes", "out_of_range"])
+def test_dense_staged_set_filter(gpu, case):
+    """Dense GROUP BY whose WHERE is a set-bitmap test (a join filter fused into the scan): the
+    kernel with the set staged in LDS (one 1024-thread block per CU, passing rows dealt out one per
+    lane and added a tile later) against the row-at-a-time kernel and the oracle. The 2^21-bit set
+    only partly fits the LDS; ragged batches give partial tiles. "dense_filter" passes 90 % of the
+    rows (more than 64 per wave and tile: earlier rounds added at once); "deletes" takes the
+    row-by-row path; "out_of_range" has keys outside the dense range, reported by both kernels."""
+    H.say(f"case dense staged set filter {case}")
+    n = 3_000_000
+    rng = np.random.default_rng(91)
+    big = (1 << 21) + 37
+    k = rng.integers(-50, big + 50, n).astype(np.int32)
+    g = rng.integers(0, 40, n).astype(np.int32)
+    v = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    p = rng.random(n) * 100
+    bits = rng.random((big + 31) // 32 * 32) < (0.9 if case == "dense_filter" else 0.05)
+    words = np.packbits(bits, bitorder="little").view(np.uint32).copy()
+    dset = ws.DevBuf.from_array(words)
+    deleted = rng.random(n) < 0.05 if case == "deletes" else None
+    t = H.Table({"k": k, "g": g, "v": v, "p": p}, [1_048_576, 5, 0, 1_200_001, 751_418], deleted)
+    b = abi.EnvBuilder()
+    sk, sg, sv, sp = b.slot(abi.T_I32), b.slot(abi.T_I32), b.slot(abi.T_I64), b.slot(abi.T_F64)
+    s0 = b.add_set(dset.p, big)
+    plan = abi.AggPlan()
+    plan.has_where = 1
+    plan.where = b.prog([(abi.OP_COL, sk), b.ilit(3), (abi.OP_SUB,), (abi.OP_INSET, s0)])
+    plan.nkeys = 1
+    plan.keys[0] = b.col(sg)
+    plan.key_int[0] = 1
+    ops = [abi.AGG_COUNT, abi.AGG_SUM, abi.AGG_SUM, abi.AGG_MIN, abi.AGG_MAX, abi.AGG_SUM, abi.AGG_MIN]
+    plan.naggs = len(ops)
+    for a, op in enumerate(ops):
+        plan.ops[a] = op
+    for a in (1, 2, 3, 4):
+        plan.aggs[a] = b.col(sv)
+        plan.agg_int[a] = 1
+    plan.agg_bits[1] = plan.agg_bits[2] = 1
+    plan.agg_part[1], plan.agg_part[2] = 1, 2
+    plan.agg_int[0] = 1
+    plan.aggs[5] = plan.aggs[6] = b.col(sp)
+    slots = [("k", abi.T_I32, 0), ("g", abi.T_I32, 0), ("v", abi.T_I64, 0), ("p", abi.T_F64, 0)]
+    dev, host = H.Bound(t, slots, True), H.Bound(t, slots, False)
+    ds = H.dense_spec(plan, [0], [30 if case == "out_of_range" else 40])
+    jit = jit_for(gpu, b.env, len(slots), plan, abi.JF_DENSE)
+    knob = C.c_int64()
+    ws.check(gpu.ws_tune_get(b"dense_staged_min_set_words", C.byref(knob)))
+    results = {}
+    try:
+        for kernel, words_min in (("staged", 0), ("plain", 1 << 40)):
+            ws.check(gpu.ws_tune_set(b"dense_staged_min_set_words", words_min))
+            gpu.ws_prof_reset()
+            gpu.ws_prof_only(b"")
+            gpu.ws_prof_enable(1)
+            try:
+                if case == "out_of_range":
+                    with pytest.raises(ws.WsError, match="outside the planned key range"):
+                        H.gpu_aggregate(plan, b.env, dev, dense=ds, jit=jit)
+                else:
+                    results[kernel] = H.gpu_aggregate(plan, b.env, dev, dense=ds, jit=jit)
+            finally:
+                prof = ws.prof_rows()
+                gpu.ws_prof_enable(0)
+            staged_ran = any(name.startswith("agg_dense_staged") for name in prof)
+            assert staged_ran == (kernel == "staged"), prof
+    finally:
+        ws.check(gpu.ws_tune_set(b"dense_staged_min_set_words", knob.value))
+    if case == "out_of_range":
+        return
+    b.env.sets[s0] = words.ctypes.data  # the oracle reads the set from host memory
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, host, dense=True)
+    assert len(ok) == 40
+    for kernel in ("staged", "plain"):
+        gk, ga, _, _ = results[kernel]
+        H.assert_groups_equal(plan, gk, ga, ok, oa)

@@ -73,16 +73,45 @@ def test_c3_star_join_full_size(gpu):
     fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
     dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
     try:
+        rows, st = Q.c3(fact, dim)  # distinct dimension keys: the join runs as the fused filter
+        Q.set_join_rewrite(False)
+        try:
+            jrows, jst = Q.c3(fact, dim)  # the hash-join plan
+        finally:
+            Q.set_join_rewrite(True)
+    finally:
+        fact.free()
+        dim.free()
+    exp = H.oracle_c3(fk, qty, price, flag, dkey, dattr)
+    assert np.array_equal(rows[np.argsort(rows[:, 0])], exp)
+    assert np.array_equal(jrows[np.argsort(jrows[:, 0])], exp)
+    sel = dattr < 50
+    assert st[0] == jst[0] == int(sel.sum())
+    assert st[2] == jst[2] == int(exp[:, 1].sum())  # every join pair lands in one group
+    assert st[1] == jst[1]
+    H.say(f"  dim={st[0]} fact_filtered={st[1]} pairs={st[2]} groups={len(exp)}")
+
+
+def test_c3_duplicate_dimension_keys_keep_the_join(gpu):
+    """A dimension whose filtered keys repeat: each fact row joins every copy, so the driver must
+    keep the hash join (the bitmap filter alone would count each row once)."""
+    H.say("case config star join, duplicate dimension keys")
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    fk, qty, price, flag, dkey, dattr = H.gen_c3(3_000_000, 200_000)
+    dkey = dkey.copy()
+    dkey[1::7] = dkey[0:-1:7][: len(dkey[1::7])]  # every 7th key repeats its neighbour's
+    dattr = dattr.copy()
+    dattr[1::7] = dattr[0:-1:7][: len(dattr[1::7])]  # with the same attribute, so both copies pass
+    fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
+    dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
+    try:
         rows, st = Q.c3(fact, dim)
     finally:
         fact.free()
         dim.free()
     exp = H.oracle_c3(fk, qty, price, flag, dkey, dattr)
     assert np.array_equal(rows[np.argsort(rows[:, 0])], exp)
-    sel = dattr < 50
-    assert st[0] == int(sel.sum())
-    assert st[2] == int(exp[:, 1].sum())  # every join pair lands in one group
-    H.say(f"  dim={st[0]} fact_filtered={st[1]} pairs={st[2]} groups={len(exp)}")
+    assert st[2] == int(exp[:, 1].sum()) and st[2] > st[1]  # more pairs than fact rows: the join ran
 
 
 def test_c4_all_256_queries(gpu):

@@ -80,6 +80,7 @@ _SIGS = {
     "ws_temp_column": (I, [P, I, P, I, I64, C.POINTER(abi.TempColSpec), C.POINTER(D), C.POINTER(D), PP]),
     "ws_eval_at": (I, [C.POINTER(ExprEnv), Prog, P, I64, P]), "ws_gather": (I, [P, I, P, I64, P]),
     "ws_gather_enc": (I, [C.POINTER(ColView), I, P, I64, P]), "ws_key_bitmap": (I, [ColView, I64, I64, I64, P]),
+    "ws_bitmap_count": (I, [P, I64, C.POINTER(I64)]),
     "ws_set_bits": (I, [P, P, I64]), "ws_gather_null": (I, [P, I, P, I64, P]),
     "ws_gather_bytes": (I, [P, I, P, I64, P, I]),
     "ws_hash_join": (I, [ColView, I64, ColView, I64, PP, PP, C.POINTER(I64)]),

@@ -43,9 +43,9 @@ __global__ void __launch_bounds__(1024) scan_counts_k(uint32_t* cnt, int64_t n, 
 __global__ void batch_counts_k(const ScanDesc* d, const uint32_t* tile_off, int stride, const u64* total, u64* per_batch) {
   batch_counts_body(*d, tile_off, stride, total, per_batch);
 }
-__global__ void __launch_bounds__(256) select_expand_k(const ScanDesc* d, const u64* masks, const uint8_t* tile_fmt,
-                                                        const uint32_t* tile_off, int stride, int32_t* out_idx) {
-  select_expand_body(*d, masks, tile_fmt, tile_off, stride, out_idx);
+__global__ void __launch_bounds__(kSelExpandThreads) select_expand_k(const u64* masks, const int64_t* tile_row,
+                                                                      const uint32_t* tile_off, int stride, int32_t* out_idx) {
+  select_expand_body(masks, (const i64*)tile_row, tile_off, stride, out_idx);
 }
 __global__ void init_table_k(GTable t, int nk, int na, ws_agg_plan plan) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= (int64_t)t.mask; i += (int64_t)gridDim.x * blockDim.x) {
@@ -932,10 +932,62 @@ TuneKnob g_tune[] = {
     {"part_pack", {1}, 0, 1},                   // packed partial-group records allowed (integer keys)
     {"select_chunks_per_cu", {8}, 1, 64},       // chunked selection: chunks of tiles per CU
     {"select_mask_min_rows", {1 << 16}, 0, (int64_t)1 << 40},  // compiled selection: ballot pass from this many rows
+    {"dense_staged_min_set_words", {1024}, 0, (int64_t)1 << 40},  // compiled dense GROUP BY: sets staged in LDS from this size
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
 enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
-       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks, kTuneMaskMinRows };
+       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks, kTuneMaskMinRows, kTuneDenseStagedWords };
+
+// Dense plans whose WHERE reads set bitmaps of at least dense_staged_min_set_words words, over at
+// least select_mask_min_rows rows: the staged kernel (one 1024-thread block per CU, up to four
+// LDS copies of the accumulators, the sets' leading words in the rest of the LDS; tiles of
+// kSelMaskRows rows per thread), then the usual fold of the block partials. False when the plan
+// does not qualify or its copies would leave the sets less than half the LDS.
+static bool run_dense_staged(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit, const ScanInput& in, ScanDesc d) {
+  const ws_agg_plan& plan = A->plan;
+  const ws_dense_spec& ds = A->dense;
+  void* jf = plan.has_where ? jit_function(jit, kJitDenseStaged) : nullptr;
+  if (!jf) return false;
+  int64_t set_words = 0, rows = 0;
+  for (int k = 0; k < WS_MAX_SETS; ++k)
+    if (env.sets[k] && env.set_size[k] > 0) set_words += ((int64_t)env.set_size[k] + 31) / 32;
+  for (int64_t n : in.n) rows += n;
+  if (set_words == 0 || set_words < tune(kTuneDenseStagedWords) || rows < tune(kTuneMaskMinRows)) return false;
+  size_t off = 0;
+  for (int a = 0; a < plan.naggs; ++a) {
+    d.slot_off[a] = (int32_t)off;
+    const size_t bytes = (size_t)ds.G * (ds.kind[a] == WS_DK_COUNT ? 4 : 8);
+    off += (bytes + 15) / 16 * 16;
+  }
+  const int C = (int)std::min<size_t>(4, std::max<size_t>(1, (16 * 1024) / std::max<size_t>(off, 1)));
+  const size_t copies = off * C;
+  const size_t stat = (size_t)jit_staged_static_lds(jit);
+  if (copies + stat > kLdsPerCu / 2) return false;
+  constexpr int kStagedBlock = 1024;
+  d.mode = 2;
+  d.lds_slots = C;
+  d.copy_stride = (int32_t)off;
+  d.tile_rows = kSelMaskRows * kStagedBlock;
+  int64_t ntiles = 0;
+  for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
+  d.set_words = (int32_t)std::min<int64_t>(set_words, (int64_t)(kLdsPerCu - stat - copies) / 4);
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ntiles, cu_count()));
+  const int64_t S = ds.G * plan.naggs;
+  DescUpload up;
+  build_desc(up, d, in, plan, &ds, env, false, 0, nullptr);
+  DevBuf<i64> parts(grid * S);
+  int* flags = A->flags;
+  i64* pp = parts.p;
+  void* args[] = {&up.ddesc, &pp, &flags};
+  jit_launch_raw(jf, (unsigned)grid, kStagedBlock, copies + 4 * (size_t)d.set_words, args, jit_prof_name(jit, "agg_dense_staged"));
+  ProfScope ps("agg_dense_reduce");
+  if (S <= 16384 && grid >= 64)
+    hipLaunchKernelGGL(dense_reduce_slot_k, dim3((unsigned)S), dim3(256), 0, stream(), plan, ds, (const i64*)pp, (int)grid, A->acc);
+  else
+    hipLaunchKernelGGL(dense_reduce_k, dim3(grid_for(S)), dim3(kBlock), 0, stream(), plan, ds, (const i64*)pp, (int)grid, A->acc);
+  check_launch("agg_dense_reduce");
+  return true;
+}
 
 struct PartPlan {
   int pbits = 0;  // 0: plain table path
@@ -1153,6 +1205,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
   }
   if (A->is_dense) {
     const ws_dense_spec& ds = A->dense;
+    if (jf && run_dense_staged(A, env, jit, in, d)) return;
     DensePlan dp = plan_dense(ds, plan.naggs);
     d.mode = dp.mode;
     d.lds_slots = dp.copies;
@@ -1764,18 +1817,18 @@ static int64_t select_launch(const ws_expr_env& env, int has_where, ws_prog wher
     constexpr int nw = 1;  // counts per tile
     DevBuf<u64> masks(ntiles * words);
     DevBuf<uint32_t> cnt(ntiles);  // per-tile counts, then their exclusive prefix
-    DevBuf<uint8_t> fmt(ntiles);   // per-tile word layout (1: raw four-row ballots)
+    DevBuf<int64_t> trow(ntiles);  // per tile: first global row * 2 + word layout (1: raw four-row ballots)
+    if (words > kSelExpandThreads || words % 4) throw Error("selection tile of " + std::to_string(d.tile_rows) + " rows", WS_ERR_INTERNAL);
     fill_u32(cnt.p, ntiles, 0);
-    fill_u8(fmt.p, ntiles, 0);
-    void* args[] = {&up.ddesc, &masks.p, &cnt.p, &fmt.p};
+    void* args[] = {&up.ddesc, &masks.p, &cnt.p, &trow.p};
     jit_launch_raw(jf, (unsigned)std::min<int64_t>(ntiles, cu_count()), kStagedBlock, lds, args,
                    jit_prof_name(jit, "select_mask"));
     ProfScope ps("select_expand");
     hipLaunchKernelGGL(scan_counts_k, dim3(1), dim3(1024), 0, stream(), cnt.p, ntiles * nw, total);
     hipLaunchKernelGGL(batch_counts_k, dim3(1), dim3(256), 0, stream(), (const ScanDesc*)up.ddesc, (const uint32_t*)cnt.p, nw,
                        (const u64*)total, pb);
-    hipLaunchKernelGGL(select_expand_k, dim3((unsigned)ntiles), dim3(words), 0, stream(), (const ScanDesc*)up.ddesc,
-                       (const u64*)masks.p, (const uint8_t*)fmt.p, (const uint32_t*)cnt.p, nw, out_idx);
+    hipLaunchKernelGGL(select_expand_k, dim3((unsigned)ntiles), dim3(words), 0, stream(), (const u64*)masks.p,
+                       (const int64_t*)trow.p, (const uint32_t*)cnt.p, nw, out_idx);
     check_launch("select_expand");
   } else if (jf) {
     void* args[] = {&up.ddesc, &st.p, &ctr, &out_idx, &pb, &total, &err};

@@ -383,6 +383,27 @@ void function(std::ostringstream& o, const Lowering& lw, const std::string& sign
   o << "  }\n";
 }
 
+// The same function over a row's loaded column words (x, of struct VRow): adds the columns it
+// reads to `cols`; clears `ok` when a read has no row form.
+std::string function_row(const Lowering& lw, const std::string& signature, const std::vector<int>& roots,
+                         const std::string& kind /* "double", "int" */, std::set<int>& cols, bool& ok) {
+  std::ostringstream o;
+  o << "  __device__ " << signature << " const {\n    switch (i) {\n";
+  for (size_t i = 0; i < roots.size(); ++i) {
+    std::ostringstream body;
+    Emitter em(lw, Source::Row, body);
+    em.emit(roots[i]);
+    std::string v = Emitter::tmp(roots[i]);
+    if (kind == "double") v = em.dbl(roots[i]);
+    else if (!lw.is_int(roots[i])) v = "(long long)" + v;
+    cols.insert(em.row_cols.begin(), em.row_cols.end());
+    ok = ok && em.row_ok;
+    o << "    case " << i << ": {\n" << body.str() << std::string(4, ' ') << "  return " << v << ";\n    }\n";
+  }
+  o << "    }\n    return 0;\n  }\n";
+  return o.str();
+}
+
 int dense_kind(int op, bool is_int) {
   if (op == WS_AGG_COUNT) return WS_DK_COUNT;
   if (op == WS_AGG_SUM) return is_int ? WS_DK_SUM_I : WS_DK_SUM_D;
@@ -551,6 +572,32 @@ std::string evaluator_source(const ws_expr_env& env, int ncols, const ws_agg_pla
   function(o, lw, "long long ikey(int i, long long r)", keys, "int");
   function(o, lw, "double agg(int i, long long r)", aggs, "double");
   function(o, lw, "long long iagg(int i, long long r)", iaggs, "int");
+  // Keys and aggregate arguments in a load half (load_vrow: the row's column words) and a compute
+  // half (ikey_v, iagg_v, agg_v), for kernels that issue a row's loads a step before they use them.
+  {
+    std::set<int> cols;
+    bool ok = true;
+    const std::string fk = function_row(lw, "long long ikey_v(int i, const VRow& x, long long r)", keys, "int", cols, ok);
+    const std::string fi = function_row(lw, "long long iagg_v(int i, const VRow& x, long long r)", iaggs, "int", cols, ok);
+    const std::string fa = function_row(lw, "double agg_v(int i, const VRow& x, long long r)", aggs, "double", cols, ok);
+    o << "  static constexpr bool kVSplit = " << (ok ? "true" : "false") << ";\n";
+    o << "  struct VRow {";
+    if (ok)
+      for (int c : cols) o << " " << (env.cols[c].enc ? "unsigned" : elem_type(env.cols[c].t)) << " w" << c << ";";
+    o << " int pad_; };\n";
+    o << "  __device__ void load_vrow(long long r, VRow& x) const {";
+    if (ok)
+      for (int c : cols)
+        o << " " << row_word(c) << " = " << (env.cols[c].enc ? "ws::enc_word(" + enc_view(c) + ",r)" : col_ptr(c) + "[r]") << ";";
+    o << " }\n";
+    if (ok) {
+      o << fk << fi << fa;
+    } else {
+      o << "  __device__ long long ikey_v(int i, const VRow&, long long r) const { return ikey(i, r); }\n";
+      o << "  __device__ long long iagg_v(int i, const VRow&, long long r) const { return iagg(i, r); }\n";
+      o << "  __device__ double agg_v(int i, const VRow&, long long r) const { return agg(i, r); }\n";
+    }
+  }
   o << "};\n";
   return o.str();
 }

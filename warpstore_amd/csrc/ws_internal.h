@@ -163,7 +163,8 @@ constexpr int kJitPartCount = 100, kJitPartScatter = 101;  // partition passes o
 constexpr int kJitBatchDense = 102;                         // batched kernel of a dense plan
 constexpr int kJitSelectChunk = 103;                        // chunked kernel of a selection plan
 constexpr int kJitSelectStaged = 104;  // persistent chunked kernel of a selection plan (sets staged in LDS)
-int jit_staged_static_lds(const ws_jit_program* p);  // static LDS bytes of that kernel
+constexpr int kJitDenseStaged = 105;   // dense GROUP BY kernel of a dense plan with its sets staged in LDS
+int jit_staged_static_lds(const ws_jit_program* p);  // static LDS bytes of either staged kernel
 // ---- learned sizing (ws_sizing.hip): appends one (features, exact groups) sample to the log
 void sizing_log(const float* features, int64_t groups);
 void jit_launch_raw(void* fn, unsigned grid, unsigned block, size_t shmem, void** args, const char* prof_name);

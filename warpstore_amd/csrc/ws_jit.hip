@@ -39,7 +39,7 @@ struct ws_jit_program {
   hipFunction_t fn_part[2] = {nullptr, nullptr};  // grouped plans: partition count / scatter kernels
   hipFunction_t fn_batch = nullptr;               // dense plans: the batched (many queries) kernel
   hipFunction_t fn_chunk = nullptr;               // selection plans: the chunked kernel
-  hipFunction_t fn_staged = nullptr;              // selection plans: the persistent chunked kernel (sets in LDS)
+  hipFunction_t fn_staged = nullptr;              // selection and dense plans: the kernel with the sets in LDS
   int staged_static_lds = 0;                      // its static LDS bytes
   std::atomic<int> state{0};  // 0 compiling, 1 ready, 2 failed
   bool from_disk = false;
@@ -90,7 +90,7 @@ thread_local std::string t_sql;
 thread_local ws_jit_query_info t_info;
 thread_local std::vector<ws_jit_program*> t_used;
 
-const char* kKernelMacro[] = {"WS_KERNEL_NOKEYS", "WS_KERNEL_GROUPED", "WS_KERNEL_DENSE", "WS_KERNEL_SELECT_JIT",
+const char* kKernelMacro[] = {"WS_KERNEL_NOKEYS", "WS_KERNEL_GROUPED", "WS_KERNEL_DENSE_JIT", "WS_KERNEL_SELECT_JIT",
                               "WS_KERNEL_SELECT_JIT"};
 const char* kKernelName[] = {"ws_q_nokeys", "ws_q_grouped", "ws_q_dense", "ws_q_select", "ws_q_select"};
 
@@ -233,7 +233,11 @@ void load_image(ws_jit_program& p, const std::string& image) {
   hipFunction_t f;
   WS_CK(hipModuleGetFunction(&f, mod, kKernelName[p.fn_kind]));
   p.fn = f;
-  if (p.fn_kind == WS_JF_DENSE) WS_CK(hipModuleGetFunction(&p.fn_batch, mod, (std::string(kKernelName[p.fn_kind]) + "_batch").c_str()));
+  if (p.fn_kind == WS_JF_DENSE) {
+    WS_CK(hipModuleGetFunction(&p.fn_batch, mod, (std::string(kKernelName[p.fn_kind]) + "_batch").c_str()));
+    WS_CK(hipModuleGetFunction(&p.fn_staged, mod, (std::string(kKernelName[p.fn_kind]) + "_staged").c_str()));
+    WS_CK(hipFuncGetAttribute(&p.staged_static_lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, p.fn_staged));
+  }
   if (p.fn_kind == WS_JF_FLAG || p.fn_kind == WS_JF_FLAGM) {
     WS_CK(hipModuleGetFunction(&p.fn_chunk, mod, (std::string(kKernelName[p.fn_kind]) + "_chunk").c_str()));
     WS_CK(hipModuleGetFunction(&p.fn_staged, mod, (std::string(kKernelName[p.fn_kind]) + "_staged").c_str()));
@@ -328,6 +332,7 @@ void* jit_function(const ws_jit_program* p, int fn) {
   if (fn == kJitBatchDense) return p->fn_kind == WS_JF_DENSE ? (void*)p->fn_batch : nullptr;
   if (fn == kJitSelectChunk) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_chunk : nullptr;
   if (fn == kJitSelectStaged) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_staged : nullptr;
+  if (fn == kJitDenseStaged) return p->fn_kind == WS_JF_DENSE ? (void*)p->fn_staged : nullptr;
   const bool sel = (fn == WS_JF_FLAG || fn == WS_JF_FLAGM) && (p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM);
   return (p->fn_kind == fn || sel) ? (void*)p->fn : nullptr;
 }

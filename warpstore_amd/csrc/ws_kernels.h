@@ -290,12 +290,47 @@ __device__ __forceinline__ const unsigned WS_LOCAL* lds_spare_word() {
   __shared__ unsigned spare;  // (never written: its value is always masked off)
   return (const unsigned WS_LOCAL*)&spare;
 }
-// Copies min(set words, words - off) words of a set to lds[off...]; returns the count.
+// Copies min(set words, words - off) words of a set to lds[off...]; returns the count. Each
+// thread issues all loads of a round before its first LDS store, so a round costs one memory
+// round trip: 16-byte loads when source and destination allow, else 4-byte ones.
 __device__ __forceinline__ int stage_set(unsigned* lds, int off, int words, const unsigned WS_GLOBAL* g, i64 size) {
   const i64 need = size > 0 ? (size + 31) >> 5 : 0;
   const i64 room = words - off > 0 ? words - off : 0;
   const int w = (int)(need < room ? need : room);
-  for (int i = threadIdx.x; i < w; i += blockDim.x) lds[off + i] = g[i];
+  constexpr int U = 8;
+  const int T = blockDim.x;
+  int done = 0;
+  if ((((unsigned long long)g | (unsigned long long)(lds + off)) & 15) == 0) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const int w4 = w >> 2;
+    for (int base = 0; base < w4; base += U * T) {
+      u4 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * T + (int)threadIdx.x;
+        if (i < w4) v[u] = ((const u4 WS_GLOBAL*)g)[i];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = base + u * T + (int)threadIdx.x;
+        if (i < w4) ((u4*)(lds + off))[i] = v[u];
+      }
+    }
+    done = w4 << 2;
+  }
+  for (int base = done; base < w; base += U * T) {
+    unsigned v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * T + (int)threadIdx.x;
+      if (i < w) v[u] = g[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * T + (int)threadIdx.x;
+      if (i < w) lds[off + i] = v[u];
+    }
+  }
   return w;
 }
 // size is the set's bit count (0 <= size < 2^31), so a member's index fits 32 bits.
@@ -336,6 +371,36 @@ __device__ __forceinline__ TileRows tile_rows(const ScanDesc& d, i64 t, int b) {
   tr.count = uni(left < d.tile_rows ? left : (i64)d.tile_rows);
   return tr;
 }
+// The batch of a block's current tile, with the fields its tile loop needs held in scalar
+// registers. The descriptor is read only when the batch changes: a read of it inside a loop that
+// stores to global memory is a vector load (the compiler cannot prove the descriptor unchanged,
+// which a scalar load needs), and waiting for one waits for every load issued before it, a
+// prefetch of the next tile's columns included.
+struct BatchCursor {
+  int b = -1;
+  i64 t_lo = 0, t_hi = 0, n = 0, tr = 0, row0 = 0;
+  const uint32_t* del = nullptr;
+  // Moves to tile t's batch; true when it differs from the previous one (block-uniform).
+  __device__ __forceinline__ bool seek(const ScanDesc& d, i64 t) {
+    if (b >= 0 && t >= t_lo && t < t_hi) return false;
+    b = b < 0 || t < t_lo ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, b);
+    t_lo = uni((i64)d.tile0[b]);
+    t_hi = uni((i64)d.tile0[b + 1]);
+    n = uni((i64)d.n[b]);
+    tr = uni((i64)d.tile_rows);
+    row0 = d.row0 ? uni((i64)d.row0[b]) : 0;
+    del = uni(d.del[b]);
+    return true;
+  }
+  __device__ __forceinline__ TileRows rows(i64 t) const {
+    TileRows r;
+    r.b = b;
+    r.first = (t - t_lo) * tr;
+    const i64 left = n - r.first;
+    r.count = left < tr ? left : tr;
+    return r;
+  }
+};
 // Block-wide: stage batch b's column views in LDS (all threads call).
 __device__ __forceinline__ void stage_batch(const ScanDesc& d, int b, ws_colview* scols) {
   __syncthreads();
@@ -478,16 +543,15 @@ __device__ void nokeys_body(const ScanDesc& d, double* out, NoKeysPartial* parts
     facc[a] = agg_identity(op);
   }
   i64 cnt = 0;
-  int cur = -1;
-  for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-    const int b = batch_of_tile(d, t, cur);
-    if (b != cur) {
-      stage_batch(d, b, scols);
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-      cur = b;
+  BatchCursor bc;
+  const i64 ntiles = uni(d.ntiles);
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    if (bc.seek(d, t)) {
+      stage_batch(d, bc.b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
-    const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = uni(d.del[b]);
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
 #pragma unroll
     for (int j = 0; j < Ev::kRows; ++j) {
       for (i64 i = threadIdx.x + (i64)j * blockDim.x; i < tr.count; i += (i64)Ev::kRows * blockDim.x) {
@@ -620,6 +684,49 @@ __device__ __forceinline__ void dense_value(const Ev& ev, const ws_agg_plan& pla
 // Final slot encoding (reference kernels.cuh:489-506, 616-630): agg_bits -> int64 bits; otherwise a
 // double ((double) of the exact integer).
 
+// The dense LDS copies (layout mode 2): d.lds_slots copies of d.copy_stride bytes, slot a's array
+// at d.slot_off[a] within a copy (u32 counts, int64 or double otherwise). Block-wide.
+__device__ void dense_init_copies(const ScanDesc& d, const ws_dense_spec& ds, int NA, unsigned char* smem) {
+  const i64 G = ds.G;
+  for (int c = 0; c < d.lds_slots; ++c) {
+    unsigned char* cb = smem + (i64)c * d.copy_stride;
+    for (int a = 0; a < NA; ++a) {
+      const uint8_t k = ds.kind[a];
+      unsigned char* arr = cb + d.slot_off[a];
+      for (i64 g = threadIdx.x; g < G; g += blockDim.x) {
+        if (k == WS_DK_COUNT) ((unsigned*)arr)[g] = 0u;
+        else if (dk_int(k)) ((i64*)arr)[g] = int_identity(dk_op(k));
+        else ((double*)arr)[g] = agg_identity(dk_op(k));
+      }
+    }
+  }
+}
+// The block's partial of every (group, slot) from its LDS copies, written coalesced to
+// out[g * NA + a] (int64 or double bits). Block-wide, after a barrier.
+__device__ void dense_write_copies(const ScanDesc& d, const ws_dense_spec& ds, int NA, const unsigned char* smem, i64* out) {
+  const i64 S = ds.G * NA;
+  const int C = d.lds_slots;
+  for (i64 s = threadIdx.x; s < S; s += blockDim.x) {
+    const i64 g = s / NA;
+    const int a = (int)(s - g * NA);
+    const uint8_t k = ds.kind[a];
+    const int op = dk_op(k);
+    if (k == WS_DK_COUNT) {
+      i64 x = 0;
+      for (int c = 0; c < C; ++c) x += ((const unsigned*)(smem + (i64)c * d.copy_stride + d.slot_off[a]))[g];
+      out[s] = x;
+    } else if (dk_int(k)) {
+      i64 x = int_identity(op);
+      for (int c = 0; c < C; ++c) x = combine_i(op, x, ((const i64*)(smem + (i64)c * d.copy_stride + d.slot_off[a]))[g]);
+      out[s] = x;
+    } else {
+      double x = agg_identity(op);
+      for (int c = 0; c < C; ++c) x = combine_d(op, x, ((const double*)(smem + (i64)c * d.copy_stride + d.slot_off[a]))[g]);
+      out[s] = dbits(x);
+    }
+  }
+}
+
 // Dense filter + GROUP BY. Three physical layouts, chosen by the host planner per launch:
 //   mode 1: per-thread private accumulators in LDS ([slot][thread], no atomics per row);
 //   mode 2: `copies` LDS copies of all group accumulators, copy = wave % copies, LDS atomics;
@@ -648,32 +755,20 @@ __device__ void dense_body(const ScanDesc& d, double* gacc, i64* parts, int* fla
       sl[(i64)s * T + tid] = init;
     }
   } else if (mode == 2) {
-    for (int c = 0; c < C; ++c) {
-      unsigned char* cb = ws_smem + (i64)c * d.copy_stride;
-      for (int a = 0; a < NA; ++a) {
-        const uint8_t k = ds.kind[a];
-        unsigned char* arr = cb + d.slot_off[a];
-        for (i64 g = tid; g < G; g += T) {
-          if (k == WS_DK_COUNT) ((unsigned*)arr)[g] = 0u;
-          else if (dk_int(k)) ((i64*)arr)[g] = int_identity(dk_op(k));
-          else ((double*)arr)[g] = agg_identity(dk_op(k));
-        }
-      }
-    }
+    dense_init_copies(d, ds, NA, ws_smem);
   }
   const int wave = tid / kWave;
   unsigned char* mine = mode == 2 ? ws_smem + (i64)(wave % C) * d.copy_stride : nullptr;
   bool bad = false;
-  int cur = -1;
-  for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-    const int b = batch_of_tile(d, t, cur);
-    if (b != cur) {
-      stage_batch(d, b, scols);
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-      cur = b;
+  BatchCursor bc;
+  const i64 ntiles = uni(d.ntiles);
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    if (bc.seek(d, t)) {
+      stage_batch(d, bc.b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
-    const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = uni(d.del[b]);
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
     for (i64 i = tid; i < tr.count; i += T) {
       const i64 r = tr.first + i;
       if (is_deleted(del, r)) continue;
@@ -752,25 +847,7 @@ __device__ void dense_body(const ScanDesc& d, double* gacc, i64* parts, int* fla
       }
     }
   } else {
-    for (i64 s = tid; s < S; s += T) {
-      const i64 g = s / NA;
-      const int a = (int)(s - g * NA);
-      const uint8_t k = ds.kind[a];
-      const int op = dk_op(k);
-      if (k == WS_DK_COUNT) {
-        i64 x = 0;
-        for (int c = 0; c < C; ++c) x += ((const unsigned*)(ws_smem + (i64)c * d.copy_stride + d.slot_off[a]))[g];
-        out[s] = x;
-      } else if (dk_int(k)) {
-        i64 x = int_identity(op);
-        for (int c = 0; c < C; ++c) x = combine_i(op, x, ((const i64*)(ws_smem + (i64)c * d.copy_stride + d.slot_off[a]))[g]);
-        out[s] = x;
-      } else {
-        double x = agg_identity(op);
-        for (int c = 0; c < C; ++c) x = combine_d(op, x, ((const double*)(ws_smem + (i64)c * d.copy_stride + d.slot_off[a]))[g]);
-        out[s] = dbits(x);
-      }
-    }
+    dense_write_copies(d, ds, NA, ws_smem, out);
   }
 }
 
@@ -932,21 +1009,20 @@ __device__ void grouped_body(const ScanDesc& d, GTable gt) {
     for (int a = 0; a < NA; ++a) lacc[(i64)i * NA + a] = slot_identity(ev.agg_bits(a), ev.op(a));
   }
   if (threadIdx.x == 0) s_new = 0;
-  int cur = -1;
-  for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+  BatchCursor bc;
+  const i64 ntiles = uni(d.ntiles);
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     // Overflowed: the whole aggregation is redone bigger, stop early (block-uniform decision).
     __syncthreads();
     if (threadIdx.x == 0) s_stop = *(volatile int*)gt.flags;
     __syncthreads();
     if (s_stop) break;
-    const int b = batch_of_tile(d, t, cur);
-    if (b != cur) {
-      stage_batch(d, b, scols);
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-      cur = b;
+    if (bc.seek(d, t)) {
+      stage_batch(d, bc.b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
-    const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = uni(d.del[b]);
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
     for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
       const i64 r = tr.first + i;
       if (is_deleted(del, r)) continue;
@@ -1222,16 +1298,14 @@ __device__ void part_count_body(const ScanDesc& d) {
   for (int k = 0; k < kPackKeys; ++k) kmin[k] = kI64Max, kmax[k] = kI64Min;
   const i64 c = blockIdx.x;
   const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
-  int cur = -1;
+  BatchCursor bc;
   for (i64 t = t0; t < t1; ++t) {
-    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
-    if (b != cur) {
-      stage_batch(d, b, scols);  // also orders the histogram reset before the first update
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-      cur = b;
+    if (bc.seek(d, t)) {
+      stage_batch(d, bc.b, scols);  // also orders the histogram reset before the first update
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
-    const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = uni(d.del[b]);
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
     // U rows per thread per step, evaluated before any LDS update so their loads overlap (the
     // interpreter's out-of-line VM calls gain nothing from it: one row there)
     constexpr int U = Ev::kRows > 1 ? kPartRows : 1;
@@ -1358,16 +1432,14 @@ __device__ void part_scatter_body(const ScanDesc& d) {
     for (int a = 0; a < NA; ++a) pacc[(i64)i * NA + a] = slot_identity(ev.agg_bits(a), ev.op(a));
   }
   const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
-  int cur = -1;
+  BatchCursor bc;
   for (i64 t = t0; t < t1; ++t) {
-    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
-    if (b != cur) {
-      stage_batch(d, b, scols);  // also orders the table set-up before the first row
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-      cur = b;
+    if (bc.seek(d, t)) {
+      stage_batch(d, bc.b, scols);  // also orders the table set-up before the first row
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
-    const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = uni(d.del[b]);
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
     // Ev::kRows rows per thread per step: all are evaluated (their loads overlap) before the LDS
     // table and the record writes see them.
     constexpr int U = Ev::kRows > 1 && Ev::kKeys + Ev::kAggs <= 6 ? kPartRows : 1;  // register budget
@@ -1660,19 +1732,18 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
     if (t0 >= d.ntiles) return;  // (block-uniform)
     u64* s_mask = (u64*)ws_smem;                         // [tiles][kSelRows][nw]
     uint32_t* s_pre = (uint32_t*)(s_mask + (t1 - t0) * per_tile);  // exclusive prefix of their counts
-    int cur = -1;
-    u64 bcount = 0;  // this wave's passing rows in batch cur, added to per_batch when the batch ends
+    BatchCursor bc;
+    u64 bcount = 0;  // this wave's passing rows in batch bc.b, added to per_batch when the batch ends
     for (i64 t = t0; t < t1; ++t) {
-      const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
-      if (b != cur) {
-        if (cur >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+      const int prev = bc.b;
+      if (bc.seek(d, t)) {
+        if (prev >= 0 && lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[prev], (unsigned long long)bcount);
         bcount = 0;
-        stage_batch(d, b, scols);
-        ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-        cur = b;
+        stage_batch(d, bc.b, scols);
+        ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
       }
-      const TileRows tr = tile_rows(d, t, b);
-      const uint32_t* del = uni(d.del[b]);
+      const TileRows tr = bc.rows(t);
+      const uint32_t* del = bc.del;
       bool pass[kSelChunkRows];
       if (Ev::kPureWhere || !del) {  // (block-uniform)
         // Every row's WHERE is evaluated (rows past the tile re-read its first row), so the loads of
@@ -1704,7 +1775,7 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
         bcount += (u64)__popcll(m);
       }
     }
-    if (lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[cur], (unsigned long long)bcount);
+    if (lane == 0 && bcount) atomicAdd((unsigned long long*)&per_batch[bc.b], (unsigned long long)bcount);
     __syncthreads();
     // Exclusive prefix of the ballots' counts in (tile, step, wave) order: per thread a run of
     // consecutive ballots, then a block scan of the run totals.
@@ -1739,12 +1810,11 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
     }
     __syncthreads();
     const u64 base = s_excl;
-    cur = -1;
+    BatchCursor wc;
     for (i64 t = t0; t < t1; ++t) {
-      const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
-      cur = b;
-      const TileRows tr = tile_rows(d, t, b);
-      const i64 g0 = d.row0[b] + tr.first;
+      wc.seek(d, t);
+      const TileRows tr = wc.rows(t);
+      const i64 g0 = wc.row0 + tr.first;
 #pragma unroll
       for (int j = 0; j < kSelChunkRows; ++j) {
         const int q = (int)(t - t0) * per_tile + j * nw + wave;
@@ -1764,7 +1834,7 @@ __device__ void select_chunk_body(const ScanDesc& d, u64* state, unsigned* chunk
 // and select_expand_body then turn the bits into ascending indices.
 constexpr int kSelMaskRows = 8;
 template <class Ev>
-__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt, uint8_t* tile_fmt) {
+__device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_cnt, i64* tile_row) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
   __shared__ ws_colview scols[WS_MAX_COLS];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
@@ -1773,58 +1843,59 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
   ev.stage_sets((unsigned*)ws_smem, d.set_words);
   __syncthreads();
   const i64 t0 = (i64)blockIdx.x * d.ntiles / gridDim.x, t1 = (i64)(blockIdx.x + 1) * d.ntiles / gridDim.x;
-  int cur = -1;
+  BatchCursor bc;
   // Compiled WHEREs in two halves (Ev::kSplit): later full tiles' column words load while this tile
   // is computed (xa, xb hold this tile's and the next one's rows when `have`).
   typename Ev::Row xa[kSelMaskRows];
   bool have = false;
   for (i64 t = t0; t < t1; ++t) {
-    const int b = cur < 0 ? batch_of_tile_bsearch(d, t) : batch_of_tile(d, t, cur);
-    if (b != cur) {
+    if (bc.seek(d, t)) {
       have = false;
-      stage_batch(d, b, scols);
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)b * d.ncols : scols);
-      cur = b;
+      stage_batch(d, bc.b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
-    const TileRows tr = tile_rows(d, t, b);
-    const uint32_t* del = uni(d.del[b]);
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
     u64* tm = masks + t * words;
     if constexpr (Ev::kSplit) {
-      if (!del && tr.count == d.tile_rows && ev.rows4_ok()) {  // (block-uniform)
-        // Four consecutive rows per lane and load: lane L of wave w takes rows w * 512 + g * 256 +
-        // 4 L + s of the tile (g = 0, 1; s = 0..3), so a wave's load instruction carries 1 KiB.
-        // Its ballots are stored as they are (bit L of word (w, g, s) is row w * 512 + g * 256 +
-        // 4 L + s) and tile_fmt[t] = 1 tells the expansion to transpose them. The next full tile
-        // of this block and batch loads before this tile's dependent lookups (without a branch:
-        // this tile's rows again when there is none).
-        const i64 tr_ = d.tile_rows;
+      if (!del && tr.count == bc.tr && ev.rows4_ok()) {  // (block-uniform)
+        // Four consecutive rows per lane and load: lane L of wave w takes rows w * 64 R + g * 256
+        // + 4 L + s of the tile (R = kSelMaskRows rows per lane, g < R / 4, s = 0..3), so a wave's
+        // load instruction carries 1 KiB. Its ballots are stored as they are (bit L of word
+        // (w, g, s) = w R + 4 g + s is row w * 64 R + g * 256 + 4 L + s) and bit 0 of tile_row[t] tells
+        // the expansion to transpose them. The next full tile of this block and batch loads before
+        // this tile's dependent lookups (without a branch: this tile's rows again when there is
+        // none).
+        constexpr int NG = kSelMaskRows / 4;
+        const i64 tr_ = bc.tr;
         const i64 rb = tr.first + (i64)wave * (64 * kSelMaskRows) + 4 * lane;
         if (!have) {
-          ev.load_rows4(rb, xa);
-          ev.load_rows4(rb + 256, xa + 4);
+#pragma unroll
+          for (int g = 0; g < NG; ++g) ev.load_rows4(rb + 256 * g, xa + 4 * g);
         }
-        const bool nxt = t + 1 < t1 && tr.first + 2 * tr_ <= uni((i64)d.n[b]);
+        const bool nxt = t + 1 < t1 && tr.first + 2 * tr_ <= bc.n;
         const i64 rn = nxt ? rb + tr_ : rb;
         typename Ev::Row xb[kSelMaskRows];
-        ev.load_rows4(rn, xb);
-        ev.load_rows4(rn + 256, xb + 4);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) ev.load_rows4(rn + 256 * g, xb + 4 * g);
         bool p[kSelMaskRows];
 #pragma unroll
         for (int j = 0; j < kSelMaskRows; ++j) p[j] = ev.where_row(xa[j], rb + (j >> 2) * 256 + (j & 3));
 #pragma unroll
         for (int j = 0; j < kSelMaskRows; ++j) xa[j] = xb[j];
         have = nxt;
+        // lane j keeps ballot j: one store instruction per wave writes them all
         uint32_t tc = 0;
+        u64 mine = 0;
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const u64 B = __ballot(p[g * 4 + q]);
-            tc += (uint32_t)__popcll(B);
-            if (lane == g * 4 + q) tm[wave * 8 + g * 4 + q] = B;
-          }
+        for (int j = 0; j < kSelMaskRows; ++j) {
+          const u64 B = __ballot(p[j]);
+          tc += (uint32_t)__popcll(B);
+          if (lane == j) mine = B;
+        }
+        if (lane < kSelMaskRows) tm[wave * kSelMaskRows + lane] = mine;
         if (lane == 0 && tc) atomicAdd(&tile_cnt[t], tc);
-        if (threadIdx.x == 0) tile_fmt[t] = 1;
+        if (threadIdx.x == 0) tile_row[t] = ((bc.row0 + tr.first) << 1) | 1;
         continue;
       }
       have = false;
@@ -1832,7 +1903,7 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
     bool pass[kSelMaskRows];
     if (Ev::kPureWhere || !del) {  // (block-uniform) every row evaluated, loads issued together
       bool w[kSelMaskRows];
-      if (tr.count == d.tile_rows) {  // (block-uniform) full tile: row = first + j * blockDim + thread
+      if (tr.count == bc.tr) {  // (block-uniform) full tile: row = first + j * blockDim + thread
         const i64 r0 = tr.first + threadIdx.x;
 #pragma unroll
         for (int j = 0; j < kSelMaskRows; ++j) w[j] = !ev.has_where() || ev.where(r0 + (i64)j * blockDim.x);
@@ -1859,14 +1930,210 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
       }
     }
     uint32_t tc = 0;
+    u64 mine = 0;
 #pragma unroll
     for (int j = 0; j < kSelMaskRows; ++j) {
       const u64 m = __ballot(pass[j]);
-      if (lane == 0) tm[j * nw + wave] = m;
+      if (lane == j) mine = m;
       tc += (uint32_t)__popcll(m);
     }
+    if (lane < kSelMaskRows) tm[lane * nw + wave] = mine;
+    if (threadIdx.x == 0) tile_row[t] = (bc.row0 + tr.first) << 1;
     if (lane == 0 && tc) atomicAdd(&tile_cnt[t], tc);  // (16 per tile address: no queue builds up)
   }
+}
+
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int select_bit(u64 m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int c = __popcll(m & ((1ull << w) - 1));
+    if (k >= c) {
+      k -= c;
+      m >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
+// Dense GROUP BY whose WHERE reads large set bitmaps (a join filter fused into the scan, an
+// IN-list over a wide key range): the ballot pass's structure, accumulating instead of storing
+// ballots. One block of 1024 threads per CU holds d.lds_slots LDS copies of the accumulators (the
+// dense_init_copies layout, copy = wave % copies) and, in the LDS left, the leading words of the
+// plan's set bitmaps (d.set_words). Each block scans a contiguous range of tiles of kSelMaskRows
+// rows per thread. On full tiles the WHERE's columns load four rows per lane, the next tile's
+// while this one is evaluated. A wave's passing rows are then dealt out one per lane (the k-th
+// passing row, in ballot order, to lane k); their keys and values load together, no load being
+// conditional (a lane without a row reads the tile's first row), and reach the LDS copies one
+// tile later, which folds the wait for them into the next tile's wait for its set lookups. Beyond
+// 64 passing rows per wave and tile, the earlier rounds of 64 are added at once. Other tiles go
+// row by row. Block partials are written as dense_body's.
+template <class Ev>
+__device__ void dense_staged_body(const ScanDesc& d, i64* parts, int* flags) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char ws_smem[];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  Ev ev(d);
+  const ws_agg_plan& plan = *d.plan;
+  const ws_dense_spec& ds = *d.ds;
+  const int NK = ev.nkeys(), NA = ev.naggs();
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int C = d.lds_slots;
+  dense_init_copies(d, ds, NA, ws_smem);
+  ev.stage_sets((unsigned*)(ws_smem + (i64)C * d.copy_stride), d.set_words);
+  // the dense spec and slot offsets in registers (reads of the descriptor in the loop would be
+  // vector loads that wait behind the column loads)
+  i64 kmin[Ev::kKeys], krange[Ev::kKeys], kstride[Ev::kKeys];
+#pragma unroll
+  for (int k = 0; k < Ev::kKeys; ++k) {
+    kmin[k] = k < NK ? uni((i64)ds.kmin[k]) : 0;
+    krange[k] = k < NK ? uni((i64)ds.range[k]) : 1;
+    kstride[k] = k < NK ? uni((i64)ds.stride[k]) : 0;
+  }
+  int soff[Ev::kAggs];
+#pragma unroll
+  for (int a = 0; a < Ev::kAggs; ++a) soff[a] = a < NA ? uni((int)d.slot_off[a]) : 0;
+  __syncthreads();
+  unsigned char* mine = ws_smem + (i64)(wave % C) * d.copy_stride;
+  bool bad = false;
+  // A row's key and aggregate column words (loaded) and the row; added to the accumulators later,
+  // so that nothing computed from the words makes the wave wait for them at the load.
+  struct Pend {
+    typename Ev::VRow x;
+    i64 r;
+    bool valid;
+  };
+  auto load = [&](i64 r, bool valid, Pend& q) {
+    ev.load_vrow(r, q.x);
+    q.r = r;
+    q.valid = valid;
+  };
+  auto add = [&](const Pend& q) {
+    if (!q.valid) return;
+    i64 g = 0;
+#pragma unroll
+    for (int k = 0; k < Ev::kKeys; ++k) {
+      if (k >= NK) break;
+      const i64 o = ev.ikey_v(k, q.x, q.r) - kmin[k];
+      if (o < 0 || o >= krange[k]) {
+        bad = true;
+        return;
+      }
+      g += o * kstride[k];
+    }
+#pragma unroll
+    for (int a = 0; a < Ev::kAggs; ++a) {
+      if (a >= NA) break;
+      const uint8_t k = ev.dkind(a, ds);
+      unsigned char* arr = mine + soff[a];
+      if (k == WS_DK_COUNT) {
+        __hip_atomic_fetch_add(&((unsigned*)arr)[g], 1u, __ATOMIC_RELAXED, WS_LDS);
+      } else if (dk_int(k)) {
+        atomic_combine_i<WS_LDS>(dk_op(k), &((i64*)arr)[g], int_part(ev, a, ev.iagg_v(a, q.x, q.r)));
+      } else {
+        atomic_combine_d<WS_LDS>(dk_op(k), &((double*)arr)[g], ev.agg_v(a, q.x, q.r));
+      }
+    }
+  };
+  Pend pend;  // the previous full tile's last round of passing rows: loaded, not yet added
+  bool pending = false;
+  const i64 t0 = (i64)blockIdx.x * d.ntiles / gridDim.x, t1 = (i64)(blockIdx.x + 1) * d.ntiles / gridDim.x;
+  BatchCursor bc;
+  bool have = false;
+  // One tile: xc holds its WHERE columns when `have`; the next tile's load into xn. The loop below
+  // alternates the two arrays (a copy from one to the other would wait for the loads).
+  typedef typename Ev::Row Rows[kSelMaskRows];
+  auto tile = [&](i64 t, Rows& xc, Rows& xn) {
+    if (bc.seek(d, t)) {
+      have = false;
+      stage_batch(d, bc.b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+    }
+    const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
+    if constexpr (Ev::kSplit) {
+      if (!del && tr.count == bc.tr && ev.rows4_ok()) {  // (block-uniform)
+        // rows as in select_mask_body: lane L of wave w takes rows w * 64 R + g * 256 + 4 L + s
+        constexpr int NG = kSelMaskRows / 4;
+        const i64 rw = tr.first + (i64)wave * (64 * kSelMaskRows);
+        const i64 rb = rw + 4 * lane;
+        if (!have) {
+#pragma unroll
+          for (int g = 0; g < NG; ++g) ev.load_rows4(rb + 256 * g, xc + 4 * g);
+        }
+        const bool nxt = t + 1 < t1 && tr.first + 2 * bc.tr <= bc.n;
+        const i64 rn = nxt ? rb + bc.tr : rb;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) ev.load_rows4(rn + 256 * g, xn + 4 * g);
+        u64 B[kSelMaskRows];
+        int pre[kSelMaskRows + 1];
+        pre[0] = 0;
+#pragma unroll
+        for (int j = 0; j < kSelMaskRows; ++j) {
+          B[j] = __ballot(ev.where_row(xc[j], rb + (j >> 2) * 256 + (j & 3)));
+          pre[j + 1] = pre[j] + __popcll(B[j]);
+        }
+        have = nxt;
+        if (pending) add(pend);  // (its loads were issued before this tile's lookups)
+        const int total = pre[kSelMaskRows];
+        // passing row k: ballot j with pre[j] <= k < pre[j + 1], then the (k - pre[j])-th set bit
+        auto row_of = [&](int k) -> i64 {
+          int j = 0;
+#pragma unroll
+          for (int q = 1; q < kSelMaskRows; ++q) j += k >= pre[q] ? 1 : 0;
+          // B[j] and pre[j] by masks, not selects (a select chain becomes an indexed load from
+          // scratch memory, whose wait would take the next tile's loads with it)
+          u64 bj = 0;
+          int pj = 0;
+#pragma unroll
+          for (int q = 0; q < kSelMaskRows; ++q) {
+            const u64 m = 0ull - (u64)(j == q);
+            bj |= B[q] & m;
+            pj |= pre[q] & (int)m;
+          }
+          return rw + (j >> 2) * 256 + 4 * select_bit(bj, k - pj) + (j & 3);
+        };
+        // rounds of 64 before the last (more than 64 passing rows in the wave): added at once
+        const int last = total > 0 ? (total - 1) & ~(kWave - 1) : 0;
+        for (int base = 0; base < last; base += kWave) {
+          Pend q;
+          load(row_of(base + lane), true, q);
+          add(q);
+        }
+        // the last round, straight-line (a loop or a branch around these loads would make the
+        // compiler wait for every load in flight, the next tile's included): added next tile
+        pending = total > 0;
+        if (pending) {  // (wave-uniform)
+          const int k = last + lane;
+          const bool valid = k < total;
+          const i64 r = row_of(valid ? k : total - 1);  // (every lane computes a row: no branch)
+          load(valid ? r : tr.first, valid, pend);
+        }
+        return;
+      }
+      have = false;
+    }
+    if (pending) add(pend);
+    pending = false;
+    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+      const i64 r = tr.first + i;
+      if (is_deleted(del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      Pend q;
+      load(r, true, q);
+      add(q);
+    }
+  };
+  Rows xa, xb;
+  for (i64 t = t0; t < t1; t += 2) {
+    tile(t, xa, xb);
+    if (t + 1 < t1) tile(t + 1, xb, xa);
+  }
+  if (pending) add(pend);
+  if (bad) flags[1] = 1;
+  __syncthreads();
+  dense_write_copies(d, ds, NA, ws_smem, parts + (i64)blockIdx.x * ds.G * NA);
 }
 
 // Per-batch counts from the scanned tile offsets (tiles never straddle batches); no atomics, which
@@ -1910,10 +2177,6 @@ __device__ void scan_counts_body(uint32_t* cnt, i64 n, u64* total) {
   if (threadIdx.x == 0) *total = sum;
 }
 
-// Expands select_mask_body's ballots of tile t = blockIdx.x (one thread per 64-row word,
-// blockDim.x == tile_rows / 64): the tile's output offset comes from the scanned counts (`stride`
-// per tile), the thread's from a block scan of the words' bit counts; rows are written as global
-// indices (row0[batch] + row), ascending.
 // Bit i of x (x < 2^16) to bit 4 i.
 __device__ __forceinline__ u64 spread4(u64 x) {
   x = (x | (x << 24)) & 0x000000FF000000FFull;
@@ -1922,22 +2185,34 @@ __device__ __forceinline__ u64 spread4(u64 x) {
   x = (x | (x << 3)) & 0x1111111111111111ull;
   return x;
 }
-// tile_fmt[t] = 1: the tile's words are raw ballots of four rows per lane (word 4 k + s, bit L =
-// row 256 k + 4 L + s); row-ordered word 4 k + m is then bit 4 i + s = bit 16 m + i of word 4 k + s.
-__device__ void select_expand_body(const ScanDesc& d, const u64* masks, const uint8_t* tile_fmt, const uint32_t* tile_off,
-                                   int stride, int32_t* out_idx) {
-  __shared__ uint32_t s_w[16];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+// Expands select_mask_body's ballots of tile t = blockIdx.x (one thread per 64-row word,
+// blockDim.x == tile_rows / 64 <= kSelExpandThreads). tile_row[t] is the tile's first global row
+// times two, plus 1 when its words are raw ballots of four rows per lane (word 4 k + s, bit L =
+// row 256 k + 4 L + s; row-ordered word 4 k + m is then bit 4 i + s = bit 16 m + i of word 4 k + s).
+// The tile's output offset comes from the scanned counts (`stride` per tile), each thread's from a
+// block scan of the words' bit counts. A sparse tile's threads write their rows directly; a dense
+// tile's selected rows go to LDS in order first, and the block then writes its run of global
+// indices with coalesced stores (each thread's own run would be one store per row, strided across
+// the wave). Every load the block needs is issued before the first result is used.
+constexpr int kSelExpandThreads = 128;
+__device__ void select_expand_body(const u64* masks, const i64* tile_row, const uint32_t* tile_off, int stride,
+                                   int32_t* out_idx) {
+  __shared__ uint32_t s_w[kSelExpandThreads / kWave];
+  __shared__ uint16_t s_pos[kSelExpandThreads * 64];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
   const i64 t = blockIdx.x;
+  const u64* b4 = masks + t * (i64)blockDim.x + (threadIdx.x & ~3);
+  const u64 w0 = b4[0], w1 = b4[1], w2 = b4[2], w3 = b4[3];
+  const i64 info = tile_row[t];
+  const u64 base = tile_off[t * stride];
+  const int q = threadIdx.x & 3;
   u64 m;
-  if (tile_fmt[t]) {  // (block-uniform)
-    const u64* b4 = masks + t * (i64)blockDim.x + (threadIdx.x & ~3);
-    const int sh = 16 * (threadIdx.x & 3);
-    m = 0;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) m |= spread4((b4[s] >> sh) & 0xFFFFull) << s;
+  if (info & 1) {  // (block-uniform)
+    const int sh = 16 * q;
+    m = spread4((w0 >> sh) & 0xFFFFull) | spread4((w1 >> sh) & 0xFFFFull) << 1 | spread4((w2 >> sh) & 0xFFFFull) << 2 |
+        spread4((w3 >> sh) & 0xFFFFull) << 3;
   } else {
-    m = masks[t * (i64)blockDim.x + threadIdx.x];
+    m = q == 0 ? w0 : q == 1 ? w1 : q == 2 ? w2 : w3;
   }
   const uint32_t c = (uint32_t)__popcll(m);
   uint32_t incl = c;
@@ -1947,17 +2222,28 @@ __device__ void select_expand_body(const ScanDesc& d, const u64* masks, const ui
   }
   if (lane == kWave - 1) s_w[wave] = incl;
   __syncthreads();
-  uint32_t wbase = 0;
-  for (int w = 0; w < wave; ++w) wbase += s_w[w];
-  const int b = batch_of_tile_bsearch(d, t);
-  const TileRows tr = tile_rows(d, t, b);
-  const i64 g = d.row0[b] + tr.first + (i64)threadIdx.x * 64;
-  int32_t* o = out_idx + (i64)tile_off[t * stride] + wbase + incl - c;
+  uint32_t wbase = 0, count = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wave) wbase += s_w[w];
+    count += s_w[w];
+  }
+  uint32_t k = wbase + incl - c;
+  const uint32_t r0 = threadIdx.x * 64;
+  const i64 g = info >> 1;
+  int32_t* o = out_idx + base;
+  if (count <= 2 * blockDim.x) {  // (block-uniform)
+    while (m) {
+      o[k++] = (int32_t)(g + r0 + (uint32_t)(__ffsll((unsigned long long)m) - 1));
+      m &= m - 1;
+    }
+    return;
+  }
   while (m) {
-    const int bit = __ffsll((unsigned long long)m) - 1;
-    *o++ = (int32_t)(g + bit);
+    s_pos[k++] = (uint16_t)(r0 + (uint32_t)(__ffsll((unsigned long long)m) - 1));
     m &= m - 1;
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) o[i] = (int32_t)(g + s_pos[i]);
 }
 
 }  // namespace ws
@@ -1981,6 +2267,13 @@ __device__ void select_expand_body(const ScanDesc& d, const u64* masks, const ui
     ws::dense_body<EV>(*d, gacc, parts, flags);                                                              \
   }                                                                                                          \
   WS_KERNEL_BATCH_DENSE(EV, NAME##_batch)
+// Compiled plans only (the evaluator stages its sets): dense GROUP BY with the sets in LDS.
+#define WS_KERNEL_DENSE_JIT(EV, NAME)                                                                       \
+  WS_KERNEL_DENSE(EV, NAME)                                                                                  \
+  extern "C" __global__ void __launch_bounds__(1024) NAME##_staged(const ws::ScanDesc* d, long long* parts,   \
+                                                                   int* flags) {                             \
+    ws::dense_staged_body<EV>(*d, parts, flags);                                                             \
+  }
 #define WS_KERNEL_GROUPED(EV, NAME)                                                                         \
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, ws::GTable gt) {            \
     ws::grouped_body<EV>(*d, gt);                                                                            \
@@ -2008,6 +2301,6 @@ __device__ void select_expand_body(const ScanDesc& d, const u64* masks, const ui
 // Compiled plans only (the evaluator stages its sets): the ballot pass of the staged selection.
 #define WS_KERNEL_SELECT_STAGED(EV, NAME)                                                                   \
   extern "C" __global__ void __launch_bounds__(1024) NAME##_staged(const ws::ScanDesc* d, unsigned long long* masks, \
-                                                                   unsigned* tile_cnt, unsigned char* tile_fmt) { \
-    ws::select_mask_body<EV>(*d, masks, tile_cnt, tile_fmt);                                                 \
+                                                                   unsigned* tile_cnt, long long* tile_row) { \
+    ws::select_mask_body<EV>(*d, masks, tile_cnt, tile_row);                                                 \
   }

@@ -201,6 +201,14 @@ __device__ __forceinline__ bool bit_value(const BitSrc& s, int64_t i, int64_t& v
   v = (int64_t)d;
   return d == d;
 }
+// Set bits of a bitmap: per-thread popcounts, a wave sum, one atomic per wave.
+__global__ void __launch_bounds__(kBlock) bitmap_count_k(const uint32_t* bits, int64_t words, unsigned long long* total) {
+  unsigned long long c = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    c += (unsigned)__popc(bits[i]);
+  for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_down(c, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(total, c);
+}
 template <bool kLds>
 __global__ void __launch_bounds__(kBlock) value_bits_k(BitSrc s, int64_t n, int64_t base, int64_t nbits, uint32_t* bits,
                                                        int* saw_null) {
@@ -587,6 +595,23 @@ int ws_key_bitmap(ws_colview key, int64_t n, int64_t lo, int64_t nbits, uint32_t
   return guard([&] {
     fill_u32(bits, (nbits + 31) / 32, 0);
     value_bits("key_bitmap", bit_src_of(key), n, lo, nbits, bits, nullptr);
+  });
+}
+int ws_bitmap_count(const uint32_t* bits, int64_t words, int64_t* count) {
+  return guard([&] {
+    if (!count) throw Error("ws_bitmap_count: null count", WS_ERR_INTERNAL);
+    *count = 0;
+    if (words <= 0) return;
+    DevBuf<unsigned long long> total(1);
+    fill_u64(total.p, 1, 0);
+    unsigned long long* tp = total.p;
+    ProfScope ps("bitmap_count");
+    hipLaunchKernelGGL(bitmap_count_k, dim3((unsigned)std::min<int64_t>(grid_for(words), 1024)), dim3(kBlock), 0, stream(), bits,
+                       words, tp);
+    check_launch("bitmap_count");
+    unsigned long long h = 0;
+    d2h_sync(&h, tp, sizeof h);
+    *count = (int64_t)h;
   });
 }
 int ws_set_bits(uint32_t* bits, const int32_t* idx, int64_t m) {

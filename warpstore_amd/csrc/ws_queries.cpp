@@ -577,6 +577,77 @@ struct C3Stats {
 // The join output model the drivers consult (null: the reference's count, scan, read-back sizing).
 std::atomic<ws_sizer*> g_join_sizer{nullptr};
 
+// Set to false to keep the hash join when the build keys are distinct (tests of both plans, and
+// the learned join sizing, which sizes that join).
+std::atomic<bool> g_join_rewrite{true};
+
+// The aggregation of C3 over rows given by `bind` (columns flag, price, qty in slots 0-2), with an
+// optional WHERE over slot 3 (fk - lo IN set 0): rows (flag, count, raw price sum, qty sum).
+struct C3Agg {
+  Comp c;
+  AggPlan plan;
+  int f_flag, f_price, f_qty, f_fk = -1;
+  C3Agg(const uint32_t* bits, int64_t lo, int64_t nbits) {
+    f_flag = c.slot(WS_T_I32);
+    f_price = c.slot(WS_T_DECIMAL);
+    f_qty = c.slot(WS_T_I64);
+    c.env.cols[f_price].scale = 2;
+    std::memset(&plan, 0, sizeof plan);
+    if (bits) {
+      f_fk = c.slot(WS_T_I32);
+      c.env.sets[0] = bits;
+      c.env.set_size[0] = (int32_t)nbits;
+      plan.has_where = 1;
+      plan.where = c.prog({c.col(f_fk), c.ilit(lo), Comp::op(WS_OP_SUB), Comp::I{WS_OP_INSET, 0, 0}});
+    }
+    plan.nkeys = 1;
+    plan.keys[0] = c.prog({c.col(f_flag)});
+    plan.key_int[0] = (uint8_t)progIsInt(c.env, plan.keys[0]);
+    plan.naggs = 4;
+    plan.ops[0] = WS_AGG_COUNT;
+    plan.ops[1] = plan.ops[2] = plan.ops[3] = WS_AGG_SUM;
+    plan.aggs[1] = c.prog({Comp::I{WS_OP_COLRAW, 1, (uint16_t)f_price}});
+    plan.aggs[2] = plan.aggs[3] = c.prog({c.col(f_qty)});
+    plan.agg_part[2] = 1, plan.agg_part[3] = 2;
+    plan.agg_int[0] = 1;
+    for (int a = 1; a < 4; ++a) plan.agg_int[a] = (uint8_t)progIsInt(c.env, plan.aggs[a]);
+    plan.agg_bits[2] = plan.agg_bits[3] = 1;  // BIGINT column SUM with GROUP BY: split halves
+  }
+  // Runs over batches added by `add` (called with the aggregator); returns the group count.
+  template <class AddBatches>
+  int64_t run(const DevColumn& flag_col, int64_t rows, const char* kind, AddBatches add, int64_t* out) {
+    DenseSpec ds;
+    const bool dense = plan_dense(plan, {{flag_col.mn, flag_col.mx}}, rows, ds);
+    plan.dense_priv = dense ? (ds.priv ? 1 : 2) : 0;
+    const JitProgram* jit = jitGet(c.env, c.ncols, plan, dense ? JF_DENSE : JF_GROUPED, kind);
+    int64_t cap = pow2_at_least(2 * 25, 1024);
+    std::vector<double> keys, accs;
+    int64_t g = 0;
+    for (;;) {
+      GpuAggregator agg(plan, cap, dense ? &ds : nullptr);
+      add(agg);
+      agg.run(c.env, jit);
+      if (agg.overflowed()) {
+        cap = pow2_at_least(std::max<int64_t>(cap * 4, agg.groupCount() * 4), 1);
+        continue;
+      }
+      g = agg.fetch(keys, accs);
+      break;
+    }
+    for (int64_t i = 0; i < g; ++i) {
+      int64_t kb, lo2, hi2;
+      std::memcpy(&kb, &keys[i], 8);
+      std::memcpy(&lo2, &accs[i * 4 + 2], 8);
+      std::memcpy(&hi2, &accs[i * 4 + 3], 8);
+      out[i * 4 + 0] = kb;
+      out[i * 4 + 1] = (int64_t)accs[i * 4 + 0];
+      out[i * 4 + 2] = (int64_t)accs[i * 4 + 1];  // exact: per-group raw sums < 2^53
+      out[i * 4 + 3] = (int64_t)((__int128)hi2 * ((__int128)1 << 32) + lo2);
+    }
+    return g;
+  }
+};
+
 int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stats* st, int64_t attr_lt = 50) {
   // dim: materialize(dim, [attr < attr_lt], {key})
   Comp cd;
@@ -592,6 +663,24 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
   kv.p = drel[0].data.p;
   kv.t = WS_T_I32;
   gpuKeyBitmap(kv, nd, lo, nbits, bits.p);
+  // Distinct build keys (as many set bits as filtered dimension rows): each fact row that passes
+  // the bitmap joins exactly one dimension row, and no dimension column is read after the join,
+  // so the join is the bitmap filter itself. The fact table is then aggregated in one pass with
+  // the bitmap test as its WHERE (the set staged in LDS): no selection vector, gathers or join.
+  if (g_join_rewrite.load() && nd > 0 && gpuBitmapCount(bits.p, (int64_t)bits.n) == nd) {
+    C3Agg A(bits.p, lo, nbits);
+    const std::vector<int> slot_col = {3, 2, 1, 0};  // flag, price, qty, fk
+    const int64_t g = A.run(fact.cols[3], fact.rows, "aggregate fact (join as filter)", [&](GpuAggregator& agg) {
+      for (size_t sg = 0; sg < fact.seg_rows.size(); ++sg) {
+        ExprEnv e = bind_segment(A.c, fact, slot_col, sg);
+        agg.addBatch(e, A.c.ncols, fact.seg_rows[sg], nullptr);
+      }
+    }, out);
+    int64_t pairs = 0;
+    for (int64_t i = 0; i < g; ++i) pairs += out[i * 4 + 1];
+    if (st) st->dim_rows = nd, st->fact_rows = pairs, st->pairs = pairs, st->predicted_pairs = 0;
+    return g;
+  }
   // fact: materialize(fact, [INSET(fk - lo, bitmap)], {fk, price, qty, flag})
   Comp cf;
   const int f_fk = cf.slot(WS_T_I32);
@@ -631,55 +720,14 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
   gather_side(drel[0], ib.p);
   for (auto& rc : frel) gather_side(rc, ip.p);
   // aggregate the joined relation: keys flag; slots COUNT(*), SUM(#RAW price), SUM(qty) lo / hi
-  Comp ca;
-  const int j_flag = ca.slot(WS_T_I32), j_price = ca.slot(WS_T_DECIMAL), j_qty = ca.slot(WS_T_I64);
-  ca.env.cols[j_price].scale = 2;
-  AggPlan plan;
-  std::memset(&plan, 0, sizeof plan);
-  plan.nkeys = 1;
-  plan.keys[0] = ca.prog({ca.col(j_flag)});
-  plan.key_int[0] = (uint8_t)progIsInt(ca.env, plan.keys[0]);
-  plan.naggs = 4;
-  plan.ops[0] = WS_AGG_COUNT;
-  plan.ops[1] = plan.ops[2] = plan.ops[3] = WS_AGG_SUM;
-  plan.aggs[1] = ca.prog({Comp::I{WS_OP_COLRAW, 1, (uint16_t)j_price}});
-  plan.aggs[2] = plan.aggs[3] = ca.prog({ca.col(j_qty)});
-  plan.agg_part[2] = 1, plan.agg_part[3] = 2;
-  plan.agg_int[0] = 1;
-  for (int a = 1; a < 4; ++a) plan.agg_int[a] = (uint8_t)progIsInt(ca.env, plan.aggs[a]);
-  plan.agg_bits[2] = plan.agg_bits[3] = 1;  // BIGINT column SUM with GROUP BY: split halves
-  DenseSpec ds;
-  const bool dense = plan_dense(plan, {{fact.cols[3].mn, fact.cols[3].mx}}, m, ds);
-  plan.dense_priv = dense ? (ds.priv ? 1 : 2) : 0;
-  const JitProgram* jit = jitGet(ca.env, ca.ncols, plan, dense ? JF_DENSE : JF_GROUPED, "aggregate join result");
-  ExprEnv e = ca.env;
-  e.cols[j_flag].p = jcols[4].p;  // flag
-  e.cols[j_price].p = jcols[2].p;  // price
-  e.cols[j_qty].p = jcols[3].p;   // qty
-  int64_t cap = pow2_at_least(2 * 25, 1024);
-  std::vector<double> keys, accs;
-  int64_t g = 0;
-  for (;;) {
-    GpuAggregator agg(plan, cap, dense ? &ds : nullptr);
-    agg.addBatch(e, ca.ncols, m, nullptr);
-    agg.run(ca.env, jit);
-    if (agg.overflowed()) {
-      cap = pow2_at_least(std::max<int64_t>(cap * 4, agg.groupCount() * 4), 1);
-      continue;
-    }
-    g = agg.fetch(keys, accs);
-    break;
-  }
-  for (int64_t i = 0; i < g; ++i) {
-    int64_t kb, lo2, hi2;
-    std::memcpy(&kb, &keys[i], 8);
-    std::memcpy(&lo2, &accs[i * 4 + 2], 8);
-    std::memcpy(&hi2, &accs[i * 4 + 3], 8);
-    out[i * 4 + 0] = kb;
-    out[i * 4 + 1] = (int64_t)accs[i * 4 + 0];
-    out[i * 4 + 2] = (int64_t)accs[i * 4 + 1];  // exact: per-group raw sums < 2^53
-    out[i * 4 + 3] = (int64_t)((__int128)hi2 * ((__int128)1 << 32) + lo2);
-  }
+  C3Agg A(nullptr, 0, 0);
+  const int64_t g = A.run(fact.cols[3], m, "aggregate join result", [&](GpuAggregator& agg) {
+    ExprEnv e = A.c.env;
+    e.cols[A.f_flag].p = jcols[4].p;   // flag
+    e.cols[A.f_price].p = jcols[2].p;  // price
+    e.cols[A.f_qty].p = jcols[3].p;    // qty
+    agg.addBatch(e, A.c.ncols, m, nullptr);
+  }, out);
   if (st) st->dim_rows = nd, st->fact_rows = nf, st->pairs = m, st->predicted_pairs = pred;
   return g;
 }
@@ -1273,6 +1321,10 @@ int wsq_c3(void* fact, void* dim, int64_t attr_lt, int64_t* out, int64_t* groups
 }
 // The join output model (null: count first, as the reference). Not owned.
 void wsq_set_join_sizer(ws_sizer* m) { g_join_sizer.store(m); }
+
+// 1 (default): a star join whose filtered build keys are distinct and whose build columns are not
+// read after the join runs as the bitmap filter fused into the aggregation; 0 keeps the hash join.
+void wsq_set_join_rewrite(int on) { g_join_rewrite.store(on != 0); }
 
 // The sizing model the drivers consult (null to go back to the range rule alone). Not owned.
 void wsq_set_sizer(ws_sizer* m) { g_sizer.store(m); }

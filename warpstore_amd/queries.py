@@ -36,6 +36,8 @@ def qlib():
         Q.wsq_c3.restype = I
         Q.wsq_set_join_sizer.argtypes = [P]
         Q.wsq_set_join_sizer.restype = None
+        Q.wsq_set_join_rewrite.argtypes = [I]
+        Q.wsq_set_join_rewrite.restype = None
         Q.wsq_c4_load.argtypes = [C.c_uint64, I64, I64, C.POINTER(C.c_void_p), C.POINTER(I64)]
         Q.wsq_c4_load.restype = I
         Q.wsq_c4_free.argtypes = [P]
@@ -118,6 +120,12 @@ def c3(fact: DeviceTable, dim: DeviceTable, attr_lt: int = 50):
     g = C.c_int64()
     _ck(qlib().wsq_c3(fact.h, dim.h, attr_lt, out.ctypes.data, C.byref(g), st.ctypes.data))
     return out[:g.value], st
+
+
+def set_join_rewrite(on: bool) -> None:
+    """True (default): the star join with distinct filtered build keys runs as a bitmap filter
+    fused into the fact aggregation; False keeps the hash join."""
+    qlib().wsq_set_join_rewrite(1 if on else 0)
 
 
 def set_sizers(groups=None, joins=None) -> None:

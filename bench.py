@@ -471,9 +471,10 @@ def bench_join(args) -> int:
         Q.c3(fact, dim)
     L.ws_sync()
     # stage times from one more (untimed) query: a warm-up's first launches also load code objects
-    warm, _ = profiled_query(ws, L, lambda s: Q.c3(fact, dim))
-    # the fact filter's launch: the persistent (sets staged in LDS) or the chunked selection kernel
-    dom = next((k for k in warm if k.endswith("[filter (fact by join keys)]")), "select_jit[filter (fact by join keys)]")
+    warm, (_, wst) = profiled_query(ws, L, lambda s: Q.c3(fact, dim))
+    # the roofline kernel: the one with the most GPU time in that query (the fused fact scan when
+    # the dimension keys are distinct, else the fact filter of the hash-join plan)
+    dom = max(warm, key=lambda k: warm[k][1])
     elapsed, lat, launches, total_ms, (rows, st) = timed_steps(ws, L, lambda s: Q.c3(fact, dim), args.steps, dom)
     O = oracle_lib()
     exp = oracle_c3(O, (fk, qty, price, flag), (dkey, dattr))
@@ -490,10 +491,17 @@ def bench_join(args) -> int:
     out["fact_rows_per_s"] = round(JOIN_FACT * args.steps / elapsed, 1)
     out["parity_vs_oracle"] = parity
     out["stage_ms"] = {k: round(v[1] / max(v[0], 1), 4) for k, v in sorted(warm.items(), key=lambda kv: -kv[1][1])[:8]}
-    out["roofline"] = roofline(dom + " (compiled fact filter: fk - lo IN dimension-key bitmap)", 4 * JOIN_FACT, launches,
-                               total_ms, pmc_traffic("ws_q_select_staged (fact filter)", "pmc_c3_star_join.json"))
+    if dom.startswith("agg_dense_staged"):
+        # fk of every fact row, flag (4 B), price and qty (8 B each) of every passing row
+        desc = " (fused fact scan: fk - lo IN dimension-key bitmap staged in LDS, GROUP BY flag of passing rows)"
+        alg, sym = 4 * JOIN_FACT + 20 * int(st[1]), "ws_q_dense_staged (fused fact scan)"
+    else:
+        desc, alg, sym = " (compiled fact filter: fk - lo IN dimension-key bitmap)", 4 * JOIN_FACT, "ws_q_select_staged (fact filter)"
+    out["roofline"] = roofline(dom + desc, alg, launches, total_ms, pmc_traffic(sym, "pmc_c3_star_join.json"))
+    out["plan"] = "join as bitmap filter fused into the fact aggregation (distinct dimension keys)" if dom.startswith(
+        "agg_dense_staged") else "hash join (filtered fact rows probe the filtered dimension)"
     if not args.no_learned:
-        out["learned_sizing"] = join_learned(ws, L, Q, fact, dim, exp, args, elapsed / args.steps)
+        out["learned_sizing"] = join_learned(ws, L, Q, fact, dim, exp, args)
     if not args.no_cpu:
         B = cpu_lib()
         res = np.zeros((256, 4), np.int64)
@@ -513,30 +521,41 @@ def bench_join(args) -> int:
 JOIN_TRAIN_FILTERS = (3, 8, 15, 25, 35, 65, 90, 130, 200, 300, 450, 700)  # dimension filters attr < x (not 50)
 
 
-def join_learned(ws, L, Q, fact, dim, exp, args, plain_s):
-    """The star join with the learned join-size model: trained on the GPU on the exact output
+def join_learned(ws, L, Q, fact, dim, exp, args):
+    """The hash-join plan of the star join (the join kept although the dimension keys are
+    distinct), plain and with the learned join-size model: trained on the GPU on the exact output
     sizes of the same join under 12 other dimension filters (the measured filter attr < 50 is held
     out), installed in the drivers, then timed like the plain path; results checked again."""
     from warpstore_amd import sizing
-    sizing.log_clear("joins")
-    for a in JOIN_TRAIN_FILTERS:
-        Q.c3(fact, dim, attr_lt=a)
-    model = sizing.train_from_log("joins", epochs=600)
-    Q.set_sizers(joins=model.h)
-    try:
+
+    def timed():
         for _ in range(max(args.warmup, 1)):
             Q.c3(fact, dim)
         L.ws_sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            rows, st = Q.c3(fact, dim)
+            out = Q.c3(fact, dim)
         L.ws_sync()
-        learned_s = (time.perf_counter() - t0) / args.steps
+        return (time.perf_counter() - t0) / args.steps, out
+
+    Q.set_join_rewrite(False)
+    try:
+        plain_s, _ = timed()
+        sizing.log_clear("joins")
+        for a in JOIN_TRAIN_FILTERS:
+            Q.c3(fact, dim, attr_lt=a)
+        model = sizing.train_from_log("joins", epochs=600)
+        Q.set_sizers(joins=model.h)
+        try:
+            learned_s, (rows, st) = timed()
+        finally:
+            Q.set_sizers(joins=None)
     finally:
-        Q.set_sizers(joins=None)
+        Q.set_join_rewrite(True)
     rows = np.asarray(rows)
     same = bool(np.array_equal(rows[np.argsort(rows[:, 0])], exp[np.argsort(exp[:, 0])]))
-    return {"model": "MLP 16-32-32-1 on join key statistics, trained on the GPU", "training_joins": model.samples,
+    return {"plan": "hash join (the default plan runs the join as the fused bitmap filter)",
+            "model": "MLP 16-32-32-1 on join key statistics, trained on the GPU", "training_joins": model.samples,
             "train_loss": round(model.loss, 6), "pairs": int(st[2]), "predicted_pairs": int(st[3]),
             "plain_ms_per_query": round(plain_s * 1e3, 4), "learned_ms_per_query": round(learned_s * 1e3, 4),
             "speedup": round(plain_s / learned_s, 3), "parity_vs_oracle": same}
@@ -598,8 +617,12 @@ def bench_serve(args) -> int:
     sizing.log_clear("joins")
     for d, r in WEB_TRAIN_FILTERS:
         Q.c2(web, 10, date_lt=d, region_lt=r)
-    for a in JOIN_TRAIN_FILTERS:
-        Q.c3(fact, dim, attr_lt=a)
+    Q.set_join_rewrite(False)  # the join sizes come from the hash-join plan
+    try:
+        for a in JOIN_TRAIN_FILTERS:
+            Q.c3(fact, dim, attr_lt=a)
+    finally:
+        Q.set_join_rewrite(True)
     gm, jm = sizing.train_from_log("groups", epochs=600), sizing.train_from_log("joins", epochs=600)
     budget = args.budget_mb << 20
     forms = (("learned_batched", 1, 2, (gm.h, jm.h)), ("plain_per_query", 0, 4, (None, None)))

@@ -51,6 +51,9 @@ void ws_trim_pool(void);                             /* gpuTrimPool    gpu.h:51 
 void ws_tmp_free(void* p);                           /* tmpFree        gpu.h:52 */
 int ws_h2d(void* dst, const void* src, size_t bytes);/* h2d            gpu.h:53 */
 int ws_d2h(void* dst, const void* src, size_t bytes);/* d2h            gpu.h:54 */
+/* No reference entry point: a device-to-host copy queued on the library stream and not waited for
+   (dst: page-locked memory from ws_pinned_alloc); the next synchronizing call completes it. */
+int ws_d2h_async(void* dst, const void* src, size_t bytes);
 int ws_d2d(void* dst, const void* src, size_t bytes);/* d2d            gpu.h:55 */
 int ws_sync(void);                                   /* gpuSync        gpu.h:56 */
 /* VRAM tiering (out of scope: every column is GPU resident here; these keep the link surface). */
@@ -234,6 +237,16 @@ int ws_set_bits(uint32_t* bits, const int32_t* idx, int64_t m);                 
 /* No reference entry point: the number of set bits among `words` words (synchronous). The star-join
    driver compares it with the filtered build rows: equal means the build keys are distinct. */
 int ws_bitmap_count(const uint32_t* bits, int64_t words, int64_t* count);
+/* No reference entry point: materialize(build, WHERE, {key}) followed by gpuKeyBitmap (gpu.h:174)
+   in one scan, for a join filter whose build rows are not needed otherwise. Zeroes the
+   ceil(nbits / 32) words of `bits`, then sets bit key - lo for every live row of the nb batches
+   that passes the WHERE and whose integer key (a program over the batch columns) lies in
+   [lo, lo + nbits); *count (device memory, may be null) gets the number of passing rows.
+   Asynchronous on the library stream. 0 < nbits <= 2^31. jit (may be null): a compiled selection
+   plan (WS_JF_FLAG / WS_JF_FLAGM) whose keys[0] is `key`; without one the interpreter runs. */
+int ws_key_bitmap_where(const ws_expr_env* env, int has_where, ws_prog where, ws_prog key, int nb, const int64_t* bn,
+                        const uint32_t* const* bdel, const ws_colview* bcols, int ncols, int64_t lo, int64_t nbits,
+                        uint32_t* bits, uint64_t* count, const ws_jit_program* jit);
 int ws_gather_null(const void* src, int ptype, const int32_t* idx, int64_t m, void* dst);        /* gpuGatherNull gpu.h:193 */
 int ws_gather_bytes(const void* src, int esize, const int32_t* idx, int64_t m, void* dst, int nulls); /* gpuGatherBytes gpu.h:195 */
 

@@ -92,6 +92,37 @@ def test_c3_star_join_full_size(gpu):
     H.say(f"  dim={st[0]} fact_filtered={st[1]} pairs={st[2]} groups={len(exp)}")
 
 
+def test_c3_filtered_keys_distinct_without_key_column(gpu):
+    """Repeated dimension keys whose extra copies all fail the filter: the key column is not a key,
+    but the filtered keys are distinct (as many bitmap bits as filtered rows), so the join still
+    runs as the fused filter, over the materialized build side."""
+    H.say("case config star join, repeated keys filtered out")
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    fk, qty, price, flag, dkey, dattr = H.gen_c3(3_000_000, 200_000)
+    dkey = dkey.copy()
+    dattr = dattr.copy()
+    dkey[1::7] = dkey[0:-1:7][: len(dkey[1::7])]
+    dattr[1::7] = 999  # the copies never pass attr < 50
+    fact = Q.DeviceTable([fk, qty, price, flag], [abi.T_I32, abi.T_I64, abi.T_DECIMAL, abi.T_I32])
+    dim = Q.DeviceTable([dkey, dattr], [abi.T_I32, abi.T_I32])
+    try:
+        gpu.ws_prof_reset()
+        gpu.ws_prof_only(b"")
+        gpu.ws_prof_enable(1)
+        try:
+            rows, st = Q.c3(fact, dim)
+        finally:
+            prof = ws.prof_rows()
+            gpu.ws_prof_enable(0)
+    finally:
+        fact.free()
+        dim.free()
+    exp = H.oracle_c3(fk, qty, price, flag, dkey, dattr)
+    assert np.array_equal(rows[np.argsort(rows[:, 0])], exp)
+    assert "bitmap_count" in prof and not any(k.startswith("join_") for k in prof), sorted(prof)
+    assert st[2] == st[1] == int(exp[:, 1].sum())
+
+
 def test_c3_duplicate_dimension_keys_keep_the_join(gpu):
     """A dimension whose filtered keys repeat: each fact row joins every copy, so the driver must
     keep the hash join (the bitmap filter alone would count each row once)."""
@@ -183,8 +214,12 @@ def test_c5_serving_full_tables(gpu):
         sizing.log_clear("joins")
         for d, r in ((19723 + 90, 64), (19723 + 200, 128), (19723 + 365, 256), (19723 + 30, 200)):
             Q.c2(web, 10, date_lt=d, region_lt=r)
-        for a in (10, 30, 100, 300):
-            Q.c3(fact, dim, attr_lt=a)
+        Q.set_join_rewrite(False)  # the join sizes come from the hash-join plan
+        try:
+            for a in (10, 30, 100, 300):
+                Q.c3(fact, dim, attr_lt=a)
+        finally:
+            Q.set_join_rewrite(True)
         gm, jm = sizing.train_from_log("groups", epochs=300), sizing.train_from_log("joins", epochs=300)
         n, rate = 120, 300.0
         Q.set_sizers(gm.h, jm.h)

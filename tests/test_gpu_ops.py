@@ -571,3 +571,53 @@ def test_select_ballot_pass_wide_and_encoded_columns(gpu, width):
     assert m > 10_000
     assert len(got) == m and np.array_equal(got, exp[:m]) and np.array_equal(per, eper)
     assert np.array_equal(plain, exp[:m]) and np.array_equal(plain_per, eper)
+
+
+@pytest.mark.parametrize("deletes", [False, True])
+def test_key_bitmap_where_matches_materialized_bitmap(gpu, deletes):
+    """ws_key_bitmap_where (the filtered build side's key bitmap in one scan) against numpy: ragged
+    batches, deleted rows, keys below and above the bitmap range (ignored), an expression key, and
+    the count of passing rows."""
+    H.say(f"case key bitmap where deletes={deletes}")
+    rng = np.random.default_rng(17)
+    n = 400_000
+    key = rng.integers(-1000, 300_000, n).astype(np.int32)
+    attr = rng.integers(0, 1000, n).astype(np.int32)
+    deleted = rng.random(n) < 0.1 if deletes else None
+    t = H.Table({"k": key, "a": attr}, [131_072, 7, 0, 200_000, 68_921], deleted)
+    b = abi.EnvBuilder()
+    sk, sa = b.slot(abi.T_I32), b.slot(abi.T_I32)
+    where = b.prog([(abi.OP_COL, sa), b.ilit(70), (abi.OP_LT,)])
+    kprog = b.prog([(abi.OP_COL, sk), b.ilit(5), (abi.OP_ADD,)])
+    slots = [("k", abi.T_I32, 0), ("a", abi.T_I32, 0)]
+    dev = H.Bound(t, slots, True)
+    lo, nbits = 100, 250_000
+    bits = ws.DevBuf(((nbits + 31) // 32) * 4)
+    cnt = ws.DevBuf(8)
+    nb = len(t.sizes)
+    live = np.ones(n, bool) if deleted is None else ~deleted
+    passing = live & (attr < 70)
+    k = key[passing].astype(np.int64) + 5 - lo
+    k = k[(k >= 0) & (k < nbits)]
+    want = np.zeros((nbits + 31) // 32 * 32, bool)
+    want[k] = True
+    kp = abi.AggPlan()
+    kp.has_where, kp.where, kp.nkeys, kp.keys[0], kp.key_int[0] = 1, where, 1, kprog, 1
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    jit = C.c_void_p()
+    ws.check(gpu.ws_jit_get(C.byref(b.env), 2, C.byref(kp), abi.JF_FLAGM, b"key bitmap", C.byref(jit)))
+    assert jit.value
+    for plan in (None, jit):  # the interpreter, then the compiled plan
+        ws.check(gpu.ws_key_bitmap_where(C.byref(b.env), 1, where, kprog, nb, ws.ptr(np.array(t.sizes, np.int64)),
+                                         C.cast(dev.dels, C.c_void_p), C.cast(dev.views, C.c_void_p), 2, lo, nbits, bits.p,
+                                         cnt.p, plan))
+        got = bits.to_array(np.uint32, (nbits + 31) // 32)
+        assert np.array_equal(got, np.packbits(want, bitorder="little").view(np.uint32))
+        assert int(cnt.to_array(np.uint64, 1)[0]) == int(passing.sum())
+    # a floating-point key is refused
+    sp = b.slot(abi.T_F64)
+    fprog = b.prog([(abi.OP_COL, sp)])
+    rc = gpu.ws_key_bitmap_where(C.byref(b.env), 1, where, fprog, nb, ws.ptr(np.array(t.sizes, np.int64)),
+                                 C.cast(dev.dels, C.c_void_p), C.cast(dev.views, C.c_void_p), 2, lo, nbits, bits.p, None,
+                                 None)
+    assert rc == ws.WS_ERR_INTERNAL

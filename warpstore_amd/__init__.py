@@ -42,7 +42,7 @@ _SIGS = {
     "ws_dev_alloc": (I, [SZ, PP]), "ws_dev_free": (None, [P]), "ws_tmp_alloc": (I, [SZ, PP]),
     "ws_trim_pool": (None, []), "ws_tmp_free": (None, [P]),
     "ws_pinned_alloc": (I, [SZ, PP]), "ws_pinned_free": (None, [P]),
-    "ws_h2d": (I, [P, P, SZ]), "ws_d2h": (I, [P, P, SZ]), "ws_d2d": (I, [P, P, SZ]), "ws_sync": (I, []),
+    "ws_h2d": (I, [P, P, SZ]), "ws_d2h": (I, [P, P, SZ]), "ws_d2h_async": (I, [P, P, SZ]), "ws_d2d": (I, [P, P, SZ]), "ws_sync": (I, []),
     "ws_set_vram_limit": (None, [I64]), "ws_vram_used": (I64, []), "ws_vram_fill": (D, []),
     "ws_place_column": (I, [P, SZ, I, C.POINTER(I), PP]), "ws_release_column": (None, [P, P, SZ, I]),
     "ws_vram_acquire": (P, [P, SZ]), "ws_vram_release": (None, [P, SZ]), "ws_host_map": (P, [P, SZ]),
@@ -81,6 +81,7 @@ _SIGS = {
     "ws_eval_at": (I, [C.POINTER(ExprEnv), Prog, P, I64, P]), "ws_gather": (I, [P, I, P, I64, P]),
     "ws_gather_enc": (I, [C.POINTER(ColView), I, P, I64, P]), "ws_key_bitmap": (I, [ColView, I64, I64, I64, P]),
     "ws_bitmap_count": (I, [P, I64, C.POINTER(I64)]),
+    "ws_key_bitmap_where": (I, [C.POINTER(ExprEnv), I, Prog, Prog, I, P, P, P, I, I64, I64, P, P, P]),
     "ws_set_bits": (I, [P, P, I64]), "ws_gather_null": (I, [P, I, P, I64, P]),
     "ws_gather_bytes": (I, [P, I, P, I64, P, I]),
     "ws_hash_join": (I, [ColView, I64, ColView, I64, PP, PP, C.POINTER(I64)]),

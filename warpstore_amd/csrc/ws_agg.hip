@@ -8,6 +8,7 @@
 #include <mutex>
 #include <utility>
 
+#include "ws_codegen.h"
 #include "ws_internal.h"
 
 // Ahead-of-time kernels: the shared bodies with the bytecode interpreter.
@@ -23,6 +24,9 @@ WS_KERNEL_NOKEYS(ws::InterpEvalCols, ws_c_nokeys)
 WS_KERNEL_DENSE(ws::InterpEvalCols, ws_c_dense)
 WS_KERNEL_GROUPED(ws::InterpEvalCols, ws_c_grouped)
 WS_KERNEL_SELECT(ws::InterpEvalCols, ws_c_select)
+WS_KERNEL_KEY_BITS(ws::InterpEval, ws_i_key_bits)
+WS_KERNEL_KEY_BITS(ws::InterpEvalLean, ws_l_key_bits)
+WS_KERNEL_KEY_BITS(ws::InterpEvalCols, ws_c_key_bits)
 
 // Interpreter variant of a plan: the full VM, the lean VM (no scalar-function opcodes), or the
 // column-only evaluator (every program one column or literal). IV_PICK names the kernel.
@@ -46,6 +50,20 @@ __global__ void batch_counts_k(const ScanDesc* d, const uint32_t* tile_off, int 
 __global__ void __launch_bounds__(kSelExpandThreads) select_expand_k(const u64* masks, const int64_t* tile_row,
                                                                       const uint32_t* tile_off, int stride, int32_t* out_idx) {
   select_expand_body(masks, (const i64*)tile_row, tile_off, stride, out_idx);
+}
+// *out = sum of a[0, n): one block.
+__global__ void __launch_bounds__(1024) sum_u64_k(const u64* a, int64_t n, u64* out) {
+  __shared__ u64 s_w[16];
+  u64 t = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) t += a[i];
+  for (int o = kWave / 2; o > 0; o >>= 1) t += __shfl_down(t, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 s = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s += s_w[w];
+    *out = s;
+  }
 }
 __global__ void init_table_k(GTable t, int nk, int na, ws_agg_plan plan) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= (int64_t)t.mask; i += (int64_t)gridDim.x * blockDim.x) {
@@ -797,9 +815,15 @@ struct ws_agg {
     uint32_t groups;
   } st{};
   bool st_valid = false;
+  // small dense aggregations: the accumulators read back with the status (host_acc_ok while
+  // st_valid), so a fetch needs no further round trip
+  std::vector<double> host_acc;
+  bool host_acc_ok = false;
   // fetch state
   int64_t fetched = -1;
   DevBuf<double> fetch_buf;
+  std::vector<double> host_keys, host_accs;  // a fetch served from host_acc
+  bool fetched_host = false;
   // the flags, counter and dense accumulators still need their initial values (deferred from
   // creation so a batch of aggregations is initialized by one launch)
   bool pending_init = false;
@@ -851,6 +875,8 @@ static void prepare(ws_agg* A) {
   }
 }
 // Reads the aggregation's small results in one round trip (cached until the next launch).
+// Dense accumulators up to this size come back with the status read (one round trip per fetch).
+constexpr size_t kSmallDenseFetchBytes = 32 * 1024;
 static const ws_agg::Status& status(ws_agg* A) {
   prepare(A);
   if (A->st_valid) return A->st;
@@ -860,6 +886,12 @@ static const ws_agg::Status& status(ws_agg* A) {
   if (A->flags) pc[n++] = D2HPiece{A->st.flags, A->flags, sizeof A->st.flags};
   if (A->count) pc[n++] = D2HPiece{&A->st.count, A->count, sizeof(u64)};
   if (A->part && A->goff) pc[n++] = D2HPiece{&A->st.groups, A->goff + A->P, sizeof(uint32_t)};
+  const size_t acc_bytes = A->is_dense && !A->part && A->acc ? sizeof(double) * (size_t)A->dense.G * A->plan.naggs : 0;
+  A->host_acc_ok = acc_bytes > 0 && acc_bytes <= kSmallDenseFetchBytes;
+  if (A->host_acc_ok) {
+    A->host_acc.resize(acc_bytes / sizeof(double));
+    pc[n++] = D2HPiece{A->host_acc.data(), A->acc, acc_bytes};
+  }
   d2h_gather_sync(pc, n);
   A->st_valid = true;
   return A->st;
@@ -1545,6 +1577,33 @@ int ws_agg_fetch_begin(ws_agg* a, int64_t* groups) {
       a->fetched = 1;
       return;
     }
+    a->fetched_host = false;
+    if (a->is_dense && !a->part) {
+      status(a);  // (small: the accumulators come with it)
+      if (a->host_acc_ok) {
+        // the groups with rows (COUNT slot > 0, as compact_dense), keys from the group index
+        const ws_dense_spec& ds = a->dense;
+        const int nk = a->plan.nkeys, na = a->plan.naggs;
+        a->host_keys.clear();
+        a->host_accs.clear();
+        for (int64_t g = 0; g < ds.G; ++g) {
+          const double* row = a->host_acc.data() + g * na;
+          if (!(row[0] > 0)) continue;
+          for (int c = 0; c < nk; ++c) {
+            const int64_t v = ds.kmin[c] + (g / ds.stride[c]) % ds.range[c];
+            double kv;
+            if (a->plan.key_int[c]) std::memcpy(&kv, &v, 8);
+            else kv = (double)v;
+            a->host_keys.push_back(kv);
+          }
+          a->host_accs.insert(a->host_accs.end(), row, row + na);
+        }
+        a->fetched_host = true;
+        a->fetched = na ? (int64_t)(a->host_accs.size() / na) : 0;
+        *groups = a->fetched;
+        return;
+      }
+    }
     a->fetched = agg_compact_into(a, a->fetch_buf, false);
     *groups = a->fetched;
   });
@@ -1566,6 +1625,13 @@ int ws_agg_fetch_copy(ws_agg* a, double* keys, double* accs) {
     }
     if (a->fetched < 0) throw Error("fetch_copy without fetch_begin", WS_ERR_INTERNAL);
     const int64_t g = a->fetched;
+    if (a->fetched_host) {
+      if (g) {
+        std::memcpy(keys, a->host_keys.data(), sizeof(double) * g * nk);
+        std::memcpy(accs, a->host_accs.data(), sizeof(double) * g * na);
+      }
+      return;
+    }
     if (g) {
       WS_CK(hipMemcpyAsync(keys, a->fetch_buf.p, sizeof(double) * g * nk, hipMemcpyDeviceToHost, stream()));
       WS_CK(hipMemcpyAsync(accs, a->fetch_buf.p + g * nk, sizeof(double) * g * na, hipMemcpyDeviceToHost, stream()));
@@ -1891,6 +1957,61 @@ int ws_select_multi(const ws_expr_env* env, int has_where, ws_prog where, int nb
     *count = select_launch(*env, has_where, where, in, false, jit, WS_JF_FLAGM, out.p, &pb);
     for (int b = 0; b < nb; ++b) per_batch[b] = pb[b];
     *out_idx = out.release();
+  });
+}
+
+int ws_key_bitmap_where(const ws_expr_env* env, int has_where, ws_prog where, ws_prog key, int nb, const int64_t* bn,
+                        const uint32_t* const* bdel, const ws_colview* bcols, int ncols, int64_t lo, int64_t nbits,
+                        uint32_t* bits, uint64_t* count, const ws_jit_program* jit) {
+  return guard([&] {
+    if (nbits <= 0 || nbits > ((int64_t)1 << 31)) throw Error("ws_key_bitmap_where: bitmap of " + std::to_string(nbits) + " bits", WS_ERR_INTERNAL);
+    if (!codegen::program_is_int(*env, key)) throw Error("ws_key_bitmap_where: the key must be an integer expression", WS_ERR_INTERNAL);
+    fill_u32(bits, (nbits + 31) / 32, 0);
+    ScanInput in;
+    in.nb = nb;
+    in.ncols = ncols;
+    int64_t total = 0;
+    for (int b = 0; b < nb; ++b) {
+      in.n.push_back(bn[b]);
+      in.del.push_back(bdel ? bdel[b] : nullptr);
+      total += bn[b];
+    }
+    in.cols.assign(bcols, bcols + (size_t)nb * ncols);
+    if (total <= 0) {
+      if (count) fill_u64((u64*)count, 1, 0);
+      return;
+    }
+    ws_agg_plan plan{};
+    plan.has_where = has_where ? 1 : 0;
+    plan.where = where;
+    plan.nkeys = 1;
+    plan.keys[0] = key;
+    plan.key_int[0] = 1;
+    // Two rows per thread and tile, enough blocks to fill the CUs: each row's loads wait, so the
+    // latency is hidden by the number of waves in flight.
+    ScanDesc d{};
+    d.tile_rows = 2 * kBlock;
+    int64_t ntiles = 0;
+    for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
+    DescUpload up;
+    build_desc(up, d, in, plan, nullptr, *env, false, 0, nullptr);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ntiles, (int64_t)cu_count() * 16));
+    long long lo_ = lo, nbits_ = nbits;
+    DevBuf<u64> parts(count ? grid : 0);  // per-block counts of passing rows, summed into *count
+    unsigned long long* pp = (unsigned long long*)parts.p;
+    if (void* jf = jit ? jit_function(jit, kJitKeyBits) : nullptr) {
+      void* args[] = {&up.ddesc, &lo_, &nbits_, &bits, &pp};
+      jit_launch_raw(jf, grid, kBlock, 0, args, jit_prof_name(jit, "key_bitmap_where"));
+    } else {
+      const int iv = interp_of(*env, plan);
+      ProfScope ps("key_bitmap_where");
+      hipLaunchKernelGGL(IV_PICK(iv, key_bits), dim3(grid), dim3(kBlock), 0, stream(), up.ddesc, lo_, nbits_, bits, pp);
+      check_launch("key_bitmap_where");
+    }
+    if (count) {
+      hipLaunchKernelGGL(sum_u64_k, dim3(1), dim3(1024), 0, stream(), (const u64*)parts.p, (int64_t)grid, (u64*)count);
+      check_launch("key_bitmap_where_count");
+    }
   });
 }
 

@@ -565,6 +565,45 @@ std::string evaluator_source(const ws_expr_env& env, int ncols, const ws_agg_pla
         }
       }
     o << " }\n";
+    // Four consecutive rows kept as the 16-byte vectors they load as (a kernel that carries the
+    // next rows across a loop iteration keeps each load's registers whole, with no copy that would
+    // wait for the load); row_of4 takes row s of them.
+    o << "  struct Row4 {";
+    if (split)
+      for (int c : em.row_cols) {
+        const ws_colview& v = env.cols[c];
+        if (v.enc) o << " ws::ws_u32x4 w" << c << ";";
+        else if (v.t == WS_T_I32 || v.t == WS_T_I32N) o << " ws::ws_i32x4 w" << c << ";";
+        else o << " " << (v.t == WS_T_F64 ? "ws::ws_f64x2" : "ws::ws_i64x2") << " w" << c << "a, w" << c << "b;";
+      }
+    o << " int pad_; };\n";
+    o << "  __device__ void load_row4(long long r, Row4& x) const {";
+    if (split)
+      for (int c : em.row_cols) {
+        const ws_colview& v = env.cols[c];
+        const std::string w = "x.w" + std::to_string(c);
+        if (v.enc) {
+          o << " " << w << " = ws::ws_u32x4{";
+          for (int q = 0; q < 4; ++q) o << (q ? "," : "") << "ws::enc_word(" << enc_view(c) << ",r+" << q << ")";
+          o << "};";
+        } else if (v.t == WS_T_I32 || v.t == WS_T_I32N) {
+          o << " " << w << " = *(const ws::ws_i32x4 WS_GLOBAL*)(" << col_ptr(c) << " + r);";
+        } else {
+          const char* vt = v.t == WS_T_F64 ? "ws::ws_f64x2" : "ws::ws_i64x2";
+          o << " " << w << "a = *(const " << vt << " WS_GLOBAL*)(" << col_ptr(c) << " + r); " << w << "b = *(const " << vt
+            << " WS_GLOBAL*)(" << col_ptr(c) << " + r + 2);";
+        }
+      }
+    o << " }\n";
+    o << "  __device__ Row row_of4(const Row4& x, int s) const { Row y;";
+    if (split)
+      for (int c : em.row_cols) {
+        const ws_colview& v = env.cols[c];
+        const std::string w = "x.w" + std::to_string(c);
+        if (v.enc || v.t == WS_T_I32 || v.t == WS_T_I32N) o << " y.w" << c << " = " << w << "[s];";
+        else o << " y.w" << c << " = s < 2 ? " << w << "a[s] : " << w << "b[s - 2];";
+      }
+    o << " y.pad_ = 0; return y; }\n";
     if (split) o << "  __device__ bool where_row(const Row& x, long long r) const {\n" << body.str() << "      return " << cond << ";\n  }\n";
     else o << "  __device__ bool where_row(const Row&, long long r) const { return where(r); }\n";
   }

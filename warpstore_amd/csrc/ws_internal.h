@@ -164,6 +164,7 @@ constexpr int kJitBatchDense = 102;                         // batched kernel of
 constexpr int kJitSelectChunk = 103;                        // chunked kernel of a selection plan
 constexpr int kJitSelectStaged = 104;  // persistent chunked kernel of a selection plan (sets staged in LDS)
 constexpr int kJitDenseStaged = 105;   // dense GROUP BY kernel of a dense plan with its sets staged in LDS
+constexpr int kJitKeyBits = 106;       // key bitmap kernel of a selection plan (keys[0] of passing rows)
 int jit_staged_static_lds(const ws_jit_program* p);  // static LDS bytes of either staged kernel
 // ---- learned sizing (ws_sizing.hip): appends one (features, exact groups) sample to the log
 void sizing_log(const float* features, int64_t groups);

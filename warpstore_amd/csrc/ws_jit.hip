@@ -39,6 +39,7 @@ struct ws_jit_program {
   hipFunction_t fn_part[2] = {nullptr, nullptr};  // grouped plans: partition count / scatter kernels
   hipFunction_t fn_batch = nullptr;               // dense plans: the batched (many queries) kernel
   hipFunction_t fn_chunk = nullptr;               // selection plans: the chunked kernel
+  hipFunction_t fn_keybits = nullptr;             // selection plans: key bitmap of the passing rows (keys[0])
   hipFunction_t fn_staged = nullptr;              // selection and dense plans: the kernel with the sets in LDS
   int staged_static_lds = 0;                      // its static LDS bytes
   std::atomic<int> state{0};  // 0 compiling, 1 ready, 2 failed
@@ -240,6 +241,7 @@ void load_image(ws_jit_program& p, const std::string& image) {
   }
   if (p.fn_kind == WS_JF_FLAG || p.fn_kind == WS_JF_FLAGM) {
     WS_CK(hipModuleGetFunction(&p.fn_chunk, mod, (std::string(kKernelName[p.fn_kind]) + "_chunk").c_str()));
+    WS_CK(hipModuleGetFunction(&p.fn_keybits, mod, (std::string(kKernelName[p.fn_kind]) + "_keybits").c_str()));
     WS_CK(hipModuleGetFunction(&p.fn_staged, mod, (std::string(kKernelName[p.fn_kind]) + "_staged").c_str()));
     WS_CK(hipFuncGetAttribute(&p.staged_static_lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, p.fn_staged));
   }
@@ -333,6 +335,7 @@ void* jit_function(const ws_jit_program* p, int fn) {
   if (fn == kJitSelectChunk) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_chunk : nullptr;
   if (fn == kJitSelectStaged) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_staged : nullptr;
   if (fn == kJitDenseStaged) return p->fn_kind == WS_JF_DENSE ? (void*)p->fn_staged : nullptr;
+  if (fn == kJitKeyBits) return p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM ? (void*)p->fn_keybits : nullptr;
   const bool sel = (fn == WS_JF_FLAG || fn == WS_JF_FLAGM) && (p->fn_kind == WS_JF_FLAG || p->fn_kind == WS_JF_FLAGM);
   return (p->fn_kind == fn || sel) ? (void*)p->fn : nullptr;
 }

@@ -106,6 +106,7 @@ __device__ __forceinline__ ws_colview uniform_view(const ws_colview& v) {
 // base + code, the top code standing for the type's NULL.
 // Builtin vector types for 16-byte loads through address-space-qualified pointers.
 typedef int ws_i32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned ws_u32x4 __attribute__((ext_vector_type(4)));
 typedef long long ws_i64x2 __attribute__((ext_vector_type(2)));
 typedef double ws_f64x2 __attribute__((ext_vector_type(2)));
 
@@ -1846,7 +1847,7 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
   BatchCursor bc;
   // Compiled WHEREs in two halves (Ev::kSplit): later full tiles' column words load while this tile
   // is computed (xa, xb hold this tile's and the next one's rows when `have`).
-  typename Ev::Row xa[kSelMaskRows];
+  typename Ev::Row4 xa[kSelMaskRows / 4];
   bool have = false;
   for (i64 t = t0; t < t1; ++t) {
     if (bc.seek(d, t)) {
@@ -1871,18 +1872,18 @@ __device__ void select_mask_body(const ScanDesc& d, u64* masks, uint32_t* tile_c
         const i64 rb = tr.first + (i64)wave * (64 * kSelMaskRows) + 4 * lane;
         if (!have) {
 #pragma unroll
-          for (int g = 0; g < NG; ++g) ev.load_rows4(rb + 256 * g, xa + 4 * g);
+          for (int g = 0; g < NG; ++g) ev.load_row4(rb + 256 * g, xa[g]);
         }
         const bool nxt = t + 1 < t1 && tr.first + 2 * tr_ <= bc.n;
         const i64 rn = nxt ? rb + tr_ : rb;
-        typename Ev::Row xb[kSelMaskRows];
+        typename Ev::Row4 xb[NG];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) ev.load_rows4(rn + 256 * g, xb + 4 * g);
+        for (int g = 0; g < NG; ++g) ev.load_row4(rn + 256 * g, xb[g]);
         bool p[kSelMaskRows];
 #pragma unroll
-        for (int j = 0; j < kSelMaskRows; ++j) p[j] = ev.where_row(xa[j], rb + (j >> 2) * 256 + (j & 3));
+        for (int j = 0; j < kSelMaskRows; ++j) p[j] = ev.where_row(ev.row_of4(xa[j >> 2], j & 3), rb + (j >> 2) * 256 + (j & 3));
 #pragma unroll
-        for (int j = 0; j < kSelMaskRows; ++j) xa[j] = xb[j];
+        for (int g = 0; g < NG; ++g) xa[g] = xb[g];
         have = nxt;
         // lane j keeps ballot j: one store instruction per wave writes them all
         uint32_t tc = 0;
@@ -2043,7 +2044,7 @@ __device__ void dense_staged_body(const ScanDesc& d, i64* parts, int* flags) {
   bool have = false;
   // One tile: xc holds its WHERE columns when `have`; the next tile's load into xn. The loop below
   // alternates the two arrays (a copy from one to the other would wait for the loads).
-  typedef typename Ev::Row Rows[kSelMaskRows];
+  typedef typename Ev::Row4 Rows[kSelMaskRows / 4];
   auto tile = [&](i64 t, Rows& xc, Rows& xn) {
     if (bc.seek(d, t)) {
       have = false;
@@ -2060,18 +2061,18 @@ __device__ void dense_staged_body(const ScanDesc& d, i64* parts, int* flags) {
         const i64 rb = rw + 4 * lane;
         if (!have) {
 #pragma unroll
-          for (int g = 0; g < NG; ++g) ev.load_rows4(rb + 256 * g, xc + 4 * g);
+          for (int g = 0; g < NG; ++g) ev.load_row4(rb + 256 * g, xc[g]);
         }
         const bool nxt = t + 1 < t1 && tr.first + 2 * bc.tr <= bc.n;
         const i64 rn = nxt ? rb + bc.tr : rb;
 #pragma unroll
-        for (int g = 0; g < NG; ++g) ev.load_rows4(rn + 256 * g, xn + 4 * g);
+        for (int g = 0; g < NG; ++g) ev.load_row4(rn + 256 * g, xn[g]);
         u64 B[kSelMaskRows];
         int pre[kSelMaskRows + 1];
         pre[0] = 0;
 #pragma unroll
         for (int j = 0; j < kSelMaskRows; ++j) {
-          B[j] = __ballot(ev.where_row(xc[j], rb + (j >> 2) * 256 + (j & 3)));
+          B[j] = __ballot(ev.where_row(ev.row_of4(xc[j >> 2], j & 3), rb + (j >> 2) * 256 + (j & 3)));
           pre[j + 1] = pre[j] + __popcll(B[j]);
         }
         have = nxt;
@@ -2134,6 +2135,46 @@ __device__ void dense_staged_body(const ScanDesc& d, i64* parts, int* flags) {
   if (bad) flags[1] = 1;
   __syncthreads();
   dense_write_copies(d, ds, NA, ws_smem, parts + (i64)blockIdx.x * ds.G * NA);
+}
+
+// Bitmap of the integer keys of the rows that pass the WHERE (the filtered build side of a join
+// filter, without materializing it): bit key - lo of `bits` for each live passing row whose key is
+// in [lo, lo + nbits); parts[block] (when given) gets the block's count of passing rows (summed
+// afterwards: atomics from every block on one address would queue). Tiles grid-stride.
+template <class Ev>
+__device__ void key_bits_body(const ScanDesc& d, i64 lo, i64 nbits, uint32_t* bits, u64* parts) {
+  __shared__ u64 s_cnt[16];
+  __shared__ ws_colview scols[WS_MAX_COLS];
+  Ev ev(d);
+  u64 passed = 0;
+  BatchCursor bc;
+  const i64 ntiles = uni(d.ntiles);
+  for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    if (bc.seek(d, t)) {
+      stage_batch(d, bc.b, scols);
+      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+    }
+    const TileRows tr = bc.rows(t);
+    for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+      const i64 r = tr.first + i;
+      if (is_deleted(bc.del, r)) continue;
+      if (ev.has_where() && !ev.where(r)) continue;
+      ++passed;
+      const u64 o = (u64)(ev.ikey(0, r) - lo);
+      if (o < (u64)nbits) atomicOr(&bits[o >> 5], 1u << (o & 31));
+    }
+  }
+  if (parts) {  // (block-uniform)
+    for (int o = kWave / 2; o > 0; o >>= 1) passed += __shfl_down(passed, o, kWave);
+    const int wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) s_cnt[wave] = passed;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u64 t = 0;
+      for (int w = 0; w < nw; ++w) t += s_cnt[w];
+      parts[blockIdx.x] = t;
+    }
+  }
 }
 
 // Per-batch counts from the scanned tile offsets (tiles never straddle batches); no atomics, which
@@ -2274,6 +2315,11 @@ __device__ void select_expand_body(const u64* masks, const i64* tile_row, const 
                                                                    int* flags) {                             \
     ws::dense_staged_body<EV>(*d, parts, flags);                                                             \
   }
+#define WS_KERNEL_KEY_BITS(EV, NAME)                                                                        \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, long long lo, long long nbits,   \
+                                                         unsigned* bits, unsigned long long* parts) {         \
+    ws::key_bits_body<EV>(*d, lo, nbits, bits, parts);                                                       \
+  }
 #define WS_KERNEL_GROUPED(EV, NAME)                                                                         \
   extern "C" __global__ void __launch_bounds__(256) NAME(const ws::ScanDesc* d, ws::GTable gt) {            \
     ws::grouped_body<EV>(*d, gt);                                                                            \
@@ -2297,7 +2343,8 @@ __device__ void select_expand_body(const u64* masks, const i64* tile_row, const 
                                                                  unsigned long long* total, int* err) {     \
     ws::select_chunk_body<EV>(*d, state, chunk_ctr, out_idx, per_batch, total, err);                         \
   }
-#define WS_KERNEL_SELECT_JIT(EV, NAME) WS_KERNEL_SELECT(EV, NAME) WS_KERNEL_SELECT_STAGED(EV, NAME)
+#define WS_KERNEL_SELECT_JIT(EV, NAME) \
+  WS_KERNEL_SELECT(EV, NAME) WS_KERNEL_SELECT_STAGED(EV, NAME) WS_KERNEL_KEY_BITS(EV, NAME##_keybits)
 // Compiled plans only (the evaluator stages its sets): the ballot pass of the staged selection.
 #define WS_KERNEL_SELECT_STAGED(EV, NAME)                                                                   \
   extern "C" __global__ void __launch_bounds__(1024) NAME##_staged(const ws::ScanDesc* d, unsigned long long* masks, \

@@ -52,6 +52,7 @@ struct DevColumn {
   std::vector<SegCol> segs;
   int64_t mn = INT64_MAX, mx = INT64_MIN;  // min / max of integer columns (colRange)
   ws_col_stats stats{};                    // merged segment statistics (computed at load)
+  bool unique = false;  // integer column without repeated values (a key), found at load when its span < 2^27
 };
 struct DevTable {
   std::vector<int64_t> seg_rows;
@@ -649,6 +650,60 @@ struct C3Agg {
 };
 
 int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stats* st, int64_t attr_lt = 50) {
+  const int64_t lo = dim.cols[0].mn, nbits = dim.cols[0].mx - dim.cols[0].mn + 1;
+  // A key column on the dimension side: each fact row that passes the bitmap of the filtered keys
+  // joins exactly one dimension row, and no dimension column is read after the join, so the join
+  // is the bitmap filter itself. The bitmap is set in one scan of the dimension (no materialized
+  // build side) and the fact table aggregated in one pass with the bitmap test as its WHERE (the
+  // set staged in LDS): no selection vector, gathers or join, and no read-back before the results.
+  if (g_join_rewrite.load() && dim.cols[0].unique && dim.rows > 0 && nbits <= ((int64_t)1 << 30)) {
+    Comp cd;
+    const int d_key = cd.slot(WS_T_I32), d_attr = cd.slot(WS_T_I32);
+    const Prog dwhere = cd.prog({cd.col(d_attr), cd.ilit(attr_lt), Comp::op(WS_OP_LT)});
+    const Prog dkey = cd.prog({cd.col(d_key)});
+    std::vector<int64_t> bn;
+    std::vector<const uint32_t*> bdel;
+    std::vector<ColView> bcols;
+    for (size_t sg = 0; sg < dim.seg_rows.size(); ++sg) {
+      bn.push_back(dim.seg_rows[sg]);
+      bdel.push_back(nullptr);
+      ExprEnv e = bind_segment(cd, dim, {0, 1}, sg);
+      for (int k = 0; k < cd.ncols; ++k) bcols.push_back(e.cols[k]);
+    }
+    AggPlan kp;  // the selection plan of the filter, with the key as keys[0] (compiled key bitmap kernel)
+    std::memset(&kp, 0, sizeof kp);
+    kp.has_where = 1;
+    kp.where = dwhere;
+    kp.nkeys = 1;
+    kp.keys[0] = dkey;
+    kp.key_int[0] = 1;
+    const JitProgram* kjit = jitGet(cd.env, cd.ncols, kp, JF_FLAGM, "key bitmap (dimension filter)");
+    DevVec<uint32_t> bits((nbits + 31) / 32);
+    DevVec<uint64_t> nsel(1);
+    wsd::ck(ws_key_bitmap_where(wsd::env(cd.env), 1, wsd::prog(dwhere), wsd::prog(dkey), (int)bn.size(), bn.data(), bdel.data(),
+                                reinterpret_cast<const ws_colview*>(bcols.data()), cd.ncols, lo, nbits, bits.p, nsel.p,
+                                wsd::handle(kjit)));
+    // the count of filtered dimension rows comes back with the aggregation's read-back
+    static thread_local uint64_t* t_nsel = nullptr;
+    if (!t_nsel) wsd::ck(ws_pinned_alloc(sizeof(uint64_t), (void**)&t_nsel));
+    wsd::ck(ws_d2h_async(t_nsel, nsel.p, sizeof(uint64_t)));
+    C3Agg A(bits.p, lo, nbits);
+    const std::vector<int> slot_col = {3, 2, 1, 0};  // flag, price, qty, fk
+    const int64_t g = A.run(fact.cols[3], fact.rows, "aggregate fact (join as filter)", [&](GpuAggregator& agg) {
+      for (size_t sg = 0; sg < fact.seg_rows.size(); ++sg) {
+        ExprEnv e = bind_segment(A.c, fact, slot_col, sg);
+        agg.addBatch(e, A.c.ncols, fact.seg_rows[sg], nullptr);
+      }
+    }, out);
+    if (st) {
+      int64_t pairs = 0;
+      for (int64_t i = 0; i < g; ++i) pairs += out[i * 4 + 1];
+      st->dim_rows = (int64_t)*t_nsel;
+      st->fact_rows = st->pairs = pairs;
+      st->predicted_pairs = 0;
+    }
+    return g;
+  }
   // dim: materialize(dim, [attr < attr_lt], {key})
   Comp cd;
   const int d_attr = cd.slot(WS_T_I32);
@@ -657,16 +712,13 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
   const int64_t nd = materialize(dim, cd, true, dwhere, {1}, {0}, drel, "filter (dimension)");
   // join filter: the dim keys were reduced below half their table, fact is large: exact bitmap
   // of the keys over the dim key range (engine_select.cpp:1015-1041)
-  const int64_t lo = dim.cols[0].mn, nbits = dim.cols[0].mx - dim.cols[0].mn + 1;
   DevVec<uint32_t> bits((nbits + 31) / 32);
   ColView kv{};
   kv.p = drel[0].data.p;
   kv.t = WS_T_I32;
   gpuKeyBitmap(kv, nd, lo, nbits, bits.p);
-  // Distinct build keys (as many set bits as filtered dimension rows): each fact row that passes
-  // the bitmap joins exactly one dimension row, and no dimension column is read after the join,
-  // so the join is the bitmap filter itself. The fact table is then aggregated in one pass with
-  // the bitmap test as its WHERE (the set staged in LDS): no selection vector, gathers or join.
+  // Distinct filtered keys on a dimension without a key column (as many set bits as filtered
+  // rows): the join is the bitmap filter, as above, over the materialized build side.
   if (g_join_rewrite.load() && nd > 0 && gpuBitmapCount(bits.p, (int64_t)bits.n) == nd) {
     C3Agg A(bits.p, lo, nbits);
     const std::vector<int> slot_col = {3, 2, 1, 0};  // flag, price, qty, fk
@@ -1284,6 +1336,17 @@ int wsq_table_load(int ncols, const int* types, const int* encode, int64_t n, co
           else ws_col_stats_merge(&col.stats, &st);
         }
         r0 += rows;
+      }
+      // distinct values (a key column): one pass over a bitmap of the value span
+      if (col.t != WS_T_F64 && n > 0 && (uint64_t)(col.mx - col.mn) < (1ull << 27)) {
+        std::vector<uint64_t> seen((size_t)((col.mx - col.mn) / 64 + 1), 0);
+        col.unique = true;
+        for (int64_t i = 0; i < n && col.unique; ++i) {
+          const int64_t v = w == 4 ? ((const int32_t*)host[c])[i] : ((const int64_t*)host[c])[i];
+          const uint64_t b = (uint64_t)(v - col.mn);
+          if (seen[b >> 6] >> (b & 63) & 1) col.unique = false;
+          seen[b >> 6] |= 1ull << (b & 63);
+        }
       }
     }
     *out = t.release();

@@ -459,6 +459,11 @@ void ws_trim_pool(void) { trim_pool(); }
 void ws_tmp_free(void* p) { tmp_free(p); }
 int ws_h2d(void* dst, const void* src, size_t b) { return guard([&] { h2d_sync(dst, src, b); }); }
 int ws_d2h(void* dst, const void* src, size_t b) { return guard([&] { d2h_sync(dst, src, b); }); }
+int ws_d2h_async(void* dst, const void* src, size_t b) {
+  return guard([&] {
+    if (b) WS_CK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, stream()));
+  });
+}
 int ws_d2d(void* dst, const void* src, size_t b) {
   return guard([&] {
     if (!b) return;

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "ws_internal.h"
@@ -80,7 +81,8 @@ static int64_t g_vram_limit = INT64_MAX;
 // is issued in order on its per-thread stream, so the block's earlier uses precede its next use
 // without any synchronization or driver call. hipMalloc runs only on a cache miss (warm-up), and
 // a miss that finds no free device memory releases the cached blocks and retries once. Blocks of
-// threads that exited are reused by others after a device synchronization.
+// threads that exited, and blocks freed by a thread other than the one that took them (its stream
+// may still be using them), are reused by others after a device synchronization.
 namespace {
 size_t size_class(size_t b) {
   if (b <= 512) return 512;
@@ -109,6 +111,8 @@ struct BlockCache {
   }
 };
 thread_local BlockCache t_cache;
+// Blocks this thread took and has not freed: a free by another thread is a cross-thread free.
+thread_local std::unordered_set<void*> t_owned;
 
 void release_cached() {  // frees this thread's cached blocks and the orphans
   (void)hipDeviceSynchronize();
@@ -139,6 +143,7 @@ void* tmp_alloc(size_t bytes) {
   if (it != t_cache.free.end() && !it->second.empty()) {
     void* p = it->second.back();
     it->second.pop_back();
+    t_owned.insert(p);
     return p;
   }
   if (g_orphan_count.load(std::memory_order_relaxed) > 0) {
@@ -153,7 +158,8 @@ void* tmp_alloc(size_t bytes) {
       }
     }
     if (p) {
-      WS_CK(hipDeviceSynchronize());  // the exited thread's last work on the block has ended
+      WS_CK(hipDeviceSynchronize());  // the previous owner's last work on the block has ended
+      t_owned.insert(p);
       return p;
     }
   }
@@ -163,8 +169,11 @@ void* tmp_alloc(size_t bytes) {
     release_cached();
     WS_CK(hipMalloc(&p, c));
   }
-  std::lock_guard<std::mutex> g(g_mem_mu);
-  g_live[p] = c;
+  {
+    std::lock_guard<std::mutex> g(g_mem_mu);
+    g_live[p] = c;
+  }
+  t_owned.insert(p);
   return p;
 }
 void tmp_free(void* p) {
@@ -175,7 +184,13 @@ void tmp_free(void* p) {
     auto it = g_live.find(p);
     if (it == g_live.end()) return;  // not a block of this allocator
     c = it->second;
+    if (!t_owned.count(p)) {  // taken by another thread: its stream may still use the block
+      g_orphans[c].push_back(p);
+      ++g_orphan_count;
+      return;
+    }
   }
+  t_owned.erase(p);
   t_cache.free[c].push_back(p);
 }
 

@@ -10,8 +10,10 @@
  *     with the same codes, as the reference does (gpu.cu:23-30);
  *   - device buffers returned by the library ("tmp" buffers) come from its caching allocator and
  *     are released with ws_tmp_free (DevVec<T> semantics, gpu.h:59-79): a freed block is reused,
- *     in stream order, by later work of the thread that freed it; ws_trim_pool returns the
- *     calling thread's cached blocks to the device;
+ *     in stream order, by later work of the thread that freed it; a block freed by another thread
+ *     than the one that took it goes to a shared pool, and its next taker waits for the device
+ *     first; ws_trim_pool returns the calling thread's cached blocks and the shared pool to the
+ *     device;
  *   - all work of a host thread goes to that thread's default stream (hipStreamPerThread), the HIP
  *     form of the reference's per-thread default stream (CMakeLists.txt:50-52).
  */

@@ -7,7 +7,9 @@
   - codegen failure (jit.cu:433-442): a plan hipRTC rejects (an invalid extra option) yields no
     program, the query runs interpreted with identical results, the plan cache marks it failed;
   - out of device memory (gpu.cu:23-30, 165-174): allocations the device cannot satisfy fail with
-    code 1041 and leave the library usable; a VRAM budget refuses columns beyond it with 1041."""
+    code 1041 and leave the library usable; a VRAM budget refuses columns beyond it with 1041;
+  - an aggregation handle run on one thread and fetched / destroyed on another (device blocks
+    freed by a thread that did not take them)."""
 import ctypes as C
 import os
 
@@ -176,3 +178,57 @@ def test_vram_budget_refuses_columns(gpu):
         assert gpu.ws_vram_used() == used
     finally:
         gpu.ws_set_vram_limit((1 << 63) - 1)
+
+
+def test_aggregation_handle_moves_between_threads(gpu):
+    """A handle built and run on one thread, fetched and destroyed on another: the destroying
+    thread's frees are cross-thread frees (the blocks go to the shared pool, whose reuse waits for
+    the device), and the blocks are reused correctly by later queries on both threads."""
+    H.say("case aggregation handle moves between threads")
+    import threading
+    t, b, plan, dev, host = _query(n=200_000)
+    ok, oa, _ = H.oracle_aggregate(plan, b.env, host)
+    for _ in range(3):
+        box = {}
+        th = threading.Thread(target=lambda: box.update(r=H.gpu_aggregate(plan, b.env, dev, cap=1 << 11, fetch=False)))
+        th.start()
+        th.join()
+        h, _cap = box["r"]
+        try:
+            g = C.c_int64()
+            ws.check(gpu.ws_agg_fetch_begin(h, C.byref(g)))
+            keys = np.zeros(max(g.value * plan.nkeys, 1))
+            accs = np.zeros(max(g.value * plan.naggs, 1))
+            ws.check(gpu.ws_agg_fetch_copy(h, ws.ptr(keys), ws.ptr(accs)))
+        finally:
+            gpu.ws_agg_destroy(h)
+        gk = keys[:g.value * plan.nkeys].reshape(g.value, plan.nkeys)
+        ga = accs[:g.value * plan.naggs].reshape(g.value, plan.naggs)
+        H.assert_groups_equal(plan, gk, ga, ok, oa)
+        # this thread takes the freed blocks from the shared pool
+        gk, ga, _, _ = H.gpu_aggregate(plan, b.env, dev, cap=1 << 11)
+        H.assert_groups_equal(plan, gk, ga, ok, oa)
+    # a raw block taken on one thread and freed on another keeps its contents until reused
+    box = {}
+    pat = np.arange(1 << 18, dtype=np.int64)
+
+    def take():
+        p = C.c_void_p()
+        ws.check(gpu.ws_tmp_alloc(pat.nbytes, C.byref(p)))
+        ws.check(gpu.ws_h2d(p, ws.ptr(pat), pat.nbytes))
+        box["p"] = p
+
+    th = threading.Thread(target=take)
+    th.start()
+    th.join()
+    back = np.zeros_like(pat)
+    ws.check(gpu.ws_d2h(ws.ptr(back), box["p"], pat.nbytes))
+    assert np.array_equal(back, pat)
+    gpu.ws_tmp_free(box["p"])
+    q = C.c_void_p()
+    ws.check(gpu.ws_tmp_alloc(pat.nbytes, C.byref(q)))
+    ws.check(gpu.ws_h2d(q, ws.ptr(pat[::-1].copy()), pat.nbytes))
+    ws.check(gpu.ws_d2h(ws.ptr(back), q, pat.nbytes))
+    assert np.array_equal(back, pat[::-1])
+    gpu.ws_tmp_free(q)
+    gpu.ws_trim_pool()

@@ -6,7 +6,6 @@
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
-#include <unordered_set>
 #include <vector>
 
 #include "ws_internal.h"
@@ -96,7 +95,13 @@ size_t size_class(size_t b) {
   return (b + step - 1) / step * step;
 }
 std::mutex g_mem_mu;
-std::unordered_map<void*, size_t> g_live;                   // every block handed out -> its class size
+struct LiveBlock {
+  size_t cls;      // size class
+  uint64_t owner;  // thread that took it last (t_id); a free by another thread is a cross-thread free
+};
+std::unordered_map<void*, LiveBlock> g_live;                // every block this allocator made
+std::atomic<uint64_t> g_thread_seq{0};
+thread_local const uint64_t t_id = ++g_thread_seq;         // never reused, unlike std::thread::id
 std::unordered_map<size_t, std::vector<void*>> g_orphans;  // cached blocks of exited threads
 std::atomic<int64_t> g_orphan_count{0};
 struct BlockCache {
@@ -111,8 +116,6 @@ struct BlockCache {
   }
 };
 thread_local BlockCache t_cache;
-// Blocks this thread took and has not freed: a free by another thread is a cross-thread free.
-thread_local std::unordered_set<void*> t_owned;
 
 void release_cached() {  // frees this thread's cached blocks and the orphans
   (void)hipDeviceSynchronize();
@@ -141,9 +144,8 @@ void* tmp_alloc(size_t bytes) {
   const size_t c = size_class(bytes);
   auto it = t_cache.free.find(c);
   if (it != t_cache.free.end() && !it->second.empty()) {
-    void* p = it->second.back();
+    void* p = it->second.back();  // cached blocks are this thread's: owner already t_id
     it->second.pop_back();
-    t_owned.insert(p);
     return p;
   }
   if (g_orphan_count.load(std::memory_order_relaxed) > 0) {
@@ -155,11 +157,11 @@ void* tmp_alloc(size_t bytes) {
         p = o->second.back();
         o->second.pop_back();
         --g_orphan_count;
+        g_live[p].owner = t_id;
       }
     }
     if (p) {
       WS_CK(hipDeviceSynchronize());  // the previous owner's last work on the block has ended
-      t_owned.insert(p);
       return p;
     }
   }
@@ -171,9 +173,8 @@ void* tmp_alloc(size_t bytes) {
   }
   {
     std::lock_guard<std::mutex> g(g_mem_mu);
-    g_live[p] = c;
+    g_live[p] = LiveBlock{c, t_id};
   }
-  t_owned.insert(p);
   return p;
 }
 void tmp_free(void* p) {
@@ -183,14 +184,13 @@ void tmp_free(void* p) {
     std::lock_guard<std::mutex> g(g_mem_mu);
     auto it = g_live.find(p);
     if (it == g_live.end()) return;  // not a block of this allocator
-    c = it->second;
-    if (!t_owned.count(p)) {  // taken by another thread: its stream may still use the block
+    c = it->second.cls;
+    if (it->second.owner != t_id) {  // taken by another thread: its stream may still use the block
       g_orphans[c].push_back(p);
       ++g_orphan_count;
       return;
     }
   }
-  t_owned.erase(p);
   t_cache.free[c].push_back(p);
 }
 

@@ -532,6 +532,8 @@ std::string evaluator_source(const ws_expr_env& env, int ncols, const ws_agg_pla
     }
     const bool split = where >= 0 && em.row_ok && !em.row_cols.empty();
     o << "  static constexpr bool kSplit = " << (split ? "true" : "false") << ";\n";
+    // where_row needs no load of its own: a split WHERE, or none
+    o << "  static constexpr bool kRowSplit = " << (split || where < 0 ? "true" : "false") << ";\n";
     o << "  struct Row {";
     if (split)
       for (int c : em.row_cols) o << " " << (env.cols[c].enc ? "unsigned" : elem_type(env.cols[c].t)) << " w" << c << ";";
@@ -617,6 +619,7 @@ std::string evaluator_source(const ws_expr_env& env, int ncols, const ws_agg_pla
     std::set<int> cols;
     bool ok = true;
     const std::string fk = function_row(lw, "long long ikey_v(int i, const VRow& x, long long r)", keys, "int", cols, ok);
+    const std::string fd = function_row(lw, "double key_v(int i, const VRow& x, long long r)", keys, "double", cols, ok);
     const std::string fi = function_row(lw, "long long iagg_v(int i, const VRow& x, long long r)", iaggs, "int", cols, ok);
     const std::string fa = function_row(lw, "double agg_v(int i, const VRow& x, long long r)", aggs, "double", cols, ok);
     o << "  static constexpr bool kVSplit = " << (ok ? "true" : "false") << ";\n";
@@ -629,10 +632,33 @@ std::string evaluator_source(const ws_expr_env& env, int ncols, const ws_agg_pla
       for (int c : cols)
         o << " " << row_word(c) << " = " << (env.cols[c].enc ? "ws::enc_word(" + enc_view(c) + ",r)" : col_ptr(c) + "[r]") << ";";
     o << " }\n";
+    // the keys alone (KRow: the key columns' words), for passes that need no aggregate argument
+    std::set<int> kcols;
+    bool kok = true;
+    const std::string kk = function_row(lw, "long long ikey_k(int i, const KRow& x, long long r)", keys, "int", kcols, kok);
+    const std::string kd = function_row(lw, "double key_k(int i, const KRow& x, long long r)", keys, "double", kcols, kok);
+    kok = kok && ok;
+    o << "  static constexpr bool kKSplit = " << (kok ? "true" : "false") << ";\n";
+    o << "  struct KRow {";
+    if (kok)
+      for (int c : kcols) o << " " << (env.cols[c].enc ? "unsigned" : elem_type(env.cols[c].t)) << " w" << c << ";";
+    o << " int pad_; };\n";
+    o << "  __device__ void load_krow(long long r, KRow& x) const {";
+    if (kok)
+      for (int c : kcols)
+        o << " " << row_word(c) << " = " << (env.cols[c].enc ? "ws::enc_word(" + enc_view(c) + ",r)" : col_ptr(c) + "[r]") << ";";
+    o << " }\n";
+    if (kok) {
+      o << kk << kd;
+    } else {
+      o << "  __device__ long long ikey_k(int i, const KRow&, long long r) const { return ikey(i, r); }\n";
+      o << "  __device__ double key_k(int i, const KRow&, long long r) const { return key(i, r); }\n";
+    }
     if (ok) {
-      o << fk << fi << fa;
+      o << fk << fd << fi << fa;
     } else {
       o << "  __device__ long long ikey_v(int i, const VRow&, long long r) const { return ikey(i, r); }\n";
+      o << "  __device__ double key_v(int i, const VRow&, long long r) const { return key(i, r); }\n";
       o << "  __device__ long long iagg_v(int i, const VRow&, long long r) const { return iagg(i, r); }\n";
       o << "  __device__ double agg_v(int i, const VRow&, long long r) const { return agg(i, r); }\n";
     }

@@ -986,6 +986,57 @@ __device__ __forceinline__ void row_values(const Ev& ev, const ws_agg_plan& plan
   }
 }
 
+// The same from a row's loaded column words (a split evaluator's VRow).
+template <class Ev>
+__device__ __forceinline__ u64 row_keys_v(const Ev& ev, int NK, const typename Ev::VRow& x, i64 r, double* kv) {
+  u64 h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+  for (int k = 0; k < Ev::kKeys; ++k) {
+    if (k >= NK) break;
+    kv[k] = ev.key_int(k) ? bitsd(ev.ikey_v(k, x, r)) : ev.key_v(k, x, r) + 0.0;
+    h = mix64(h ^ (u64)dbits(kv[k])) + (u64)k;
+  }
+  return h == kEmpty ? h ^ 1 : h;
+}
+template <class Ev>
+__device__ __forceinline__ void row_values_v(const Ev& ev, int NA, const typename Ev::VRow& x, i64 r, double* av) {
+#pragma unroll
+  for (int a = 0; a < Ev::kAggs; ++a) {
+    if (a >= NA) break;
+    av[a] = ev.agg_bits(a) ? bitsd(int_part(ev, a, ev.iagg_v(a, x, r))) : ev.agg_v(a, x, r);
+  }
+}
+// Whether an evaluator has load halves for a whole row (WHERE words without loads of their own,
+// key and aggregate words): compiled evaluators; the interpreters have none.
+template <class Ev, class = void>
+struct RowSplit {
+  static constexpr bool value = false;
+};
+template <class Ev>
+struct RowSplit<Ev, decltype((void)Ev::kRowSplit)> {
+  static constexpr bool value = Ev::kRowSplit && Ev::kVSplit && Ev::kPureKeys && Ev::kPureVals && Ev::kPureWhere;
+};
+// The same for the WHERE and the keys alone (KRow).
+template <class Ev, class = void>
+struct KeySplit {
+  static constexpr bool value = false;
+};
+template <class Ev>
+struct KeySplit<Ev, decltype((void)Ev::kKSplit)> {
+  static constexpr bool value = Ev::kRowSplit && Ev::kKSplit && Ev::kPureKeys && Ev::kPureWhere;
+};
+template <class Ev>
+__device__ __forceinline__ u64 row_keys_k(const Ev& ev, int NK, const typename Ev::KRow& x, i64 r, double* kv) {
+  u64 h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+  for (int k = 0; k < Ev::kKeys; ++k) {
+    if (k >= NK) break;
+    kv[k] = ev.key_int(k) ? bitsd(ev.ikey_k(k, x, r)) : ev.key_k(k, x, r) + 0.0;
+    h = mix64(h ^ (u64)dbits(kv[k])) + (u64)k;
+  }
+  return h == kEmpty ? h ^ 1 : h;
+}
+
 // Hash filter + GROUP BY. Group identity is the 64-bit hash of the keys (as in the reference,
 // kernels.cuh:340-347): h = fold(mix64(h ^ bits(key_k)) + k) from 0x9e3779b97f4a7c15.
 // Each block keeps an open-addressing table of `lds_slots` groups in LDS for the whole launch
@@ -1299,53 +1350,118 @@ __device__ void part_count_body(const ScanDesc& d) {
   for (int k = 0; k < kPackKeys; ++k) kmin[k] = kI64Max, kmax[k] = kI64Min;
   const i64 c = blockIdx.x;
   const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
-  BatchCursor bc;
-  for (i64 t = t0; t < t1; ++t) {
-    if (bc.seek(d, t)) {
-      stage_batch(d, bc.b, scols);  // also orders the histogram reset before the first update
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+  // U rows per thread per step, evaluated before any LDS update so their loads overlap (the
+  // interpreter's out-of-line VM calls gain nothing from it: one row there)
+  constexpr int U = Ev::kRows > 1 ? kPartRows : 1;
+  auto widen = [&](const double* kv) {
+#pragma unroll
+    for (int k = 0; k < kPackKeys; ++k) {
+      if (k >= NK || k >= Ev::kKeys) break;
+      const i64 v = dbits(kv[k]);
+      kmin[k] = v < kmin[k] ? v : kmin[k];
+      kmax[k] = v > kmax[k] ? v : kmax[k];
     }
-    const TileRows tr = bc.rows(t);
-    const uint32_t* del = bc.del;
-    // U rows per thread per step, evaluated before any LDS update so their loads overlap (the
-    // interpreter's out-of-line VM calls gain nothing from it: one row there)
-    constexpr int U = Ev::kRows > 1 ? kPartRows : 1;
-    for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
+  };
+  BatchCursor bc;
+  if constexpr (KeySplit<Ev>::value) {
+    // Compiled plans in load and compute halves: tile t + 1's WHERE and key words load while tile
+    // t's rows are hashed and counted (one tile = one step of U rows per thread; the loop
+    // alternates two register sets, as a copy between them would wait for the loads).
+    struct Step {
+      typename Ev::Row w[U];
+      typename Ev::KRow k[U];
+    };
+    auto load_step = [&](const TileRows& tr, Step& x) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 ii = threadIdx.x + (i64)u * blockDim.x;
+        const i64 r = tr.first + (ii < tr.count ? ii : 0);
+        ev.load_row(r, x.w[u]);
+        ev.load_krow(r, x.k[u]);
+      }
+    };
+    bool have = false;
+    auto tile = [&](i64 t, Step& xc, Step& xn) {
+      if (bc.seek(d, t)) {
+        have = false;
+        stage_batch(d, bc.b, scols);  // also orders the histogram reset before the first update
+        ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+      }
+      const TileRows tr = bc.rows(t);
+      if (bc.del || bc.tr != (i64)blockDim.x * U) {  // (block-uniform) row by row
+        have = false;
+        for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+          const i64 r = tr.first + i;
+          if (is_deleted(bc.del, r)) continue;
+          double kv[Ev::kKeys];
+          const u64 h = row_keys(ev, NK, r, kv);
+          if (ev.has_where() && !ev.where(r)) continue;
+          if (ranged) widen(kv);
+          __hip_atomic_fetch_add(&hist[h >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
+        }
+        return;
+      }
+      if (!have) load_step(tr, xc);
+      // the next tile of this batch (else this tile again: the loads stay unconditional)
+      const bool nxt = t + 1 < t1 && t + 1 < bc.t_hi;
+      load_step(nxt ? bc.rows(t + 1) : tr, xn);
       u64 h[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const i64 ii = i + (i64)u * blockDim.x;
-        const i64 r = tr.first + ii;
-        h[u] = kEmpty;
+        const i64 ii = threadIdx.x + (i64)u * blockDim.x;
+        const i64 r = tr.first + (ii < tr.count ? ii : 0);
         double kv[Ev::kKeys];
-        if (Ev::kPureKeys && Ev::kPureWhere && !del) {
-          // (block-uniform) no side effects, no deleted rows: evaluated without a branch (rows
-          // past the tile re-read its first row), so the step's rows keep their loads together
-          const i64 rr = tr.first + (ii < tr.count ? ii : 0);
-          const u64 hk = row_keys(ev, NK, rr, kv);
-          const bool w = !ev.has_where() || ev.where(rr);
-          h[u] = ii < tr.count && w ? hk : kEmpty;
-        } else if (Ev::kPureKeys) {  // key loads issued with the WHERE loads, not after them
-          if (ii < tr.count) {
-            const u64 hk = row_keys(ev, NK, r, kv);
-            if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
-          }
-        } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
-          h[u] = row_keys(ev, NK, r, kv);
-        }
-        if (ranged && h[u] != kEmpty) {
-#pragma unroll
-          for (int k = 0; k < kPackKeys; ++k) {
-            if (k >= NK || k >= Ev::kKeys) break;
-            const i64 v = dbits(kv[k]);
-            kmin[k] = v < kmin[k] ? v : kmin[k];
-            kmax[k] = v > kmax[k] ? v : kmax[k];
-          }
-        }
+        const u64 hk = row_keys_k(ev, NK, xc.k[u], r, kv);
+        h[u] = ii < tr.count && ev.where_row(xc.w[u], r) ? hk : kEmpty;
+        if (ranged && h[u] != kEmpty) widen(kv);
       }
+      have = nxt;
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (h[u] != kEmpty) __hip_atomic_fetch_add(&hist[h[u] >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
+    };
+    Step xa, xb;
+    for (i64 t = t0; t < t1; t += 2) {
+      tile(t, xa, xb);
+      if (t + 1 < t1) tile(t + 1, xb, xa);
+    }
+  } else {
+    for (i64 t = t0; t < t1; ++t) {
+      if (bc.seek(d, t)) {
+        stage_batch(d, bc.b, scols);  // also orders the histogram reset before the first update
+        ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+      }
+      const TileRows tr = bc.rows(t);
+      const uint32_t* del = bc.del;
+      for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
+        u64 h[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const i64 ii = i + (i64)u * blockDim.x;
+          const i64 r = tr.first + ii;
+          h[u] = kEmpty;
+          double kv[Ev::kKeys];
+          if (Ev::kPureKeys && Ev::kPureWhere && !del) {
+            // (block-uniform) no side effects, no deleted rows: evaluated without a branch (rows
+            // past the tile re-read its first row), so the step's rows keep their loads together
+            const i64 rr = tr.first + (ii < tr.count ? ii : 0);
+            const u64 hk = row_keys(ev, NK, rr, kv);
+            const bool w = !ev.has_where() || ev.where(rr);
+            h[u] = ii < tr.count && w ? hk : kEmpty;
+          } else if (Ev::kPureKeys) {  // key loads issued with the WHERE loads, not after them
+            if (ii < tr.count) {
+              const u64 hk = row_keys(ev, NK, r, kv);
+              if (!is_deleted(del, r) && (!ev.has_where() || ev.where(r))) h[u] = hk;
+            }
+          } else if (ii < tr.count && !is_deleted(del, r) && (!ev.has_where() || ev.where(r))) {
+            h[u] = row_keys(ev, NK, r, kv);
+          }
+          if (ranged && h[u] != kEmpty) widen(kv);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (h[u] != kEmpty) __hip_atomic_fetch_add(&hist[h[u] >> shift], 1u, __ATOMIC_RELAXED, WS_LDS);
+      }
     }
   }
   if (ranged) {  // one LDS atomic per wave and bound
@@ -1433,25 +1549,54 @@ __device__ void part_scatter_body(const ScanDesc& d) {
     for (int a = 0; a < NA; ++a) pacc[(i64)i * NA + a] = slot_identity(ev.agg_bits(a), ev.op(a));
   }
   const i64 t0 = c * d.chunk_tiles, t1 = t0 + d.chunk_tiles < d.ntiles ? t0 + d.chunk_tiles : d.ntiles;
-  BatchCursor bc;
-  for (i64 t = t0; t < t1; ++t) {
-    if (bc.seek(d, t)) {
-      stage_batch(d, bc.b, scols);  // also orders the table set-up before the first row
-      ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+  // Ev::kRows rows per thread per step: all are evaluated (their loads overlap) before the LDS
+  // table and the record writes see them.
+  constexpr int U = Ev::kRows > 1 && Ev::kKeys + Ev::kAggs <= 6 ? kPartRows : 1;  // register budget
+  // A run that would overflow is noted in a register and reported once at the end: a store through
+  // the descriptor's generic pointer inside the loop would be a flat store, after which the
+  // compiler waits for every load in flight (the next tile's included) at the next use of any.
+  bool perr = false;
+  // One row into the pre-aggregation table (two probes; a plain LDS read first, so a full table
+  // costs reads, not compare-and-swaps), else a record in its partition's run.
+  auto insert = [&](u64 h, const double* kv, const double* av) {
+    unsigned s = (unsigned)(h & (u64)(S - 1));
+    for (int probe = 0; probe < 2; ++probe) {
+      u64 prev = __hip_atomic_load(&pk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      bool claimed = false;
+      if (prev == kEmpty) {
+        claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (claimed)
+          for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[k];
+      }
+      if (claimed || prev == h) {
+#pragma unroll
+        for (int a = 0; a < Ev::kAggs; ++a) {
+          if (a >= NA) break;
+          atomic_combine_slot<WS_LDS>(ev.agg_bits(a), ev.op(a), &pacc[(i64)s * NA + a], av[a]);
+        }
+        return;
+      }
+      s = (s + 1) & (unsigned)(S - 1);
     }
-    const TileRows tr = bc.rows(t);
-    const uint32_t* del = bc.del;
-    // Ev::kRows rows per thread per step: all are evaluated (their loads overlap) before the LDS
-    // table and the record writes see them.
-    constexpr int U = Ev::kRows > 1 && Ev::kKeys + Ev::kAggs <= 6 ? kPartRows : 1;  // register budget
+    const unsigned p = (unsigned)(h >> shift);
+    const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
+    if (pos >= end_pos[p]) {
+      perr = true;
+      return;
+    }
+    put_record(L, recs + (i64)pos * W, h, NK, NA, kv, av);
+  };
+  // The rows of tile tr, steps of U rows per thread (the evaluation of a step's rows issues all
+  // their loads before any result is needed when the plan allows).
+  auto tile_rows = [&](const TileRows& tr, const uint32_t* del) {
     for (i64 i = threadIdx.x; i < tr.count; i += (i64)blockDim.x * U) {
       double kv[U][Ev::kKeys];
       double av[U][Ev::kAggs];
       u64 h[U];
       if (Ev::kPureKeys && Ev::kPureVals && Ev::kPureWhere && !del) {
         // (block-uniform) No side effects and no deleted rows: every row of the step is evaluated
-        // (rows past the tile re-read its first row) with no branch between them, so the loads
-        // of all U rows issue before any result is needed.
+        // (rows past the tile re-read its first row) with no branch between them.
         bool w[U];
         u64 hk[U];
 #pragma unroll
@@ -1483,40 +1628,91 @@ __device__ void part_scatter_body(const ScanDesc& d) {
         }
       }
 #pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (h[u] != kEmpty) insert(h[u], kv[u], av[u]);
+    }
+  };
+  BatchCursor bc;
+  if constexpr (RowSplit<Ev>::value) {
+    // Compiled plans in load and compute halves: while tile t's rows go into the table and their
+    // records are written, tile t + 1's column words are already loading (its loads are issued
+    // before this tile's record stores, so waiting for them does not wait for those stores). A tile
+    // is one step (tile rows = U per thread); the loop alternates two register sets (a copy from
+    // one to the other would wait for the loads).
+    struct Step {
+      typename Ev::Row w[U];
+      typename Ev::VRow v[U];
+    };
+    auto load_step = [&](const TileRows& tr, Step& x) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 ii = threadIdx.x + (i64)u * blockDim.x;
+        const i64 r = tr.first + (ii < tr.count ? ii : 0);
+        ev.load_row(r, x.w[u]);
+        ev.load_vrow(r, x.v[u]);
+      }
+    };
+    bool have = false;
+    auto tile = [&](i64 t, Step& xc, Step& xn) {
+      if (bc.seek(d, t)) {
+        have = false;
+        stage_batch(d, bc.b, scols);  // also orders the table set-up before the first row
+        ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+      }
+      const TileRows tr = bc.rows(t);
+      if (bc.del || bc.tr != (i64)blockDim.x * U) {  // (block-uniform)
+        // row by row (both register sets stay live across this branch: no room for U rows)
+        have = false;
+        for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
+          const i64 r = tr.first + i;
+          if (is_deleted(bc.del, r)) continue;
+          double kv[Ev::kKeys], av[Ev::kAggs];
+          const u64 h = row_keys(ev, NK, r, kv);
+          row_values(ev, plan, NA, r, av);
+          if (!ev.has_where() || ev.where(r)) insert(h, kv, av);
+        }
+        return;
+      }
+      if (!have) load_step(tr, xc);
+      // the next tile of this batch (else this tile again: the loads stay unconditional, since a
+      // branch around them would make the compiler wait for every load in flight)
+      const bool nxt = t + 1 < t1 && t + 1 < bc.t_hi;
+      load_step(nxt ? bc.rows(t + 1) : tr, xn);
+      have = nxt;
+      // every row's hash and WHERE first (the waits for this tile's words then come before any
+      // record store, which the same counter would otherwise make them wait for), then one row at
+      // a time into the table (only one row's keys and values live at once)
+      u64 h[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 ii = threadIdx.x + (i64)u * blockDim.x;
+        const i64 r = tr.first + (ii < tr.count ? ii : 0);
+        double kv[Ev::kKeys];
+        const u64 hk = row_keys_v(ev, NK, xc.v[u], r, kv);
+        h[u] = ii < tr.count && ev.where_row(xc.w[u], r) ? hk : kEmpty;
+      }
+#pragma unroll
       for (int u = 0; u < U; ++u) {
         if (h[u] == kEmpty) continue;
-        // Two probes; a plain LDS read first, so a full table costs reads, not compare-and-swaps.
-        unsigned s = (unsigned)(h[u] & (u64)(S - 1));
-        bool done = false;
-        for (int probe = 0; probe < 2; ++probe) {
-          u64 prev = __hip_atomic_load(&pk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          bool claimed = false;
-          if (prev == kEmpty) {
-            claimed = __hip_atomic_compare_exchange_strong(&pk[s], &prev, h[u], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (claimed)
-              for (int k = 0; k < NK; ++k) pkv[(i64)s * NK + k] = kv[u][k];
-          }
-          if (claimed || prev == h[u]) {
-#pragma unroll
-            for (int a = 0; a < Ev::kAggs; ++a) {
-              if (a >= NA) break;
-              atomic_combine_slot<WS_LDS>(ev.agg_bits(a), ev.op(a), &pacc[(i64)s * NA + a], av[u][a]);
-            }
-            done = true;
-            break;
-          }
-          s = (s + 1) & (unsigned)(S - 1);
-        }
-        if (done) continue;
-        const unsigned p = (unsigned)(h[u] >> shift);
-        const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
-        if (pos >= end_pos[p]) {
-          *d.part_err = 1;
-          continue;
-        }
-        put_record(L, recs + (i64)pos * W, h[u], NK, NA, kv[u], av[u]);
+        const i64 r = tr.first + threadIdx.x + (i64)u * blockDim.x;
+        double kv[Ev::kKeys], av[Ev::kAggs];
+        row_keys_v(ev, NK, xc.v[u], r, kv);
+        row_values_v(ev, NA, xc.v[u], r, av);
+        insert(h[u], kv, av);
       }
+    };
+    Step xa, xb;
+    for (i64 t = t0; t < t1; t += 2) {
+      tile(t, xa, xb);
+      if (t + 1 < t1) tile(t + 1, xb, xa);
+    }
+  } else {
+    for (i64 t = t0; t < t1; ++t) {
+      if (bc.seek(d, t)) {
+        stage_batch(d, bc.b, scols);  // also orders the table set-up before the first row
+        ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
+      }
+      tile_rows(bc.rows(t), bc.del);
     }
   }
   __syncthreads();
@@ -1526,11 +1722,12 @@ __device__ void part_scatter_body(const ScanDesc& d) {
     const unsigned p = (unsigned)(h >> shift);
     const uint32_t pos = __hip_atomic_fetch_add(&cur_pos[p], 1u, __ATOMIC_RELAXED, WS_LDS);
     if (pos >= end_pos[p]) {
-      *d.part_err = 1;
+      perr = true;
       continue;
     }
     put_record(L, recs + (i64)pos * W, h, NK, NA, &pkv[(i64)i * NK], &pacc[(i64)i * NA]);
   }
+  if (perr) *d.part_err = 1;
   __syncthreads();
   for (int p = threadIdx.x; p < P; p += blockDim.x)
     for (uint32_t pos = cur_pos[p]; pos < end_pos[p]; ++pos) recs[(i64)pos * W] = kEmpty;

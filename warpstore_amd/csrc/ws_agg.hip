@@ -72,8 +72,14 @@ __global__ void init_table_k(GTable t, int nk, int na, ws_agg_plan plan) {
   }
 }
 __global__ void __launch_bounds__(256) batch_init_k(const BatchQuery* qs) { batch_init_body(qs); }
-__global__ void init_dense_k(double* acc, int64_t G, int na, ws_agg_plan plan) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < G * na; i += (int64_t)gridDim.x * blockDim.x) {
+// An aggregator's initial values in one launch: flags and group counter cleared, dense
+// accumulators (acc may be null) at their slots' identities.
+__global__ void agg_prepare_k(uint32_t* flags, u64* count, double* acc, int64_t G, int na, ws_agg_plan plan) {
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i0 < 4) flags[i0] = 0;
+  if (i0 == 0) *count = 0;
+  if (!acc) return;
+  for (int64_t i = i0; i < G * na; i += (int64_t)gridDim.x * blockDim.x) {
     const int a = (int)(i % na);
     acc[i] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
   }
@@ -866,13 +872,10 @@ static void ensure_table(ws_agg* A, int64_t slots) {
 static void prepare(ws_agg* A) {
   if (!A->pending_init) return;
   A->pending_init = false;
-  fill_u32((uint32_t*)A->flags, 4, 0);
-  fill_u64(A->count, 1, 0);
-  if (A->is_dense) {
-    hipLaunchKernelGGL(init_dense_k, dim3(grid_for(A->cap * std::max(A->plan.naggs, 1))), dim3(256), 0, stream(), A->acc, A->cap,
-                       A->plan.naggs, A->plan);
-    check_launch("init_dense");
-  }
+  const int64_t n = A->is_dense ? A->cap * std::max(A->plan.naggs, 1) : 4;
+  hipLaunchKernelGGL(agg_prepare_k, dim3(grid_for(std::max<int64_t>(n, 4))), dim3(256), 0, stream(), (uint32_t*)A->flags, A->count,
+                     A->is_dense ? A->acc : nullptr, A->cap, A->plan.naggs, A->plan);
+  check_launch("agg_prepare");
 }
 // Reads the aggregation's small results in one round trip (cached until the next launch).
 // Dense accumulators up to this size come back with the status read (one round trip per fetch).

@@ -221,13 +221,21 @@ struct Stage {
 };
 static thread_local Stage t_stage;
 
+// Initial capacity of each ring buffer, all allocated at a thread's first upload (a page-locked
+// allocation costs tens of microseconds: none should land inside a query once the ring is warm).
+constexpr size_t kStageInitBytes = 128 << 10;
 uint8_t* pinned_stage(size_t bytes) {
   ensure_init();
   Stage& st = t_stage;
+  if (!st.slot[0].ev)
+    for (StageSlot& s : st.slot) {
+      WS_CK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+      s.cap = kStageInitBytes;
+      WS_CK(hipHostMalloc((void**)&s.buf, s.cap, hipHostMallocDefault));
+    }
   StageSlot& s = st.slot[st.next];
   st.cur = st.next;
   st.next = (st.next + 1) % kStageSlots;
-  if (!s.ev) WS_CK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
   if (s.pending) {
     WS_CK(hipEventSynchronize(s.ev));
     s.pending = false;
@@ -235,7 +243,7 @@ uint8_t* pinned_stage(size_t bytes) {
   if (bytes > s.cap) {
     if (s.buf) WS_CK(hipHostFree(s.buf));
     s.buf = nullptr;
-    s.cap = std::max<size_t>(bytes, 64 << 10);
+    s.cap = std::max<size_t>(bytes, kStageInitBytes);
     WS_CK(hipHostMalloc((void**)&s.buf, s.cap, hipHostMallocDefault));
   }
   return s.buf;

@@ -116,15 +116,14 @@ struct Comp {
   static I op(int o) { return I{(uint8_t)o, 0, 0}; }
 };
 
-// Per batch: the env with the slots bound to the segment's columns (bindView, engine_core.h:4073).
-ExprEnv bind_segment(const Comp& c, const DevTable& t, const std::vector<int>& slot_col, size_t seg) {
-  ExprEnv e = c.env;
+// Per batch: the slots of e (the plan's environment, copied once per loop of batches) bound to
+// segment seg's columns (bindView, engine_core.h:4073).
+void bind_segment(ExprEnv& e, const Comp& c, const DevTable& t, const std::vector<int>& slot_col, size_t seg) {
   for (size_t s = 0; s < slot_col.size(); ++s) {
     ColView v = t.view(slot_col[s], seg);
     v.t = c.env.cols[s].t;
     e.cols[s] = v;
   }
-  return e;
 }
 
 int dense_kind(int op, bool is_int) {
@@ -221,8 +220,9 @@ int64_t query_c1(const DevTable& t, int64_t* out_i, double* out_d, int64_t cap_r
   int64_t g = 0;
   for (;;) {
     GpuAggregator agg(plan, cap, dense ? &ds : nullptr);
+    ExprEnv e = c.env;
     for (size_t s = 0; s < t.seg_rows.size(); ++s) {
-      ExprEnv e = bind_segment(c, t, slot_col, s);
+      bind_segment(e, c, t, slot_col, s);
       agg.addBatch(e, c.ncols, t.seg_rows[s], nullptr);
     }
     agg.run(c.env, jit);
@@ -402,8 +402,9 @@ int64_t query_c2(const DevTable& t, int64_t* out, int64_t limit, C2Stats* st, in
     GpuAggregator a1(plan1, cap1, nullptr);
     ws_agg_set_features(a1.handle(), f1);
     if (pred1 > 0) wsd::ck(ws_agg_set_strategy(a1.handle(), WS_STRAT_AUTO, pred1 + pred1 / 4));
+    ExprEnv e = c.env;
     for (size_t s = 0; s < t.seg_rows.size(); ++s) {
-      ExprEnv e = bind_segment(c, t, slot_col, s);
+      bind_segment(e, c, t, slot_col, s);
       a1.addBatch(e, c.ncols, t.seg_rows[s], nullptr);
     }
     a1.run(c.env, jit1);
@@ -541,10 +542,11 @@ int64_t materialize(const DevTable& t, Comp& c, bool has_where, Prog where, cons
   std::vector<int64_t> bn;
   std::vector<const uint32_t*> bdel;
   std::vector<ColView> bcols;
+  ExprEnv e = c.env;
   for (size_t s = 0; s < t.seg_rows.size(); ++s) {
     bn.push_back(t.seg_rows[s]);
     bdel.push_back(nullptr);
-    ExprEnv e = bind_segment(c, t, slot_col, s);
+    bind_segment(e, c, t, slot_col, s);
     for (int k = 0; k < c.ncols; ++k) bcols.push_back(e.cols[k]);
   }
   DevVec<int32_t> sel;
@@ -664,10 +666,11 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
     std::vector<int64_t> bn;
     std::vector<const uint32_t*> bdel;
     std::vector<ColView> bcols;
+    ExprEnv e = cd.env;
     for (size_t sg = 0; sg < dim.seg_rows.size(); ++sg) {
       bn.push_back(dim.seg_rows[sg]);
       bdel.push_back(nullptr);
-      ExprEnv e = bind_segment(cd, dim, {0, 1}, sg);
+      bind_segment(e, cd, dim, {0, 1}, sg);
       for (int k = 0; k < cd.ncols; ++k) bcols.push_back(e.cols[k]);
     }
     AggPlan kp;  // the selection plan of the filter, with the key as keys[0] (compiled key bitmap kernel)
@@ -690,8 +693,9 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
     C3Agg A(bits.p, lo, nbits);
     const std::vector<int> slot_col = {3, 2, 1, 0};  // flag, price, qty, fk
     const int64_t g = A.run(fact.cols[3], fact.rows, "aggregate fact (join as filter)", [&](GpuAggregator& agg) {
+      ExprEnv e = A.c.env;
       for (size_t sg = 0; sg < fact.seg_rows.size(); ++sg) {
-        ExprEnv e = bind_segment(A.c, fact, slot_col, sg);
+        bind_segment(e, A.c, fact, slot_col, sg);
         agg.addBatch(e, A.c.ncols, fact.seg_rows[sg], nullptr);
       }
     }, out);
@@ -723,8 +727,9 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
     C3Agg A(bits.p, lo, nbits);
     const std::vector<int> slot_col = {3, 2, 1, 0};  // flag, price, qty, fk
     const int64_t g = A.run(fact.cols[3], fact.rows, "aggregate fact (join as filter)", [&](GpuAggregator& agg) {
+      ExprEnv e = A.c.env;
       for (size_t sg = 0; sg < fact.seg_rows.size(); ++sg) {
-        ExprEnv e = bind_segment(A.c, fact, slot_col, sg);
+        bind_segment(e, A.c, fact, slot_col, sg);
         agg.addBatch(e, A.c.ncols, fact.seg_rows[sg], nullptr);
       }
     }, out);
@@ -879,8 +884,9 @@ uint64_t c4_alone(const C4Query& q, int64_t* groups, bool digest = true) {
   int64_t cap = p.cap, g = 0;
   for (;;) {
     GpuAggregator agg(p.plan, cap, p.dense ? &p.ds : nullptr);
+    ExprEnv e = p.c.env;
     for (size_t s = 0; s < q.t->seg_rows.size(); ++s) {
-      ExprEnv e = bind_segment(p.c, *q.t, kC4Slots, s);
+      bind_segment(e, p.c, *q.t, kC4Slots, s);
       agg.addBatch(e, p.c.ncols, q.t->seg_rows[s], nullptr);
     }
     agg.run(p.c.env, jit);
@@ -958,8 +964,9 @@ void c4_batched(const std::vector<const C4Query*>& qs, std::vector<uint64_t>* di
   for (size_t i : dense_idx) {
     const C4Plan& p = plans[i];
     aggs.emplace_back(new GpuAggregator(p.plan, p.cap, &p.ds));
+    ExprEnv e = p.c.env;
     for (size_t s = 0; s < qs[i]->t->seg_rows.size(); ++s) {
-      ExprEnv e = bind_segment(p.c, *qs[i]->t, kC4Slots, s);
+      bind_segment(e, p.c, *qs[i]->t, kC4Slots, s);
       aggs.back()->addBatch(e, p.c.ncols, qs[i]->t->seg_rows[s], nullptr);
     }
     handles.push_back(aggs.back()->handle());
@@ -1005,8 +1012,9 @@ C4Prepared* c4_prepare(const std::vector<const C4Query*>& qs) {
   for (size_t i = 0; i < qs.size(); ++i) {
     const C4Plan& p = P->plans[i];
     P->aggs.emplace_back(new GpuAggregator(p.plan, p.cap, &p.ds));
+    ExprEnv e = p.c.env;
     for (size_t s = 0; s < qs[i]->t->seg_rows.size(); ++s) {
-      ExprEnv e = bind_segment(p.c, *qs[i]->t, kC4Slots, s);
+      bind_segment(e, p.c, *qs[i]->t, kC4Slots, s);
       P->aggs.back()->addBatch(e, p.c.ncols, qs[i]->t->seg_rows[s], nullptr);
     }
     handles.push_back(P->aggs.back()->handle());

@@ -121,34 +121,61 @@ struct FetchQuery {
   ws_dense_spec ds;     // keys are rebuilt from the group index
   int64_t G;
 };
-// Block q: number of groups of query q with rows (slot 0 = COUNT(*) > 0).
-__global__ void __launch_bounds__(256) dense_count_many_k(const FetchQuery* qs, int na, uint32_t* counts) {
-  __shared__ uint32_t s[256];
-  const FetchQuery& Q = qs[blockIdx.x];
-  uint32_t c = 0;
-  for (int64_t g = threadIdx.x; g < Q.G; g += blockDim.x) c += Q.acc[g * na] > 0;
-  s[threadIdx.x] = c;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
+// A batched fetch works in chunks of kFetchChunk groups (many blocks per large query, so the
+// largest query does not set the time): chunk j covers groups [g0, g0 + kFetchChunk) of query q.
+constexpr int64_t kFetchChunk = 4096;
+struct FetchChunk {
+  int32_t q, pad;
+  int64_t g0;
+};
+// Chunks of queries with G[0..n) groups, query after query; cb[q] = first chunk of query q.
+static std::vector<FetchChunk> fetch_chunks(const std::vector<FetchQuery>& fq, std::vector<int64_t>& cb) {
+  std::vector<FetchChunk> ch;
+  cb.assign(fq.size() + 1, 0);
+  for (size_t q = 0; q < fq.size(); ++q) {
+    cb[q] = (int64_t)ch.size();
+    for (int64_t g0 = 0; g0 < fq[q].G; g0 += kFetchChunk) ch.push_back(FetchChunk{(int32_t)q, 0, g0});
   }
-  if (threadIdx.x == 0) counts[blockIdx.x] = s[0];
+  cb[fq.size()] = (int64_t)ch.size();
+  return ch;
 }
-// Block q: its groups with rows, ascending group index, row-major keys | accs from row offs[q].
-__global__ void __launch_bounds__(256) dense_pack_many_k(const FetchQuery* qs, ws_agg_plan plan, int nk, int na,
-                                                         const uint32_t* offs, double* out_k, double* out_a) {
+// Block j: number of groups of chunk j with rows (slot 0 = COUNT(*) > 0).
+__global__ void __launch_bounds__(256) dense_count_many_k(const FetchQuery* qs, const FetchChunk* ch, int na, uint32_t* counts) {
   __shared__ uint32_t s_w[4];
-  __shared__ uint32_t s_base;
-  const FetchQuery& Q = qs[blockIdx.x];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_base = offs[blockIdx.x];
+  const FetchChunk c = ch[blockIdx.x];
+  const FetchQuery& Q = qs[c.q];
+  const int64_t g1 = c.g0 + kFetchChunk < Q.G ? c.g0 + kFetchChunk : Q.G;
+  uint32_t n = 0;
+#pragma unroll 16
+  for (int64_t g = c.g0 + threadIdx.x; g < g1; g += blockDim.x) n += Q.acc[g * na] > 0;
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = n;
   __syncthreads();
-  for (int64_t g0 = 0; g0 < Q.G; g0 += blockDim.x) {
-    const int64_t g = g0 + threadIdx.x;
-    const bool has = g < Q.G && Q.acc[g * na] > 0;
-    const u64 m = __ballot(has);
+  if (threadIdx.x == 0) counts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+// Block j: chunk j's groups with rows, ascending group index, row-major keys | accs from row
+// offs[j]. Every thread loads the count slots of its groups of the chunk first (their loads in
+// flight together), then the chunk is ranked 256 groups at a time.
+__global__ void __launch_bounds__(256) dense_pack_many_k(const FetchQuery* qs, const FetchChunk* ch, ws_agg_plan plan, int nk,
+                                                         int na, const uint32_t* offs, double* out_k, double* out_a) {
+  constexpr int kPer = (int)(kFetchChunk / 256);
+  __shared__ uint32_t s_w[4];
+  const FetchChunk c = ch[blockIdx.x];
+  const FetchQuery& Q = qs[c.q];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  bool has[kPer];
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int64_t g = c.g0 + it * 256 + threadIdx.x;
+    has[it] = g < Q.G && Q.acc[g * na] > 0;
+  }
+  uint32_t base = offs[blockIdx.x];
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int64_t g = c.g0 + it * 256 + threadIdx.x;
+    const u64 m = __ballot(has[it]);
     const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    __syncthreads();  // the previous step's s_w reads are done
     if (lane == 0) s_w[wv] = (uint32_t)__popcll(m);
     __syncthreads();
     uint32_t before = 0, total = 0;
@@ -156,17 +183,15 @@ __global__ void __launch_bounds__(256) dense_pack_many_k(const FetchQuery* qs, w
       if (j < wv) before += s_w[j];
       total += s_w[j];
     }
-    if (has) {
-      const int64_t i = (int64_t)(s_base + before + rank);
+    if (has[it]) {
+      const int64_t i = (int64_t)(base + before + rank);
       for (int k = 0; k < nk; ++k) {
         const i64 v = Q.ds.kmin[k] + (g / Q.ds.stride[k]) % Q.ds.range[k];
         out_k[i * nk + k] = plan.key_int[k] ? bitsd(v) : (double)v;
       }
       for (int a = 0; a < na; ++a) out_a[i * na + a] = Q.acc[g * na + a];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) s_base += total;
-    __syncthreads();
+    base += total;
   }
 }
 
@@ -1474,6 +1499,9 @@ struct ws_agg_batch_plan {
   ws_agg_plan plan{};
   BatchBuild b;
   ws::DevBuf<ws::FetchQuery> fq;
+  ws::DevBuf<ws::FetchChunk> ch;  // fetch chunks; cb[q] = first chunk of query q
+  std::vector<int64_t> cb;
+  int64_t nch = 0;
   ws::DevBuf<uint32_t> cnt, offs;
   ws::DevBuf<double> pk, pa;
   int64_t gcap = 0;
@@ -1696,21 +1724,31 @@ int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* tota
         throw Error("batched fetch: every aggregation must be dense with the same plan shape", WS_ERR_INTERNAL);
       fq[q] = FetchQuery{A->acc, A->dense, A->dense.G};
     }
+    std::vector<int64_t> cb;
+    const std::vector<FetchChunk> ch = fetch_chunks(fq, cb);
+    const int64_t nch = (int64_t)ch.size();
+    if (nch == 0) {
+      for (int q = 0; q < n; ++q) counts[q] = 0;
+      return;
+    }
     DevBuf<FetchQuery> dq(n);
+    DevBuf<FetchChunk> dch(nch);
     h2d_async_staged(dq.p, fq.data(), sizeof(FetchQuery) * n);
-    DevBuf<uint32_t> cnt(n), offs(n + 1);
-    hipLaunchKernelGGL(dense_count_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)dq.p, na, cnt.p);
-    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)cnt.p, offs.p, n);
+    h2d_async_staged(dch.p, ch.data(), sizeof(FetchChunk) * nch);
+    DevBuf<uint32_t> cnt(nch), offs(nch + 1);
+    hipLaunchKernelGGL(dense_count_many_k, dim3((unsigned)nch), dim3(256), 0, stream(), (const FetchQuery*)dq.p,
+                       (const FetchChunk*)dch.p, na, cnt.p);
+    hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)cnt.p, offs.p, (int)nch);
     check_launch("fetch_many_count");
-    std::vector<uint32_t> h(n + 1);
-    d2h_sync(h.data(), offs.p, sizeof(uint32_t) * (n + 1));
-    for (int q = 0; q < n; ++q) counts[q] = (int64_t)(h[q + 1] - h[q]);
-    *total = h[n];
+    std::vector<uint32_t> h(nch + 1);
+    d2h_sync(h.data(), offs.p, sizeof(uint32_t) * (nch + 1));
+    for (int q = 0; q < n; ++q) counts[q] = (int64_t)(h[cb[q + 1]] - h[cb[q]]);
+    *total = h[nch];
     if (*total > cap) throw Error("batched fetch: more groups than the buffers hold", WS_ERR_INTERNAL);
     if (!*total) return;
     DevBuf<double> pk(*total * nk), pa(*total * na);
-    hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)dq.p, aggs[0]->plan, nk,
-                       na, (const uint32_t*)offs.p, pk.p, pa.p);
+    hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)nch), dim3(256), 0, stream(), (const FetchQuery*)dq.p,
+                       (const FetchChunk*)dch.p, aggs[0]->plan, nk, na, (const uint32_t*)offs.p, pk.p, pa.p);
     check_launch("fetch_many_pack");
     // straight into the caller's arrays (pinned ones, e.g. from ws_pinned_alloc, copy at link speed)
     if (nk) WS_CK(hipMemcpyAsync(keys, pk.p, sizeof(double) * (size_t)(*total * nk), hipMemcpyDeviceToHost, stream()));
@@ -1738,10 +1776,15 @@ int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* e
       P->gcap += aggs[q]->dense.G;
       aggs[q]->pending_init = false;  // every replay gives the tables their initial values
     }
+    const std::vector<FetchChunk> ch = fetch_chunks(fq, P->cb);
+    P->nch = (int64_t)ch.size();
+    if (P->nch == 0) throw Error("prepared batch: no groups", WS_ERR_INTERNAL);
     P->fq = DevBuf<FetchQuery>(n);
     h2d_sync(P->fq.p, fq.data(), sizeof(FetchQuery) * n);
-    P->cnt = DevBuf<uint32_t>(n);
-    P->offs = DevBuf<uint32_t>(n + 1);
+    P->ch = DevBuf<FetchChunk>(P->nch);
+    h2d_sync(P->ch.p, ch.data(), sizeof(FetchChunk) * P->nch);
+    P->cnt = DevBuf<uint32_t>(P->nch);
+    P->offs = DevBuf<uint32_t>(P->nch + 1);
     P->pk = DevBuf<double>(std::max<int64_t>(P->gcap * P->nk, 1));
     P->pa = DevBuf<double>(std::max<int64_t>(P->gcap * P->na, 1));
     WS_CK(hipStreamSynchronize(stream()));
@@ -1750,11 +1793,11 @@ int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* e
     try {
       StreamOverride so(P->cs);
       launch_batch(P->b, n, false);
-      hipLaunchKernelGGL(dense_count_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)P->fq.p, P->na,
-                         P->cnt.p);
-      hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)P->cnt.p, P->offs.p, n);
-      hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)n), dim3(256), 0, stream(), (const FetchQuery*)P->fq.p, P->plan,
-                         P->nk, P->na, (const uint32_t*)P->offs.p, P->pk.p, P->pa.p);
+      hipLaunchKernelGGL(dense_count_many_k, dim3((unsigned)P->nch), dim3(256), 0, stream(), (const FetchQuery*)P->fq.p,
+                         (const FetchChunk*)P->ch.p, P->na, P->cnt.p);
+      hipLaunchKernelGGL(part_start_k, dim3(1), dim3(1024), 0, stream(), (const uint32_t*)P->cnt.p, P->offs.p, (int)P->nch);
+      hipLaunchKernelGGL(dense_pack_many_k, dim3((unsigned)P->nch), dim3(256), 0, stream(), (const FetchQuery*)P->fq.p,
+                         (const FetchChunk*)P->ch.p, P->plan, P->nk, P->na, (const uint32_t*)P->offs.p, P->pk.p, P->pa.p);
       check_launch("batch_plan_capture");
     } catch (...) {
       hipGraph_t g = nullptr;
@@ -1771,10 +1814,10 @@ int ws_agg_batch_replay(ws_agg_batch_plan* P, int64_t* counts, int64_t* total, i
   return guard([&] {
     *total = 0;
     WS_CK(hipGraphLaunch(P->exec, stream()));
-    std::vector<uint32_t> h((size_t)P->n + 1);
+    std::vector<uint32_t> h((size_t)P->nch + 1);
     d2h_sync(h.data(), P->offs.p, sizeof(uint32_t) * h.size());
-    for (int q = 0; q < P->n; ++q) counts[q] = (int64_t)(h[q + 1] - h[q]);
-    *total = h[P->n];
+    for (int q = 0; q < P->n; ++q) counts[q] = (int64_t)(h[P->cb[q + 1]] - h[P->cb[q]]);
+    *total = h[P->nch];
     if (*total > cap) throw Error("prepared batch: more groups than the buffers hold", WS_ERR_INTERNAL);
     if (!*total) return;
     if (P->nk) WS_CK(hipMemcpyAsync(keys, P->pk.p, sizeof(double) * (size_t)(*total * P->nk), hipMemcpyDeviceToHost, stream()));

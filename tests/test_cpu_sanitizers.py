@@ -60,7 +60,8 @@ host = H.Bound(t, [("k", abi.T_I32, 0), ("v", abi.T_I64, 0), ("p", abi.T_F64, 0)
 for dense in (False, True):
     H.oracle_aggregate(plan, b.env, host, dense=dense, max_groups=2000)
 out = np.zeros(len(k), np.int32); per = np.zeros(3, np.int64)
-O.or_select(C.byref(b.env), 1, plan.where, 3, ws.ptr(np.array(t.sizes, np.int64)), None, C.cast(host.views, C.c_void_p), 3,
+sizes = np.array(t.sizes, np.int64)
+O.or_select(C.byref(b.env), 1, plan.where, 3, ws.ptr(sizes), None, C.cast(host.views, C.c_void_p), 3,
             ws.ptr(out), ws.ptr(per))
 bk = dkey[:2000].astype(np.int32); pk = fk[:20000].astype(np.int32)
 bv, pv = abi.ColView(), abi.ColView()

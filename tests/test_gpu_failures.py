@@ -227,8 +227,9 @@ def test_aggregation_handle_moves_between_threads(gpu):
     gpu.ws_tmp_free(box["p"])
     q = C.c_void_p()
     ws.check(gpu.ws_tmp_alloc(pat.nbytes, C.byref(q)))
-    ws.check(gpu.ws_h2d(q, ws.ptr(pat[::-1].copy()), pat.nbytes))
+    rev = pat[::-1].copy()  # held in a name: ws.ptr of a temporary would point at freed memory
+    ws.check(gpu.ws_h2d(q, ws.ptr(rev), pat.nbytes))
     ws.check(gpu.ws_d2h(ws.ptr(back), q, pat.nbytes))
-    assert np.array_equal(back, pat[::-1])
+    assert np.array_equal(back, rev)
     gpu.ws_tmp_free(q)
     gpu.ws_trim_pool()

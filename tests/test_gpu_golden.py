@@ -282,7 +282,8 @@ def test_checked_overflow_raises_flag(gpu, mode):
     da = ws.DevBuf.from_array(a)
     flag = ws.DevBuf.from_array(np.zeros(1, np.int32))
     for big, want in ((4, 1), (1, 0)):
-        ws.check(gpu.ws_h2d(flag.p, ws.ptr(np.zeros(1, np.int32)), 4))
+        zero = np.zeros(1, np.int32)
+        ws.check(gpu.ws_h2d(flag.p, ws.ptr(zero), 4))
         b = abi.EnvBuilder()
         s = b.slot(abi.T_I64)
         b.env.cols[s].p = da.p

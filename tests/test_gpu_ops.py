@@ -607,8 +607,9 @@ def test_key_bitmap_where_matches_materialized_bitmap(gpu, deletes):
     jit = C.c_void_p()
     ws.check(gpu.ws_jit_get(C.byref(b.env), 2, C.byref(kp), abi.JF_FLAGM, b"key bitmap", C.byref(jit)))
     assert jit.value
+    sizes = np.array(t.sizes, np.int64)
     for plan in (None, jit):  # the interpreter, then the compiled plan
-        ws.check(gpu.ws_key_bitmap_where(C.byref(b.env), 1, where, kprog, nb, ws.ptr(np.array(t.sizes, np.int64)),
+        ws.check(gpu.ws_key_bitmap_where(C.byref(b.env), 1, where, kprog, nb, ws.ptr(sizes),
                                          C.cast(dev.dels, C.c_void_p), C.cast(dev.views, C.c_void_p), 2, lo, nbits, bits.p,
                                          cnt.p, plan))
         got = bits.to_array(np.uint32, (nbits + 31) // 32)
@@ -617,7 +618,7 @@ def test_key_bitmap_where_matches_materialized_bitmap(gpu, deletes):
     # a floating-point key is refused
     sp = b.slot(abi.T_F64)
     fprog = b.prog([(abi.OP_COL, sp)])
-    rc = gpu.ws_key_bitmap_where(C.byref(b.env), 1, where, fprog, nb, ws.ptr(np.array(t.sizes, np.int64)),
+    rc = gpu.ws_key_bitmap_where(C.byref(b.env), 1, where, fprog, nb, ws.ptr(sizes),
                                  C.cast(dev.dels, C.c_void_p), C.cast(dev.views, C.c_void_p), 2, lo, nbits, bits.p, None,
                                  None)
     assert rc == ws.WS_ERR_INTERNAL

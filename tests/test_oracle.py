@@ -379,7 +379,8 @@ def test_select_matches_numpy_mask():
     for has_where, sel in ((1, (res == 1) & ~deleted), (0, ~deleted)):
         out = np.zeros(n, np.int32)
         per = np.zeros(len(sizes), np.int64)
-        m = O.or_select(C.byref(b.env), has_where, where, len(sizes), ws.ptr(np.array(sizes, np.int64)),
+        sz = np.array(sizes, np.int64)
+        m = O.or_select(C.byref(b.env), has_where, where, len(sizes), ws.ptr(sz),
                         C.cast(host.dels, C.c_void_p), C.cast(host.views, C.c_void_p), host.nslots, ws.ptr(out), ws.ptr(per))
         exp = np.nonzero(sel)[0]
         assert np.array_equal(out[:m], exp)
@@ -391,7 +392,8 @@ def test_select_matches_numpy_mask():
     t2 = H.Table({"x": np.array([np.nan, 0.0, -0.0, 2.5, -1e-300])}, 5)
     h2 = H.Bound(t2, [("x", abi.T_F64, 0)], False)
     out = np.zeros(5, np.int32)
-    m = O.or_select(C.byref(b.env), 1, b.col(sx), 1, ws.ptr(np.array([5], np.int64)), C.cast(h2.dels, C.c_void_p),
+    five = np.array([5], np.int64)
+    m = O.or_select(C.byref(b.env), 1, b.col(sx), 1, ws.ptr(five), C.cast(h2.dels, C.c_void_p),
                     C.cast(h2.views, C.c_void_p), 1, ws.ptr(out), None)
     assert out[:m].tolist() == [3, 4]
 

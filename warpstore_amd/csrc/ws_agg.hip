@@ -84,12 +84,17 @@ __global__ void agg_prepare_k(uint32_t* flags, u64* count, double* acc, int64_t 
     acc[i] = slot_identity(plan.agg_bits[a] != 0, plan.ops[a]);
   }
 }
-__global__ void dense_reduce_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts, int nparts, double* gacc) {
-  dense_reduce_body(plan, ds, parts, nparts, gacc);
+__global__ void __launch_bounds__(kReduceWaves * kWave) dense_reduce_cols_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts,
+                                                                         int nparts, double* gacc) {
+  dense_reduce_cols_body(plan, ds, parts, nparts, gacc);
 }
-__global__ void __launch_bounds__(256) dense_reduce_slot_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts, int nparts,
-                                                           double* gacc) {
-  dense_reduce_slot_body(plan, ds, parts, nparts, gacc);
+// Folds nparts block partials ([part][G * NA]) into the accumulators, in block order.
+static void dense_reduce(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int64_t nparts, double* gacc) {
+  const int64_t S = ds.G * plan.naggs;
+  if (S <= 0 || nparts <= 0) return;
+  hipLaunchKernelGGL(dense_reduce_cols_k, dim3((unsigned)((S + kWave - 1) / kWave)), dim3(kReduceWaves * kWave), 0, stream(), plan,
+                     ds, parts, (int)nparts, gacc);
+  check_launch("agg_dense_reduce");
 }
 // Compacted groups, row-major (fetch) or column-major (compact): rows [off, off + g) of a result
 // of G groups, taken from slots idx[i] of a row-major table (idx null: slot i).
@@ -904,7 +909,7 @@ static void prepare(ws_agg* A) {
 }
 // Reads the aggregation's small results in one round trip (cached until the next launch).
 // Dense accumulators up to this size come back with the status read (one round trip per fetch).
-constexpr size_t kSmallDenseFetchBytes = 32 * 1024;
+constexpr size_t kSmallDenseFetchBytes = 56 * 1024;
 static const ws_agg::Status& status(ws_agg* A) {
   prepare(A);
   if (A->st_valid) return A->st;
@@ -1041,11 +1046,7 @@ static bool run_dense_staged(ws_agg* A, const ws_expr_env& env, const ws_jit_pro
   void* args[] = {&up.ddesc, &pp, &flags};
   jit_launch_raw(jf, (unsigned)grid, kStagedBlock, copies + 4 * (size_t)d.set_words, args, jit_prof_name(jit, "agg_dense_staged"));
   ProfScope ps("agg_dense_reduce");
-  if (S <= 16384 && grid >= 64)
-    hipLaunchKernelGGL(dense_reduce_slot_k, dim3((unsigned)S), dim3(256), 0, stream(), plan, ds, (const i64*)pp, (int)grid, A->acc);
-  else
-    hipLaunchKernelGGL(dense_reduce_k, dim3(grid_for(S)), dim3(kBlock), 0, stream(), plan, ds, (const i64*)pp, (int)grid, A->acc);
-  check_launch("agg_dense_reduce");
+  dense_reduce(plan, ds, (const i64*)pp, grid, A->acc);
   return true;
 }
 
@@ -1293,13 +1294,7 @@ static void agg_run(ws_agg* A, const ws_expr_env& env, const ws_jit_program* jit
     }
     if (dp.mode != 0) {
       ProfScope ps("agg_dense_reduce");
-      if (S <= 16384 && grid >= 64)  // few slots, many partials: a block per slot
-        hipLaunchKernelGGL(dense_reduce_slot_k, dim3((unsigned)S), dim3(256), 0, stream(), plan, ds, (const i64*)pp, (int)grid,
-                           gacc);
-      else
-        hipLaunchKernelGGL(dense_reduce_k, dim3(grid_for(S)), dim3(kBlock), 0, stream(), plan, ds, (const i64*)pp, (int)grid,
-                           gacc);
-      check_launch("agg_dense_reduce");
+      dense_reduce(plan, ds, (const i64*)pp, grid, gacc);
     }
     return;
   }

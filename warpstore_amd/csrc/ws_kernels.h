@@ -853,89 +853,60 @@ __device__ void dense_body(const ScanDesc& d, double* gacc, i64* parts, int* fla
 }
 
 // Folds the dense block partials in block order into gacc (row-major [g][NA]), on top of what
-// earlier windows left there. A block whose count for g is 0 contributes nothing to g, as in the
-// reference's per-block flush, so slot values of groups without rows keep their identities.
-__device__ void dense_reduce_body(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int nparts,
-                                  double* gacc) {
-  const int NA = plan.naggs;
-  const i64 S = ds.G * NA;
-  for (i64 s = blockIdx.x * (i64)blockDim.x + threadIdx.x; s < S; s += (i64)gridDim.x * blockDim.x) {
-    const i64 g = s / NA;
-    const int a = (int)(s - g * NA);
-    const uint8_t k = ds.kind[a];
-    const int op = dk_op(k);
-    const bool bits = plan.agg_bits[a] != 0;
-    if (dk_int(k)) {
-      i64 x = op == WS_AGG_SUM ? 0 : int_identity(op);
-      bool any = false;
-      for (int p = 0; p < nparts; ++p) {
-        if (parts[(i64)p * S + g * NA] == 0) continue;  // the block saw no row of g
-        any = true;
-        x = combine_i(op, x, parts[(i64)p * S + s]);
-      }
-      if (!any) continue;
-      if (bits) gacc[s] = bitsd(combine_i(op, dbits(gacc[s]), x));
-      else gacc[s] = combine_d(a == 0 ? WS_AGG_SUM : op, gacc[s], (double)x);
-    } else {
-      double x = agg_identity(op);
-      bool any = false;
-      for (int p = 0; p < nparts; ++p) {
-        if (parts[(i64)p * S + g * NA] == 0) continue;
-        any = true;
-        x = combine_d(op, x, bitsd(parts[(i64)p * S + s]));
-      }
-      if (!any) continue;
-      gacc[s] = combine_d(op, gacc[s], x);
-    }
-  }
-}
-
-// The same fold with one block per slot (few slots, many partials): the block's threads combine
-// strided partials, then a fixed-order tree in LDS (deterministic for doubles) finishes the slot.
-__device__ void dense_reduce_slot_body(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int nparts,
+// earlier windows left there. A block whose count for g is 0 contributes nothing to g. Lane l of a
+// 1024-thread block owns slot blockIdx.x * 64 + l, wave w folds partials w, w + 16, ... in order
+// (adjacent lanes read adjacent slots of a partial: coalesced), then wave 0 folds the 16 wave
+// results in wave order (a fixed order: deterministic for doubles).
+constexpr int kReduceWaves = 16;
+__device__ void dense_reduce_cols_body(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int nparts,
                                        double* gacc) {
-  __shared__ i64 s_v[256];
-  __shared__ int s_any[256];
+  __shared__ i64 s_v[kReduceWaves][kWave];
+  __shared__ unsigned char s_any[kReduceWaves][kWave];
   const int NA = plan.naggs;
   const i64 S = ds.G * NA;
-  const i64 s = blockIdx.x;
-  const i64 g = s / NA;
-  const int a = (int)(s - g * NA);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const i64 s = (i64)blockIdx.x * kWave + lane;
+  const bool live = s < S;
+  const i64 g = live ? s / NA : 0;
+  const int a = live ? (int)(s - g * NA) : 0;
   const uint8_t k = ds.kind[a];
   const int op = dk_op(k);
   const bool isint = dk_int(k);
   i64 xi = op == WS_AGG_SUM ? 0 : int_identity(op);
   double xd = agg_identity(op);
   bool any = false;
-  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-    if (parts[(i64)p * S + g * NA] == 0) continue;  // the block saw no row of g
-    any = true;
-    if (isint) xi = combine_i(op, xi, parts[(i64)p * S + s]);
-    else xd = combine_d(op, xd, bitsd(parts[(i64)p * S + s]));
-  }
-  s_v[threadIdx.x] = isint ? xi : dbits(xd);
-  s_any[threadIdx.x] = any ? 1 : 0;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w && s_any[threadIdx.x + w]) {
-      const i64 o = s_v[threadIdx.x + w];
-      s_v[threadIdx.x] = !s_any[threadIdx.x] ? o
-                         : isint         ? combine_i(op, s_v[threadIdx.x], o)
-                                         : dbits(combine_d(op, bitsd(s_v[threadIdx.x]), bitsd(o)));
-      s_any[threadIdx.x] = 1;
+  if (live) {
+#pragma unroll 4
+    for (int p = w; p < nparts; p += kReduceWaves) {
+      const i64* pp = parts + (i64)p * S;
+      if (pp[g * NA] == 0) continue;  // the block saw no row of g
+      any = true;
+      if (isint) xi = combine_i(op, xi, pp[s]);
+      else xd = combine_d(op, xd, bitsd(pp[s]));
     }
-    __syncthreads();
   }
-  if (threadIdx.x != 0 || !s_any[0]) return;
-  const bool bits = plan.agg_bits[a] != 0;
+  s_v[w][lane] = isint ? xi : dbits(xd);
+  s_any[w][lane] = any ? 1 : 0;
+  __syncthreads();
+  if (w != 0 || !live) return;
+  bool seen = false;
+  for (int v = 0; v < kReduceWaves; ++v) {
+    if (!s_any[v][lane]) continue;
+    const i64 o = s_v[v][lane];
+    if (!seen) xi = o, xd = bitsd(o);
+    else if (isint) xi = combine_i(op, xi, o);
+    else xd = combine_d(op, xd, bitsd(o));
+    seen = true;
+  }
+  if (!seen) return;
   if (isint) {
-    const i64 x = s_v[0];
-    if (bits) gacc[s] = bitsd(combine_i(op, dbits(gacc[s]), x));
-    else gacc[s] = combine_d(a == 0 ? WS_AGG_SUM : op, gacc[s], (double)x);
+    if (plan.agg_bits[a]) gacc[s] = bitsd(combine_i(op, dbits(gacc[s]), xi));
+    else gacc[s] = combine_d(a == 0 ? WS_AGG_SUM : op, gacc[s], (double)xi);
   } else {
-    gacc[s] = combine_d(op, gacc[s], bitsd(s_v[0]));
+    gacc[s] = combine_d(op, gacc[s], xd);
   }
 }
+
 
 // ------------------------------------------------------------------ hash GROUP BY
 

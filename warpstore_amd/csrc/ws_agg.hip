@@ -88,12 +88,20 @@ __global__ void __launch_bounds__(kReduceWaves * kWave) dense_reduce_cols_k(ws_a
                                                                          int nparts, double* gacc) {
   dense_reduce_cols_body(plan, ds, parts, nparts, gacc);
 }
-// Folds nparts block partials ([part][G * NA]) into the accumulators, in block order.
+__global__ void __launch_bounds__(256) dense_reduce_slot_k(ws_agg_plan plan, ws_dense_spec ds, const i64* parts, int nparts,
+                                                           double* gacc) {
+  dense_reduce_slot_body(plan, ds, parts, nparts, gacc);
+}
+// Folds nparts block partials ([part][G * NA]) into the accumulators, in block order: a block per
+// slot when the slots are few (all of a slot's partials load at once), else (slot, partial) tiles.
 static void dense_reduce(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int64_t nparts, double* gacc) {
   const int64_t S = ds.G * plan.naggs;
   if (S <= 0 || nparts <= 0) return;
-  hipLaunchKernelGGL(dense_reduce_cols_k, dim3((unsigned)((S + kWave - 1) / kWave)), dim3(kReduceWaves * kWave), 0, stream(), plan,
-                     ds, parts, (int)nparts, gacc);
+  if (S <= 2048 && nparts >= 64)
+    hipLaunchKernelGGL(dense_reduce_slot_k, dim3((unsigned)S), dim3(256), 0, stream(), plan, ds, parts, (int)nparts, gacc);
+  else
+    hipLaunchKernelGGL(dense_reduce_cols_k, dim3((unsigned)((S + kWave - 1) / kWave)), dim3(kReduceWaves * kWave), 0, stream(),
+                       plan, ds, parts, (int)nparts, gacc);
   check_launch("agg_dense_reduce");
 }
 // Compacted groups, row-major (fetch) or column-major (compact): rows [off, off + g) of a result

@@ -876,13 +876,24 @@ __device__ void dense_reduce_cols_body(const ws_agg_plan& plan, const ws_dense_s
   double xd = agg_identity(op);
   bool any = false;
   if (live) {
-#pragma unroll 4
-    for (int p = w; p < nparts; p += kReduceWaves) {
-      const i64* pp = parts + (i64)p * S;
-      if (pp[g * NA] == 0) continue;  // the block saw no row of g
-      any = true;
-      if (isint) xi = combine_i(op, xi, pp[s]);
-      else xd = combine_d(op, xd, bitsd(pp[s]));
+    // 8 partials per step, their count and value loads issued before any is used
+    constexpr int U = 8;
+    for (int p0 = w; p0 < nparts; p0 += kReduceWaves * U) {
+      i64 c[U], v[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int p = p0 + j * kReduceWaves;
+        const i64* pp = parts + (i64)(p < nparts ? p : w) * S;
+        c[j] = p < nparts ? pp[g * NA] : 0;
+        v[j] = pp[s];
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (c[j] == 0) continue;  // the block saw no row of g (or past the last partial)
+        any = true;
+        if (isint) xi = combine_i(op, xi, v[j]);
+        else xd = combine_d(op, xd, bitsd(v[j]));
+      }
     }
   }
   s_v[w][lane] = isint ? xi : dbits(xd);
@@ -907,6 +918,53 @@ __device__ void dense_reduce_cols_body(const ws_agg_plan& plan, const ws_dense_s
   }
 }
 
+
+// The same fold with one block per slot (few slots, many partials): the block's threads combine
+// strided partials, then a fixed-order tree in LDS (deterministic for doubles) finishes the slot.
+__device__ void dense_reduce_slot_body(const ws_agg_plan& plan, const ws_dense_spec& ds, const i64* parts, int nparts,
+                                       double* gacc) {
+  __shared__ i64 s_v[256];
+  __shared__ int s_any[256];
+  const int NA = plan.naggs;
+  const i64 S = ds.G * NA;
+  const i64 s = blockIdx.x;
+  const i64 g = s / NA;
+  const int a = (int)(s - g * NA);
+  const uint8_t k = ds.kind[a];
+  const int op = dk_op(k);
+  const bool isint = dk_int(k);
+  i64 xi = op == WS_AGG_SUM ? 0 : int_identity(op);
+  double xd = agg_identity(op);
+  bool any = false;
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+    if (parts[(i64)p * S + g * NA] == 0) continue;  // the block saw no row of g
+    any = true;
+    if (isint) xi = combine_i(op, xi, parts[(i64)p * S + s]);
+    else xd = combine_d(op, xd, bitsd(parts[(i64)p * S + s]));
+  }
+  s_v[threadIdx.x] = isint ? xi : dbits(xd);
+  s_any[threadIdx.x] = any ? 1 : 0;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && s_any[threadIdx.x + w]) {
+      const i64 o = s_v[threadIdx.x + w];
+      s_v[threadIdx.x] = !s_any[threadIdx.x] ? o
+                         : isint         ? combine_i(op, s_v[threadIdx.x], o)
+                                         : dbits(combine_d(op, bitsd(s_v[threadIdx.x]), bitsd(o)));
+      s_any[threadIdx.x] = 1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0 || !s_any[0]) return;
+  const bool bits = plan.agg_bits[a] != 0;
+  if (isint) {
+    const i64 x = s_v[0];
+    if (bits) gacc[s] = bitsd(combine_i(op, dbits(gacc[s]), x));
+    else gacc[s] = combine_d(a == 0 ? WS_AGG_SUM : op, gacc[s], (double)x);
+  } else {
+    gacc[s] = combine_d(op, gacc[s], bitsd(s_v[0]));
+  }
+}
 
 // ------------------------------------------------------------------ hash GROUP BY
 

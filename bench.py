@@ -228,6 +228,8 @@ def main() -> int:
                     help="set a tuning knob of the physical plans (ws_tune_set) before running")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: skip the autotuner, use the knobs as set")
     ap.add_argument("--no-learned", action="store_true", help="join / web workloads: skip the learned-sizing comparison")
+    ap.add_argument("--heavy-workers", type=int, default=3,
+                    help="serve: threads running web / join queries beside the batching thread (continuous-batching form)")
     ap.add_argument("--budget-mb", type=int, default=1536,
                     help="serve workload: HBM budget for the requests' estimated temporaries (0: none)")
     ap.add_argument("--calibrate", action="store_true",
@@ -625,7 +627,9 @@ def bench_serve(args) -> int:
         Q.set_join_rewrite(True)
     gm, jm = sizing.train_from_log("groups", epochs=600), sizing.train_from_log("joins", epochs=600)
     budget = args.budget_mb << 20
-    forms = (("learned_batched", 1, 2, (gm.h, jm.h)), ("plain_per_query", 0, 4, (None, None)))
+    # the batched form: one batching thread + heavy_workers heavy-query threads; the per-query
+    # form: the reference's at most 4 queries at a time
+    forms = (("learned_batched", 1, args.heavy_workers, (gm.h, jm.h)), ("plain_per_query", 0, 4, (None, None)))
     for name, mode, workers, models in forms:  # warm-up: compile plans
         Q.set_sizers(*models)
         Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=mode, workers=workers, seed=1, budget_bytes=budget)
@@ -636,7 +640,8 @@ def bench_serve(args) -> int:
             Q.set_sizers(*models)
             st, _, _ = Q.serve(pool, web, fact, dim, rate=rate, count=count, mode=mode, workers=workers, max_batch=256, seed=5,
                                budget_bytes=budget)
-            st.update({"offered_rate": rate, "form": name, "budget_mb": args.budget_mb})
+            st.update({"offered_rate": rate, "form": name, "budget_mb": args.budget_mb,
+                       "gpu_threads": workers + 1 if mode == 1 else workers})
             rows.append(st)
             print(json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}), flush=True)
     Q.set_sizers(None, None)

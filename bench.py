@@ -362,7 +362,8 @@ def bench_batch(args) -> int:
     distinct values, WHERE val > theta_q; 12 B/row scanned. One step = all 256 queries, run as one
     batched launch + one batched fetch into page-locked memory, and (separately timed) one by one
     through the per-query path, and as a batch prepared once whose launches replay from a HIP graph
-    (the repeated-dashboard form). Every timed form fetches every group to the host; the digests
+    (the repeated-dashboard form), one replay at a time and pipelined over two prepared copies (each
+    batch's groups copied to the host while the next batch's kernels run). Every timed form fetches every group to the host; the digests
     that compare the forms are computed after the timing. Unless --no-autotune, the autotuner first
     picks the batched form's tile, LDS table and grid knobs on this workload."""
     import warpstore_amd as ws
@@ -402,10 +403,22 @@ def bench_batch(args) -> int:
     L.ws_sync()
     res["replay"] = (time.perf_counter() - t0) / args.steps
     dr, gr = pr.run()
+    # two prepared copies in a pipeline: each batch's groups are copied to the host while the next
+    # batch's kernels run (every batch still fetches all its groups)
+    pr2 = s.prepare()
+    pr.pipeline(pr2, max(args.warmup, 2), digests=False)
+    L.ws_sync()
+    t0 = time.perf_counter()
+    pr.pipeline(pr2, args.steps, digests=False)
+    L.ws_sync()
+    res["pipelined"] = (time.perf_counter() - t0) / args.steps
+    dl, gl = pr.pipeline(pr2, 3)
+    pr2.free()
     pr.free()
     db, gb = s.run(batched=True)
     dp, gp = s.run(batched=False)
-    same = bool(np.array_equal(db, dp) and np.array_equal(gb, gp) and np.array_equal(dr, dp) and np.array_equal(gr, gp))
+    same = bool(np.array_equal(db, dp) and np.array_equal(gb, gp) and np.array_equal(dr, dp) and np.array_equal(gr, gp)
+                and np.array_equal(dl, dp) and np.array_equal(gl, gp))
     bt, pt = res["batched"], res["per_query"]
     scan = s.rows * 12
     out = {
@@ -417,6 +430,7 @@ def bench_batch(args) -> int:
         "per_query_queries_per_s": round(256 / pt, 1), "speedup_vs_per_query": round(pt / bt, 3),
         "results_equal_per_query": same, "groups": int(gb.sum()), "autotuner": tuned,
         "graph_replay_ms_per_step": round(res["replay"] * 1e3, 4), "graph_replay_value": round(scan / res["replay"] / 1e9, 3),
+        "pipelined_ms_per_step": round(res["pipelined"] * 1e3, 4), "pipelined_value": round(scan / res["pipelined"] / 1e9, 3),
         "roofline": roofline("agg_batch_dense_jit (all 256 queries' filter + dense GROUP BY in one launch)", scan, launches,
                              total_ms, None),
     }

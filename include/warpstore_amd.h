@@ -121,6 +121,15 @@ typedef struct ws_agg_batch_plan ws_agg_batch_plan;
 int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* envs, const ws_jit_program* jit,
                          ws_agg_batch_plan** out);
 int ws_agg_batch_replay(ws_agg_batch_plan* p, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs);
+/* A replay in two halves (no reference counterpart: a pipelined batch loop). ws_agg_batch_launch
+   enqueues the plan's graph and the read-back of its group counts on the plan's own stream (after
+   the calling thread's earlier work) and returns; ws_agg_batch_finish waits for the counts, copies
+   the groups out as ws_agg_batch_replay does, and returns when they are in keys / accs. Between the
+   two the host can launch another plan, whose kernels then run while this one's groups are copied
+   (pinned keys / accs, e.g. from ws_pinned_alloc, copy without a staging pass). A plan holds one
+   launch at a time: launching it again before its finish, or finishing it unlaunched, fails. */
+int ws_agg_batch_launch(ws_agg_batch_plan* p);
+int ws_agg_batch_finish(ws_agg_batch_plan* p, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs);
 void ws_agg_batch_release(ws_agg_batch_plan* p);
 int ws_agg_fetch_many(int n, ws_agg* const* aggs, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs);
 /* Aggregation strategy of a hash GROUP BY (no reference counterpart: the hook through which a

@@ -55,10 +55,25 @@ def make_queries(nq, seed, max_rows=300_000):
     return qs
 
 
+@pytest.mark.parametrize("ranges", [0, 64])
 @pytest.mark.parametrize("mode", ["interp", "jit"])
-def test_batched_dense_equals_alone_and_oracle(gpu, mode):
-    H.say(f"case batched dense {mode}")
+def test_batched_dense_equals_alone_and_oracle(gpu, mode, ranges):
+    """ranges: the batch_key_ranges knob. 0 sends the groups of queries with more groups than the
+    kernel's LDS table to device-scope atomics; 64 reads those queries' rows once per table-sized
+    key range (1e5 keys over 512-slot tables is 196 ranges, so the largest queries still take the
+    atomics)."""
+    H.say(f"case batched dense {mode} key ranges {ranges}")
     L = gpu
+    saved = C.c_int64()
+    ws.check(L.ws_tune_get(b"batch_key_ranges", C.byref(saved)))
+    ws.check(L.ws_tune_set(b"batch_key_ranges", ranges))
+    try:
+        _batched_case(L, mode)
+    finally:
+        ws.check(L.ws_tune_set(b"batch_key_ranges", saved.value))
+
+
+def _batched_case(L, mode):
     qs = make_queries(40, seed=44)
     jit = None
     if mode == "jit":

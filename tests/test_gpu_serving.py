@@ -1,10 +1,13 @@
 """Serving forms: the many-small-aggregations workload batched into one launch equals the plain
 per-query path query by query, and in the continuous-batching serving loop every request's result
 is the same as when it runs alone (small aggregates, web COUNT DISTINCT queries, star joins)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
 import helpers as H
+import warpstore_amd as ws
 from warpstore_amd import abi, queries as Q
 
 pytestmark = pytest.mark.gpu
@@ -105,3 +108,45 @@ def test_prepared_batch_graph_replays(gpu):
         if pr is not None:
             pr.free()
         s.free()
+
+
+def test_prepared_batch_pipeline(gpu):
+    """Two prepared copies of one set alternate in a pipeline (each batch's groups copied to the
+    host while the next batch's kernels run): the last batch of 1 to 5 steps equals the per-query
+    path, and so does a plain replay of either copy afterwards. The halves' misuse (a finish with no
+    launch, a second launch before the finish) fails with an error and leaves the plan usable."""
+    H.say("case prepared batch: pipelined launch / finish")
+    gpu.ws_jit_set_mode(abi.JIT_SYNC)
+    s = Q.C4Set(seed=23, nq=40)
+    a = b = None
+    try:
+        dp, gp = s.run(batched=False)
+        a, b = s.prepare(), s.prepare()
+        for steps in (1, 2, 5):
+            d, g = a.pipeline(b, steps)
+            assert np.array_equal(d, dp) and np.array_equal(g, gp), steps
+        for p in (a, b):
+            d, g = p.run()
+            assert np.array_equal(d, dp) and np.array_equal(g, gp)
+        counts = np.zeros(40, np.int64)
+        total = C.c_int64()
+        assert gpu.ws_agg_batch_finish(C.c_void_p(_plan_of(a)), ws.ptr(counts), C.byref(total), 0, None, None) != 0
+        assert gpu.ws_agg_batch_launch(C.c_void_p(_plan_of(a))) == 0
+        assert gpu.ws_agg_batch_launch(C.c_void_p(_plan_of(a))) != 0
+        cap = int(gp.sum())
+        keys, accs = np.zeros(cap), np.zeros(cap * 5)
+        ws.check(gpu.ws_agg_batch_finish(C.c_void_p(_plan_of(a)), ws.ptr(counts), C.byref(total), cap, ws.ptr(keys),
+                                         ws.ptr(accs)))
+        assert total.value == cap and np.array_equal(counts, gp)
+        d, g = a.run()
+        assert np.array_equal(d, dp)
+    finally:
+        for p in (a, b):
+            if p is not None:
+                p.free()
+        s.free()
+
+
+def _plan_of(prepared) -> int:
+    """The ws_agg_batch_plan of a prepared C4 set."""
+    return Q.qlib().wsq_c4_plan_handle(prepared.h)

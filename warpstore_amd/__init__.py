@@ -67,6 +67,7 @@ _SIGS = {
     "ws_sizer_groups": (I, [P, P, C.POINTER(I64)]), "ws_sizing_features": (I, [P, I, P, I64, C.c_double, P]),
     "ws_agg_set_features": (I, [P, P]), "ws_agg_run_batch": (I, [I, P, P, P]),
     "ws_agg_batch_prepare": (I, [I, P, P, P, PP]), "ws_agg_batch_replay": (I, [P, P, C.POINTER(I64), I64, P, P]),
+    "ws_agg_batch_launch": (I, [P]), "ws_agg_batch_finish": (I, [P, P, C.POINTER(I64), I64, P, P]),
     "ws_agg_batch_release": (None, [P]),
     "ws_agg_fetch_many": (I, [I, P, P, C.POINTER(I64), I64, P, P]), "ws_sizing_log_size": (I64, []), "ws_sizing_log_copy": (I64, [P, P, I64]),
     "ws_sizing_log_clear": (None, []),

@@ -1006,10 +1006,11 @@ TuneKnob g_tune[] = {
     {"select_chunks_per_cu", {8}, 1, 64},       // chunked selection: chunks of tiles per CU
     {"select_mask_min_rows", {1 << 16}, 0, (int64_t)1 << 40},  // compiled selection: ballot pass from this many rows
     {"dense_staged_min_set_words", {1024}, 0, (int64_t)1 << 40},  // compiled dense GROUP BY: sets staged in LDS from this size
+    {"batch_key_ranges", {0}, 0, 64},           // batched dense run: most key ranges per query beyond the table (0: none)
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
 enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
-       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks, kTuneMaskMinRows, kTuneDenseStagedWords };
+       kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks, kTuneMaskMinRows, kTuneDenseStagedWords, kTuneBatchRanges };
 
 // Dense plans whose WHERE reads set bitmaps of at least dense_staged_min_set_words words, over at
 // least select_mask_min_rows rows: the staged kernel (one 1024-thread block per CU, up to four
@@ -1384,7 +1385,21 @@ static int64_t agg_compact_into(ws_agg* A, DevBuf<double>& buf, bool colmajor, D
 
 // A batched run's launch description: the blob [BatchQuery n][BatchTile T] then per query plan,
 // dense spec, env, n[], del[], cols[], tile0[], with device addresses resolved against `dev`.
+// LDS bytes of the batched kernel's table of `slots` slots, and its slot count: the largest power
+// of two up to the batch_lds_slots knob that fits the LDS.
+static size_t batch_lds(int slots, int na) {
+  return (((size_t)slots * 4 + 15) & ~(size_t)15) + (size_t)slots * std::max(na, 1) * 8;
+}
+static int batch_slots(int na) {
+  int S = 64;
+  while (S * 2 <= (int)tune(kTuneBatchSlots) && batch_lds(S * 2, na) + kStaticLds <= kLdsPerCu) S *= 2;
+  return S;
+}
+// A query with more groups than the table, in at most batch_key_ranges table-sized key ranges, reads
+// its rows once per range (groups in LDS) instead of sending them to device-scope atomics.
+
 struct BatchBuild {
+  int S = 64;  // the kernel's LDS table slots
   Blob blob;
   uint8_t* dev = nullptr;
   const BatchQuery* dq = nullptr;
@@ -1411,11 +1426,19 @@ static BatchBuild build_batch(int n, ws_agg* const* aggs, const ws_expr_env* con
                 WS_ERR_NEEDS_PLAN);
   r.jf = jit ? jit_function(jit, kJitBatchDense) : nullptr;
   const int tile_rows = (int)tune(kTuneBatchTile);
+  r.S = batch_slots(na);
   std::vector<BatchTile> tiles;
-  for (int q = 0; q < n; ++q)
-    for (size_t b = 0; b < aggs[q]->bn.size(); ++b)
-      for (int64_t f = 0; f < aggs[q]->bn[b]; f += tile_rows)
-        tiles.push_back(BatchTile{q, (int32_t)b, (int32_t)std::min<int64_t>(tile_rows, aggs[q]->bn[b] - f), 0, f});
+  for (int q = 0; q < n; ++q) {
+    const int64_t G = aggs[q]->dense.G;
+    const int64_t ranges = G > r.S ? (G + r.S - 1) / r.S : 0;
+    const bool split = ranges > 0 && ranges <= tune(kTuneBatchRanges);
+    for (int64_t k = 0; k < (split ? ranges : 1); ++k) {
+      const int32_t glo = split ? (int32_t)(k * r.S) : -1;
+      for (size_t b = 0; b < aggs[q]->bn.size(); ++b)
+        for (int64_t f = 0; f < aggs[q]->bn[b]; f += tile_rows)
+          tiles.push_back(BatchTile{q, (int32_t)b, (int32_t)std::min<int64_t>(tile_rows, aggs[q]->bn[b] - f), glo, f});
+    }
+  }
   r.ntiles = (int64_t)tiles.size();
   Blob& blob = r.blob;
   const size_t o_q = blob.reserve(sizeof(BatchQuery) * n);
@@ -1472,11 +1495,8 @@ static void launch_batch(const BatchBuild& b, int n, bool prof) {
     hipLaunchKernelGGL(batch_init_k, dim3(gx, (unsigned)n), dim3(256), 0, stream(), b.dq);
     check_launch("batch_init");
   }
-  const int na = b.na;
-  int S = 64;  // the kernel's per-tile tables are powers of two up to S, within the LDS
-  auto lds_of = [&](int slots) { return (((size_t)slots * 4 + 15) & ~(size_t)15) + (size_t)slots * std::max(na, 1) * 8; };
-  while (S * 2 <= (int)tune(kTuneBatchSlots) && lds_of(S * 2) + kStaticLds <= kLdsPerCu) S *= 2;
-  const size_t lds = lds_of(S);
+  const int S = b.S;
+  const size_t lds = batch_lds(S, b.na);
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(b.ntiles, (int64_t)cu_count() * tune(kTuneBatchBlocksPerCu)));
   const BatchQuery* dq = b.dq;
   const BatchTile* dt = b.dt;
@@ -1508,10 +1528,17 @@ struct ws_agg_batch_plan {
   ws::DevBuf<uint32_t> cnt, offs;
   ws::DevBuf<double> pk, pa;
   int64_t gcap = 0;
-  hipStream_t cs = nullptr;
+  hipStream_t cs = nullptr;  // the plan's stream: its graph and read-backs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_in = nullptr, ev_cnt = nullptr;  // the caller's earlier work; the counts read back
+  uint32_t* hoffs = nullptr;                     // pinned: the group offsets of the last launch
+  bool launched = false;
   ~ws_agg_batch_plan() {
+    if (launched && cs) (void)hipStreamSynchronize(cs);
+    if (hoffs) (void)hipHostFree(hoffs);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_cnt) (void)hipEventDestroy(ev_cnt);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cs) (void)hipStreamDestroy(cs);
@@ -1810,23 +1837,49 @@ int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* e
     }
     WS_CK(hipStreamEndCapture(P->cs, &P->graph));
     WS_CK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    WS_CK(hipEventCreateWithFlags(&P->ev_in, hipEventDisableTiming));
+    WS_CK(hipEventCreateWithFlags(&P->ev_cnt, hipEventDisableTiming));
+    WS_CK(hipHostMalloc((void**)&P->hoffs, sizeof(uint32_t) * (size_t)(P->nch + 1), hipHostMallocDefault));
     *out = P.release();
   });
 }
+// The halves of a replay (ws_agg_batch_replay is launch + finish): the graph and the read-back of
+// the group offsets on the plan's stream, ordered after the calling thread's stream; then, on the
+// host, the wait for those offsets and the copies of the groups.
+static void batch_plan_launch(ws_agg_batch_plan* P) {
+  if (P->launched) throw Error("prepared batch: launched again before its finish", WS_ERR_INTERNAL);
+  WS_CK(hipEventRecord(P->ev_in, stream()));
+  WS_CK(hipStreamWaitEvent(P->cs, P->ev_in, 0));
+  WS_CK(hipGraphLaunch(P->exec, P->cs));
+  WS_CK(hipMemcpyAsync(P->hoffs, P->offs.p, sizeof(uint32_t) * (size_t)(P->nch + 1), hipMemcpyDeviceToHost, P->cs));
+  WS_CK(hipEventRecord(P->ev_cnt, P->cs));
+  P->launched = true;
+}
+static void batch_plan_finish(ws_agg_batch_plan* P, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs) {
+  *total = 0;
+  if (!P->launched) throw Error("prepared batch: finished without a launch", WS_ERR_INTERNAL);
+  P->launched = false;
+  WS_CK(hipEventSynchronize(P->ev_cnt));
+  const uint32_t* h = P->hoffs;
+  for (int q = 0; q < P->n; ++q) counts[q] = (int64_t)(h[P->cb[q + 1]] - h[P->cb[q]]);
+  *total = h[P->nch];
+  if (*total > cap) throw Error("prepared batch: more groups than the buffers hold", WS_ERR_INTERNAL);
+  if (!*total) return;
+  if (P->nk) WS_CK(hipMemcpyAsync(keys, P->pk.p, sizeof(double) * (size_t)(*total * P->nk), hipMemcpyDeviceToHost, P->cs));
+  if (P->na) WS_CK(hipMemcpyAsync(accs, P->pa.p, sizeof(double) * (size_t)(*total * P->na), hipMemcpyDeviceToHost, P->cs));
+  WS_CK(hipStreamSynchronize(P->cs));
+}
 int ws_agg_batch_replay(ws_agg_batch_plan* P, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs) {
   return guard([&] {
-    *total = 0;
-    WS_CK(hipGraphLaunch(P->exec, stream()));
-    std::vector<uint32_t> h((size_t)P->nch + 1);
-    d2h_sync(h.data(), P->offs.p, sizeof(uint32_t) * h.size());
-    for (int q = 0; q < P->n; ++q) counts[q] = (int64_t)(h[P->cb[q + 1]] - h[P->cb[q]]);
-    *total = h[P->nch];
-    if (*total > cap) throw Error("prepared batch: more groups than the buffers hold", WS_ERR_INTERNAL);
-    if (!*total) return;
-    if (P->nk) WS_CK(hipMemcpyAsync(keys, P->pk.p, sizeof(double) * (size_t)(*total * P->nk), hipMemcpyDeviceToHost, stream()));
-    if (P->na) WS_CK(hipMemcpyAsync(accs, P->pa.p, sizeof(double) * (size_t)(*total * P->na), hipMemcpyDeviceToHost, stream()));
-    WS_CK(hipStreamSynchronize(stream()));
+    batch_plan_launch(P);
+    batch_plan_finish(P, counts, total, cap, keys, accs);
   });
+}
+int ws_agg_batch_launch(ws_agg_batch_plan* P) {
+  return guard([&] { batch_plan_launch(P); });
+}
+int ws_agg_batch_finish(ws_agg_batch_plan* P, int64_t* counts, int64_t* total, int64_t cap, double* keys, double* accs) {
+  return guard([&] { batch_plan_finish(P, counts, total, cap, keys, accs); });
 }
 void ws_agg_batch_release(ws_agg_batch_plan* P) { delete P; }
 

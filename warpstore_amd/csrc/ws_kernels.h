@@ -1186,7 +1186,7 @@ __device__ inline void batch_init_body(const BatchQuery* qs) {
 struct BatchTile {
   int32_t q, b;   // query, batch of the query
   int32_t count;  // rows
-  int32_t pad;
+  int32_t glo;    // -1; or a key-range tile: only groups [glo, glo + S) of the query, in LDS
   int64_t first;  // first row in the batch
 };
 
@@ -1230,6 +1230,7 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
   const i64 per = (ntiles + gridDim.x - 1) / gridDim.x;
   const i64 t0 = (i64)blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
   int tab_q = -1, tab_St = 0;  // the query whose groups the table holds, and its slots in use
+  int tab_glo = -1;            // and its key range (key-range tiles)
   bool tab_keep = false;       // the table holds a direct layout that the query's next tile continues
   for (i64 t = t0; t < t1; ++t) {
     const BatchTile bt = tiles[t];
@@ -1243,17 +1244,22 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
     // the query's tiles); else 2 x the tile's possible groups, hashed (a power of two, >= 64,
     // <= S); else none
     const i64 G = ds.G, reach = bt.count < G ? (i64)bt.count : G;
-    const bool direct = G <= (i64)S;
+    // a key-range tile: the groups [glo, glo + S) of a query with more groups than S, directly
+    // at slot g - glo (the host gives such a query one tile list per range: each range reads the
+    // query's rows again instead of sending them to device-scope atomics)
+    const int glo = bt.glo;
+    const bool ranged = glo >= 0;
+    const bool direct = ranged || G <= (i64)S;
     // a few groups: C copies of the direct table, lane l using copy l % C, so the lanes of one LDS
     // atomic spread over C x G slots instead of contending for G
-    const int C = direct && G <= kCopyGroups ? (int)(S / G < 8 ? S / G : 8) : 1;
-    int St = direct ? (int)G * C : 0;
+    const int C = !ranged && direct && G <= kCopyGroups ? (int)(S / G < 8 ? S / G : 8) : 1;
+    int St = ranged ? (int)(G - glo < (i64)S ? G - glo : (i64)S) : direct ? (int)G * C : 0;
     if (!direct && G <= 4 * (i64)bt.count && 2 * reach <= S) {
       St = 64;
       while (St < S && St < 2 * reach) St <<= 1;
       St = St < S ? St : S;
     }
-    const bool cont = tab_keep && tab_q == bt.q;
+    const bool cont = tab_keep && tab_q == bt.q && tab_glo == glo;
     __syncthreads();  // the previous tile's rows are folded; its column views are no longer read
     if (!cont) {
       if (tab_q >= 0) {
@@ -1266,6 +1272,7 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
       }
       tab_q = St > 0 ? bt.q : -1;
       tab_St = St;
+      tab_glo = glo;
       tab_keep = direct;
     }
     for (int i = threadIdx.x; i < d.ncols; i += blockDim.x) scols[i] = d.cols[(i64)bt.b * d.ncols + i];
@@ -1303,7 +1310,11 @@ __device__ void batch_dense_body(const BatchQuery* qs, const BatchTile* tiles, i
         const bool valid = g >= 0;
         if (!valid) continue;
         int slot = -1;
-        if (direct) {
+        if (ranged) {
+          if (g < glo || g >= glo + St) continue;  // another range's group
+          slot = (int)g - glo;
+          if (lg[slot] < 0) lg[slot] = (int)g;
+        } else if (direct) {
           slot = copy_base + (int)g;
           if (lg[slot] < 0) lg[slot] = (int)g;  // occupied (a plain store: every writer stores the same)
         } else if (St > 0) {

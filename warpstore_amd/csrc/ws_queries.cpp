@@ -1040,6 +1040,46 @@ void c4_replay(C4Prepared& P, std::vector<uint64_t>* digests, std::vector<int64_
   }
 }
 
+// Steps of a prepared set as a pipeline over two prepared copies of it (their own aggregators,
+// graphs and streams): step s launches copy s % 2, then finishes step s - 1, whose groups are copied
+// to the host while step s's kernels run. digests / groups: of the last step.
+void c4_pipeline(C4Prepared& A, C4Prepared& B, int steps, std::vector<uint64_t>* digests, std::vector<int64_t>& groups) {
+  const size_t n = A.plans.size();
+  if (B.plans.size() != n) throw SqlError("pipelined batch: the two prepared copies differ in size", 1105);
+  C4Prepared* P[2] = {&A, &B};
+  std::vector<int64_t> counts(n, 0);
+  t_results.reserve(std::max<int64_t>(std::max(A.cap, B.cap), 1), 1, 5);
+  bool out[2] = {false, false};  // launched, not yet finished
+  auto finish = [&](int k) {
+    int64_t total = 0;
+    out[k] = false;
+    wsd::ck(ws_agg_batch_finish(P[k]->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs));
+  };
+  try {
+    for (int s = 0; s < steps; ++s) {
+      wsd::ck(ws_agg_batch_launch(P[s & 1]->plan));
+      out[s & 1] = true;
+      if (s > 0) finish((s - 1) & 1);
+    }
+    if (steps > 0) finish((steps - 1) & 1);
+  } catch (...) {
+    for (int k = 0; k < 2; ++k)
+      if (out[k]) {
+        int64_t total = 0;
+        (void)ws_agg_batch_finish(P[k]->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs);
+      }
+    throw;
+  }
+  if (digests) digests->assign(n, 0);
+  groups.assign(n, 0);
+  int64_t off = 0;
+  for (size_t i = 0; i < n && steps > 0; ++i) {
+    if (digests) (*digests)[i] = c4_digest(t_results.keys + off, t_results.accs + off * 5, counts[i]);
+    groups[i] = counts[i];
+    off += counts[i];
+  }
+}
+
 // ------------------------------------------------------------------ serving loop (open-loop stream)
 
 // A request of the mixed stream: a small aggregate over one of the pool's segments, the web
@@ -1487,6 +1527,20 @@ int wsq_c4_replay(void* prepared, uint64_t* digests, int64_t* groups) {
     }
   });
 }
+// steps batches of the set, pipelined over two prepared copies of it (see c4_pipeline); digests and
+// groups of the last step.
+int wsq_c4_pipeline(void* a, void* b, int steps, uint64_t* digests, int64_t* groups) {
+  return guard([&] {
+    std::vector<uint64_t> d;
+    std::vector<int64_t> g;
+    c4_pipeline(*(C4Prepared*)a, *(C4Prepared*)b, steps, digests ? &d : nullptr, g);
+    for (size_t i = 0; i < g.size(); ++i) {
+      if (digests) digests[i] = d[i];
+      groups[i] = g[i];
+    }
+  });
+}
+void* wsq_c4_plan_handle(void* prepared) { return ((C4Prepared*)prepared)->plan; }
 void wsq_c4_release(void* prepared) { delete (C4Prepared*)prepared; }
 
 // The autotuner (see autotune()) on the batched workload of a C4 set: knobs batch_tile_rows,

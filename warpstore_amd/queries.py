@@ -53,6 +53,10 @@ def qlib():
         Q.wsq_c4_prepare.restype = I
         Q.wsq_c4_replay.argtypes = [P, P, P]
         Q.wsq_c4_replay.restype = I
+        Q.wsq_c4_pipeline.argtypes = [P, P, I, P, P]
+        Q.wsq_c4_pipeline.restype = I
+        Q.wsq_c4_plan_handle.argtypes = [P]
+        Q.wsq_c4_plan_handle.restype = C.c_void_p
         Q.wsq_c4_release.argtypes = [P]
         Q.wsq_c4_release.restype = None
         for f in ("wsq_c4_autotune", "wsq_c2_autotune"):
@@ -185,6 +189,15 @@ class PreparedC4:
         d = np.zeros(self.nq, np.uint64)
         g = np.zeros(self.nq, np.int64)
         _ck(qlib().wsq_c4_replay(self.h, d.ctypes.data if digests else None, g.ctypes.data))
+        return (d if digests else None), g
+
+    def pipeline(self, other: "PreparedC4", steps: int, digests: bool = True):
+        """steps batches alternating between this copy and other (a second prepare() of the same
+        set): each batch's groups are copied to the host while the next batch's kernels run.
+        (digests, group counts) of the last batch."""
+        d = np.zeros(self.nq, np.uint64)
+        g = np.zeros(self.nq, np.int64)
+        _ck(qlib().wsq_c4_pipeline(self.h, other.h, steps, d.ctypes.data if digests else None, g.ctypes.data))
         return (d if digests else None), g
 
     def free(self) -> None:

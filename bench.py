@@ -516,6 +516,7 @@ def bench_join(args) -> int:
     out["roofline"] = roofline(dom + desc, alg, launches, total_ms, pmc_traffic(sym, "pmc_c3_star_join.json"))
     out["plan"] = "join as bitmap filter fused into the fact aggregation (distinct dimension keys)" if dom.startswith(
         "agg_dense_staged") else "hash join (filtered fact rows probe the filtered dimension)"
+    out["concurrent_sessions"] = join_concurrent(L, Q, fact, dim, rows, bytes_read, args)
     if not args.no_learned:
         out["learned_sizing"] = join_learned(ws, L, Q, fact, dim, exp, args)
     if not args.no_cpu:
@@ -532,6 +533,54 @@ def bench_join(args) -> int:
     dim.free()
     print(json.dumps(out), flush=True)
     return 0 if parity else 1
+
+
+def join_concurrent(L, Q, fact, dim, want, bytes_read, args, sessions: int = 4) -> dict:
+    """The star join from `sessions` host threads at once, as the reference engine runs at most
+    four queries concurrently (engine.cpp:1262-1337): each thread issues its launches on its own
+    stream, so one query's host-side launch work overlaps another's fact scan. Whole-job figures
+    over sessions x steps queries (timed from the release of the warmed-up threads to the end of
+    the last query); every thread's last result is checked against the single-session one."""
+    import threading
+    steps = max(args.steps, 1)
+    start, done = threading.Barrier(sessions + 1), threading.Barrier(sessions + 1)
+    got, errors = [None] * sessions, []
+
+    def worker(i):
+        try:
+            for _ in range(max(args.warmup, 1)):
+                Q.c3(fact, dim)
+            L.ws_sync()
+            start.wait()
+            for _ in range(steps):
+                r, _ = Q.c3(fact, dim)
+            got[i] = np.asarray(r)
+            done.wait()
+        except BaseException as e:  # noqa: BLE001 - reported below, the barriers released
+            errors.append(repr(e))
+            start.abort()
+            done.abort()
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(sessions)]
+    for t in threads:
+        t.start()
+    try:
+        start.wait()
+        t0 = time.perf_counter()
+        done.wait()
+        elapsed = time.perf_counter() - t0
+    except threading.BrokenBarrierError:
+        elapsed = None
+    for t in threads:
+        t.join()
+    if errors or elapsed is None:
+        return {"sessions": sessions, "error": errors[:1]}
+    key = np.argsort(want[:, 0])
+    same = all(g is not None and np.array_equal(g[np.argsort(g[:, 0])], want[key]) for g in got)
+    q = sessions * steps
+    return {"sessions": sessions, "queries": q, "queries_per_s": round(q / elapsed, 1),
+            "value": round(bytes_read * q / elapsed / 1e9, 3), "unit": "GB/s", "ms_per_query": round(elapsed / q * 1e3, 4),
+            "results_equal_single_session": bool(same)}
 
 
 JOIN_TRAIN_FILTERS = (3, 8, 15, 25, 35, 65, 90, 130, 200, 300, 450, 700)  # dimension filters attr < x (not 50)

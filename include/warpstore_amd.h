@@ -151,7 +151,12 @@ int ws_agg_partitioned(const ws_agg* a); /* 1 when the last run produced a parti
  * default 4), "part_pack" (0 or 1, default 1: packed partial-group records for integer keys),
  * "select_chunks_per_cu" (1..64, default 8: chunked selection of long inputs),
  * "select_mask_min_rows" (default 2^16: compiled selections of at least this many rows take the
- * ballot pass with set bitmaps staged in LDS, then a count scan and an expansion).
+ * ballot pass with set bitmaps staged in LDS, then a count scan and an expansion),
+ * "dense_staged_min_set_words" (default 1024: compiled dense GROUP BYs whose WHERE reads set
+ * bitmaps of at least this many words stage them in LDS), "batch_key_ranges" (0..64, default 0:
+ * batched queries with more groups than the LDS table, in at most this many table-sized key
+ * ranges, read their rows once per range), "topk_steps" (1..64, default 8) and
+ * "topk_blocks_per_cu" (1..16, default 1): the first pass of top-k selection.
  * Unknown keys and values out of range return WS_ERR_INTERNAL. */
 int ws_tune_set(const char* key, int64_t value);
 int ws_tune_get(const char* key, int64_t* value);

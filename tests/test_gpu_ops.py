@@ -369,6 +369,31 @@ def test_topk_perm_is_sort_prefix(gpu, m, k, nk):
     assert np.array_equal(got, exp[:kk])
 
 
+@pytest.mark.parametrize("order,k,nk", [("rising", 10, 1), ("rising", 200, 1), ("rising", 33, 2), ("falling", 10, 1),
+                                         ("sawtooth", 64, 1), ("constant", 10, 1)])
+def test_topk_orders_that_defeat_the_threshold(gpu, order, k, nk):
+    """ORDER BY ... DESC LIMIT k over inputs where every step brings many new candidates (keys
+    rising in row order: each row beats the kept k; a sawtooth of short rising runs; all keys equal,
+    so the row index decides), so the selection sorts its steps' new entries instead of ranking them
+    one by one; and keys falling, where steps bring none. Against the oracle's stable sort."""
+    H.say(f"case topk order={order} k={k} nk={nk}")
+    m = 300_000
+    rng = np.random.default_rng(k + nk)
+    i = np.arange(m, dtype=np.float64)
+    first = {"rising": i, "falling": -i, "sawtooth": i % 700, "constant": np.zeros(m)}[order]
+    keys = [first] + [np.floor(rng.random(m) * 5) for _ in range(nk - 1)]
+    desc = np.ones(nk, np.uint8)
+    dks = [ws.DevBuf.from_array(v) for v in keys]
+    kp = (C.c_void_p * nk)(*[d.p for d in dks])
+    perm = C.c_void_p()
+    ws.check(gpu.ws_topk_perm(kp, ws.ptr(desc), nk, m, k, C.byref(perm)))
+    got = ws.DevBuf(k * 4, tmp=True, adopt=perm.value).to_array(np.int32, k)
+    exp = np.zeros(m, np.int32)
+    hp = (C.c_void_p * nk)(*[v.ctypes.data for v in keys])
+    H.oracle().or_sort_perm(hp, ws.ptr(desc), nk, m, ws.ptr(exp))
+    assert np.array_equal(got, exp[:k])
+
+
 @pytest.mark.parametrize("ptype,dtype", [(abi.P_I32, np.int32), (abi.P_I64, np.int64)])
 def test_gather_multi_raw(gpu, ptype, dtype):
     """Gather across raw batches (the LDS-staged kernel): ragged batches, sorted and unsorted

@@ -1007,8 +1007,15 @@ TuneKnob g_tune[] = {
     {"select_mask_min_rows", {1 << 16}, 0, (int64_t)1 << 40},  // compiled selection: ballot pass from this many rows
     {"dense_staged_min_set_words", {1024}, 0, (int64_t)1 << 40},  // compiled dense GROUP BY: sets staged in LDS from this size
     {"batch_key_ranges", {0}, 0, 64},           // batched dense run: most key ranges per query beyond the table (0: none)
+    {"topk_steps", {8}, 1, 64},                 // top-k selection, first pass: steps of 256 rows per block
+    {"topk_blocks_per_cu", {1}, 1, 16},         // and at most this many blocks per CU
 };
 static int64_t tune(int i) { return g_tune[i].value.load(std::memory_order_relaxed); }
+int64_t ws::tune_knob(const char* name) {
+  for (const TuneKnob& k : g_tune)
+    if (std::strcmp(k.name, name) == 0) return k.value.load(std::memory_order_relaxed);
+  throw Error(std::string("tune: unknown knob ") + name, WS_ERR_INTERNAL);
+}
 enum { kTuneMergeLds, kTuneScatterLds, kTuneChunks, kTuneAutoRows, kTuneAutoGroups, kTuneBatchTile, kTuneBatchSlots,
        kTuneBatchBlocksPerCu, kTunePack, kTuneSelectChunks, kTuneMaskMinRows, kTuneDenseStagedWords, kTuneBatchRanges };
 

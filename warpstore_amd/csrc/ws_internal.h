@@ -49,6 +49,8 @@ int guard(F&& f) {
 // ---- device
 void ensure_init();
 int cu_count();
+// The current value of a tuning knob (ws_tune_set), by name; throws for an unknown name.
+int64_t tune_knob(const char* name);
 const std::string& arch_name();  // e.g. "gfx950:sramecc+:xnack-"
 // The stream a thread's work goes to: its per-thread stream, or a capture stream while the thread
 // records a graph (set through StreamOverride).

@@ -232,6 +232,8 @@ template <int NK, bool kFirst>
 __device__ __forceinline__ TopRow<NK> top_load(const TopIn& in, int64_t r, int64_t r1) {
   TopRow<NK> e;
   e.i = INT32_MAX;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) e.k[j] = ~0ull;  // no row: defined, sorts last
   if (r < r1) {
     if (kFirst) {
 #pragma unroll
@@ -288,6 +290,89 @@ __device__ __forceinline__ int top_rank(const TopSlots<NK>& b, int x, int cnt, i
   return rank;
 }
 
+// A step's new entries (slots [cnt, cnt + c) of b, any order, c <= kTopThreads) sorted in place by a
+// bitonic sort over the block, one entry per thread (strides below a wave by shuffles, the two
+// wider ones through `tmp`); slots past the new entries sort last (no row: the largest key and
+// index). Every thread of the block calls it.
+template <int NK>
+__device__ __forceinline__ void top_sort_new(TopSlots<NK>& b, TopSlots<NK>& tmp, int cnt, int c) {
+  const int t = threadIdx.x;
+  TopRow<NK> me;
+  if (t < c) {
+    me = b.get(cnt + t);
+  } else {
+#pragma unroll
+    for (int j = 0; j < NK; ++j) me.k[j] = ~0ull;
+    me.i = INT32_MAX;
+  }
+  for (int size = 2; size <= kTopThreads; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      TopRow<NK> other;
+      if (stride < kWave) {
+#pragma unroll
+        for (int j = 0; j < NK; ++j) other.k[j] = (u64)__shfl_xor((long long)me.k[j], stride, kWave);
+        other.i = __shfl_xor(me.i, stride, kWave);
+      } else {
+        __syncthreads();  // the previous exchange's reads are done
+        tmp.put(t, me);
+        __syncthreads();
+        other = tmp.get(t ^ stride);
+      }
+      // pair (t, t ^ stride): the lower thread keeps the smaller entry in an ascending run of
+      // `size`, the larger in a descending one (the whole block is one ascending run at the end)
+      const bool up = (t & size) == 0, lower = (t & stride) == 0;
+      const bool take = lower == up ? top_before(other, me) : top_before(me, other);
+      if (take) me = other;
+    }
+  }
+  __syncthreads();  // every unsorted entry has been read (and the exchanges are done)
+  if (t < c) b.put(cnt + t, me);
+  __syncthreads();
+}
+// Rank of slot x in the union of two ascending runs of b, [0, cnt) (the list) and [cnt, n) (the
+// step's sorted new entries): its index in its run plus the entries of the other run before it.
+template <int NK>
+__device__ __forceinline__ int top_rank_sorted(const TopSlots<NK>& b, int x, int cnt, int n) {
+  const TopRow<NK> me = b.get(x);
+  const bool old = x < cnt;
+  int lo = old ? cnt : 0, hi = old ? n : cnt;
+  const int base = old ? x : x - cnt;
+  const int first = lo;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (top_before(b.get(mid), me)) lo = mid + 1;
+    else hi = mid;
+  }
+  return base + (lo - first);
+}
+// Steps bringing more new entries than this sort them (top_sort_new) instead of ranking each
+// against all the others.
+constexpr int kTopSortFrom = 64;
+
+// Whether entry e (pass: it is a candidate) is among the k best candidates of its wave: a
+// candidate with k better ones in its own wave cannot be among the block's k best. The wave's
+// candidates are compared through lane reads (no LDS, no barrier); a wave with at most k
+// candidates keeps them all.
+template <int NK>
+__device__ __forceinline__ bool top_wave_keep(const TopRow<NK>& e, bool pass, int k) {
+  const u64 pm = __ballot(pass);
+  if (__popcll(pm) <= k) return pass;  // (wave-uniform)
+  int better = 0;
+#pragma unroll 16
+  for (int j = 0; j < kWave; ++j) {
+    TopRow<NK> o;
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)e.k[q], j);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(e.k[q] >> 32), j);
+      o.k[q] = ((u64)hi << 32) | lo;
+    }
+    o.i = __builtin_amdgcn_readlane(e.i, j);
+    better += ((pm >> j) & 1) && top_before(o, e) ? 1 : 0;
+  }
+  return pass && better < k;
+}
+
 template <int NK, bool kFirst>
 __global__ void __launch_bounds__(kTopThreads) topk_k(TopIn in, int64_t m, int64_t per_block, int k, u64* out_ok,
                                                       int32_t* out_idx) {
@@ -303,7 +388,8 @@ __global__ void __launch_bounds__(kTopThreads) topk_k(TopIn in, int64_t m, int64
   for (int64_t base = r0; base < r1; base += kTopThreads) {
     const TopRow<NK> e = nxt;
     nxt = top_load<NK, kFirst>(in, base + kTopThreads + t, r1);  // next step's row, in flight meanwhile
-    if (e.i != INT32_MAX && (cnt < k || top_before(e, thr)))
+    const bool pass = e.i != INT32_MAX && (cnt < k || top_before(e, thr));
+    if (top_wave_keep(e, pass, k))
       s_b[cur].put(cnt + __hip_atomic_fetch_add(&s_c, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), e);
     __syncthreads();
     const int c = s_c;
@@ -311,9 +397,17 @@ __global__ void __launch_bounds__(kTopThreads) topk_k(TopIn in, int64_t m, int64
     if (c == 0) continue;  // (uniform)
     if (t == 0) s_c = 0;
     const int n = cnt + c, o = cur ^ 1;
-    for (int x = t; x < n; x += kTopThreads) {
-      const int rank = top_rank(s_b[cur], x, cnt, n);
-      if (rank < k) s_b[o].put(rank, s_b[cur].get(x));
+    if (c > kTopSortFrom) {  // (uniform)
+      top_sort_new(s_b[cur], s_b[o], cnt, c);
+      for (int x = t; x < n; x += kTopThreads) {
+        const int rank = top_rank_sorted(s_b[cur], x, cnt, n);
+        if (rank < k) s_b[o].put(rank, s_b[cur].get(x));
+      }
+    } else {
+      for (int x = t; x < n; x += kTopThreads) {
+        const int rank = top_rank(s_b[cur], x, cnt, n);
+        if (rank < k) s_b[o].put(rank, s_b[cur].get(x));
+      }
     }
     __syncthreads();
     cur = o;
@@ -332,8 +426,12 @@ __global__ void __launch_bounds__(kTopThreads) topk_k(TopIn in, int64_t m, int64
 
 template <int NK>
 static void topk_launch(TopIn in, int64_t m, int64_t kk, int32_t* out) {
-  // first pass: about 8 steps per block, at most one block per CU; second pass: one block
-  const int64_t want = std::max<int64_t>(1, std::min<int64_t>((m + 8 * kTopThreads - 1) / (8 * kTopThreads), cu_count()));
+  // first pass: about topk_steps steps of kTopThreads rows per block, at most topk_blocks_per_cu
+  // blocks per CU (each step waits for its rows and three barriers, so fewer steps per block
+  // shorten the pass while the candidates it leaves stay few); then passes of one step per block
+  const int64_t steps = tune_knob("topk_steps"), bpc = tune_knob("topk_blocks_per_cu");
+  const int64_t want = std::max<int64_t>(
+      1, std::min<int64_t>((m + steps * kTopThreads - 1) / (steps * kTopThreads), cu_count() * bpc));
   const int64_t per = (m + want - 1) / want;
   const int64_t nb = (m + per - 1) / per;
   DevBuf<u64> cok(nb * kk * NK), fok(kk * NK);

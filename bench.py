@@ -228,6 +228,8 @@ def main() -> int:
                     help="set a tuning knob of the physical plans (ws_tune_set) before running")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: skip the autotuner, use the knobs as set")
     ap.add_argument("--no-learned", action="store_true", help="join / web workloads: skip the learned-sizing comparison")
+    ap.add_argument("--no-concurrent", action="store_true",
+                    help="join workload: skip the concurrent-sessions figure (profiles of the single-session query)")
     ap.add_argument("--heavy-workers", type=int, default=3,
                     help="serve: threads running web / join queries beside the batching thread (continuous-batching form)")
     ap.add_argument("--budget-mb", type=int, default=1536,
@@ -516,7 +518,7 @@ def bench_join(args) -> int:
     out["roofline"] = roofline(dom + desc, alg, launches, total_ms, pmc_traffic(sym, "pmc_c3_star_join.json"))
     out["plan"] = "join as bitmap filter fused into the fact aggregation (distinct dimension keys)" if dom.startswith(
         "agg_dense_staged") else "hash join (filtered fact rows probe the filtered dimension)"
-    out["concurrent_sessions"] = join_concurrent(L, Q, fact, dim, rows, bytes_read, args)
+    out["concurrent_sessions"] = None if args.no_concurrent else join_concurrent(L, Q, fact, dim, rows, bytes_read, args)
     if not args.no_learned:
         out["learned_sizing"] = join_learned(ws, L, Q, fact, dim, exp, args)
     if not args.no_cpu:

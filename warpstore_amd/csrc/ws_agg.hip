@@ -2096,10 +2096,10 @@ int ws_key_bitmap_where(const ws_expr_env* env, int has_where, ws_prog where, ws
     plan.nkeys = 1;
     plan.keys[0] = key;
     plan.key_int[0] = 1;
-    // Two rows per thread and tile, enough blocks to fill the CUs: each row's loads wait, so the
-    // latency is hidden by the number of waves in flight.
+    // kKeyBitsRows rows per thread and tile (their loads issue together), a block per tile up to
+    // 16 per CU.
     ScanDesc d{};
-    d.tile_rows = 2 * kBlock;
+    d.tile_rows = kKeyBitsRows * kBlock;
     int64_t ntiles = 0;
     for (int64_t n : in.n) ntiles += (n + d.tile_rows - 1) / d.tile_rows;
     DescUpload up;

@@ -2377,7 +2377,11 @@ __device__ void dense_staged_body(const ScanDesc& d, i64* parts, int* flags) {
 // Bitmap of the integer keys of the rows that pass the WHERE (the filtered build side of a join
 // filter, without materializing it): bit key - lo of `bits` for each live passing row whose key is
 // in [lo, lo + nbits); parts[block] (when given) gets the block's count of passing rows (summed
-// afterwards: atomics from every block on one address would queue). Tiles grid-stride.
+// afterwards: atomics from every block on one address would queue). Tiles grid-stride; a tile is
+// kKeyBitsRows rows per thread, whose WHERE and key loads all issue before any bit is set when
+// the plan has no side effects (rows past the tile re-read its first row), so a thread waits
+// about once per tile instead of once per row.
+constexpr int kKeyBitsRows = 8;
 template <class Ev>
 __device__ void key_bits_body(const ScanDesc& d, i64 lo, i64 nbits, uint32_t* bits, u64* parts) {
   __shared__ u64 s_cnt[16];
@@ -2386,15 +2390,38 @@ __device__ void key_bits_body(const ScanDesc& d, i64 lo, i64 nbits, uint32_t* bi
   u64 passed = 0;
   BatchCursor bc;
   const i64 ntiles = uni(d.ntiles);
+  constexpr int U = kKeyBitsRows;
   for (i64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
     if (bc.seek(d, t)) {
       stage_batch(d, bc.b, scols);
       ev.bind(Ev::kGlobalCols ? d.cols + (i64)bc.b * d.ncols : scols);
     }
     const TileRows tr = bc.rows(t);
+    const uint32_t* del = bc.del;
+    if (Ev::kPureKeys && Ev::kPureWhere && !del) {  // (block-uniform)
+      for (i64 i0 = threadIdx.x; i0 < tr.count; i0 += (i64)blockDim.x * U) {
+        bool w[U];
+        i64 key[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const i64 i = i0 + (i64)u * blockDim.x;
+          const i64 r = tr.first + (i < tr.count ? i : 0);
+          w[u] = i < tr.count && (!ev.has_where() || ev.where(r));
+          key[u] = ev.ikey(0, r);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!w[u]) continue;
+          ++passed;
+          const u64 o = (u64)(key[u] - lo);
+          if (o < (u64)nbits) atomicOr(&bits[o >> 5], 1u << (o & 31));
+        }
+      }
+      continue;
+    }
     for (i64 i = threadIdx.x; i < tr.count; i += blockDim.x) {
       const i64 r = tr.first + i;
-      if (is_deleted(bc.del, r)) continue;
+      if (is_deleted(del, r)) continue;
       if (ev.has_where() && !ev.where(r)) continue;
       ++passed;
       const u64 o = (u64)(ev.ikey(0, r) - lo);

@@ -590,6 +590,7 @@ struct C3Agg {
   Comp c;
   AggPlan plan;
   int f_flag, f_price, f_qty, f_fk = -1;
+  std::function<void()> after_launch;  // called once the aggregation is launched, before its results are read
   C3Agg(const uint32_t* bits, int64_t lo, int64_t nbits) {
     f_flag = c.slot(WS_T_I32);
     f_price = c.slot(WS_T_DECIMAL);
@@ -630,6 +631,7 @@ struct C3Agg {
       GpuAggregator agg(plan, cap, dense ? &ds : nullptr);
       add(agg);
       agg.run(c.env, jit);
+      if (after_launch) after_launch();
       if (agg.overflowed()) {
         cap = pow2_at_least(std::max<int64_t>(cap * 4, agg.groupCount() * 4), 1);
         continue;
@@ -686,11 +688,12 @@ int64_t query_c3(const DevTable& fact, const DevTable& dim, int64_t* out, C3Stat
     wsd::ck(ws_key_bitmap_where(wsd::env(cd.env), 1, wsd::prog(dwhere), wsd::prog(dkey), (int)bn.size(), bn.data(), bdel.data(),
                                 reinterpret_cast<const ws_colview*>(bcols.data()), cd.ncols, lo, nbits, bits.p, nsel.p,
                                 wsd::handle(kjit)));
-    // the count of filtered dimension rows comes back with the aggregation's read-back
+    // the count comes back with the aggregation's read-back: its copy is queued after the
+    // aggregation's launches, off the path to the fact scan
     static thread_local uint64_t* t_nsel = nullptr;
     if (!t_nsel) wsd::ck(ws_pinned_alloc(sizeof(uint64_t), (void**)&t_nsel));
-    wsd::ck(ws_d2h_async(t_nsel, nsel.p, sizeof(uint64_t)));
     C3Agg A(bits.p, lo, nbits);
+    A.after_launch = [&] { wsd::ck(ws_d2h_async(t_nsel, nsel.p, sizeof(uint64_t))); };
     const std::vector<int> slot_col = {3, 2, 1, 0};  // flag, price, qty, fk
     const int64_t g = A.run(fact.cols[3], fact.rows, "aggregate fact (join as filter)", [&](GpuAggregator& agg) {
       ExprEnv e = A.c.env;

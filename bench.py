@@ -364,8 +364,8 @@ def bench_batch(args) -> int:
     distinct values, WHERE val > theta_q; 12 B/row scanned. One step = all 256 queries, run as one
     batched launch + one batched fetch into page-locked memory, and (separately timed) one by one
     through the per-query path, and as a batch prepared once whose launches replay from a HIP graph
-    (the repeated-dashboard form), one replay at a time and pipelined over two prepared copies (each
-    batch's groups copied to the host while the next batch's kernels run). Every timed form fetches every group to the host; the digests
+    (the repeated-dashboard form), one replay at a time and pipelined over three prepared copies
+    (each batch's groups copied to the host while the next two batches' kernels run). Every timed form fetches every group to the host; the digests
     that compare the forms are computed after the timing. Unless --no-autotune, the autotuner first
     picks the batched form's tile, LDS table and grid knobs on this workload."""
     import warpstore_amd as ws
@@ -405,17 +405,18 @@ def bench_batch(args) -> int:
     L.ws_sync()
     res["replay"] = (time.perf_counter() - t0) / args.steps
     dr, gr = pr.run()
-    # two prepared copies in a pipeline: each batch's groups are copied to the host while the next
-    # batch's kernels run (every batch still fetches all its groups)
-    pr2 = s.prepare()
-    pr.pipeline(pr2, max(args.warmup, 2), digests=False)
+    # three prepared copies in a pipeline: each batch's groups are copied to the host while the
+    # next two batches' kernels run (every batch still fetches all its groups)
+    more = [s.prepare(), s.prepare()]
+    pr.pipeline(more, max(args.warmup, 3), digests=False)
     L.ws_sync()
     t0 = time.perf_counter()
-    pr.pipeline(pr2, args.steps, digests=False)
+    pr.pipeline(more, args.steps, digests=False)
     L.ws_sync()
     res["pipelined"] = (time.perf_counter() - t0) / args.steps
-    dl, gl = pr.pipeline(pr2, 3)
-    pr2.free()
+    dl, gl = pr.pipeline(more, 4)
+    for p in more:
+        p.free()
     pr.free()
     db, gb = s.run(batched=True)
     dp, gp = s.run(batched=False)

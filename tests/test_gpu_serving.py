@@ -111,21 +111,24 @@ def test_prepared_batch_graph_replays(gpu):
 
 
 def test_prepared_batch_pipeline(gpu):
-    """Two prepared copies of one set alternate in a pipeline (each batch's groups copied to the
-    host while the next batch's kernels run): the last batch of 1 to 5 steps equals the per-query
-    path, and so does a plain replay of either copy afterwards. The halves' misuse (a finish with no
+    """Two, then three, prepared copies of one set take turns in a pipeline (each batch's groups
+    copied to the host while the later batches' kernels run): the last batch of 1 to 7 steps equals
+    the per-query path, and so does a plain replay of every copy afterwards. The halves' misuse (a finish with no
     launch, a second launch before the finish) fails with an error and leaves the plan usable."""
     H.say("case prepared batch: pipelined launch / finish")
     gpu.ws_jit_set_mode(abi.JIT_SYNC)
     s = Q.C4Set(seed=23, nq=40)
-    a = b = None
+    a = b = c = None
     try:
         dp, gp = s.run(batched=False)
-        a, b = s.prepare(), s.prepare()
+        a, b, c = s.prepare(), s.prepare(), s.prepare()
         for steps in (1, 2, 5):
             d, g = a.pipeline(b, steps)
             assert np.array_equal(d, dp) and np.array_equal(g, gp), steps
-        for p in (a, b):
+        for steps in (1, 2, 3, 7):  # three copies: two batches in flight during each copy-out
+            d, g = a.pipeline([b, c], steps)
+            assert np.array_equal(d, dp) and np.array_equal(g, gp), ("three copies", steps)
+        for p in (a, b, c):
             d, g = p.run()
             assert np.array_equal(d, dp) and np.array_equal(g, gp)
         counts = np.zeros(40, np.int64)
@@ -141,7 +144,7 @@ def test_prepared_batch_pipeline(gpu):
         d, g = a.run()
         assert np.array_equal(d, dp)
     finally:
-        for p in (a, b):
+        for p in (a, b, c):
             if p is not None:
                 p.free()
         s.free()

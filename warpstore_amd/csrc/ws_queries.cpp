@@ -1043,33 +1043,39 @@ void c4_replay(C4Prepared& P, std::vector<uint64_t>* digests, std::vector<int64_
   }
 }
 
-// Steps of a prepared set as a pipeline over two prepared copies of it (their own aggregators,
-// graphs and streams): step s launches copy s % 2, then finishes step s - 1, whose groups are copied
-// to the host while step s's kernels run. digests / groups: of the last step.
-void c4_pipeline(C4Prepared& A, C4Prepared& B, int steps, std::vector<uint64_t>* digests, std::vector<int64_t>& groups) {
-  const size_t n = A.plans.size();
-  if (B.plans.size() != n) throw SqlError("pipelined batch: the two prepared copies differ in size", 1105);
-  C4Prepared* P[2] = {&A, &B};
+// Steps of a prepared set as a pipeline over D prepared copies of it (their own aggregators,
+// graphs and streams): step s launches copy s % D, then finishes step s - (D - 1), whose groups are
+// copied to the host while the D - 1 later steps' kernels run (D = 3 keeps two batches' kernels
+// queued during each copy). digests / groups: of the last step.
+void c4_pipeline(const std::vector<C4Prepared*>& P, int steps, std::vector<uint64_t>* digests, std::vector<int64_t>& groups) {
+  const int D = (int)P.size();
+  if (D < 1) throw SqlError("pipelined batch: no prepared copies", 1105);
+  const size_t n = P[0]->plans.size();
+  int64_t cap = 1;
+  for (const C4Prepared* p : P) {
+    if (p->plans.size() != n) throw SqlError("pipelined batch: the prepared copies differ in size", 1105);
+    cap = std::max(cap, p->cap);
+  }
   std::vector<int64_t> counts(n, 0);
-  t_results.reserve(std::max<int64_t>(std::max(A.cap, B.cap), 1), 1, 5);
-  bool out[2] = {false, false};  // launched, not yet finished
+  t_results.reserve(cap, 1, 5);
+  std::vector<bool> out((size_t)D, false);  // launched, not yet finished
   auto finish = [&](int k) {
     int64_t total = 0;
-    out[k] = false;
-    wsd::ck(ws_agg_batch_finish(P[k]->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs));
+    out[(size_t)k] = false;
+    wsd::ck(ws_agg_batch_finish(P[(size_t)k]->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs));
   };
   try {
     for (int s = 0; s < steps; ++s) {
-      wsd::ck(ws_agg_batch_launch(P[s & 1]->plan));
-      out[s & 1] = true;
-      if (s > 0) finish((s - 1) & 1);
+      wsd::ck(ws_agg_batch_launch(P[(size_t)(s % D)]->plan));
+      out[(size_t)(s % D)] = true;
+      if (s >= D - 1) finish((s - (D - 1)) % D);
     }
-    if (steps > 0) finish((steps - 1) & 1);
+    for (int s = std::max(0, steps - (D - 1)); s < steps; ++s) finish(s % D);
   } catch (...) {
-    for (int k = 0; k < 2; ++k)
-      if (out[k]) {
+    for (int k = 0; k < D; ++k)
+      if (out[(size_t)k]) {
         int64_t total = 0;
-        (void)ws_agg_batch_finish(P[k]->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs);
+        (void)ws_agg_batch_finish(P[(size_t)k]->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs);
       }
     throw;
   }
@@ -1530,13 +1536,15 @@ int wsq_c4_replay(void* prepared, uint64_t* digests, int64_t* groups) {
     }
   });
 }
-// steps batches of the set, pipelined over two prepared copies of it (see c4_pipeline); digests and
-// groups of the last step.
-int wsq_c4_pipeline(void* a, void* b, int steps, uint64_t* digests, int64_t* groups) {
+// steps batches of the set, pipelined over ncopies prepared copies of it (see c4_pipeline); digests
+// and groups of the last step.
+int wsq_c4_pipeline(void* const* copies, int ncopies, int steps, uint64_t* digests, int64_t* groups) {
   return guard([&] {
     std::vector<uint64_t> d;
     std::vector<int64_t> g;
-    c4_pipeline(*(C4Prepared*)a, *(C4Prepared*)b, steps, digests ? &d : nullptr, g);
+    std::vector<C4Prepared*> P;
+    for (int k = 0; k < ncopies; ++k) P.push_back((C4Prepared*)copies[k]);
+    c4_pipeline(P, steps, digests ? &d : nullptr, g);
     for (size_t i = 0; i < g.size(); ++i) {
       if (digests) digests[i] = d[i];
       groups[i] = g[i];

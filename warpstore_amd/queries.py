@@ -53,7 +53,7 @@ def qlib():
         Q.wsq_c4_prepare.restype = I
         Q.wsq_c4_replay.argtypes = [P, P, P]
         Q.wsq_c4_replay.restype = I
-        Q.wsq_c4_pipeline.argtypes = [P, P, I, P, P]
+        Q.wsq_c4_pipeline.argtypes = [P, I, I, P, P]
         Q.wsq_c4_pipeline.restype = I
         Q.wsq_c4_plan_handle.argtypes = [P]
         Q.wsq_c4_plan_handle.restype = C.c_void_p
@@ -191,13 +191,16 @@ class PreparedC4:
         _ck(qlib().wsq_c4_replay(self.h, d.ctypes.data if digests else None, g.ctypes.data))
         return (d if digests else None), g
 
-    def pipeline(self, other: "PreparedC4", steps: int, digests: bool = True):
-        """steps batches alternating between this copy and other (a second prepare() of the same
-        set): each batch's groups are copied to the host while the next batch's kernels run.
-        (digests, group counts) of the last batch."""
+    def pipeline(self, others, steps: int, digests: bool = True):
+        """steps batches taking this copy and others (further prepare()s of the same set) in turn:
+        with D copies, each batch's groups are copied to the host while the next D - 1 batches'
+        kernels run. (digests, group counts) of the last batch."""
+        if isinstance(others, PreparedC4):
+            others = [others]
+        hs = (C.c_void_p * (1 + len(others)))(self.h, *[o.h for o in others])
         d = np.zeros(self.nq, np.uint64)
         g = np.zeros(self.nq, np.int64)
-        _ck(qlib().wsq_c4_pipeline(self.h, other.h, steps, d.ctypes.data if digests else None, g.ctypes.data))
+        _ck(qlib().wsq_c4_pipeline(hs, len(hs), steps, d.ctypes.data if digests else None, g.ctypes.data))
         return (d if digests else None), g
 
     def free(self) -> None:

@@ -700,7 +700,9 @@ def bench_serve(args) -> int:
         Q.set_sizers(*models)
         Q.serve(pool, web, fact, dim, rate=200.0, count=60, mode=mode, workers=workers, seed=1, budget_bytes=budget)
     rows = []
-    for rate in (400.0, 1600.0, 3200.0, 6400.0):
+    # (12800/s saturates both forms: at 6400 the batched form drains only just behind the arrivals,
+    # so its q/s there is bounded by the offered rate, not by its capacity)
+    for rate in (400.0, 1600.0, 3200.0, 6400.0, 12800.0):
         count = int(min(4000, max(400, rate * 1.5)))
         for name, mode, workers, models in forms:
             Q.set_sizers(*models)
@@ -714,10 +716,15 @@ def bench_serve(args) -> int:
     # capacity at equal p99: the highest sustained q/s of each form among the rates where its p99
     # stays within the plain form's p99 at its own highest sustained rate
     summary = {}
+    plain = [r for r in rows if r["form"] == "plain_per_query"]
+    p99_cap = max(plain, key=lambda r: r["qps"])["p99_ms"]
     for name in ("learned_batched", "plain_per_query"):
         rs = [r for r in rows if r["form"] == name]
-        summary[name] = {"max_qps": round(max(r["qps"] for r in rs), 1),
-                         "p99_ms_at_max": round(max(rs, key=lambda r: r["qps"])["p99_ms"], 3)}
+        ok = [r for r in rs if r["p99_ms"] <= p99_cap] or rs
+        best = max(ok, key=lambda r: r["qps"])
+        summary[name] = {"max_qps": round(best["qps"], 1), "p99_ms_at_max": round(best["p99_ms"], 3),
+                         "offered_rate_at_max": best["offered_rate"]}
+    summary["p99_bound_ms"] = round(p99_cap, 3)
     summary["qps_ratio"] = round(summary["learned_batched"]["max_qps"] / summary["plain_per_query"]["max_qps"], 3)
     print(json.dumps({"serving_summary": summary, "models": {"groups_trained_on": gm.samples, "joins_trained_on": jm.samples}}),
           flush=True)

@@ -592,7 +592,18 @@ int ws_copy_kernel(void* dst, const void* src, size_t bytes) {
 
 // ------------------------------------------------------------------ C ABI: profiling
 
-void ws_prof_enable(int on) { t_prof = on != 0; }
+void ws_prof_enable(int on) {
+  t_prof = on != 0;
+  // events for the next 256 profiled launches made now, so that none is created inside a timed loop
+  while (t_prof && t_prof_free.size() < 512) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {  // (no error out of a void entry point: profiling then creates as it goes)
+      (void)hipGetLastError();
+      break;
+    }
+    t_prof_free.push_back(e);
+  }
+}
 void ws_prof_only(const char* kernel) { t_prof_only = kernel ? kernel : ""; }
 void ws_prof_reset(void) {
   for (auto& r : t_prof_recs) {

@@ -365,7 +365,8 @@ def bench_batch(args) -> int:
     batched launch + one batched fetch into page-locked memory, and (separately timed) one by one
     through the per-query path, and as a batch prepared once whose launches replay from a HIP graph
     (the repeated-dashboard form), one replay at a time and pipelined over three prepared copies
-    (each batch's groups copied to the host while the next two batches' kernels run). Every timed form fetches every group to the host; the digests
+    (each batch's groups copied to the host while the next two batches' kernels run), and pipelined
+    three deep with every batch built from scratch (plans, aggregators, descriptors and graph). Every timed form fetches every group to the host; the digests
     that compare the forms are computed after the timing. Unless --no-autotune, the autotuner first
     picks the batched form's tile, LDS table and grid knobs on this workload."""
     import warpstore_amd as ws
@@ -418,10 +419,18 @@ def bench_batch(args) -> int:
     for p in more:
         p.free()
     pr.free()
+    # every batch built from scratch (plans, aggregators, descriptors, graph), pipelined 3 deep
+    s.fresh_pipeline(max(args.warmup, 3), 3, digests=False)
+    L.ws_sync()
+    t0 = time.perf_counter()
+    s.fresh_pipeline(args.steps, 3, digests=False)
+    L.ws_sync()
+    res["fresh_pipelined"] = (time.perf_counter() - t0) / args.steps
+    df, gf = s.fresh_pipeline(4, 3)
     db, gb = s.run(batched=True)
     dp, gp = s.run(batched=False)
     same = bool(np.array_equal(db, dp) and np.array_equal(gb, gp) and np.array_equal(dr, dp) and np.array_equal(gr, gp)
-                and np.array_equal(dl, dp) and np.array_equal(gl, gp))
+                and np.array_equal(dl, dp) and np.array_equal(gl, gp) and np.array_equal(df, dp) and np.array_equal(gf, gp))
     bt, pt = res["batched"], res["per_query"]
     scan = s.rows * 12
     out = {
@@ -434,6 +443,8 @@ def bench_batch(args) -> int:
         "results_equal_per_query": same, "groups": int(gb.sum()), "autotuner": tuned,
         "graph_replay_ms_per_step": round(res["replay"] * 1e3, 4), "graph_replay_value": round(scan / res["replay"] / 1e9, 3),
         "pipelined_ms_per_step": round(res["pipelined"] * 1e3, 4), "pipelined_value": round(scan / res["pipelined"] / 1e9, 3),
+        "fresh_pipelined_ms_per_step": round(res["fresh_pipelined"] * 1e3, 4),
+        "fresh_pipelined_value": round(scan / res["fresh_pipelined"] / 1e9, 3),
         "roofline": roofline("agg_batch_dense_jit (all 256 queries' filter + dense GROUP BY in one launch)", scan, launches,
                              total_ms, None),
     }

@@ -128,6 +128,9 @@ def test_prepared_batch_pipeline(gpu):
         for steps in (1, 2, 3, 7):  # three copies: two batches in flight during each copy-out
             d, g = a.pipeline([b, c], steps)
             assert np.array_equal(d, dp) and np.array_equal(g, gp), ("three copies", steps)
+        for steps, depth in ((1, 3), (4, 3), (3, 1)):  # every batch built from scratch
+            d, g = s.fresh_pipeline(steps, depth)
+            assert np.array_equal(d, dp) and np.array_equal(g, gp), ("fresh", steps, depth)
         for p in (a, b, c):
             d, g = p.run()
             assert np.array_equal(d, dp) and np.array_equal(g, gp)

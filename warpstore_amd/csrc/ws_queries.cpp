@@ -1089,6 +1089,49 @@ void c4_pipeline(const std::vector<C4Prepared*>& P, int steps, std::vector<uint6
   }
 }
 
+// steps batches of the queries, each built from scratch (plans, aggregators, descriptor blob and
+// graph: c4_prepare) and pipelined `depth` deep: batch s is launched before batch s - (depth - 1)
+// is finished and released, so one batch's build, result copy and release overlap the kernels of
+// the batches after it. digests / groups: of the last batch.
+void c4_fresh_pipeline(const std::vector<const C4Query*>& qs, int steps, int depth, std::vector<uint64_t>* digests,
+                       std::vector<int64_t>& groups) {
+  if (depth < 1) throw SqlError("pipelined batch: depth below 1", 1105);
+  const size_t n = qs.size();
+  std::vector<int64_t> counts(n, 0);
+  std::deque<std::unique_ptr<C4Prepared>> inflight;  // launched, not yet finished
+  auto finish_front = [&] {
+    std::unique_ptr<C4Prepared> P = std::move(inflight.front());
+    inflight.pop_front();
+    t_results.reserve(std::max<int64_t>(P->cap, 1), 1, 5);
+    int64_t total = 0;
+    wsd::ck(ws_agg_batch_finish(P->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs));
+  };
+  try {
+    for (int s = 0; s < steps; ++s) {
+      std::unique_ptr<C4Prepared> P(c4_prepare(qs));
+      wsd::ck(ws_agg_batch_launch(P->plan));
+      inflight.push_back(std::move(P));
+      if ((int)inflight.size() >= depth) finish_front();
+    }
+    while (!inflight.empty()) finish_front();
+  } catch (...) {
+    while (!inflight.empty()) {  // the plans' destructors wait for their streams
+      int64_t total = 0;
+      (void)ws_agg_batch_finish(inflight.front()->plan, counts.data(), &total, t_results.cap, t_results.keys, t_results.accs);
+      inflight.pop_front();
+    }
+    throw;
+  }
+  if (digests) digests->assign(n, 0);
+  groups.assign(n, 0);
+  int64_t off = 0;
+  for (size_t i = 0; i < n && steps > 0; ++i) {
+    if (digests) (*digests)[i] = c4_digest(t_results.keys + off, t_results.accs + off * 5, counts[i]);
+    groups[i] = counts[i];
+    off += counts[i];
+  }
+}
+
 // ------------------------------------------------------------------ serving loop (open-loop stream)
 
 // A request of the mixed stream: a small aggregate over one of the pool's segments, the web
@@ -1545,6 +1588,24 @@ int wsq_c4_pipeline(void* const* copies, int ncopies, int steps, uint64_t* diges
     std::vector<C4Prepared*> P;
     for (int k = 0; k < ncopies; ++k) P.push_back((C4Prepared*)copies[k]);
     c4_pipeline(P, steps, digests ? &d : nullptr, g);
+    for (size_t i = 0; i < g.size(); ++i) {
+      if (digests) digests[i] = d[i];
+      groups[i] = g[i];
+    }
+  });
+}
+// steps batches of the set, each built from scratch and pipelined depth deep (see
+// c4_fresh_pipeline); digests and groups of the last batch.
+int wsq_c4_fresh_pipeline(void* set, int steps, int depth, uint64_t* digests, int64_t* groups) {
+  return guard([&] {
+    auto* v = (std::vector<std::pair<DevTable*, int64_t>>*)set;
+    std::vector<C4Query> qs;
+    for (auto& e : *v) qs.push_back(C4Query{e.first, e.second});
+    std::vector<const C4Query*> qp;
+    for (auto& q : qs) qp.push_back(&q);
+    std::vector<uint64_t> d;
+    std::vector<int64_t> g;
+    c4_fresh_pipeline(qp, steps, depth, digests ? &d : nullptr, g);
     for (size_t i = 0; i < g.size(); ++i) {
       if (digests) digests[i] = d[i];
       groups[i] = g[i];

@@ -55,6 +55,8 @@ def qlib():
         Q.wsq_c4_replay.restype = I
         Q.wsq_c4_pipeline.argtypes = [P, I, I, P, P]
         Q.wsq_c4_pipeline.restype = I
+        Q.wsq_c4_fresh_pipeline.argtypes = [P, I, I, P, P]
+        Q.wsq_c4_fresh_pipeline.restype = I
         Q.wsq_c4_plan_handle.argtypes = [P]
         Q.wsq_c4_plan_handle.restype = C.c_void_p
         Q.wsq_c4_release.argtypes = [P]
@@ -154,6 +156,14 @@ class C4Set:
         d = np.zeros(self.nq, np.uint64)
         g = np.zeros(self.nq, np.int64)
         _ck(qlib().wsq_c4_run(self.h, 1 if batched else 0, d.ctypes.data if digests else None, g.ctypes.data))
+        return (d if digests else None), g
+
+    def fresh_pipeline(self, steps: int, depth: int = 3, digests: bool = True):
+        """steps batches of the set, each built from scratch (plans, aggregators, descriptors and
+        graph) and pipelined depth deep; (digests, group counts) of the last batch."""
+        d = np.zeros(self.nq, np.uint64)
+        g = np.zeros(self.nq, np.int64)
+        _ck(qlib().wsq_c4_fresh_pipeline(self.h, steps, depth, d.ctypes.data if digests else None, g.ctypes.data))
         return (d if digests else None), g
 
     def prepare(self) -> "PreparedC4":

@@ -1524,6 +1524,50 @@ static void launch_batch(const BatchBuild& b, int n, bool prof) {
 // A batch of dense aggregations prepared once and replayed: its descriptor blob stays resident,
 // and its launches (initial values, the batched kernel, then the fetch's count, scan and pack
 // kernels) are recorded in a HIP graph; a replay is one graph launch and two read-backs.
+// A prepared batch's stream, events and page-locked offsets buffer, kept for the next plan on the
+// same device once the plan is released (a pipeline that prepares a plan per batch would otherwise
+// create and destroy them per batch, and page-locking memory is a slow call). Idle ones only: a
+// plan's stream is synchronized before its resources come back.
+struct PlanRes {
+  int dev = -1;
+  hipStream_t cs = nullptr;
+  hipEvent_t ev_in = nullptr, ev_cnt = nullptr;
+  uint32_t* hoffs = nullptr;
+  size_t hcap = 0;  // words of hoffs
+};
+static std::mutex g_plan_res_mu;
+static std::vector<PlanRes> g_plan_res;
+static PlanRes plan_res_acquire(size_t words) {
+  int dev = 0;
+  WS_CK(hipGetDevice(&dev));
+  PlanRes r;
+  {
+    std::lock_guard<std::mutex> g(g_plan_res_mu);
+    for (size_t i = g_plan_res.size(); i-- > 0;)
+      if (g_plan_res[i].dev == dev) {
+        r = g_plan_res[i];
+        g_plan_res.erase(g_plan_res.begin() + (std::ptrdiff_t)i);
+        break;
+      }
+  }
+  r.dev = dev;
+  if (!r.cs) WS_CK(hipStreamCreateWithFlags(&r.cs, hipStreamNonBlocking));
+  if (!r.ev_in) WS_CK(hipEventCreateWithFlags(&r.ev_in, hipEventDisableTiming));
+  if (!r.ev_cnt) WS_CK(hipEventCreateWithFlags(&r.ev_cnt, hipEventDisableTiming));
+  if (r.hcap < words) {
+    if (r.hoffs) (void)hipHostFree(r.hoffs);
+    r.hoffs = nullptr;
+    r.hcap = 0;
+    WS_CK(hipHostMalloc((void**)&r.hoffs, sizeof(uint32_t) * words, hipHostMallocDefault));
+    r.hcap = words;
+  }
+  return r;
+}
+static void plan_res_release(const PlanRes& r) {
+  std::lock_guard<std::mutex> g(g_plan_res_mu);
+  g_plan_res.push_back(r);
+}
+
 struct ws_agg_batch_plan {
   int n = 0, nk = 0, na = 0;
   ws_agg_plan plan{};
@@ -1535,6 +1579,7 @@ struct ws_agg_batch_plan {
   ws::DevBuf<uint32_t> cnt, offs;
   ws::DevBuf<double> pk, pa;
   int64_t gcap = 0;
+  PlanRes res;               // from the pool (returned on release)
   hipStream_t cs = nullptr;  // the plan's stream: its graph and read-backs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1542,13 +1587,10 @@ struct ws_agg_batch_plan {
   uint32_t* hoffs = nullptr;                     // pinned: the group offsets of the last launch
   bool launched = false;
   ~ws_agg_batch_plan() {
-    if (launched && cs) (void)hipStreamSynchronize(cs);
-    if (hoffs) (void)hipHostFree(hoffs);
-    if (ev_in) (void)hipEventDestroy(ev_in);
-    if (ev_cnt) (void)hipEventDestroy(ev_cnt);
+    if (cs) (void)hipStreamSynchronize(cs);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (cs) (void)hipStreamDestroy(cs);
+    if (res.cs) plan_res_release(res);
     if (b.dev) ws::tmp_free(b.dev);
   }
 };
@@ -1825,7 +1867,11 @@ int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* e
     P->pk = DevBuf<double>(std::max<int64_t>(P->gcap * P->nk, 1));
     P->pa = DevBuf<double>(std::max<int64_t>(P->gcap * P->na, 1));
     WS_CK(hipStreamSynchronize(stream()));
-    WS_CK(hipStreamCreateWithFlags(&P->cs, hipStreamNonBlocking));
+    P->res = plan_res_acquire((size_t)P->nch + 1);
+    P->cs = P->res.cs;
+    P->ev_in = P->res.ev_in;
+    P->ev_cnt = P->res.ev_cnt;
+    P->hoffs = P->res.hoffs;
     WS_CK(hipStreamBeginCapture(P->cs, hipStreamCaptureModeThreadLocal));
     try {
       StreamOverride so(P->cs);
@@ -1844,9 +1890,6 @@ int ws_agg_batch_prepare(int n, ws_agg* const* aggs, const ws_expr_env* const* e
     }
     WS_CK(hipStreamEndCapture(P->cs, &P->graph));
     WS_CK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
-    WS_CK(hipEventCreateWithFlags(&P->ev_in, hipEventDisableTiming));
-    WS_CK(hipEventCreateWithFlags(&P->ev_cnt, hipEventDisableTiming));
-    WS_CK(hipHostMalloc((void**)&P->hoffs, sizeof(uint32_t) * (size_t)(P->nch + 1), hipHostMallocDefault));
     *out = P.release();
   });
 }

@@ -361,8 +361,10 @@ def bench_filter(args) -> int:
 
 def bench_batch(args) -> int:
     """256 filter + GROUP BY queries (seed 4): rows lognormal in [4096, 4M], keys with 10 to 1e5
-    distinct values, WHERE val > theta_q; 12 B/row scanned. One step = all 256 queries, run as one
-    batched launch + one batched fetch into page-locked memory, and (separately timed) one by one
+    distinct values, WHERE val > theta_q; 12 B/row scanned. One step = all 256 queries, their plans
+    built, run as one batched launch and every group fetched into page-locked memory; `value` times
+    steps pipelined three deep (a step's build and result copy overlap later steps' kernels), and
+    the same steps one at a time are reported beside it. Also timed: the queries one by one
     through the per-query path, and as a batch prepared once whose launches replay from a HIP graph
     (the repeated-dashboard form), one replay at a time and pipelined over three prepared copies
     (each batch's groups copied to the host while the next two batches' kernels run), and pipelined
@@ -431,15 +433,20 @@ def bench_batch(args) -> int:
     dp, gp = s.run(batched=False)
     same = bool(np.array_equal(db, dp) and np.array_equal(gb, gp) and np.array_equal(dr, dp) and np.array_equal(gr, gp)
                 and np.array_equal(dl, dp) and np.array_equal(gl, gp) and np.array_equal(df, dp) and np.array_equal(gf, gp))
-    bt, pt = res["batched"], res["per_query"]
+    bt, pt, ft = res["batched"], res["per_query"], res["fresh_pipelined"]
     scan = s.rows * 12
+    # value: the whole job's throughput, batches built from scratch and pipelined (the same work per
+    # step as one batch at a time, steps overlapped); the roofline is the batched kernel measured
+    # in the one-at-a-time steps, where no other batch's kernel shares the GPU
     out = {
-        "metric": "GB/s scanned by 256 batched filter+GROUP BY queries", "value": round(scan / bt / 1e9, 3),
-        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(bt * 1e3, 4),
+        "metric": "GB/s scanned by 256 batched filter+GROUP BY queries", "value": round(scan / ft / 1e9, 3),
+        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ft * 1e3, 4),
         "higher_is_better": True, "dtype": "int64", "data": "synthetic (seeded generator, seed 4)",
-        "config": {"workload": "256 filter + GROUP BY queries, one batched launch", "rows": s.rows, "bytes_per_row": 12},
-        "queries_per_s": round(256 / bt, 1), "per_query_ms_per_step": round(pt * 1e3, 4),
-        "per_query_queries_per_s": round(256 / pt, 1), "speedup_vs_per_query": round(pt / bt, 3),
+        "config": {"workload": "256 filter + GROUP BY queries per step, each step's plans built from scratch, one batched "
+                               "launch per step, steps pipelined 3 deep", "rows": s.rows, "bytes_per_row": 12},
+        "queries_per_s": round(256 / ft, 1), "per_query_ms_per_step": round(pt * 1e3, 4),
+        "per_query_queries_per_s": round(256 / pt, 1), "speedup_vs_per_query": round(pt / ft, 3),
+        "one_at_a_time_ms_per_step": round(bt * 1e3, 4), "one_at_a_time_value": round(scan / bt / 1e9, 3),
         "results_equal_per_query": same, "groups": int(gb.sum()), "autotuner": tuned,
         "graph_replay_ms_per_step": round(res["replay"] * 1e3, 4), "graph_replay_value": round(scan / res["replay"] / 1e9, 3),
         "pipelined_ms_per_step": round(res["pipelined"] * 1e3, 4), "pipelined_value": round(scan / res["pipelined"] / 1e9, 3),
